@@ -1,0 +1,217 @@
+#!/usr/bin/env python3
+"""Tet10 elasticity DOF/s (assemble + PCG to 1e-10), mixed vs FP64, one GPU.
+
+Workload (default "tet10-160x20x20"): cantilever box 0.4 x 0.05 x 0.05 m,
+160 x 20 x 20 Kuhn cells -> 384,000 Tet10 elements, 539,601 nodes,
+1,618,803 DOF; E = 210 GPa, nu = 0.3, clamped at x = 0, uniform traction
+1 MPa in -z on x = 0.4.  Synthetic mesh, generated on the host before the
+timed region and resident in HBM when it starts.
+
+One step = the whole drop-in path on device-resident inputs: element
+geometry, node-pair CSR pattern (radix sort), stiffness values, traction
+load, Dirichlet-masked PCG to ||b_f - A_ff x_f|| / ||b_f|| <= 1e-10 (FP64
+residual).  `value` is the mixed-precision path (FP32 inner CG on the
+Jacobi-scaled NB3 matrix + FP64 residual replacement); the FP64 path on the
+same mesh is timed beside it (`fp64`).  DOF counts all nodal DOF (3N).
+
+The roofline object describes the dominant kernel of the mixed path, the
+FP32 NB3 SpMV inside PCG: its average duration is measured with HIP events
+on sampled launches during the timed solves (cf_spmv_profile), and its
+algorithmic bytes per launch are 40 nnz_b + 4 (N+1) + 8 n (FP32 3x3 blocks
++ int32 column per block, row pointers, x read once, y written once).
+"""
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+WORKLOADS = {
+    # name: (nx, ny, nz)
+    "tet10-160x20x20": (160, 20, 20),
+    "tet10-80x10x10": (80, 10, 10),
+}
+LIMS = dict(xlim=(0.0, 0.4), ylim=(0.0, 0.05), zlim=(0.0, 0.05))
+E_MOD, NU, TRACTION, TOL = 210e9, 0.3, -1e6, 1e-10
+HBM_PEAK_GBS = 8000.0
+CPU_SAMPLE = (40, 5, 5)
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="tet10-160x20x20", choices=sorted(WORKLOADS))
+    ap.add_argument("--precond64", default="jacobi", choices=["jacobi", "block3"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-only", action="store_true", help=argparse.SUPPRESS)
+    return ap.parse_args()
+
+
+def cpu_baseline():
+    """Oracle (NumPy/SciPy restatement of the reference path: per-element
+    Python assembly, coo->csr, apply_dirichlet_vec3 + SuperLU spsolve) on a
+    bounded sample of the same cantilever, single-threaded."""
+    import numpy as np
+    from oracle import elastic as oel
+    from oracle import meshes as om
+    t0 = time.perf_counter()
+    m = om.box_mesh_tet10(*CPU_SAMPLE, **LIMS)
+    t1 = time.perf_counter()
+    A, rhs = oel.assemble_elasticity_tet10(m, E_MOD, NU, traction={"xmax": (0.0, 0.0, TRACTION)})
+    t2 = time.perf_counter()
+    u = oel.apply_dirichlet_vec3(A, rhs, m, {"xmin": (0.0, 0.0, 0.0)})
+    t3 = time.perf_counter()
+    assert np.isfinite(u).all()
+    n = 3 * m.n_nodes
+    return {"value": n / (t3 - t1), "unit": "DOF/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle assemble + apply_dirichlet_vec3/spsolve on the same cantilever "
+                       f"at {CPU_SAMPLE[0]}x{CPU_SAMPLE[1]}x{CPU_SAMPLE[2]} cells ({n} DOF): "
+                       f"assemble {t2 - t1:.2f} s, BC+solve {t3 - t2:.2f} s; SuperLU fill "
+                       "grows superlinearly, so the full-size rate is lower")}
+
+
+def run_cpu_baseline_subprocess():
+    env = dict(os.environ, OPENBLAS_NUM_THREADS="1", OMP_NUM_THREADS="1", MKL_NUM_THREADS="1")
+    out = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-only"], env=env,
+                         capture_output=True, text=True, timeout=600)
+    if out.returncode != 0:
+        return {"value": None, "error": out.stderr[-500:]}
+    return json.loads(out.stdout.strip().splitlines()[-1])
+
+
+def main():
+    args = parse()
+    if args.cpu_only:
+        print(json.dumps(cpu_baseline()))
+        return 0
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from cloudfem_amd import _native as N
+    from cloudfem_amd.coef import dirichlet_vec3
+    from cloudfem_amd.device import DeviceMesh, build_incidence, to_dev
+    from cloudfem_amd.mesh import box_mesh_tet10
+    from cloudfem_amd.mesh.faces import boundary_faces10
+    from cloudfem_amd.physics.elastic3d import lame
+    from cloudfem_amd.solver import DeviceMatrix, pcg
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    lib = N.load()
+
+    # ---- inputs (untimed): mesh, clamped DOFs, traction faces, all in HBM
+    dims = WORKLOADS[args.workload]
+    mesh = box_mesh_tet10(*dims, **LIMS)
+    n_nodes, n_dof = mesh.n_nodes, 3 * mesh.n_nodes
+    xyz = to_dev(mesh.nodes.T.copy(), torch.float64, dev)
+    conn = to_dev(mesh.elements.astype(np.int32), torch.int32, dev)
+    faces_np = boundary_faces10(mesh, "xmax")
+    faces = to_dev(faces_np.astype(np.int32), torch.int32, dev)
+    coef = torch.zeros((faces_np.shape[0], 3, 3), dtype=torch.float64, device=dev)
+    coef[:, :, 2] = TRACTION
+    mask, fv = dirichlet_vec3(mesh, {"xmin": (0.0, 0.0, 0.0)})
+    fixed = to_dev(mask.astype(np.uint8), torch.uint8, dev)
+    x0 = to_dev(np.where(mask, fv, 0.0), torch.float64, dev)
+    lam, mu = lame(E_MOD, NU)
+    torch.cuda.synchronize()
+
+    def step(precision, precond):
+        dm = DeviceMesh.from_device(xyz[0], xyz[1], xyz[2], conn)
+        vals = dm.matrix_values(N.OP_T10_ELAST, lam=lam, mu=mu)
+        rhs = torch.zeros(n_dof, dtype=torch.float64, device=dev)
+        dm.add_face_rhs_device(N.FACE_T10_TRACTION, faces, coef, rhs,
+                               build_incidence(faces.reshape(-1), n_nodes, dev))
+        x = x0.clone()
+        A = DeviceMatrix(N.FMT_NB3, dm.pattern(), vals)
+        stats = pcg(A, rhs, x, fixed, precision=precision, precond=precond, tol=TOL)
+        return x, stats, dm.pattern().nnz
+
+    def timed(precision, precond):
+        for _ in range(args.warmup):
+            step(precision, precond)
+        torch.cuda.synchronize()
+        lib.cf_spmv_profile(1, None, None, None, None)
+        times, last = [], None
+        for _ in range(args.steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            last = step(precision, precond)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        a64, a32 = ctypes_double(), ctypes_double()
+        c64, c32 = ctypes_int64(), ctypes_int64()
+        lib.cf_spmv_profile(0, byref(a64), byref(c64), byref(a32), byref(c32))
+        return times, last, (a64.value, c64.value, a32.value, c32.value)
+
+    import ctypes
+    ctypes_double, ctypes_int64, byref = ctypes.c_double, ctypes.c_int64, ctypes.byref
+
+    t_mx, (x_mx, st_mx, nnz_b), prof_mx = timed("mixed", "jacobi")
+    t_64, (x_64, st_64, _), prof_64 = timed("fp64", args.precond64)
+
+    ms_mx = 1e3 * float(np.mean(t_mx))
+    ms_64 = 1e3 * float(np.mean(t_64))
+    value = n_dof / (ms_mx / 1e3)
+    diff = float(torch.linalg.vector_norm(x_mx - x_64) / torch.linalg.vector_norm(x_64))
+    u = x_64.cpu().numpy()
+    tip = float(u[2::3][mesh.boundaries["xmax"]].mean())
+
+    # dominant kernel of the mixed path: FP32 NB3 SpMV
+    bytes_32 = 40 * nnz_b + 4 * (n_nodes + 1) + 8 * n_dof
+    avg32_ms = prof_mx[2]
+    achieved = bytes_32 / (avg32_ms / 1e3) / 1e9 if avg32_ms > 0 else None
+    bytes_64 = 76 * nnz_b + 4 * (n_nodes + 1) + 16 * n_dof
+    avg64_ms = prof_64[0]
+
+    line = {
+        "metric": "Tet10 elasticity DOF/s (assemble+PCG to 1e-10), mixed vs FP64 on 1 GPU",
+        "value": value,
+        "unit": "DOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_mx,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload} Tet10 cantilever 0.4x0.05x0.05 m, "
+                               f"{mesh.n_elements} elements, {n_dof} DOF, E=210GPa nu=0.3, "
+                               "clamped x=0, 1 MPa -z traction at x=0.4",
+                   "n_dof": n_dof, "nnz_blocks": nnz_b, "tol": TOL},
+        "fp64": {"value": n_dof / (ms_64 / 1e3), "ms_per_step": ms_64,
+                 "iterations": st_64["iterations"], "rel_residual": st_64["rel_residual"],
+                 "precond": st_64["precond"], "spmv_avg_ms": avg64_ms,
+                 "spmv_gbs": bytes_64 / (avg64_ms / 1e3) / 1e9 if avg64_ms > 0 else None},
+        "mixed": {"iterations": st_mx["iterations"], "residual_updates": st_mx["residual_updates"],
+                  "rel_residual": st_mx["rel_residual"], "rel_diff_vs_fp64": diff},
+        "speedup_mixed_vs_fp64": ms_64 / ms_mx,
+        "tip_uz": tip,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                     "kernel": "spmv_nb3<float,float,float,32> (FP32 NB3 SpMV in PCG)",
+                     "avg_ms": avg32_ms, "launches_sampled": prof_mx[3],
+                     "algorithmic_bytes": bytes_32},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = run_cpu_baseline_subprocess()
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

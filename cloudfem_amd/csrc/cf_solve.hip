@@ -1,0 +1,1051 @@
+// Sparse matrix-vector product and preconditioned CG on the free-DOF system.
+//
+// Replaces scipy's SuperLU spsolve behind apply_dirichlet_vec3
+// (elastic3d.py:220-232) and solve_tet_poisson (tet_poisson.py:121).  The
+// free system A_ff x_f = b_f - A_fF x_F is never formed: fixed DOFs are
+// masked (their rows of A p are zeroed and their entries of r, z, p stay 0),
+// which is the same arithmetic as CG on A[free][:, free].
+//
+// Kernels per CG iteration (all grid-stride over a fixed grid, so the
+// per-block partial sums -- and therefore every dot product -- are
+// bitwise reproducible; the last block to finish folds the partials in a
+// fixed order, no floating-point atomics):
+//   1. spmv      Ap = A p (masked), p.Ap             -> alpha
+//   2. update    x += alpha p, r -= alpha Ap, z = M^-1 r, r.r, r.z -> beta
+//   3. direction p = z + beta p
+// A device-side halt flag makes the remaining kernels of a launched batch
+// exit at once after convergence, so the host polls only once per batch.
+//
+// Mixed precision (CF_PREC_MIXED): the inner iteration runs entirely in FP32
+// on the Jacobi-scaled matrix S A S (S = diag(A)^-1/2, FP32 copy of the
+// values, 4 bytes per stored entry instead of 8) with FP64 accumulation of
+// dot products.  The solution is accumulated in FP64 and the residual is
+// periodically replaced by the FP64 residual b - A x computed with the FP64
+// matrix ("reliable updates", Clark et al., Comput. Phys. Commun. 181 (2010)
+// 1517), so the stopping test is applied to the true FP64 residual.
+#include <cmath>
+#include <vector>
+
+#include "cf_common.hpp"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr int kXcd = 8;  // MI355X: 8 accelerator complex dies, 32 CUs each
+
+enum Status : int32_t { kRunning = 0, kConverged = 1, kMaxit = 2, kBreakdown = 3, kRefresh = 4 };
+
+struct PcgState {
+  double rz;      // r.z of the current residual
+  double rr;      // r.r (recursive residual; scaled residual in mixed mode)
+  double pAp;
+  double alpha;
+  double beta;
+  double stop2;   // stop when rr <= stop2
+  double refresh2;// mixed: request an FP64 residual replacement when rr <= refresh2
+  double pad;
+  int32_t iter;
+  int32_t maxit;
+  int32_t halt;   // Status
+  uint32_t ticket;
+};
+
+// ------------------------------------------------------------- reductions
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <int L>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int off = L / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <int L>
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+  for (int off = L / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Block sum of NV values; result valid in every thread.
+template <int NV>
+__device__ void block_sum(double (&v)[NV]) {
+  __shared__ double sh[NV][kWaves];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sh[k][wave] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) s += sh[k][w];
+    v[k] = s;
+  }
+}
+
+// Grid-wide deterministic sum.  Every block contributes its partials; the
+// block that arrives last sums them in fixed order and returns true (in all
+// of its threads) with the totals in v.
+template <int NV>
+__device__ bool grid_sum(double (&v)[NV], double* __restrict__ partials,
+                         uint32_t* ticket) {
+  __shared__ bool is_last;
+  block_sum<NV>(v);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+      __hip_atomic_store(&partials[k * gridDim.x + blockIdx.x], v[k], __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    uint32_t t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    is_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return false;
+  __threadfence();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = 0.0;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x)
+      s += __hip_atomic_load(&partials[k * gridDim.x + b], __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    v[k] = s;
+  }
+  block_sum<NV>(v);
+  if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return true;
+}
+
+// Contiguous row range of a block, with blocks of one XCD covering one
+// contiguous eighth of the rows (the hardware deals workgroups round-robin
+// to the XCDs), so each XCD's L2 holds the part of x its rows gather.
+__device__ __forceinline__ void block_rows(int64_t n_rows, int64_t& r0, int64_t& r1) {
+  const int64_t G = gridDim.x;
+  int64_t b = blockIdx.x;
+  if (G % kXcd == 0) b = (b % kXcd) * (G / kXcd) + b / kXcd;
+  const int64_t chunk = (n_rows + G - 1) / G;
+  r0 = b * chunk;
+  r1 = r0 + chunk < n_rows ? r0 + chunk : n_rows;
+}
+
+// ------------------------------------------------------------- SpMV
+
+template <class T>
+__device__ __forceinline__ T ld(const T* p) { return __builtin_nontemporal_load(p); }
+
+// y = A x for NB3 storage, L lanes per node row.  If `state` is set this is
+// the CG product: y is masked, p.Ap is reduced and alpha = rz / pAp.
+template <class VT, class XT, class AT, int L>
+__global__ void __launch_bounds__(kBlock)
+spmv_nb3(int64_t n_nodes, const int32_t* __restrict__ ptr,
+         const int32_t* __restrict__ idx, const VT* __restrict__ val,
+         const XT* __restrict__ x, XT* __restrict__ y,
+         const uint8_t* __restrict__ fixed, PcgState* state,
+         double* __restrict__ partials) {
+  if (state && state->halt) return;
+  const int lane = threadIdx.x % L;
+  const int group = threadIdx.x / L;
+  constexpr int kGroups = kBlock / L;
+  int64_t r0, r1;
+  block_rows(n_nodes, r0, r1);
+  double dot = 0.0;
+  for (int64_t i = r0 + group; i < r1; i += kGroups) {
+    const int32_t base = ptr[i];
+    const int32_t len = ptr[i + 1] - base;
+    const int32_t l3 = 3 * len, nv = 9 * len;
+    const VT* v = val + int64_t(9) * base;
+    const int32_t* col = idx + base;
+    AT a0 = 0, a1 = 0, a2 = 0;
+    for (int32_t f = lane; f < nv; f += L) {
+      const int c = (f >= l3) + (f >= 2 * l3);
+      const int32_t q = f - c * l3;
+      const int32_t kq = q / 3;
+      const int cp = q - 3 * kq;
+      const AT prod = AT(ld(v + f)) * AT(x[3 * int64_t(col[kq]) + cp]);
+      a0 += c == 0 ? prod : AT(0);
+      a1 += c == 1 ? prod : AT(0);
+      a2 += c == 2 ? prod : AT(0);
+    }
+    a0 = group_sum<L>(a0);
+    a1 = group_sum<L>(a1);
+    a2 = group_sum<L>(a2);
+    if (lane == 0) {
+      AT o[3] = {a0, a1, a2};
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int64_t d = 3 * i + c;
+        if (fixed && fixed[d]) o[c] = AT(0);
+        y[d] = XT(o[c]);
+        if (state) dot += double(x[d]) * double(XT(o[c]));
+      }
+    }
+  }
+  if (!state) return;
+  double v1[1] = {dot};
+  if (grid_sum<1>(v1, partials, &state->ticket) && threadIdx.x == 0) {
+    state->pAp = v1[0];
+    if (!(v1[0] > 0.0) || !isfinite(v1[0])) {
+      state->halt = kBreakdown;
+    } else {
+      state->alpha = state->rz / v1[0];
+    }
+  }
+}
+
+template <class VT, class XT, class AT, int L>
+__global__ void __launch_bounds__(kBlock)
+spmv_csr(int64_t n_rows, const int32_t* __restrict__ ptr,
+         const int32_t* __restrict__ idx, const VT* __restrict__ val,
+         const XT* __restrict__ x, XT* __restrict__ y,
+         const uint8_t* __restrict__ fixed, PcgState* state,
+         double* __restrict__ partials) {
+  if (state && state->halt) return;
+  const int lane = threadIdx.x % L;
+  const int group = threadIdx.x / L;
+  constexpr int kGroups = kBlock / L;
+  int64_t r0, r1;
+  block_rows(n_rows, r0, r1);
+  double dot = 0.0;
+  for (int64_t i = r0 + group; i < r1; i += kGroups) {
+    const int32_t b = ptr[i], e = ptr[i + 1];
+    AT a = 0;
+    for (int32_t t = b + lane; t < e; t += L) a += AT(ld(val + t)) * AT(x[idx[t]]);
+    a = group_sum<L>(a);
+    if (lane == 0) {
+      if (fixed && fixed[i]) a = AT(0);
+      y[i] = XT(a);
+      if (state) dot += double(x[i]) * double(XT(a));
+    }
+  }
+  if (!state) return;
+  double v1[1] = {dot};
+  if (grid_sum<1>(v1, partials, &state->ticket) && threadIdx.x == 0) {
+    state->pAp = v1[0];
+    if (!(v1[0] > 0.0) || !isfinite(v1[0])) {
+      state->halt = kBreakdown;
+    } else {
+      state->alpha = state->rz / v1[0];
+    }
+  }
+}
+
+// ------------------------------------------------------------- preconditioners
+
+// Jacobi: dinv[i] = 1 / A_ii (0 on fixed DOFs).  Block3: per node the inverse
+// of the 3x3 diagonal block restricted to the free components.
+__global__ void diag_csr(int64_t n, const int32_t* __restrict__ ptr,
+                         const int32_t* __restrict__ idx, const double* __restrict__ val,
+                         const uint8_t* __restrict__ fixed, double* __restrict__ dinv,
+                         int32_t* __restrict__ bad) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    double d = 0.0;
+    for (int32_t t = ptr[i]; t < ptr[i + 1]; ++t)
+      if (idx[t] == i) d = val[t];
+    if (fixed && fixed[i]) {
+      dinv[i] = 0.0;
+    } else if (d > 0.0) {
+      dinv[i] = 1.0 / d;
+    } else {
+      dinv[i] = 0.0;
+      atomicAdd(bad, 1);
+    }
+  }
+}
+
+__device__ __forceinline__ bool inv3(const double a[3][3], double o[3][3]) {
+  const double c00 = a[1][1] * a[2][2] - a[1][2] * a[2][1];
+  const double c01 = a[1][2] * a[2][0] - a[1][0] * a[2][2];
+  const double c02 = a[1][0] * a[2][1] - a[1][1] * a[2][0];
+  const double det = a[0][0] * c00 + a[0][1] * c01 + a[0][2] * c02;
+  if (!(det > 0.0)) return false;
+  const double r = 1.0 / det;
+  o[0][0] = c00 * r;
+  o[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) * r;
+  o[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) * r;
+  o[1][0] = c01 * r;
+  o[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) * r;
+  o[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) * r;
+  o[2][0] = c02 * r;
+  o[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) * r;
+  o[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) * r;
+  return true;
+}
+
+__global__ void diag_nb3(int64_t n_nodes, const int32_t* __restrict__ ptr,
+                         const int32_t* __restrict__ idx, const double* __restrict__ val,
+                         const uint8_t* __restrict__ fixed, int block3,
+                         double* __restrict__ dinv, int32_t* __restrict__ bad) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n_nodes;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int32_t base = ptr[i], len = ptr[i + 1] - base;
+    int32_t kd = -1;
+    for (int32_t t = 0; t < len; ++t)
+      if (idx[base + t] == i) kd = t;
+    double blk[3][3] = {};
+    if (kd >= 0)
+      for (int c = 0; c < 3; ++c)
+        for (int cp = 0; cp < 3; ++cp)
+          blk[c][cp] = val[int64_t(9) * base + c * 3 * len + 3 * kd + cp];
+    bool fx[3];
+    for (int c = 0; c < 3; ++c) fx[c] = fixed && fixed[3 * i + c];
+    if (!block3) {
+      for (int c = 0; c < 3; ++c) {
+        const double d = blk[c][c];
+        if (fx[c]) {
+          dinv[3 * i + c] = 0.0;
+        } else if (d > 0.0) {
+          dinv[3 * i + c] = 1.0 / d;
+        } else {
+          dinv[3 * i + c] = 0.0;
+          atomicAdd(bad, 1);
+        }
+      }
+      continue;
+    }
+    // Restrict to free components: fixed rows/columns -> identity, then zero
+    // them in the inverse.
+    for (int c = 0; c < 3; ++c)
+      if (fx[c])
+        for (int t = 0; t < 3; ++t) blk[c][t] = blk[t][c] = (c == t) ? 1.0 : 0.0;
+    double o[3][3];
+    if (!inv3(blk, o)) {
+      atomicAdd(bad, 1);
+      for (int t = 0; t < 9; ++t) dinv[9 * i + t] = 0.0;
+      continue;
+    }
+    for (int c = 0; c < 3; ++c)
+      for (int cp = 0; cp < 3; ++cp) dinv[9 * i + 3 * c + cp] = (fx[c] || fx[cp]) ? 0.0 : o[c][cp];
+  }
+}
+
+template <int PC>
+__device__ __forceinline__ void precond(const double* __restrict__ dinv,
+                                        const double* __restrict__ r, int64_t node,
+                                        double z[3]) {
+  if constexpr (PC == CF_PC_BLOCK3) {
+    const double* m = dinv + 9 * node;
+    const double r0 = r[3 * node], r1 = r[3 * node + 1], r2 = r[3 * node + 2];
+    z[0] = m[0] * r0 + m[1] * r1 + m[2] * r2;
+    z[1] = m[3] * r0 + m[4] * r1 + m[5] * r2;
+    z[2] = m[6] * r0 + m[7] * r1 + m[8] * r2;
+  } else {
+    z[0] = dinv[node] * r[node];
+  }
+}
+
+// ------------------------------------------------------------- FP64 CG
+
+// x += alpha p; r -= alpha Ap; z = M^-1 r; reduce r.r and r.z.
+// Units: DOFs (Jacobi) or nodes (block3, 3 DOFs each).
+template <int PC>
+__global__ void __launch_bounds__(kBlock)
+cg_update(int64_t n_units, double* __restrict__ x, double* __restrict__ r,
+          const double* __restrict__ p, const double* __restrict__ Ap,
+          const double* __restrict__ dinv, PcgState* state,
+          double* __restrict__ partials) {
+  if (state->halt) return;
+  constexpr int W = (PC == CF_PC_BLOCK3) ? 3 : 1;
+  const double alpha = state->alpha;
+  double rr = 0.0, rz = 0.0;
+  for (int64_t u = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; u < n_units;
+       u += int64_t(gridDim.x) * blockDim.x) {
+#pragma unroll
+    for (int c = 0; c < W; ++c) {
+      const int64_t d = W * u + c;
+      x[d] += alpha * p[d];
+      const double rn = r[d] - alpha * Ap[d];
+      r[d] = rn;
+      rr += rn * rn;
+    }
+    double z[3];
+    precond<PC>(dinv, r, u, z);
+#pragma unroll
+    for (int c = 0; c < W; ++c) rz += r[W * u + c] * z[c];
+  }
+  double v[2] = {rr, rz};
+  if (grid_sum<2>(v, partials, &state->ticket) && threadIdx.x == 0) {
+    state->iter += 1;
+    if (!isfinite(v[0]) || !isfinite(v[1])) {
+      state->halt = kBreakdown;
+    } else if (v[0] <= state->stop2) {
+      state->rr = v[0];
+      state->halt = kConverged;
+    } else if (state->iter >= state->maxit) {
+      state->rr = v[0];
+      state->halt = kMaxit;
+    } else {
+      state->beta = v[1] / state->rz;
+      state->rz = v[1];
+      state->rr = v[0];
+    }
+  }
+}
+
+template <int PC>
+__global__ void __launch_bounds__(kBlock)
+cg_direction(int64_t n_units, const double* __restrict__ r, double* __restrict__ p,
+             const double* __restrict__ dinv, const PcgState* state) {
+  if (state->halt) return;
+  constexpr int W = (PC == CF_PC_BLOCK3) ? 3 : 1;
+  const double beta = state->beta;
+  for (int64_t u = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; u < n_units;
+       u += int64_t(gridDim.x) * blockDim.x) {
+    double z[3];
+    precond<PC>(dinv, r, u, z);
+#pragma unroll
+    for (int c = 0; c < W; ++c) p[W * u + c] = z[c] + beta * p[W * u + c];
+  }
+}
+
+// ------------------------------------------------------------- setup helpers
+
+// out = (fixed ? keep_fixed : keep_free) ? x : 0
+__global__ void split_vector(int64_t n, const double* __restrict__ x,
+                             const uint8_t* __restrict__ fixed, int want_fixed,
+                             double* __restrict__ out) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const bool f = fixed && fixed[i];
+    out[i] = (f == bool(want_fixed)) ? x[i] : 0.0;
+  }
+}
+
+// r = (b - Ax) on free DOFs, 0 on fixed ones; reduce r.r into state->pad
+// (mode 0: also store as the reference norm) or into rr (mode 1).
+__global__ void __launch_bounds__(kBlock)
+residual(int64_t n, const double* __restrict__ b, const double* __restrict__ Ax,
+         const uint8_t* __restrict__ fixed, double* __restrict__ r, PcgState* state,
+         double* __restrict__ partials, double* __restrict__ out_norm2) {
+  double rr = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const double v = (fixed && fixed[i]) ? 0.0 : b[i] - Ax[i];
+    r[i] = v;
+    rr += v * v;
+  }
+  double v1[1] = {rr};
+  if (grid_sum<1>(v1, partials, &state->ticket) && threadIdx.x == 0) *out_norm2 = v1[0];
+}
+
+// p = z = M^-1 r, state->rz = r.z, state->rr = r.r
+template <int PC>
+__global__ void __launch_bounds__(kBlock)
+cg_start(int64_t n_units, const double* __restrict__ r, double* __restrict__ p,
+         const double* __restrict__ dinv, PcgState* state, double* __restrict__ partials) {
+  constexpr int W = (PC == CF_PC_BLOCK3) ? 3 : 1;
+  double rr = 0.0, rz = 0.0;
+  for (int64_t u = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; u < n_units;
+       u += int64_t(gridDim.x) * blockDim.x) {
+    double z[3];
+    precond<PC>(dinv, r, u, z);
+#pragma unroll
+    for (int c = 0; c < W; ++c) {
+      p[W * u + c] = z[c];
+      rz += r[W * u + c] * z[c];
+      rr += r[W * u + c] * r[W * u + c];
+    }
+  }
+  double v[2] = {rr, rz};
+  if (grid_sum<2>(v, partials, &state->ticket) && threadIdx.x == 0) {
+    state->rr = v[0];
+    state->rz = v[1];
+  }
+}
+
+// ------------------------------------------------------------- mixed precision
+
+// S_i = 1/sqrt(A_ii) on free DOFs, 0 on fixed (from the Jacobi dinv).
+__global__ void make_scale(int64_t n, const double* __restrict__ dinv, double* __restrict__ s) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    s[i] = dinv[i] > 0.0 ? sqrt(dinv[i]) : 0.0;
+}
+
+// FP32 values of S A S in the same storage order.
+__global__ void scale_nb3_to_f32(int64_t n_nodes, const int32_t* __restrict__ ptr,
+                                 const int32_t* __restrict__ idx,
+                                 const double* __restrict__ val, const double* __restrict__ s,
+                                 float* __restrict__ out) {
+  // one node row per 64-lane wave
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (blockIdx.x * int64_t(blockDim.x) + threadIdx.x) >> 6;
+  const int64_t nwaves = (int64_t(gridDim.x) * blockDim.x) >> 6;
+  for (int64_t i = wave; i < n_nodes; i += nwaves) {
+    const int32_t base = ptr[i], len = ptr[i + 1] - base, l3 = 3 * len;
+    for (int32_t f = lane; f < 9 * len; f += 64) {
+      const int c = (f >= l3) + (f >= 2 * l3);
+      const int32_t q = f - c * l3, kq = q / 3;
+      const int cp = q - 3 * kq;
+      const int64_t t = int64_t(9) * base + f;
+      out[t] = float(s[3 * i + c] * val[t] * s[3 * int64_t(idx[base + kq]) + cp]);
+    }
+  }
+}
+
+__global__ void scale_csr_to_f32(int64_t n, const int32_t* __restrict__ ptr,
+                                 const int32_t* __restrict__ idx,
+                                 const double* __restrict__ val, const double* __restrict__ s,
+                                 float* __restrict__ out) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    for (int32_t t = ptr[i]; t < ptr[i + 1]; ++t) out[t] = float(s[i] * val[t] * s[idx[t]]);
+}
+
+// rh = fp32(S r), ph = rh (if start), rr = rh.rh; used at start and at every
+// residual replacement.  `state->refresh2` is set from the new rr.
+__global__ void __launch_bounds__(kBlock)
+mx_load_residual(int64_t n, const double* __restrict__ r, const double* __restrict__ s,
+                 float* __restrict__ rh, float* __restrict__ ph, int start,
+                 PcgState* state, double* __restrict__ partials, double refresh_factor2) {
+  double rr = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const float v = float(s[i] * r[i]);
+    rh[i] = v;
+    if (start) ph[i] = v;
+    rr += double(v) * double(v);
+  }
+  double v1[1] = {rr};
+  if (grid_sum<1>(v1, partials, &state->ticket) && threadIdx.x == 0) {
+    if (start) {
+      state->rz = v1[0];
+    } else {
+      // keep the search direction: beta = new r.r / previous r.r
+      state->beta = v1[0] / state->rz;
+      state->rz = v1[0];
+    }
+    state->rr = v1[0];
+    state->refresh2 = v1[0] * refresh_factor2;
+  }
+}
+
+// y += alpha p; r -= alpha Ap; reduce r.r (FP32 vectors, FP64 sums).
+__global__ void __launch_bounds__(kBlock)
+mx_update(int64_t n, float* __restrict__ y, float* __restrict__ rh,
+          const float* __restrict__ ph, const float* __restrict__ Aph, PcgState* state,
+          double* __restrict__ partials) {
+  if (state->halt) return;
+  const float alpha = float(state->alpha);
+  double rr = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    y[i] += alpha * ph[i];
+    const float rn = rh[i] - alpha * Aph[i];
+    rh[i] = rn;
+    rr += double(rn) * double(rn);
+  }
+  double v1[1] = {rr};
+  if (grid_sum<1>(v1, partials, &state->ticket) && threadIdx.x == 0) {
+    state->iter += 1;
+    const double rn = v1[0];
+    if (!isfinite(rn)) {
+      state->halt = kBreakdown;
+    } else if (rn <= state->stop2 || rn <= state->refresh2 || state->iter >= state->maxit) {
+      // leave rz at the previous r.r: after the replacement the direction
+      // update uses beta = (S r_true).(S r_true) / previous r.r
+      state->rr = rn;
+      state->halt = kRefresh;
+    } else {
+      state->beta = rn / state->rz;
+      state->rz = rn;
+      state->rr = rn;
+    }
+  }
+}
+
+__global__ void mx_direction(int64_t n, const float* __restrict__ rh, float* __restrict__ ph,
+                             const PcgState* state) {
+  if (state->halt) return;
+  const float beta = float(state->beta);
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    ph[i] = rh[i] + beta * ph[i];
+}
+
+// After a residual replacement the direction continues with the new beta.
+__global__ void mx_direction_forced(int64_t n, const float* __restrict__ rh,
+                                    float* __restrict__ ph, const PcgState* state) {
+  const float beta = float(state->beta);
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    ph[i] = rh[i] + beta * ph[i];
+}
+
+// x += S y; y = 0
+__global__ void mx_accumulate(int64_t n, double* __restrict__ x, float* __restrict__ y,
+                              const double* __restrict__ s) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    x[i] += s[i] * double(y[i]);
+    y[i] = 0.0f;
+  }
+}
+
+// ------------------------------------------------------------- host side
+
+// Optional timing of the CG product kernels: HIP events around every 8th
+// SpMV launch of a solve, on the solve's stream.  Averages are kept per
+// value type (FP64 / FP32) until cf_spmv_profile_reset().
+struct SpmvProfile {
+  bool enabled = false;
+  double ms[2] = {0.0, 0.0};
+  int64_t launches[2] = {0, 0};
+  std::vector<hipEvent_t> pool;
+  std::vector<std::pair<int, int>> pending;  // (event index, precision slot)
+  size_t used = 0;
+};
+SpmvProfile g_prof;
+constexpr int kProfileEvery = 8;
+
+hipEvent_t prof_event() {
+  if (g_prof.used == g_prof.pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    g_prof.pool.push_back(e);
+  }
+  return g_prof.pool[g_prof.used++];
+}
+
+void prof_begin(int slot, int iter, hipStream_t s, int* token) {
+  *token = -1;
+  if (!g_prof.enabled || iter % kProfileEvery) return;
+  hipEvent_t a = prof_event(), b = prof_event();
+  if (!a || !b) return;
+  hipEventRecord(a, s);
+  *token = int(g_prof.used - 2);
+  g_prof.pending.push_back({*token, slot});
+}
+
+void prof_end(int token, hipStream_t s) {
+  if (token >= 0) hipEventRecord(g_prof.pool[token + 1], s);
+}
+
+// Called after the stream is synchronised: fold finished pairs in.
+void prof_collect() {
+  for (auto& pr : g_prof.pending) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, g_prof.pool[pr.first], g_prof.pool[pr.first + 1]) ==
+        hipSuccess) {
+      g_prof.ms[pr.second] += ms;
+      g_prof.launches[pr.second] += 1;
+    }
+  }
+  g_prof.pending.clear();
+  g_prof.used = 0;
+}
+
+int n_dofs(const cf_matrix* A) { return A->format == CF_FMT_NB3 ? 3 * A->n_rows : A->n_rows; }
+
+// Lanes per row for the SpMV kernels: NB3 rows of Tet10 meshes hold 10..65
+// blocks (90..585 values), P1 scalar rows 5..15 entries.
+constexpr int kLanesNb3 = 32;
+constexpr int kLanesCsr = 8;
+
+template <class VT, class XT, class AT>
+int launch_spmv(const cf_matrix* A, const VT* vals, const XT* x, XT* y, const uint8_t* fixed,
+                PcgState* state, double* partials, int grid, hipStream_t s) {
+  if (A->format == CF_FMT_NB3) {
+    spmv_nb3<VT, XT, AT, kLanesNb3><<<grid, kBlock, 0, s>>>(A->n_rows, A->indptr, A->indices,
+                                                            vals, x, y, fixed, state, partials);
+  } else {
+    spmv_csr<VT, XT, AT, kLanesCsr><<<grid, kBlock, 0, s>>>(A->n_rows, A->indptr, A->indices,
+                                                            vals, x, y, fixed, state, partials);
+  }
+  CF_LAUNCHED();
+  return CF_OK;
+}
+
+struct Layout {
+  double* r;
+  double* p;
+  double* Ap;
+  double* dinv;
+  double* partials;
+  PcgState* state;
+  int32_t* bad;
+  // mixed only
+  double* s;
+  float* rh;
+  float* ph;
+  float* Aph;
+  float* yh;
+  float* vals32;
+  size_t total;
+};
+
+int pcg_grid() {
+  int g = cf::cu_count() * 8;
+  return g - g % kXcd;
+}
+
+Layout layout(const cf_matrix* A, int precision, int precond, void* base) {
+  const int64_t n = n_dofs(A);
+  const int64_t nval = A->format == CF_FMT_NB3 ? 9 * A->nnz : A->nnz;
+  cf::Carve c(base);
+  Layout L{};
+  L.r = c.take<double>(n);
+  L.p = c.take<double>(n);
+  L.Ap = c.take<double>(n);
+  L.dinv = c.take<double>(precond == CF_PC_BLOCK3 ? 3 * n : n);
+  L.partials = c.take<double>(4 * size_t(pcg_grid()));
+  L.state = c.take<PcgState>(1);
+  L.bad = c.take<int32_t>(1);
+  if (precision == CF_PREC_MIXED) {
+    L.s = c.take<double>(n);
+    L.rh = c.take<float>(n);
+    L.ph = c.take<float>(n);
+    L.Aph = c.take<float>(n);
+    L.yh = c.take<float>(n);
+    L.vals32 = c.take<float>(nval);
+  }
+  L.total = c.used;
+  return L;
+}
+
+int check_matrix(const cf_matrix* A) {
+  CF_REQUIRE(A && A->indptr && A->values && A->n_rows > 0 && A->nnz >= 0,
+             "cf_matrix: bad matrix");
+  CF_REQUIRE(A->format == CF_FMT_CSR || A->format == CF_FMT_NB3, "cf_matrix: bad format %d",
+             A->format);
+  return CF_OK;
+}
+
+int read_state(const PcgState* d, PcgState* h, hipStream_t s) {
+  CF_HIP(hipMemcpyAsync(h, d, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+  CF_HIP(hipStreamSynchronize(s));
+  return CF_OK;
+}
+
+}  // namespace
+
+extern "C" int cf_spmv(const cf_matrix* A, const double* x, double* y, const uint8_t* fixed,
+                       void* stream) {
+  int rc = check_matrix(A);
+  if (rc) return rc;
+  CF_REQUIRE(x && y, "cf_spmv: null vector");
+  hipStream_t s = cf::as_stream(stream);
+  return launch_spmv<double, double, double>(A, A->values, x, y, fixed, nullptr, nullptr,
+                                             pcg_grid(), s);
+}
+
+extern "C" int cf_pcg_workspace_bytes(const cf_matrix* A, int32_t precision, int32_t precond,
+                                      size_t* bytes) {
+  int rc = check_matrix(A);
+  if (rc) return rc;
+  CF_REQUIRE(bytes, "cf_pcg_workspace_bytes: null output");
+  *bytes = layout(A, precision, precond, nullptr).total;
+  return CF_OK;
+}
+
+namespace {
+
+template <int PC>
+int pcg_fp64(const cf_matrix* A, const double* b, double* x, const uint8_t* fixed, double tol,
+             int maxit, const Layout& L, cf_pcg_stats* st, hipStream_t s) {
+  const int64_t n = n_dofs(A);
+  const int64_t units = PC == CF_PC_BLOCK3 ? A->n_rows : n;
+  const int G = pcg_grid();
+  int rc;
+  PcgState h{};
+  // reference rhs: b_f - A_fF x_F
+  split_vector<<<G, kBlock, 0, s>>>(n, x, fixed, 1, L.p);
+  CF_LAUNCHED();
+  if ((rc = launch_spmv<double, double, double>(A, A->values, L.p, L.Ap, fixed, nullptr,
+                                                nullptr, G, s)))
+    return rc;
+  residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->pad);
+  CF_LAUNCHED();
+  // r = b - A x  (full x, warm start included)
+  if ((rc = launch_spmv<double, double, double>(A, A->values, x, L.Ap, fixed, nullptr, nullptr,
+                                                G, s)))
+    return rc;
+  residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->rr);
+  CF_LAUNCHED();
+  cg_start<PC><<<G, kBlock, 0, s>>>(units, L.r, L.p, L.dinv, L.state, L.partials);
+  CF_LAUNCHED();
+  if ((rc = read_state(L.state, &h, s))) return rc;
+  const double bb = h.pad;
+  st->b_norm = std::sqrt(bb);
+  st->iterations = 0;
+  st->residual_updates = 0;
+  if (!std::isfinite(bb) || !std::isfinite(h.rr)) {
+    cf::set_error("cf_pcg_solve: non-finite right-hand side");
+    return CF_ERR_BREAKDOWN;
+  }
+  if (bb == 0.0) {
+    // zero reduced right-hand side: the solution of the free block is 0
+    split_vector<<<G, kBlock, 0, s>>>(n, x, fixed, 1, x);
+    CF_LAUNCHED();
+    CF_HIP(hipStreamSynchronize(s));
+    st->rel_residual = 0.0;
+    return CF_OK;
+  }
+  h.stop2 = tol * tol * bb;
+  h.maxit = maxit;
+  h.iter = 0;
+  h.halt = h.rr <= h.stop2 ? kConverged : kRunning;
+  h.ticket = 0;
+  CF_HIP(hipMemcpyAsync(L.state, &h, sizeof(PcgState), hipMemcpyHostToDevice, s));
+  int batch = 8;
+  double prev_rr = h.rr;
+  int prev_it = 0;
+  while (h.halt == kRunning) {
+    for (int t = 0; t < batch; ++t) {
+      int tok;
+      prof_begin(0, h.iter + t, s, &tok);
+      if ((rc = launch_spmv<double, double, double>(A, A->values, L.p, L.Ap, fixed, L.state,
+                                                    L.partials, G, s)))
+        return rc;
+      prof_end(tok, s);
+      cg_update<PC><<<G, kBlock, 0, s>>>(units, x, L.r, L.p, L.Ap, L.dinv, L.state, L.partials);
+      CF_LAUNCHED();
+      cg_direction<PC><<<G, kBlock, 0, s>>>(units, L.r, L.p, L.dinv, L.state);
+      CF_LAUNCHED();
+    }
+    if ((rc = read_state(L.state, &h, s))) return rc;
+    prof_collect();
+    // size the next batch from the observed convergence rate
+    const int done_it = h.iter - prev_it;
+    if (h.halt == kRunning && done_it > 0 && h.rr < prev_rr) {
+      const double rate = std::log(h.rr / prev_rr) / done_it;  // < 0
+      const double need = std::log(h.stop2 / h.rr) / rate;
+      int nb = int(need * 0.9) + 4;
+      nb = nb < 8 ? 8 : (nb > 512 ? 512 : nb);
+      batch = nb;
+    } else {
+      batch = batch * 2 > 512 ? 512 : batch * 2;
+    }
+    prev_rr = h.rr;
+    prev_it = h.iter;
+  }
+  st->iterations = h.iter;
+  // true FP64 residual of the free system
+  if ((rc = launch_spmv<double, double, double>(A, A->values, x, L.Ap, fixed, nullptr, nullptr,
+                                                G, s)))
+    return rc;
+  residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->pad);
+  CF_LAUNCHED();
+  PcgState fin{};
+  if ((rc = read_state(L.state, &fin, s))) return rc;
+  st->rel_residual = std::sqrt(fin.pad / bb);
+  if (h.halt == kBreakdown) {
+    cf::set_error("cf_pcg_solve: breakdown after %d iterations (NaN/Inf or p.Ap <= 0)", h.iter);
+    return CF_ERR_BREAKDOWN;
+  }
+  if (h.halt == kMaxit) {
+    cf::set_error("cf_pcg_solve: not converged after %d iterations (rel. residual %.3e)",
+                  h.iter, st->rel_residual);
+    return CF_ERR_NOT_CONVERGED;
+  }
+  return CF_OK;
+}
+
+int pcg_mixed(const cf_matrix* A, const double* b, double* x, const uint8_t* fixed, double tol,
+              int maxit, const Layout& L, cf_pcg_stats* st, hipStream_t s) {
+  const int64_t n = n_dofs(A);
+  const int G = pcg_grid();
+  // Replace the residual once the FP32 recursion has dropped it by this
+  // factor (squared) since the last replacement.
+  const double kRefresh2 = 1e-4;
+  int rc;
+  PcgState h{};
+  make_scale<<<G, kBlock, 0, s>>>(n, L.dinv, L.s);
+  CF_LAUNCHED();
+  if (A->format == CF_FMT_NB3) {
+    scale_nb3_to_f32<<<G, kBlock, 0, s>>>(A->n_rows, A->indptr, A->indices, A->values, L.s,
+                                          L.vals32);
+  } else {
+    scale_csr_to_f32<<<G, kBlock, 0, s>>>(A->n_rows, A->indptr, A->indices, A->values, L.s,
+                                          L.vals32);
+  }
+  CF_LAUNCHED();
+  cf_matrix A32 = *A;
+  // reference norm ||b_f - A_fF x_F||
+  split_vector<<<G, kBlock, 0, s>>>(n, x, fixed, 1, L.p);
+  CF_LAUNCHED();
+  if ((rc = launch_spmv<double, double, double>(A, A->values, L.p, L.Ap, fixed, nullptr,
+                                                nullptr, G, s)))
+    return rc;
+  residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->pad);
+  CF_LAUNCHED();
+  // FP64 residual of the initial guess
+  if ((rc = launch_spmv<double, double, double>(A, A->values, x, L.Ap, fixed, nullptr, nullptr,
+                                                G, s)))
+    return rc;
+  residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->rr);
+  CF_LAUNCHED();
+  if ((rc = read_state(L.state, &h, s))) return rc;
+  const double bb = h.pad;
+  st->b_norm = std::sqrt(bb);
+  st->iterations = 0;
+  st->residual_updates = 0;
+  if (!std::isfinite(bb) || !std::isfinite(h.rr)) {
+    cf::set_error("cf_pcg_solve: non-finite right-hand side");
+    return CF_ERR_BREAKDOWN;
+  }
+  if (bb == 0.0) {
+    split_vector<<<G, kBlock, 0, s>>>(n, x, fixed, 1, x);
+    CF_LAUNCHED();
+    CF_HIP(hipStreamSynchronize(s));
+    st->rel_residual = 0.0;
+    return CF_OK;
+  }
+  const double stop_true2 = tol * tol * bb;
+  double true_rr = h.rr;
+  if (true_rr <= stop_true2) {
+    st->rel_residual = std::sqrt(true_rr / bb);
+    return CF_OK;
+  }
+  CF_HIP(hipMemsetAsync(L.yh, 0, sizeof(float) * n, s));
+  mx_load_residual<<<G, kBlock, 0, s>>>(n, L.r, L.s, L.rh, L.ph, 1, L.state, L.partials,
+                                        kRefresh2);
+  CF_LAUNCHED();
+  if ((rc = read_state(L.state, &h, s))) return rc;
+  int iters = 0, updates = 0;
+  for (;;) {
+    // Map the true stopping test onto the scaled FP32 residual through the
+    // ratio observed at the last replacement.
+    const double ratio = true_rr / h.rr;  // ||r||^2 / ||S r||^2
+    h.stop2 = stop_true2 / ratio;
+    h.maxit = maxit;
+    h.iter = iters;
+    h.halt = kRunning;
+    h.ticket = 0;
+    CF_HIP(hipMemcpyAsync(L.state, &h, sizeof(PcgState), hipMemcpyHostToDevice, s));
+    int batch = 16;
+    while (h.halt == kRunning) {
+      for (int t = 0; t < batch; ++t) {
+        int tok;
+        prof_begin(1, h.iter + t, s, &tok);
+        if ((rc = launch_spmv<float, float, float>(&A32, L.vals32, L.ph, L.Aph, nullptr,
+                                                   L.state, L.partials, G, s)))
+          return rc;
+        prof_end(tok, s);
+        mx_update<<<G, kBlock, 0, s>>>(n, L.yh, L.rh, L.ph, L.Aph, L.state, L.partials);
+        CF_LAUNCHED();
+        mx_direction<<<G, kBlock, 0, s>>>(n, L.rh, L.ph, L.state);
+        CF_LAUNCHED();
+      }
+      if ((rc = read_state(L.state, &h, s))) return rc;
+      prof_collect();
+      batch = batch * 2 > 256 ? 256 : batch * 2;
+    }
+    iters = h.iter;
+    if (h.halt == kBreakdown) {
+      cf::set_error("cf_pcg_solve: breakdown after %d iterations (NaN/Inf or p.Ap <= 0)", iters);
+      st->iterations = iters;
+      return CF_ERR_BREAKDOWN;
+    }
+    // residual replacement: x += S y, r = b - A x in FP64
+    mx_accumulate<<<G, kBlock, 0, s>>>(n, x, L.yh, L.s);
+    CF_LAUNCHED();
+    if ((rc = launch_spmv<double, double, double>(A, A->values, x, L.Ap, fixed, nullptr,
+                                                  nullptr, G, s)))
+      return rc;
+    residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->pad);
+    CF_LAUNCHED();
+    PcgState t{};
+    if ((rc = read_state(L.state, &t, s))) return rc;
+    true_rr = t.pad;
+    ++updates;
+    if (!std::isfinite(true_rr)) {
+      cf::set_error("cf_pcg_solve: non-finite residual after %d iterations", iters);
+      st->iterations = iters;
+      return CF_ERR_BREAKDOWN;
+    }
+    if (true_rr <= stop_true2 || iters >= maxit) break;
+    mx_load_residual<<<G, kBlock, 0, s>>>(n, L.r, L.s, L.rh, L.ph, 0, L.state, L.partials,
+                                          kRefresh2);
+    CF_LAUNCHED();
+    mx_direction_forced<<<G, kBlock, 0, s>>>(n, L.rh, L.ph, L.state);
+    CF_LAUNCHED();
+    if ((rc = read_state(L.state, &h, s))) return rc;
+  }
+  st->iterations = iters;
+  st->residual_updates = updates;
+  st->rel_residual = std::sqrt(true_rr / bb);
+  if (true_rr > stop_true2) {
+    cf::set_error("cf_pcg_solve: not converged after %d iterations (rel. residual %.3e)", iters,
+                  st->rel_residual);
+    return CF_ERR_NOT_CONVERGED;
+  }
+  return CF_OK;
+}
+
+}  // namespace
+
+extern "C" int cf_pcg_solve(const cf_matrix* A, const double* b, double* x,
+                            const uint8_t* fixed, int32_t precision, int32_t precond,
+                            double tol, int32_t maxit, void* ws, size_t ws_bytes,
+                            cf_pcg_stats* stats_host, void* stream) {
+  int rc = check_matrix(A);
+  if (rc) return rc;
+  CF_REQUIRE(b && x && ws && stats_host, "cf_pcg_solve: null argument");
+  CF_REQUIRE(precision == CF_PREC_FP64 || precision == CF_PREC_MIXED,
+             "cf_pcg_solve: bad precision %d", precision);
+  CF_REQUIRE(precond == CF_PC_JACOBI || precond == CF_PC_BLOCK3, "cf_pcg_solve: bad precond %d",
+             precond);
+  CF_REQUIRE(!(precond == CF_PC_BLOCK3 && A->format != CF_FMT_NB3),
+             "cf_pcg_solve: block3 preconditioner needs an NB3 matrix");
+  CF_REQUIRE(!(precond == CF_PC_BLOCK3 && precision == CF_PREC_MIXED),
+             "cf_pcg_solve: mixed precision uses Jacobi scaling");
+  CF_REQUIRE(tol > 0.0 && maxit > 0, "cf_pcg_solve: tol and maxit must be positive");
+  Layout L = layout(A, precision, precond, ws);
+  if (ws_bytes < L.total) {
+    cf::set_error("cf_pcg_solve: workspace %zu < %zu bytes", ws_bytes, L.total);
+    return CF_ERR_WORKSPACE;
+  }
+  hipStream_t s = cf::as_stream(stream);
+  const int G = pcg_grid();
+  *stats_host = cf_pcg_stats{};
+  CF_HIP(hipMemsetAsync(L.state, 0, sizeof(PcgState), s));
+  CF_HIP(hipMemsetAsync(L.bad, 0, sizeof(int32_t), s));
+  if (A->format == CF_FMT_NB3) {
+    diag_nb3<<<G, kBlock, 0, s>>>(A->n_rows, A->indptr, A->indices, A->values, fixed,
+                                  precond == CF_PC_BLOCK3, L.dinv, L.bad);
+  } else {
+    diag_csr<<<G, kBlock, 0, s>>>(A->n_rows, A->indptr, A->indices, A->values, fixed, L.dinv,
+                                  L.bad);
+  }
+  CF_LAUNCHED();
+  int32_t bad = 0;
+  CF_HIP(hipMemcpyAsync(&bad, L.bad, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  CF_HIP(hipStreamSynchronize(s));
+  if (bad) {
+    cf::set_error("cf_pcg_solve: %d free diagonal entries/blocks are not positive "
+                  "(matrix not SPD on the free DOFs)",
+                  bad);
+    return CF_ERR_BREAKDOWN;
+  }
+  if (precision == CF_PREC_MIXED)
+    return pcg_mixed(A, b, x, fixed, tol, maxit, L, stats_host, s);
+  if (precond == CF_PC_BLOCK3)
+    return pcg_fp64<CF_PC_BLOCK3>(A, b, x, fixed, tol, maxit, L, stats_host, s);
+  return pcg_fp64<CF_PC_JACOBI>(A, b, x, fixed, tol, maxit, L, stats_host, s);
+}
+
+extern "C" int cf_spmv_profile(int32_t enable, double* avg_ms_fp64, int64_t* n_fp64,
+                               double* avg_ms_fp32, int64_t* n_fp32) {
+  if (avg_ms_fp64) *avg_ms_fp64 = g_prof.launches[0] ? g_prof.ms[0] / g_prof.launches[0] : 0.0;
+  if (n_fp64) *n_fp64 = g_prof.launches[0];
+  if (avg_ms_fp32) *avg_ms_fp32 = g_prof.launches[1] ? g_prof.ms[1] / g_prof.launches[1] : 0.0;
+  if (n_fp32) *n_fp32 = g_prof.launches[1];
+  if (enable >= 0) {
+    g_prof.enabled = enable != 0;
+    g_prof.ms[0] = g_prof.ms[1] = 0.0;
+    g_prof.launches[0] = g_prof.launches[1] = 0;
+  }
+  return CF_OK;
+}

@@ -1,0 +1,51 @@
+"""Exterior boundary faces of a named node set (host-side mesh processing).
+
+Same rule and order as the reference's dictionary scans
+(tet_heat.py:219-242, tet10.py:329-352, elastic_tet10.py:68-91): a face is
+exterior iff its sorted corner triple occurs in exactly one element; it is
+kept iff all three corners are in the set; faces are listed in order of
+first occurrence (element order, then local face order) with the node
+order of that occurrence.  Done with one sort of 64-bit face keys.
+"""
+
+import numpy as np
+
+# P1 local faces (a,b,c), (a,b,d), (a,c,d), (b,c,d) (tet_heat.py:233)
+P1_FACES = ((0, 1, 2), (0, 1, 3), (0, 2, 3), (1, 2, 3))
+# Tet10 faces as corners + edge midpoints (tet10.py:287-292)
+T10_FACES = (((0, 1, 2), (4, 5, 6)), ((0, 3, 1), (7, 8, 4)),
+             ((0, 2, 3), (6, 9, 7)), ((1, 3, 2), (8, 9, 5)))
+
+
+def _exterior(tris, n_nodes):
+    """Mask of faces (rows of `tris`, in occurrence order) whose sorted
+    triple occurs exactly once."""
+    s = np.sort(tris, axis=1).astype(np.int64)
+    n = np.int64(max(n_nodes, 1))
+    keys = (s[:, 0] * n + s[:, 1]) * n + s[:, 2]
+    _, inv, counts = np.unique(keys, return_inverse=True, return_counts=True)
+    return counts[inv.ravel()] == 1
+
+
+def boundary_faces(mesh, name):
+    """(F, 3) P1 faces of boundary `name` (KeyError if unknown)."""
+    ids = np.asarray(mesh.boundaries[name], dtype=int)
+    els = np.asarray(mesh.elements)[:, :4]
+    n = mesh.n_nodes
+    tris = np.stack([els[:, list(f)] for f in P1_FACES], axis=1).reshape(-1, 3)
+    member = np.zeros(n, dtype=bool)
+    member[ids] = True
+    keep = _exterior(tris, n) & member[tris].all(axis=1)
+    return tris[keep].astype(int).reshape(-1, 3)
+
+
+def boundary_faces10(mesh, name):
+    """(F, 6) Tet10 faces [c0, c1, c2, m01, m12, m20] of boundary `name`."""
+    ids = np.asarray(mesh.boundaries[name], dtype=int)
+    els = np.asarray(mesh.elements)
+    n = mesh.n_nodes
+    full = np.stack([els[:, list(c + m)] for c, m in T10_FACES], axis=1).reshape(-1, 6)
+    member = np.zeros(n, dtype=bool)
+    member[ids] = True
+    keep = _exterior(full[:, :3], n) & member[full[:, :3]].all(axis=1)
+    return full[keep].astype(int).reshape(-1, 6)

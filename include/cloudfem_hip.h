@@ -1,0 +1,281 @@
+/*
+ * cloudfem_hip.h -- C ABI of libcloudfem_hip.so, the MI355X (gfx950) kernels
+ * behind the tetrahedral P1/Tet10 assembly and sparse SPD solve of CloudFEM.
+ *
+ * The reference (CloudFEM) is pure Python; its boundary for this path is the
+ * set of Python functions in cloudfem/fem/{tet_poisson,tet_heat,tet10}.py and
+ * cloudfem/physics/{elastic_tet10,elastic3d,modal3d}.py.  Each entry point
+ * below names the reference code whose arithmetic it replaces.  The Python
+ * package cloudfem_amd binds these with ctypes and presents the reference's
+ * function signatures.
+ *
+ * Conventions
+ *  - Every function returns an int status (CF_OK = 0); cf_last_error() gives
+ *    the message of the last failure on the calling thread.
+ *  - All array arguments are DEVICE pointers unless the name ends in _host.
+ *    Device buffers are owned by the caller; the library allocates no device
+ *    memory.  Scratch space is passed in as (ws, ws_bytes); the matching
+ *    *_workspace_bytes() call gives the size.
+ *  - `stream` is a hipStream_t (NULL = default stream).  Calls are
+ *    asynchronous on that stream unless stated otherwise.
+ *  - Indices are int32 (scipy's CSR index type for these sizes); node
+ *    coordinates are FP64 structure-of-arrays x[N], y[N], z[N];
+ *    connectivity is int32 row-major (n_elem, k), k = 4 (P1) or 10 (Tet10).
+ *  - Node-pair pattern: the CSR pattern of the node graph induced by the
+ *    elements (row = node, sorted unique columns, duplicates merged, explicit
+ *    zeros kept) -- exactly the pattern scipy's coo_matrix(...).tocsr()
+ *    produces for the reference's element triplets.
+ *  - Vector (3 DOF per node, interleaved DOF 3i+c) matrices are stored in
+ *    "NB3" order: the value array of the scipy CSR of the DOF matrix, whose
+ *    node row i occupies values[9*ptr[i] .. 9*ptr[i+1]), DOF row 3i+c holding
+ *    the 3*len_i entries block(i,j_0)[c][0..2], block(i,j_1)[c][0..2], ...
+ *    The node pattern (ptr, idx) plus this array is enough for SpMV; the
+ *    DOF-level indptr/indices are produced only for export.
+ */
+#ifndef CLOUDFEM_HIP_H
+#define CLOUDFEM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define CF_OK 0
+#define CF_ERR_ARG 1            /* bad argument (size, null pointer, op)    */
+#define CF_ERR_HIP 2            /* HIP runtime error                        */
+#define CF_ERR_WORKSPACE 3      /* workspace too small                      */
+#define CF_ERR_DEGENERATE 4     /* zero-volume element (LinAlgError)        */
+#define CF_ERR_NOT_CONVERGED 5  /* PCG hit maxit                            */
+#define CF_ERR_BREAKDOWN 6      /* PCG breakdown: NaN/Inf or p.Ap <= 0      */
+
+/* ---- matrix operators (cf_assemble_matrix) ------------------------------ */
+#define CF_OP_P1_STIFF 1   /* k_e V G G^T              tet_poisson.py:51-66   */
+#define CF_OP_P1_MASS 2    /* rho_e V (1+d_ab)/20      tet_heat.py:194-216    */
+#define CF_OP_T10_STIFF 3  /* V sum_q w_q k_q G G^T    tet10.py:138-165       */
+#define CF_OP_T10_MASS 4   /* rho_e V MASS_REF         tet10.py:185-249       */
+#define CF_OP_T10_ELAST 5  /* V sum_q w_q B^T D B      elastic_tet10.py:164-173 */
+#define CF_OP_P1_ELAST 6   /* V B^T D B                elastic3d.py:145-155   */
+#define CF_OP_P1_VMASS 7   /* rho V (1+d)/20 per comp  modal3d.py:62-75       */
+
+/* ---- right-hand-side operators (cf_assemble_rhs) ------------------------- */
+#define CF_RHS_P1_SOURCE 1    /* f_e V/4                  tet_poisson.py:62      */
+#define CF_RHS_T10_SOURCE 2   /* V sum_q w_q f_q N_q      tet10.py:155           */
+#define CF_RHS_P1_BODY 3      /* b_e,c V/4                elastic3d.py:157-162   */
+#define CF_RHS_T10_BODY 4     /* V sum_q w_q N_q b_q,c    elastic_tet10.py:175-179 */
+#define CF_RHS_P1_THERMAL 5   /* V (3l+2m) s_e g_a        elastic3d.py:164-169   */
+#define CF_RHS_T10_THERMAL 6  /* V sum_q w_q (3l+2m) s_q g_aq  elastic_tet10.py:181-190 */
+
+/* ---- face operators (cf_face_rhs, cf_face_matrix) ------------------------ */
+#define CF_FACE_P1_FLUX 1       /* q_f A/3                tet_heat.py:271-276    */
+#define CF_FACE_P1_ROBIN_K 2    /* h_f A (1+d)/12         tet_heat.py:278-290    */
+#define CF_FACE_T10_FLUX 3      /* A sum_q w6 q_q N6      tet10.py:393-401       */
+#define CF_FACE_T10_ROBIN_K 4   /* A sum_q w6 h_q N6 N6^T tet10.py:413-427       */
+#define CF_FACE_P1_TRACTION 5   /* t_f,c A/3              elastic3d.py:171-186   */
+#define CF_FACE_T10_TRACTION 6  /* A sum_q w3 N3_q t_q,c  elastic_tet10.py:192-206 */
+
+/* ---- solver options ------------------------------------------------------ */
+#define CF_FMT_CSR 0    /* scalar CSR: rows = DOFs                           */
+#define CF_FMT_NB3 1    /* 3x3 node blocks in NB3 order: rows = nodes        */
+#define CF_PREC_FP64 0  /* plain FP64 PCG                                    */
+#define CF_PREC_MIXED 1 /* FP32 inner CG on the Jacobi-scaled matrix, FP64
+                           residual replacement + solution accumulation      */
+#define CF_PC_JACOBI 0
+#define CF_PC_BLOCK3 1  /* 3x3 node-block Jacobi (NB3 only, FP64 only)       */
+
+typedef struct cf_matrix {
+  int32_t format;        /* CF_FMT_CSR or CF_FMT_NB3                          */
+  int32_t n_rows;        /* CSR: DOF rows; NB3: node rows                     */
+  int64_t nnz;           /* CSR: stored entries; NB3: stored node blocks      */
+  const int32_t* indptr; /* n_rows + 1                                        */
+  const int32_t* indices;/* nnz (node columns for NB3)                        */
+  const double* values;  /* CSR: nnz; NB3: 9*nnz                              */
+} cf_matrix;
+
+typedef struct cf_pcg_stats {
+  int32_t iterations;        /* inner iterations performed                   */
+  int32_t residual_updates;  /* FP64 residual replacements (mixed only)      */
+  double rel_residual;       /* ||b_f - A_ff x_f|| / ||b_f||, FP64, at exit  */
+  double b_norm;             /* ||b_f||                                      */
+} cf_pcg_stats;
+
+/* ---- library ------------------------------------------------------------ */
+int cf_version(void);
+const char* cf_last_error(void);
+
+/* ---- node-pair pattern (replaces the pattern half of
+ *      scipy coo_matrix(...).tocsr() at tet_poisson.py:68, tet_heat.py:216,
+ *      tet10.py:162-164 / 247-249, elastic_tet10.py:208-210,
+ *      elastic3d.py:188, modal3d.py:75) ------------------------------------
+ * Two phases around one host read of nnz:
+ *  1. cf_pattern_sort: radix-sorts all n_elem*k*k element node pairs by
+ *     (row, col), stable in pair id p = e*k*k + a*k + b, writes the sorted
+ *     pair ids to contrib[] (the contributor list of every stored entry, in
+ *     ascending element order) and nnz to *nnz_dev.
+ *  2. cf_pattern_finalize (same ws): indptr[N+1], indices[nnz] and the
+ *     contributor segment offsets seg[nnz+1] (entry s sums contrib[seg[s] ..
+ *     seg[s+1])).                                                           */
+int cf_pattern_workspace_bytes(int64_t n_elem, int32_t k, int64_t n_nodes,
+                               size_t* bytes);
+int cf_pattern_sort(const int32_t* conn, int64_t n_elem, int32_t k,
+                    int64_t n_nodes, void* ws, size_t ws_bytes,
+                    uint32_t* contrib, int64_t* nnz_dev, void* stream);
+int cf_pattern_finalize(int64_t n_elem, int32_t k, int64_t n_nodes,
+                        int64_t nnz, const void* ws, int32_t* indptr,
+                        int32_t* indices, int32_t* seg, void* stream);
+
+/* ---- node incidence: for every node the (element, local slot) items that
+ *      touch it, ascending; used for deterministic RHS gathers
+ *      (replaces rhs[el] += ... at tet_poisson.py:62, tet10.py:161,
+ *      elastic_tet10.py:179/190/206, tet_heat.py:276/291) ---------------- */
+int cf_incidence_workspace_bytes(int64_t n_items, int64_t n_nodes,
+                                 size_t* bytes);
+int cf_incidence_build(const int32_t* targets, int64_t n_items,
+                       int64_t n_nodes, void* ws, size_t ws_bytes,
+                       uint32_t* perm, int32_t* offs, void* stream);
+
+/* ---- element geometry: inverse corner Jacobian (row-major, Jinv[m][i] =
+ *      d xi_m / d x_i) and volume |det J|/6 per element, geo[e*10 + 0..9].
+ *      Replaces _tet_gradients (tet_poisson.py:17-32) and the Jacobian part
+ *      of element_operators (tet10.py:113-124).  *n_bad_dev receives the
+ *      number of elements with det J == 0 (the reference raises LinAlgError).
+ */
+int cf_tet_geometry(const double* x, const double* y, const double* z,
+                    const int32_t* conn, int64_t n_elem, int32_t k,
+                    double* geo, int32_t* n_bad_dev, void* stream);
+
+/* ---- matrix values: one thread per stored node pair sums its
+ *      contributors' local entries, recomputed from geo + constant
+ *      reference tables, in ascending element order (deterministic, no
+ *      atomics).  coef: per-element (P1_STIFF, P1_MASS, T10_MASS, P1_VMASS)
+ *      or per-element-per-point [e*5+q] (T10_STIFF); unused for *_ELAST,
+ *      which take Lame lam, mu.  Output: scalar ops write values[nnz];
+ *      *_ELAST write NB3 values[9*nnz]; P1_VMASS writes the
+ *      component-diagonal layout values[3*nnz] (DOF row 3i+c holds
+ *      block(i,j)[c][c] for the row's columns j).                          */
+int cf_assemble_matrix(int32_t op, const int32_t* indptr,
+                       const int32_t* indices, const int32_t* seg,
+                       const uint32_t* contrib, int64_t n_nodes, int64_t nnz,
+                       int32_t k, const double* geo, const double* coef,
+                       double lam, double mu, double* values, void* stream);
+
+/* ---- right-hand sides: rhs[i] (or rhs[3i+c]) += sum over the node's
+ *      incidence list; coef layouts: P1_SOURCE f[e]; T10_SOURCE f[e*5+q];
+ *      P1_BODY b[e*3+c]; T10_BODY b[(e*5+q)*3+c]; P1_THERMAL s[e];
+ *      T10_THERMAL s[e*5+q] with s = alpha (T - T_ref).                      */
+int cf_assemble_rhs(int32_t op, const int32_t* offs, const uint32_t* perm,
+                    int64_t n_nodes, int32_t k, const double* geo,
+                    const double* coef, double lam, double mu, double* rhs,
+                    void* stream);
+
+/* ---- boundary-face loads: faces (n_faces, nf) int32 with nf = 3 (P1) or
+ *      6 (Tet10, [c0,c1,c2,m01,m12,m20]); offs/perm = cf_incidence_build of
+ *      the face array.  coef: P1_FLUX q[f]; T10_FLUX q[f*6+q];
+ *      P1_TRACTION t[f*3+c]; T10_TRACTION t[(f*3+q)*3+c].  Adds into rhs.  */
+int cf_face_rhs(int32_t op, const double* x, const double* y,
+                const double* z, const int32_t* faces, int64_t n_faces,
+                int32_t nf, const int32_t* offs, const uint32_t* perm,
+                int64_t n_nodes, const double* coef, double* rhs,
+                void* stream);
+
+/* ---- boundary-face matrices (Robin): values of the face pattern built by
+ *      cf_pattern_* over the face array.  coef: P1_ROBIN_K h[f];
+ *      T10_ROBIN_K h[f*6+q].                                                */
+int cf_face_matrix(int32_t op, const double* x, const double* y,
+                   const double* z, const int32_t* faces, int32_t nf,
+                   const int32_t* indptr, const int32_t* indices,
+                   const int32_t* seg, const uint32_t* contrib,
+                   int64_t n_nodes, int64_t nnz, const double* coef,
+                   double* values, void* stream);
+
+/* ---- sparse add of a sub-pattern into a super-pattern (scalar CSR):
+ *      for every stored (i, j) of B, A[i, j] += alpha * B[i, j].  Every
+ *      stored entry of B must be stored in A (the face pattern is a subset
+ *      of the element pattern).  Replaces K + K_add (tet_heat.py:352).      */
+int cf_csr_add_subpattern(int64_t n_rows, const int32_t* a_ptr,
+                          const int32_t* a_idx, double* a_val,
+                          const int32_t* b_ptr, const int32_t* b_idx,
+                          const double* b_val, double alpha,
+                          int32_t* n_missing_dev, void* stream);
+
+/* ---- element strain/stress at the centroid, Voigt [xx,yy,zz,yz,xz,xy],
+ *      engineering shear: eps = B u_e, sig = D (eps - s_e (1,1,1,0,0,0)) with
+ *      isotropic D(lam, mu); s_thermal[e] = alpha (T_c - T_ref) or NULL.
+ *      Replaces element_stress_strain_3d (elastic3d.py:252-289) for k = 4
+ *      and element_stress_tet10 (elastic_tet10.py:232-279) for k = 10.
+ *      u is the interleaved (3N) displacement; sig, eps are (n_elem, 6).    */
+int cf_element_stress(const int32_t* conn, int64_t n_elem, int32_t k,
+                      const double* geo, const double* u,
+                      const double* s_thermal, double lam, double mu,
+                      double* sig, double* eps, void* stream);
+
+/* ---- export: DOF-level CSR indptr/indices of a vector matrix from its node
+ *      pattern.  dense3 = 1: full 3x3 blocks (elasticity, 9*nnz entries);
+ *      dense3 = 0: component-diagonal blocks (vector mass, 3*nnz entries). */
+int cf_expand_pattern3(int64_t n_nodes, const int32_t* ptr,
+                       const int32_t* idx, int32_t dense3,
+                       int32_t* dof_ptr, int32_t* dof_idx, void* stream);
+
+/* ---- scalar Dirichlet elimination with the reference's structure
+ *      (tet_poisson.apply_dirichlet, tet_poisson.py:72-110): fixed rows become
+ *      identity rows, rhs_out = rhs - A[:, F] v on free rows and v on fixed
+ *      rows, and every row stores every fixed column (explicit zeros), as
+ *      scipy's A[:, F] = 0 does.  A must be canonical CSR.  fixed_val[i] is the
+ *      value of fixed node i; fixed_list = sorted F.  Two phases around one
+ *      host read of the output nnz, like cf_pattern_*.                      */
+int cf_dirichlet_workspace_bytes(int64_t n, size_t* bytes);
+int cf_dirichlet_count(int64_t n, const int32_t* ptr, const int32_t* idx,
+                       const double* val, const double* rhs,
+                       const uint8_t* fixed, const double* fixed_val,
+                       int64_t n_fixed, void* ws, size_t ws_bytes,
+                       int32_t* ptr_out, double* rhs_out, int64_t* nnz_dev,
+                       void* stream);
+int cf_dirichlet_fill(int64_t n, const int32_t* ptr, const int32_t* idx,
+                      const double* val, const uint8_t* fixed,
+                      const int32_t* fixed_list, int64_t n_fixed,
+                      const int32_t* ptr_out, int32_t* idx_out,
+                      double* val_out, void* stream);
+
+/* ---- y = a x + b y (theta-method operators C + th h K, per-step loads;
+ *      replaces the scipy sums at tet_heat.py:361-362, 386).              */
+int cf_axpby(int64_t n, double a, const double* x, double b, double* y,
+             void* stream);
+/* ---- dst[i] = src[i] where mask[i] (re-imposing Dirichlet values,
+ *      tet_heat.py:388-392).                                              */
+int cf_copy_masked(int64_t n, const uint8_t* mask, const double* src,
+                   double* dst, void* stream);
+
+/* ---- y = A x (FP64), optional row mask (fixed[i] != 0 -> y_i = 0). ------ */
+int cf_spmv(const cf_matrix* A, const double* x, double* y,
+            const uint8_t* fixed, void* stream);
+
+/* ---- preconditioned CG on the free-DOF system (replaces spsolve /
+ *      apply_dirichlet_vec3 at elastic3d.py:220-232, tet_poisson.py:121,
+ *      tet10.py:177, elastic_tet10.py:225-228).
+ *  fixed (nullable): fixed[i] != 0 marks a Dirichlet DOF.  On entry x holds
+ *  the fixed values at fixed DOFs and the initial guess at free DOFs; on exit
+ *  x[free] solves A_ff x_f = b_f - A_fF x_F to ||res||/||b_f - A_fF x_F||
+ *  <= tol (FP64 residual) and x[fixed] is unchanged.  Synchronous: returns
+ *  after the solve, stats_host filled.  CF_ERR_NOT_CONVERGED if maxit is
+ *  reached, CF_ERR_BREAKDOWN on NaN/Inf or a non-positive curvature.       */
+int cf_pcg_workspace_bytes(const cf_matrix* A, int32_t precision,
+                           int32_t precond, size_t* bytes);
+int cf_pcg_solve(const cf_matrix* A, const double* b, double* x,
+                 const uint8_t* fixed, int32_t precision, int32_t precond,
+                 double tol, int32_t maxit, void* ws, size_t ws_bytes,
+                 cf_pcg_stats* stats_host, void* stream);
+
+/* ---- timing of the CG product kernel (no reference counterpart; used by
+ *      the benchmark to report achieved bandwidth).  Returns the average
+ *      duration (ms) and count of the sampled FP64 and FP32 SpMV launches of
+ *      cf_pcg_solve since the last reset; enable >= 0 then resets the
+ *      counters and switches sampling on (1) or off (0).  Not thread-safe. */
+int cf_spmv_profile(int32_t enable, double* avg_ms_fp64, int64_t* n_fp64,
+                    double* avg_ms_fp32, int64_t* n_fp32);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CLOUDFEM_HIP_H */

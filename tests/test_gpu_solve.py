@@ -1,0 +1,124 @@
+"""GPU solves against SciPy SuperLU on the oracle's systems and against the
+reference's analytic validations (tests/test_tet.py, tests/test_tet10.py)."""
+
+import numpy as np
+import pytest
+
+from oracle import elastic as oel
+from oracle import meshes as om
+from oracle import p1 as op1
+from tests._helpers import FACES, assert_csr_parity, assert_vec_close, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def test_p1_affine_exact(gpu):
+    from cloudfem_amd.fem.tet_poisson import solve_tet_poisson
+    from cloudfem_amd.mesh import box_mesh
+    ex = lambda x, y, z: x + 2.0 * y + 3.0 * z - 1.0  # noqa: E731
+    m = box_mesh(6, 6, 6)
+    T = solve_tet_poisson(m, k=1.0, f=0.0, bc={n: ex for n in FACES})
+    assert np.sqrt(np.mean((T - ex(*m.nodes.T)) ** 2)) < 1e-10
+
+
+def test_p1_manufactured_rates(gpu):
+    from cloudfem_amd.fem.tet_poisson import solve_tet_poisson
+    from cloudfem_amd.mesh import box_mesh
+    ue = lambda x, y, z: np.sin(np.pi * x) * np.sin(np.pi * y) * np.sin(np.pi * z)  # noqa: E731
+    errs = []
+    for n in (4, 8, 16):
+        m = box_mesh(n, n, n)
+        T = solve_tet_poisson(m, 1.0, lambda x, y, z: 3 * np.pi ** 2 * ue(x, y, z),
+                              {f: 0.0 for f in FACES})
+        errs.append(np.sqrt(np.mean((T - ue(*m.nodes.T)) ** 2)))
+    rates = [np.log(errs[i] / errs[i + 1]) / np.log(2.0) for i in range(2)]
+    assert all(1.7 < r < 2.3 for r in rates), rates
+
+
+def test_p1_cfg1_solution_matches_spsolve(gpu):
+    """20^3 Kuhn box, f = 3 pi^2 sin sin sin, T = 0 on all faces."""
+    from cloudfem_amd.fem.tet_poisson import solve_tet_poisson
+    from cloudfem_amd.mesh import box_mesh
+    src = lambda x, y, z: 3 * np.pi ** 2 * np.sin(np.pi * x) * np.sin(np.pi * y) * np.sin(np.pi * z)  # noqa: E731
+    bc = {f: 0.0 for f in FACES}
+    u = solve_tet_poisson(box_mesh(20, 20, 20), 1.0, src, bc)
+    u0 = op1.solve_tet_poisson(om.box_mesh(20, 20, 20), 1.0, src, bc)
+    assert rel_err(u, u0) < 1e-10
+
+
+def test_apply_dirichlet_structure(gpu):
+    from cloudfem_amd.fem import tet_poisson
+    from cloudfem_amd.mesh import box_mesh
+    bc = {"xmin": 1.0, "ymax": lambda x, y, z: x + z}
+    m, m0 = box_mesh(4, 3, 3), om.box_mesh(4, 3, 3)
+    A, rhs = tet_poisson.assemble(m, 1.0, 2.0)
+    A0, rhs0 = op1.assemble(m0, 1.0, 2.0)
+    B, r = tet_poisson.apply_dirichlet(A, rhs, m, bc)
+    B0, r0 = op1.apply_dirichlet(A0, rhs0, m0, bc)
+    assert_csr_parity(B, B0)
+    assert_vec_close(r, r0)
+
+
+@pytest.mark.parametrize("precision,precond", [("fp64", "jacobi"), ("fp64", "block3"),
+                                               ("mixed", "jacobi")])
+def test_tet10_elasticity_matches_spsolve(gpu, precision, precond):
+    from cloudfem_amd import solver_options
+    from cloudfem_amd.mesh import box_mesh_tet10
+    from cloudfem_amd.physics import elastic_tet10
+    lims = dict(xlim=(0.0, 0.4), ylim=(0.0, 0.05), zlim=(0.0, 0.05))
+    trac = {"xmax": (0.0, 0.0, -1e6)}
+    bc = {"xmin": (0.0, 0.0, 0.0)}
+    m = box_mesh_tet10(16, 2, 2, **lims)
+    with solver_options(precision=precision, precond=precond):
+        u = np.stack(elastic_tet10.solve_elasticity_tet10(m, 210e9, 0.3, bc=bc, traction=trac),
+                     axis=1).ravel()
+    m0 = om.box_mesh_tet10(16, 2, 2, **lims)
+    A0, rhs0 = oel.assemble_elasticity_tet10(m0, 210e9, 0.3, traction=trac)
+    u0 = oel.apply_dirichlet_vec3(A0, rhs0, m0, bc)
+    assert rel_err(u, u0) < 1e-10
+
+
+def test_tet10_cantilever_vs_beam_theory(gpu):
+    """BIG-M6 geometry, Tet10 (50, 2, 4): tip within 0.5 % of Euler-Bernoulli
+    (tests/test_tet10.py:225-256, 424-438)."""
+    from cloudfem_amd.mesh import box_mesh_tet10
+    from cloudfem_amd.physics import elastic_tet10
+    E, nu, L, B, H, P = 200e9, 0.3, 0.5, 0.05, 0.05, 1000.0
+    m = box_mesh_tet10(50, 2, 4, xlim=(0, L), ylim=(0, B), zlim=(0, H))
+    _, _, uz = elastic_tet10.solve_elasticity_tet10(m, E, nu, bc={"xmin": (0.0, 0.0, 0.0)},
+                                                    traction={"xmax": (0.0, 0.0, -P / (B * H))})
+    tip = abs(float(uz[m.boundaries["xmax"]].mean()))
+    d_eb = P * L ** 3 / (3 * E * B * H ** 3 / 12.0)
+    assert abs(tip - d_eb) / d_eb < 0.005
+
+
+def test_tet10_constant_strain_patch(gpu):
+    from cloudfem_amd.mesh import box_mesh_tet10
+    from cloudfem_amd.physics import elastic_tet10
+    A = np.array([[1e-3, 2e-4, 0.0], [0.0, -5e-4, 3e-4], [1e-4, 0.0, 8e-4]])
+    c = np.array([1e-3, -2e-3, 5e-4])
+    bc = {s: tuple((lambda x, y, z, r=r: A[r, 0] * x + A[r, 1] * y + A[r, 2] * z + c[r])
+                   for r in range(3)) for s in FACES}
+    m = box_mesh_tet10(2, 2, 2)
+    u = np.stack(elastic_tet10.solve_elasticity_tet10(m, 200e9, 0.3, bc=bc), axis=1)
+    assert np.abs(u - (m.nodes @ A.T + c)).max() < 1e-12
+    sig, _ = elastic_tet10.element_stress_tet10(m, tuple(u.T), 200e9, 0.3)
+    eps_v = np.array([A[0, 0], A[1, 1], A[2, 2], A[1, 2] + A[2, 1], A[0, 2] + A[2, 0],
+                      A[0, 1] + A[1, 0]])
+    sig_ex = oel.constitutive_3d(200e9, 0.3) @ eps_v
+    assert np.abs(sig - sig_ex).max() / np.abs(sig_ex).max() < 1e-9
+
+
+def test_solver_failures_raise(gpu):
+    from cloudfem_amd import solver_options
+    from cloudfem_amd._native import SolverBreakdown, SolverNotConverged
+    from cloudfem_amd.mesh import box_mesh_tet10
+    from cloudfem_amd.physics import elastic_tet10
+    m = box_mesh_tet10(8, 2, 2)
+    args = dict(bc={"xmin": (0.0, 0.0, 0.0)}, traction={"xmax": (0.0, 0.0, -1.0)})
+    with solver_options(maxit=3):
+        with pytest.raises(SolverNotConverged):
+            elastic_tet10.solve_elasticity_tet10(m, 1.0, 0.3, **args)
+    with pytest.raises(SolverBreakdown):
+        elastic_tet10.solve_elasticity_tet10(m, 1.0, 0.3, bc=args["bc"],
+                                             traction={"xmax": (0.0, 0.0, np.nan)})
