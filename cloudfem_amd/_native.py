@@ -61,7 +61,7 @@ EXPORTED = (
     "cf_csr_add_subpattern", "cf_expand_pattern3", "cf_spmv",
     "cf_pcg_workspace_bytes", "cf_pcg_solve", "cf_dirichlet_workspace_bytes",
     "cf_dirichlet_count", "cf_dirichlet_fill", "cf_axpby", "cf_copy_masked",
-    "cf_element_stress", "cf_spmv_profile",
+    "cf_element_stress", "cf_spmv_profile", "cf_adjoint_dk",
 )
 
 
@@ -122,6 +122,7 @@ _SIGS = {
     "cf_dirichlet_fill": ([_I64, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P], _I32),
     "cf_axpby": ([_I64, _D, _P, _D, _P, _P], _I32),
     "cf_copy_masked": ([_I64, _P, _P, _P, _P], _I32),
+    "cf_adjoint_dk": ([_P, _I64, _I32, _P, _P, _P, _P, _P], _I32),
     "cf_spmv_profile": ([_I32, ctypes.POINTER(_D), ctypes.POINTER(_I64), ctypes.POINTER(_D),
                          ctypes.POINTER(_I64)], _I32),
     "cf_element_stress": ([_P, _I64, _I32, _P, _P, _P, _D, _D, _P, _P, _P], _I32),

@@ -1,0 +1,73 @@
+"""Differentiable steady diffusion solve: dT/dk through a hand-written adjoint.
+
+No reference counterpart (CloudFEM has no adjoint or autograd).  For
+K(k) T = F with Dirichlet data (K from tet_poisson.assemble / assemble_tet10
+with a per-element conductivity k_e) and any scalar loss J(T):
+
+    K lambda = dJ/dT on the free DOFs, lambda = 0 on the fixed ones,
+    dJ/dk_e = -lambda_e^T (dK_e / dk_e) T_e          (cf_adjoint_dk)
+
+F does not depend on k.  Both solves are the GPU PCG of cloudfem_amd.solver.
+"""
+
+import numpy as np
+import torch
+
+from . import _native as N
+from .coef import centroids, dirichlet_scalar, eval_scalar
+from .device import DeviceMesh, ptr, stream_handle, to_dev
+from .solver import DeviceMatrix, pcg
+
+
+class _SteadyDiffusion(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, kappa, dm, rhs, fixed, x0, tol):
+        op = N.OP_P1_STIFF if dm.k == 4 else N.OP_T10_STIFF
+        coef = kappa.detach().to(torch.float64).contiguous()
+        if dm.k == 10:
+            coef = coef.repeat_interleave(5)   # k constant over the 5 Keast points
+        vals = dm.matrix_values(op, coef=coef)
+        A = DeviceMatrix(N.FMT_CSR, dm.pattern(), vals)
+        T = x0.clone()
+        pcg(A, rhs, T, fixed, tol=tol)
+        ctx.dm, ctx.A, ctx.fixed, ctx.tol = dm, A, fixed, tol
+        ctx.save_for_backward(T)
+        return T
+
+    @staticmethod
+    def backward(ctx, grad_T):
+        (T,) = ctx.saved_tensors
+        dm = ctx.dm
+        g = grad_T.detach().to(torch.float64).contiguous()
+        lam = torch.zeros_like(T)
+        # lambda = 0 on fixed DOFs (x holds the fixed values = 0 on entry)
+        pcg(ctx.A, g, lam, ctx.fixed, tol=ctx.tol)
+        dk = torch.empty(dm.n_elem, dtype=torch.float64, device=dm.dev)
+        N.check(N.load().cf_adjoint_dk(ptr(dm.conn), dm.n_elem, dm.k, ptr(dm.geometry()),
+                                       ptr(T), ptr(lam), ptr(dk), stream_handle()),
+                "cf_adjoint_dk")
+        return dk, None, None, None, None, None
+
+
+def steady_diffusion(mesh, kappa, f=0.0, bc=None, tol=1e-12, dm=None):
+    """T (N,) solving -div(k grad T) = f, differentiable in kappa.
+
+    mesh  : P1 or Tet10 mesh; kappa: (M,) tensor of per-element conductivities
+            on the GPU (requires_grad as needed); f: float / callable (centroid for
+            P1, Keast points for Tet10, as the reference samples it); bc: Dirichlet dict.
+    """
+    dm = dm or DeviceMesh(mesh)
+    if not bc:
+        raise ValueError("steady_diffusion needs Dirichlet data (K is singular otherwise)")
+    rhs = torch.zeros(dm.n_nodes, dtype=torch.float64, device=dm.dev)
+    if dm.k == 4:
+        fv = eval_scalar(f, centroids(mesh))
+        dm.add_rhs(N.RHS_P1_SOURCE, to_dev(fv, torch.float64, dm.dev), rhs)
+    else:
+        from .fem.tet10 import _qpoints
+        fv = eval_scalar(f, _qpoints(mesh).reshape(-1, 3))
+        dm.add_rhs(N.RHS_T10_SOURCE, to_dev(fv, torch.float64, dm.dev), rhs)
+    mask, vals = dirichlet_scalar(mesh, bc)
+    fixed = to_dev(mask.astype(np.uint8), torch.uint8, dm.dev)
+    x0 = to_dev(np.where(mask, vals, 0.0), torch.float64, dm.dev)
+    return _SteadyDiffusion.apply(kappa, dm, rhs, fixed, x0, tol)

@@ -482,6 +482,51 @@ __global__ void element_stress(const int32_t* __restrict__ conn, int64_t m,
   }
 }
 
+
+// ------------------------------------------------------------- adjoint
+
+// dJ/dk_e = -lambda_e^T (dK_e/dk_e) T_e for a diffusion coefficient that is
+// constant per element: P1 -V grad(lambda).grad(T); Tet10
+// -V sum_q w_q grad(lambda)_q.grad(T)_q (K_e is linear in k_e).
+template <int K>
+__global__ void adjoint_dk(const int32_t* __restrict__ conn, int64_t m,
+                           const double* __restrict__ geo, const double* __restrict__ T,
+                           const double* __restrict__ lam, double* __restrict__ out) {
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < m;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    Geo g = load_geo(geo, e);
+    double tv[K], lv[K];
+#pragma unroll
+    for (int a = 0; a < K; ++a) {
+      const int64_t nd = conn[e * K + a];
+      tv[a] = T[nd];
+      lv[a] = lam[nd];
+    }
+    double acc = 0.0;
+    constexpr int NQ = K == 4 ? 1 : 5;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      double gt[3] = {0.0, 0.0, 0.0}, gl[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+      for (int a = 0; a < K; ++a) {
+        double ga[3];
+        if constexpr (K == 4)
+          p1_grad(g, a, ga);
+        else
+          t10_grad(g, q, a, ga);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          gt[i] += ga[i] * tv[a];
+          gl[i] += ga[i] * lv[a];
+        }
+      }
+      const double w = K == 4 ? 1.0 : kT10.w[q];
+      acc += w * (gl[0] * gt[0] + gl[1] * gt[1] + gl[2] * gt[2]);
+    }
+    out[e] = -g.v * acc;
+  }
+}
+
 }  // namespace
 
 // ------------------------------------------------------------- C ABI
@@ -669,6 +714,21 @@ extern "C" int cf_element_stress(const int32_t* conn, int64_t n_elem, int32_t k,
   else
     element_stress<10><<<grid, kBlock, 0, s>>>(conn, n_elem, geo, u, s_thermal, lam, mu, sig,
                                                eps);
+  CF_LAUNCHED();
+  return CF_OK;
+}
+
+extern "C" int cf_adjoint_dk(const int32_t* conn, int64_t n_elem, int32_t k, const double* geo,
+                             const double* T, const double* lam, double* dk, void* stream) {
+  CF_REQUIRE(geo && T && lam && dk && (conn || n_elem == 0) && (k == 4 || k == 10),
+             "cf_adjoint_dk: bad arguments");
+  if (n_elem == 0) return CF_OK;
+  hipStream_t s = cf::as_stream(stream);
+  const int grid = cf::stride_grid(n_elem, kBlock);
+  if (k == 4)
+    adjoint_dk<4><<<grid, kBlock, 0, s>>>(conn, n_elem, geo, T, lam, dk);
+  else
+    adjoint_dk<10><<<grid, kBlock, 0, s>>>(conn, n_elem, geo, T, lam, dk);
   CF_LAUNCHED();
   return CF_OK;
 }

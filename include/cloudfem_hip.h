@@ -211,6 +211,16 @@ int cf_element_stress(const int32_t* conn, int64_t n_elem, int32_t k,
                       const double* s_thermal, double lam, double mu,
                       double* sig, double* eps, void* stream);
 
+/* ---- adjoint sensitivity of a steady diffusion solve to a per-element
+ *      conductivity (no reference counterpart; the reference has no
+ *      adjoint): dk[e] = -lambda_e^T (dK_e/dk_e) T_e, i.e. -V grad(lambda).grad(T)
+ *      for P1 (k = 4) and -V sum_q w_q grad(lambda)_q.grad(T)_q for Tet10
+ *      (k = 10), where K lambda = dJ/dT on the free DOFs, lambda = 0 on fixed
+ *      ones.  T, lambda are nodal (N); dk is (n_elem).                       */
+int cf_adjoint_dk(const int32_t* conn, int64_t n_elem, int32_t k,
+                  const double* geo, const double* T, const double* lam,
+                  double* dk, void* stream);
+
 /* ---- export: DOF-level CSR indptr/indices of a vector matrix from its node
  *      pattern.  dense3 = 1: full 3x3 blocks (elasticity, 9*nnz entries);
  *      dense3 = 0: component-diagonal blocks (vector mass, 3*nnz entries). */
