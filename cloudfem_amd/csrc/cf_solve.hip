@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "cf_common.hpp"
+#include "cf_tables.hpp"
 
 namespace {
 
@@ -473,7 +474,10 @@ __global__ void make_scale(int64_t n, const double* __restrict__ dinv, double* _
     s[i] = dinv[i] > 0.0 ? sqrt(dinv[i]) : 0.0;
 }
 
-// FP32 values of S A S in the same storage order.
+// FP32 values of S A S in "row-planar" NB3 order: node row i keeps its 9*len
+// values at the same offset 9*ptr[i], as 9 planes of len entries, plane
+// q = 3c + c' holding block(i, j_k)[c][c'] for k = 0..len-1.  A lane that
+// owns block k then reads each plane at consecutive addresses.
 __global__ void scale_nb3_to_f32(int64_t n_nodes, const int32_t* __restrict__ ptr,
                                  const int32_t* __restrict__ idx,
                                  const double* __restrict__ val, const double* __restrict__ s,
@@ -489,7 +493,60 @@ __global__ void scale_nb3_to_f32(int64_t n_nodes, const int32_t* __restrict__ pt
       const int32_t q = f - c * l3, kq = q / 3;
       const int cp = q - 3 * kq;
       const int64_t t = int64_t(9) * base + f;
-      out[t] = float(s[3 * i + c] * val[t] * s[3 * int64_t(idx[base + kq]) + cp]);
+      out[int64_t(9) * base + (3 * c + cp) * len + kq] =
+          float(s[3 * i + c] * val[t] * s[3 * int64_t(idx[base + kq]) + cp]);
+    }
+  }
+}
+
+// CG product on the row-planar FP32 copy (no mask: S zeroes fixed rows and
+// columns), lane per 3x3 block, L lanes per node row.
+template <int L>
+__global__ void __launch_bounds__(kBlock)
+spmv_nb3_planar(int64_t n_nodes, const int32_t* __restrict__ ptr,
+                const int32_t* __restrict__ idx, const float* __restrict__ val,
+                const float* __restrict__ x, float* __restrict__ y, PcgState* state,
+                double* __restrict__ partials) {
+  if (state && state->halt) return;
+  const int lane = threadIdx.x % L;
+  const int group = threadIdx.x / L;
+  constexpr int kGroups = kBlock / L;
+  int64_t r0, r1;
+  block_rows(n_nodes, r0, r1);
+  double dot = 0.0;
+  for (int64_t i = r0 + group; i < r1; i += kGroups) {
+    const int32_t base = ptr[i];
+    const int32_t len = ptr[i + 1] - base;
+    const float* v = val + int64_t(9) * base;
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+    for (int32_t k = lane; k < len; k += L) {
+      const int64_t c3 = 3 * int64_t(idx[base + k]);
+      const float x0 = x[c3], x1 = x[c3 + 1], x2 = x[c3 + 2];
+      const float* b = v + k;
+      a0 += ld(b) * x0 + ld(b + len) * x1 + ld(b + 2 * len) * x2;
+      a1 += ld(b + 3 * len) * x0 + ld(b + 4 * len) * x1 + ld(b + 5 * len) * x2;
+      a2 += ld(b + 6 * len) * x0 + ld(b + 7 * len) * x1 + ld(b + 8 * len) * x2;
+    }
+    a0 = group_sum<L>(a0);
+    a1 = group_sum<L>(a1);
+    a2 = group_sum<L>(a2);
+    if (lane == 0) {
+      y[3 * i] = a0;
+      y[3 * i + 1] = a1;
+      y[3 * i + 2] = a2;
+      if (state)
+        dot += double(x[3 * i]) * double(a0) + double(x[3 * i + 1]) * double(a1) +
+               double(x[3 * i + 2]) * double(a2);
+    }
+  }
+  if (!state) return;
+  double v1[1] = {dot};
+  if (grid_sum<1>(v1, partials, &state->ticket) && threadIdx.x == 0) {
+    state->pAp = v1[0];
+    if (!(v1[0] > 0.0) || !isfinite(v1[0])) {
+      state->halt = kBreakdown;
+    } else {
+      state->alpha = state->rz / v1[0];
     }
   }
 }
@@ -593,6 +650,198 @@ __global__ void mx_accumulate(int64_t n, double* __restrict__ x, float* __restri
   }
 }
 
+// ------------------------------------------------------------- matrix-free Tet10
+
+// Reference tables for the matrix-free operator (same values as the
+// assembly kernels' copy; each translation unit owns its constant memory).
+__constant__ cf::Tet10Tables kT10m = cf::build_tet10_tables();
+
+// Element forces f_e = K_e x_e of isotropic Tet10 elasticity, evaluated at
+// the 5 Keast points: H_q = grad u = (sum_a x_a d_aq^T) Jinv,
+// sigma_q = lam tr(H) I + mu (H + H^T), f_a += V w_q sigma_q Jinv^T d_aq.
+// Exact for affine elements, same integrand and rule as
+// elastic_tet10.py:164-170, ~1.2k FMA per element.  fe[e][a][c] is summed
+// into the nodes by mf_gather in fixed incidence order (no atomics).
+__global__ void __launch_bounds__(kBlock)
+mf_t10_elem(int64_t m, const int32_t* __restrict__ conn, const double* __restrict__ geo,
+            double lam, double mu, const double* __restrict__ x, double* __restrict__ fe,
+            const PcgState* state) {
+  if (state && state->halt) return;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < m;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    const double* g = geo + e * 10;
+    double J[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) J[r][c] = g[3 * r + c];
+    const double V = g[9];
+    double xa[10][3];
+#pragma unroll
+    for (int a = 0; a < 10; ++a) {
+      const int64_t nd = conn[e * 10 + a];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) xa[a][c] = x[3 * nd + c];
+    }
+    double f[10][3] = {};
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      double U[3][3] = {};
+#pragma unroll
+      for (int a = 0; a < 10; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) U[c][k] += xa[a][c] * kT10m.D[q][a][k];
+      double H[3][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) H[c][i] = U[c][0] * J[0][i] + U[c][1] * J[1][i] + U[c][2] * J[2][i];
+      const double ltr = lam * (H[0][0] + H[1][1] + H[2][2]);
+      const double vw = V * kT10m.w[q];
+      double P[3][3];  // P[c][k] = V w_q sum_i sigma[c][i] Jinv[k][i]
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        double sg[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) sg[i] = mu * (H[c][i] + H[i][c]) + (c == i ? ltr : 0.0);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) P[c][k] = vw * (sg[0] * J[k][0] + sg[1] * J[k][1] + sg[2] * J[k][2]);
+      }
+#pragma unroll
+      for (int a = 0; a < 10; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          f[a][c] += kT10m.D[q][a][0] * P[c][0] + kT10m.D[q][a][1] * P[c][1] +
+                     kT10m.D[q][a][2] * P[c][2];
+    }
+    double* o = fe + e * 30;
+#pragma unroll
+    for (int a = 0; a < 10; ++a)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) o[3 * a + c] = f[a][c];
+  }
+}
+
+// y_i = sum over the node's incidence items (e, a) of fe[e][a] (fixed
+// rows zeroed); with `state`: p.Ap and alpha as in the assembled SpMV.
+__global__ void __launch_bounds__(kBlock)
+mf_gather(int64_t n_nodes, const int32_t* __restrict__ offs, const uint32_t* __restrict__ perm,
+          const double* __restrict__ fe, const uint8_t* __restrict__ fixed,
+          const double* __restrict__ x, double* __restrict__ y, PcgState* state,
+          double* __restrict__ partials) {
+  if (state && state->halt) return;
+  int64_t r0, r1;
+  block_rows(n_nodes, r0, r1);
+  double dot = 0.0;
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int32_t j = offs[i]; j < offs[i + 1]; ++j) {
+      const double* v = fe + int64_t(perm[j]) * 3;
+      a0 += v[0];
+      a1 += v[1];
+      a2 += v[2];
+    }
+    double o[3] = {a0, a1, a2};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t d = 3 * i + c;
+      if (fixed && fixed[d]) o[c] = 0.0;
+      y[d] = o[c];
+      if (state) dot += x[d] * o[c];
+    }
+  }
+  if (!state) return;
+  double v1[1] = {dot};
+  if (grid_sum<1>(v1, partials, &state->ticket) && threadIdx.x == 0) {
+    state->pAp = v1[0];
+    if (!(v1[0] > 0.0) || !isfinite(v1[0])) {
+      state->halt = kBreakdown;
+    } else {
+      state->alpha = state->rz / v1[0];
+    }
+  }
+}
+
+// Diagonal 3x3 node blocks of the Tet10 elasticity matrix without
+// assembling it: block_i = sum_(e,a) V (lam M + mu M^T + mu tr(M) I),
+// M = Jinv^T R_aa Jinv.
+__global__ void mf_t10_diag(int64_t n_nodes, const int32_t* __restrict__ offs,
+                            const uint32_t* __restrict__ perm, const double* __restrict__ geo,
+                            double lam, double mu, double* __restrict__ blocks) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n_nodes;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    double acc[3][3] = {};
+    for (int32_t j = offs[i]; j < offs[i + 1]; ++j) {
+      const uint32_t item = perm[j];
+      const int64_t e = item / 10u;
+      const int a = int(item - uint32_t(e) * 10u);
+      const double* g = geo + e * 10;
+      double T[3][3], M[3][3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          T[k][c] = kT10m.R[a][a][k][0] * g[c] + kT10m.R[a][a][k][1] * g[3 + c] +
+                    kT10m.R[a][a][k][2] * g[6 + c];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) M[r][c] = g[r] * T[0][c] + g[3 + r] * T[1][c] + g[6 + r] * T[2][c];
+      const double lv = lam * g[9], mv = mu * g[9];
+      const double tr = mv * (M[0][0] + M[1][1] + M[2][2]);
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[r][c] += lv * M[r][c] + mv * M[c][r] + (r == c ? tr : 0.0);
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) blocks[9 * i + 3 * r + c] = acc[r][c];
+  }
+}
+
+// Preconditioner from diagonal node blocks (same rules as diag_nb3).
+__global__ void blocks_to_precond(int64_t n_nodes, const double* __restrict__ blocks,
+                                  const uint8_t* __restrict__ fixed, int block3,
+                                  double* __restrict__ dinv, int32_t* __restrict__ bad) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n_nodes;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    double blk[3][3];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) blk[r][c] = blocks[9 * i + 3 * r + c];
+    bool fx[3];
+    for (int c = 0; c < 3; ++c) fx[c] = fixed && fixed[3 * i + c];
+    if (!block3) {
+      for (int c = 0; c < 3; ++c) {
+        const double d = blk[c][c];
+        if (fx[c]) {
+          dinv[3 * i + c] = 0.0;
+        } else if (d > 0.0) {
+          dinv[3 * i + c] = 1.0 / d;
+        } else {
+          dinv[3 * i + c] = 0.0;
+          atomicAdd(bad, 1);
+        }
+      }
+      continue;
+    }
+    for (int c = 0; c < 3; ++c)
+      if (fx[c])
+        for (int t = 0; t < 3; ++t) blk[c][t] = blk[t][c] = (c == t) ? 1.0 : 0.0;
+    double o[3][3];
+    if (!inv3(blk, o)) {
+      atomicAdd(bad, 1);
+      for (int t = 0; t < 9; ++t) dinv[9 * i + t] = 0.0;
+      continue;
+    }
+    for (int c = 0; c < 3; ++c)
+      for (int cp = 0; cp < 3; ++cp) dinv[9 * i + 3 * c + cp] = (fx[c] || fx[cp]) ? 0.0 : o[c][cp];
+  }
+}
+
 // ------------------------------------------------------------- host side
 
 // Optional timing of the CG product kernels: HIP events around every 8th
@@ -652,6 +901,7 @@ int n_dofs(const cf_matrix* A) { return A->format == CF_FMT_NB3 ? 3 * A->n_rows 
 // blocks (90..585 values), P1 scalar rows 5..15 entries.
 constexpr int kLanesNb3 = 32;
 constexpr int kLanesCsr = 8;
+constexpr int kLanesPlanar = 16;
 
 template <class VT, class XT, class AT>
 int launch_spmv(const cf_matrix* A, const VT* vals, const XT* x, XT* y, const uint8_t* fixed,
@@ -751,26 +1001,25 @@ extern "C" int cf_pcg_workspace_bytes(const cf_matrix* A, int32_t precision, int
 
 namespace {
 
-template <int PC>
-int pcg_fp64(const cf_matrix* A, const double* b, double* x, const uint8_t* fixed, double tol,
-             int maxit, const Layout& L, cf_pcg_stats* st, hipStream_t s) {
-  const int64_t n = n_dofs(A);
-  const int64_t units = PC == CF_PC_BLOCK3 ? A->n_rows : n;
+// FP64 PCG on the masked free system.  `apply(xin, y, state, partials)`
+// launches y = A xin with fixed rows zeroed; when `state` is set it also
+// reduces p.Ap and sets alpha (the CG product).  `dinv` holds the
+// preconditioner (Jacobi 1/A_ii, or inverse 3x3 node blocks).
+template <int PC, class Apply>
+int pcg_fp64_core(Apply&& apply, int64_t n, int64_t units, const double* b, double* x,
+                  const uint8_t* fixed, double tol, int maxit, const Layout& L, cf_pcg_stats* st,
+                  hipStream_t s) {
   const int G = pcg_grid();
   int rc;
   PcgState h{};
   // reference rhs: b_f - A_fF x_F
   split_vector<<<G, kBlock, 0, s>>>(n, x, fixed, 1, L.p);
   CF_LAUNCHED();
-  if ((rc = launch_spmv<double, double, double>(A, A->values, L.p, L.Ap, fixed, nullptr,
-                                                nullptr, G, s)))
-    return rc;
+  if ((rc = apply(L.p, L.Ap, nullptr, nullptr))) return rc;
   residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->pad);
   CF_LAUNCHED();
   // r = b - A x  (full x, warm start included)
-  if ((rc = launch_spmv<double, double, double>(A, A->values, x, L.Ap, fixed, nullptr, nullptr,
-                                                G, s)))
-    return rc;
+  if ((rc = apply(x, L.Ap, nullptr, nullptr))) return rc;
   residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->rr);
   CF_LAUNCHED();
   cg_start<PC><<<G, kBlock, 0, s>>>(units, L.r, L.p, L.dinv, L.state, L.partials);
@@ -805,9 +1054,7 @@ int pcg_fp64(const cf_matrix* A, const double* b, double* x, const uint8_t* fixe
     for (int t = 0; t < batch; ++t) {
       int tok;
       prof_begin(0, h.iter + t, s, &tok);
-      if ((rc = launch_spmv<double, double, double>(A, A->values, L.p, L.Ap, fixed, L.state,
-                                                    L.partials, G, s)))
-        return rc;
+      if ((rc = apply(L.p, L.Ap, L.state, L.partials))) return rc;
       prof_end(tok, s);
       cg_update<PC><<<G, kBlock, 0, s>>>(units, x, L.r, L.p, L.Ap, L.dinv, L.state, L.partials);
       CF_LAUNCHED();
@@ -832,9 +1079,7 @@ int pcg_fp64(const cf_matrix* A, const double* b, double* x, const uint8_t* fixe
   }
   st->iterations = h.iter;
   // true FP64 residual of the free system
-  if ((rc = launch_spmv<double, double, double>(A, A->values, x, L.Ap, fixed, nullptr, nullptr,
-                                                G, s)))
-    return rc;
+  if ((rc = apply(x, L.Ap, nullptr, nullptr))) return rc;
   residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->pad);
   CF_LAUNCHED();
   PcgState fin{};
@@ -852,13 +1097,30 @@ int pcg_fp64(const cf_matrix* A, const double* b, double* x, const uint8_t* fixe
   return CF_OK;
 }
 
+template <int PC>
+int pcg_fp64(const cf_matrix* A, const double* b, double* x, const uint8_t* fixed, double tol,
+             int maxit, const Layout& L, cf_pcg_stats* st, hipStream_t s) {
+  const int64_t n = n_dofs(A);
+  const int64_t units = PC == CF_PC_BLOCK3 ? A->n_rows : n;
+  const int G = pcg_grid();
+  auto apply = [&](const double* xin, double* y, PcgState* state, double* partials) {
+    return launch_spmv<double, double, double>(A, A->values, xin, y, fixed, state, partials, G,
+                                               s);
+  };
+  return pcg_fp64_core<PC>(apply, n, units, b, x, fixed, tol, maxit, L, st, s);
+}
+
 int pcg_mixed(const cf_matrix* A, const double* b, double* x, const uint8_t* fixed, double tol,
               int maxit, const Layout& L, cf_pcg_stats* st, hipStream_t s) {
   const int64_t n = n_dofs(A);
   const int G = pcg_grid();
-  // Replace the residual once the FP32 recursion has dropped it by this
-  // factor (squared) since the last replacement.
-  const double kRefresh2 = 1e-4;
+  // Replace the residual once the FP32 recursion has dropped it by 10x
+  // since the last replacement; restart the direction when the replaced
+  // residual is more than 2x the recursive one (the FP32 recursion has
+  // drifted); give up after 8 replacements without a 2x gain.
+  const double kRefresh2 = 1e-2;
+  const double kRestart2 = 4.0;
+  const int kStallLimit = 8;
   int rc;
   PcgState h{};
   make_scale<<<G, kBlock, 0, s>>>(n, L.dinv, L.s);
@@ -913,7 +1175,8 @@ int pcg_mixed(const cf_matrix* A, const double* b, double* x, const uint8_t* fix
                                         kRefresh2);
   CF_LAUNCHED();
   if ((rc = read_state(L.state, &h, s))) return rc;
-  int iters = 0, updates = 0;
+  int iters = 0, updates = 0, stall = 0;
+  double best_rr = true_rr;
   for (;;) {
     // Map the true stopping test onto the scaled FP32 residual through the
     // ratio observed at the last replacement.
@@ -929,9 +1192,15 @@ int pcg_mixed(const cf_matrix* A, const double* b, double* x, const uint8_t* fix
       for (int t = 0; t < batch; ++t) {
         int tok;
         prof_begin(1, h.iter + t, s, &tok);
-        if ((rc = launch_spmv<float, float, float>(&A32, L.vals32, L.ph, L.Aph, nullptr,
-                                                   L.state, L.partials, G, s)))
+        if (A->format == CF_FMT_NB3) {
+          spmv_nb3_planar<kLanesPlanar><<<G, kBlock, 0, s>>>(A->n_rows, A->indptr, A->indices,
+                                                             L.vals32, L.ph, L.Aph, L.state,
+                                                             L.partials);
+          CF_LAUNCHED();
+        } else if ((rc = launch_spmv<float, float, float>(&A32, L.vals32, L.ph, L.Aph, nullptr,
+                                                          L.state, L.partials, G, s))) {
           return rc;
+        }
         prof_end(tok, s);
         mx_update<<<G, kBlock, 0, s>>>(n, L.yh, L.rh, L.ph, L.Aph, L.state, L.partials);
         CF_LAUNCHED();
@@ -966,19 +1235,36 @@ int pcg_mixed(const cf_matrix* A, const double* b, double* x, const uint8_t* fix
       return CF_ERR_BREAKDOWN;
     }
     if (true_rr <= stop_true2 || iters >= maxit) break;
+    if (true_rr < 0.25 * best_rr) {
+      best_rr = true_rr;
+      stall = 0;
+    } else if (++stall >= kStallLimit) {
+      break;
+    }
+    const double rec_rr = h.rr;  // recursive FP32 r.r before the replacement
     mx_load_residual<<<G, kBlock, 0, s>>>(n, L.r, L.s, L.rh, L.ph, 0, L.state, L.partials,
                                           kRefresh2);
     CF_LAUNCHED();
-    mx_direction_forced<<<G, kBlock, 0, s>>>(n, L.rh, L.ph, L.state);
-    CF_LAUNCHED();
+    if ((rc = read_state(L.state, &h, s))) return rc;
+    if (h.rr > kRestart2 * rec_rr) {
+      // restart: p = r (mx_load_residual with start = 1 resets p and rz)
+      mx_load_residual<<<G, kBlock, 0, s>>>(n, L.r, L.s, L.rh, L.ph, 1, L.state, L.partials,
+                                            kRefresh2);
+      CF_LAUNCHED();
+    } else {
+      mx_direction_forced<<<G, kBlock, 0, s>>>(n, L.rh, L.ph, L.state);
+      CF_LAUNCHED();
+    }
     if ((rc = read_state(L.state, &h, s))) return rc;
   }
   st->iterations = iters;
   st->residual_updates = updates;
   st->rel_residual = std::sqrt(true_rr / bb);
   if (true_rr > stop_true2) {
-    cf::set_error("cf_pcg_solve: not converged after %d iterations (rel. residual %.3e)", iters,
-                  st->rel_residual);
+    cf::set_error("cf_pcg_solve: mixed precision not converged after %d iterations and %d "
+                  "FP64 residual replacements (rel. residual %.3e%s)",
+                  iters, updates, st->rel_residual,
+                  stall >= kStallLimit ? ", stagnated: FP32 cannot resolve this system" : "");
     return CF_ERR_NOT_CONVERGED;
   }
   return CF_OK;
@@ -1048,4 +1334,129 @@ extern "C" int cf_spmv_profile(int32_t enable, double* avg_ms_fp64, int64_t* n_f
     g_prof.launches[0] = g_prof.launches[1] = 0;
   }
   return CF_OK;
+}
+
+// ------------------------------------------------------------- matrix-free C ABI
+
+namespace {
+
+struct MfArgs {
+  int64_t n_nodes, n_elem;
+  const int32_t* conn;
+  const double* geo;
+  double lam, mu;
+  const int32_t* offs;
+  const uint32_t* perm;
+};
+
+int mf_check(const MfArgs& a) {
+  CF_REQUIRE(a.n_nodes > 0 && a.n_elem > 0 && a.conn && a.geo && a.offs && a.perm,
+             "matrix-free Tet10: bad arguments");
+  return CF_OK;
+}
+
+size_t mf_layout(int64_t n_nodes, int64_t n_elem, int precond, void* base, Layout* L,
+                 double** fe, double** blocks) {
+  cf::Carve c(base);
+  const int64_t n = 3 * n_nodes;
+  L->r = c.take<double>(n);
+  L->p = c.take<double>(n);
+  L->Ap = c.take<double>(n);
+  L->dinv = c.take<double>(precond == CF_PC_BLOCK3 ? 3 * n : n);
+  L->partials = c.take<double>(4 * size_t(pcg_grid()));
+  L->state = c.take<PcgState>(1);
+  L->bad = c.take<int32_t>(1);
+  *fe = c.take<double>(30 * n_elem);
+  *blocks = c.take<double>(9 * n_nodes);
+  L->total = c.used;
+  return c.used;
+}
+
+int mf_apply(const MfArgs& a, const double* xin, double* y, const uint8_t* fixed, double* fe,
+             PcgState* state, double* partials, hipStream_t s) {
+  const int G = pcg_grid();
+  mf_t10_elem<<<cf::stride_grid(a.n_elem, kBlock), kBlock, 0, s>>>(a.n_elem, a.conn, a.geo,
+                                                                    a.lam, a.mu, xin, fe, state);
+  CF_LAUNCHED();
+  mf_gather<<<G, kBlock, 0, s>>>(a.n_nodes, a.offs, a.perm, fe, fixed, xin, y, state, partials);
+  CF_LAUNCHED();
+  return CF_OK;
+}
+
+}  // namespace
+
+extern "C" int cf_mf_t10_workspace_bytes(int64_t n_nodes, int64_t n_elem, int32_t precond,
+                                         size_t* bytes) {
+  CF_REQUIRE(bytes && n_nodes > 0 && n_elem > 0, "cf_mf_t10_workspace_bytes: bad arguments");
+  Layout L{};
+  double *fe, *blocks;
+  *bytes = mf_layout(n_nodes, n_elem, precond, nullptr, &L, &fe, &blocks);
+  return CF_OK;
+}
+
+extern "C" int cf_mf_t10_apply(int64_t n_nodes, int64_t n_elem, const int32_t* conn,
+                               const double* geo, double lam, double mu, const int32_t* offs,
+                               const uint32_t* perm, const double* x, double* y,
+                               const uint8_t* fixed, void* ws, size_t ws_bytes, void* stream) {
+  MfArgs a{n_nodes, n_elem, conn, geo, lam, mu, offs, perm};
+  int rc = mf_check(a);
+  if (rc) return rc;
+  CF_REQUIRE(x && y && ws, "cf_mf_t10_apply: null argument");
+  Layout L{};
+  double *fe, *blocks;
+  const size_t need = mf_layout(n_nodes, n_elem, CF_PC_JACOBI, ws, &L, &fe, &blocks);
+  if (ws_bytes < need) {
+    cf::set_error("cf_mf_t10_apply: workspace %zu < %zu bytes", ws_bytes, need);
+    return CF_ERR_WORKSPACE;
+  }
+  return mf_apply(a, x, y, fixed, fe, nullptr, nullptr, cf::as_stream(stream));
+}
+
+extern "C" int cf_pcg_solve_mf_t10(int64_t n_nodes, int64_t n_elem, const int32_t* conn,
+                                   const double* geo, double lam, double mu,
+                                   const int32_t* offs, const uint32_t* perm, const double* b,
+                                   double* x, const uint8_t* fixed, int32_t precond, double tol,
+                                   int32_t maxit, void* ws, size_t ws_bytes,
+                                   cf_pcg_stats* stats_host, void* stream) {
+  MfArgs a{n_nodes, n_elem, conn, geo, lam, mu, offs, perm};
+  int rc = mf_check(a);
+  if (rc) return rc;
+  CF_REQUIRE(b && x && ws && stats_host, "cf_pcg_solve_mf_t10: null argument");
+  CF_REQUIRE(precond == CF_PC_JACOBI || precond == CF_PC_BLOCK3,
+             "cf_pcg_solve_mf_t10: bad precond %d", precond);
+  CF_REQUIRE(tol > 0.0 && maxit > 0, "cf_pcg_solve_mf_t10: tol and maxit must be positive");
+  Layout L{};
+  double *fe, *blocks;
+  const size_t need = mf_layout(n_nodes, n_elem, precond, ws, &L, &fe, &blocks);
+  if (ws_bytes < need) {
+    cf::set_error("cf_pcg_solve_mf_t10: workspace %zu < %zu bytes", ws_bytes, need);
+    return CF_ERR_WORKSPACE;
+  }
+  hipStream_t s = cf::as_stream(stream);
+  const int G = pcg_grid();
+  *stats_host = cf_pcg_stats{};
+  CF_HIP(hipMemsetAsync(L.state, 0, sizeof(PcgState), s));
+  CF_HIP(hipMemsetAsync(L.bad, 0, sizeof(int32_t), s));
+  mf_t10_diag<<<cf::stride_grid(n_nodes, kBlock), kBlock, 0, s>>>(n_nodes, offs, perm, geo, lam,
+                                                                  mu, blocks);
+  CF_LAUNCHED();
+  blocks_to_precond<<<cf::stride_grid(n_nodes, kBlock), kBlock, 0, s>>>(
+      n_nodes, blocks, fixed, precond == CF_PC_BLOCK3, L.dinv, L.bad);
+  CF_LAUNCHED();
+  int32_t bad = 0;
+  CF_HIP(hipMemcpyAsync(&bad, L.bad, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  CF_HIP(hipStreamSynchronize(s));
+  if (bad) {
+    cf::set_error("cf_pcg_solve_mf_t10: %d free diagonal entries/blocks are not positive", bad);
+    return CF_ERR_BREAKDOWN;
+  }
+  (void)G;
+  auto apply = [&](const double* xin, double* y, PcgState* state, double* partials) {
+    return mf_apply(a, xin, y, fixed, fe, state, partials, s);
+  };
+  const int64_t n = 3 * n_nodes;
+  if (precond == CF_PC_BLOCK3)
+    return pcg_fp64_core<CF_PC_BLOCK3>(apply, n, n_nodes, b, x, fixed, tol, maxit, L, stats_host,
+                                       s);
+  return pcg_fp64_core<CF_PC_JACOBI>(apply, n, n, b, x, fixed, tol, maxit, L, stats_host, s);
 }

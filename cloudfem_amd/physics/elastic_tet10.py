@@ -19,6 +19,7 @@ from ..device import DeviceMesh, to_dev, vector_csr
 from ..fem.tet10 import quadrature, shape_functions
 from ..mesh.faces import T10_FACES as FACES10
 from ..mesh.faces import boundary_faces10
+from .. import solver as solver_mod
 from ..solver import DeviceMatrix, pcg
 from .elastic3d import constitutive_3d, lame
 
@@ -81,17 +82,28 @@ def assemble_elasticity_tet10(mesh, E, nu, body=None, traction=None, T=None, alp
     return vector_csr(dm.pattern(), vals, dense3=True), rhs.cpu().numpy()
 
 
-def solve_device(dm, vals, rhs, mesh, bc, **pcg_kw):
-    """Solve the assembled system on the device; returns (u (3N,) tensor, stats)."""
+def solve_device(dm, vals, rhs, mesh, bc, lam=None, mu=None, **pcg_kw):
+    """Solve on the device; returns (u (3N,) tensor, stats).
+
+    With solver option operator="matfree" (FP64 only) the stiffness is applied
+    matrix-free from the geometry record (needs lam, mu); otherwise the
+    assembled NB3 matrix `vals` is used.
+    """
     n3 = 3 * dm.n_nodes
-    A = DeviceMatrix(N.FMT_NB3, dm.pattern(), vals)
     if bc:
         mask, fv = dirichlet_vec3(mesh, bc)
     else:
         mask, fv = np.zeros(n3, dtype=bool), np.zeros(n3)
     x = to_dev(np.where(mask, fv, 0.0), torch.float64, dm.dev)
     fixed = to_dev(mask.astype(np.uint8), torch.uint8, dm.dev) if mask.any() else None
-    stats = pcg(A, rhs, x, fixed, **pcg_kw)
+    o = solver_mod.options()
+    if (o["operator"] == "matfree" and lam is not None and
+            pcg_kw.get("precision", o["precision"]) == "fp64"):
+        kw = {k: v for k, v in pcg_kw.items() if k != "precision"}
+        stats = solver_mod.pcg_matfree_t10(dm, lam, mu, rhs, x, fixed, **kw)
+    else:
+        A = DeviceMatrix(N.FMT_NB3, dm.pattern(), vals)
+        stats = pcg(A, rhs, x, fixed, **pcg_kw)
     return x, stats
 
 
@@ -99,7 +111,8 @@ def solve_elasticity_tet10(mesh, E, nu, bc=None, body=None, traction=None, T=Non
                            T_ref=0.0):
     """(ux, uy, uz) over all Tet10 nodes (elastic_tet10.py:214-229)."""
     dm, vals, rhs = _assemble_device(mesh, E, nu, body, traction, T, alpha, T_ref)
-    x, _ = solve_device(dm, vals, rhs, mesh, bc)
+    lam, mu = lame(E, nu)
+    x, _ = solve_device(dm, vals, rhs, mesh, bc, lam=lam, mu=mu)
     u = x.cpu().numpy()
     return u[0::3].copy(), u[1::3].copy(), u[2::3].copy()
 

@@ -14,6 +14,9 @@ problems:
              problems, FP64 only)
   tol        relative residual ||b_f - A_ff x_f|| / ||b_f|| (default 1e-12)
   maxit      iteration cap (default 200000)
+  operator   "assembled" (default: CSR/NB3 SpMV) or "matfree" (Tet10
+             elasticity only: element forces recomputed from the geometry
+             record every product, cf_pcg_solve_mf_t10)
 """
 
 import contextlib
@@ -25,7 +28,8 @@ import torch
 from . import _native as N
 from .device import ptr, stream_handle, workspace
 
-_DEFAULTS = {"precision": "fp64", "precond": "jacobi", "tol": 1e-12, "maxit": 200000}
+_DEFAULTS = {"precision": "fp64", "precond": "jacobi", "tol": 1e-12, "maxit": 200000,
+             "operator": "assembled"}
 _local = threading.local()
 
 
@@ -105,3 +109,45 @@ def pcg(A, b, x, fixed=None, precision=None, precond=None, tol=None, maxit=None)
     last_stats.value = stats
     N.check(rc, "cf_pcg_solve")
     return stats
+
+
+def pcg_matfree_t10(dm, lam, mu, b, x, fixed=None, precond=None, tol=None, maxit=None):
+    """PCG with the matrix-free Tet10 elasticity operator (FP64)."""
+    o = options()
+    precond = precond or o["precond"]
+    tol = float(tol if tol is not None else o["tol"])
+    maxit = int(maxit if maxit is not None else o["maxit"])
+    pc = {"jacobi": N.PC_JACOBI, "block3": N.PC_BLOCK3}[precond]
+    lib = N.load()
+    offs, perm = dm.incidence()
+    nb = ctypes.c_size_t()
+    N.check(lib.cf_mf_t10_workspace_bytes(dm.n_nodes, dm.n_elem, pc, ctypes.byref(nb)),
+            "cf_mf_t10_workspace_bytes")
+    ws = workspace(nb.value, b.device)
+    st = N.CfPcgStats()
+    rc = lib.cf_pcg_solve_mf_t10(dm.n_nodes, dm.n_elem, ptr(dm.conn), ptr(dm.geometry()),
+                                 float(lam), float(mu), ptr(offs), ptr(perm), ptr(b), ptr(x),
+                                 ptr(fixed), pc, tol, maxit, ptr(ws), nb.value,
+                                 ctypes.byref(st), stream_handle())
+    stats = {"iterations": st.iterations, "residual_updates": 0,
+             "rel_residual": st.rel_residual, "b_norm": st.b_norm, "precision": "fp64",
+             "precond": precond, "tol": tol, "operator": "matfree"}
+    last_stats.value = stats
+    N.check(rc, "cf_pcg_solve_mf_t10")
+    return stats
+
+
+def matfree_t10_apply(dm, lam, mu, x, fixed=None):
+    """y = K x with the matrix-free Tet10 elasticity operator."""
+    lib = N.load()
+    offs, perm = dm.incidence()
+    nb = ctypes.c_size_t()
+    N.check(lib.cf_mf_t10_workspace_bytes(dm.n_nodes, dm.n_elem, N.PC_JACOBI, ctypes.byref(nb)),
+            "cf_mf_t10_workspace_bytes")
+    ws = workspace(nb.value, x.device)
+    y = torch.empty_like(x)
+    N.check(lib.cf_mf_t10_apply(dm.n_nodes, dm.n_elem, ptr(dm.conn), ptr(dm.geometry()),
+                                float(lam), float(mu), ptr(offs), ptr(perm), ptr(x), ptr(y),
+                                ptr(fixed), ptr(ws), nb.value, stream_handle()),
+            "cf_mf_t10_apply")
+    return y
