@@ -52,33 +52,42 @@ __global__ void store_count(const int32_t* __restrict__ incl, int64_t n,
   if (blockIdx.x == 0 && threadIdx.x == 0) *nnz = n > 0 ? int64_t(incl[n - 1]) : 0;
 }
 
-// One thread per sorted pair; run heads write the column, the contributor
-// segment start and the row pointers of every row they open (rows with no
-// entries get the pointer of the next stored entry).
+// One thread per sorted pair; run heads write the column and the contributor
+// segment start of their stored entry.
 __global__ void pattern_fill(const uint64_t* __restrict__ keys,
                              const int32_t* __restrict__ incl, int64_t n,
-                             int sh, int64_t n_rows, int64_t nnz,
-                             int32_t* __restrict__ indptr,
-                             int32_t* __restrict__ indices,
+                             int sh, int64_t nnz, int32_t* __restrict__ indices,
                              int32_t* __restrict__ seg) {
   const uint64_t mask = (uint64_t(1) << sh) - 1;
   for (int64_t j = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; j < n;
        j += int64_t(gridDim.x) * blockDim.x) {
-    uint64_t key = keys[j];
-    bool head = (j == 0 || keys[j - 1] != key);
-    if (!head) continue;
-    int32_t s = incl[j] - 1;
+    const uint64_t key = keys[j];
+    if (j > 0 && keys[j - 1] == key) continue;
+    const int32_t s = incl[j] - 1;
     indices[s] = int32_t(key & mask);
     seg[s] = int32_t(j);
-    int64_t row = int64_t(key >> sh);
-    int64_t prev = (j == 0) ? -1 : int64_t(keys[j - 1] >> sh);
-    for (int64_t r = prev + 1; r <= row; ++r) indptr[r] = s;
   }
-  // rows after the last stored entry, and the closing segment offset
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    int64_t last_row = n > 0 ? int64_t(keys[n - 1] >> sh) : -1;
-    for (int64_t r = last_row + 1; r <= n_rows; ++r) indptr[r] = int32_t(nnz);
-    seg[nnz] = int32_t(n);
+  if (blockIdx.x == 0 && threadIdx.x == 0) seg[nnz] = int32_t(n);
+}
+
+// indptr[r] = slot of the first stored entry with row >= r: a lower bound
+// over the sorted keys (that entry is always a run head), nnz past the end.
+// One binary search per row, so empty and trailing rows cost nothing extra.
+__global__ void pattern_rowptr(const uint64_t* __restrict__ keys,
+                               const int32_t* __restrict__ incl, int64_t n, int sh,
+                               int64_t n_rows, int64_t nnz, int32_t* __restrict__ indptr) {
+  for (int64_t r = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; r <= n_rows;
+       r += int64_t(gridDim.x) * blockDim.x) {
+    int64_t lo = 0, hi = n;
+    const uint64_t first = uint64_t(r) << sh;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (keys[mid] < first)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    indptr[r] = lo < n ? incl[lo] - 1 : int32_t(nnz);
   }
 }
 
@@ -92,17 +101,20 @@ __global__ void item_keys(const int32_t* __restrict__ targets, int64_t n,
   }
 }
 
+// offs[q] = first position whose sorted key is >= q (binary search per node).
 __global__ void incidence_offsets(const uint32_t* __restrict__ keys, int64_t n,
                                   int64_t n_nodes, int32_t* __restrict__ offs) {
-  for (int64_t j = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; j < n;
-       j += int64_t(gridDim.x) * blockDim.x) {
-    int64_t r = keys[j];
-    int64_t prev = (j == 0) ? -1 : int64_t(keys[j - 1]);
-    for (int64_t q = prev + 1; q <= r; ++q) offs[q] = int32_t(j);
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    int64_t last = n > 0 ? int64_t(keys[n - 1]) : -1;
-    for (int64_t q = last + 1; q <= n_nodes; ++q) offs[q] = int32_t(n);
+  for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q <= n_nodes;
+       q += int64_t(gridDim.x) * blockDim.x) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (int64_t(keys[mid]) < q)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    offs[q] = int32_t(lo);
   }
 }
 
@@ -206,8 +218,11 @@ extern "C" int cf_pattern_finalize(int64_t n_elem, int32_t k, int64_t n_nodes,
   if (rc) return rc;
   hipStream_t s = cf::as_stream(stream);
   int grid = cf::stride_grid(n > 0 ? n : 1, kBlock);
-  pattern_fill<<<grid, kBlock, 0, s>>>(w.keys_out, w.incl, n, sh, n_nodes, nnz,
-                                       indptr, indices, seg);
+  pattern_fill<<<grid, kBlock, 0, s>>>(w.keys_out, w.incl, n, sh, nnz, indices, seg);
+  CF_LAUNCHED();
+  pattern_rowptr<<<cf::stride_grid(n_nodes + 1, kBlock), kBlock, 0, s>>>(w.keys_out, w.incl, n,
+                                                                          sh, n_nodes, nnz,
+                                                                          indptr);
   CF_LAUNCHED();
   return CF_OK;
 }
@@ -273,7 +288,8 @@ extern "C" int cf_incidence_build(const int32_t* targets, int64_t n_items,
     CF_HIP(rocprim::radix_sort_pairs(w.tmp, tb, w.keys_in, w.keys_out, w.ids_in,
                                      perm, size_t(n_items), 0, end_bit, s));
   }
-  incidence_offsets<<<grid, kBlock, 0, s>>>(w.keys_out, n_items, n_nodes, offs);
+  incidence_offsets<<<cf::stride_grid(n_nodes + 1, kBlock), kBlock, 0, s>>>(w.keys_out, n_items,
+                                                                           n_nodes, offs);
   CF_LAUNCHED();
   return CF_OK;
 }

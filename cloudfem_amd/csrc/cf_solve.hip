@@ -95,37 +95,18 @@ __device__ void block_sum(double (&v)[NV]) {
   }
 }
 
-// Grid-wide deterministic sum.  Every block contributes its partials; the
-// block that arrives last sums them in fixed order and returns true (in all
-// of its threads) with the totals in v.
+// Per-block partial sums for a grid-wide reduction: thread 0 of each block
+// stores its block's sums to partials[k * gridDim.x + blockIdx.x] with plain
+// stores.  A 1-block `finalize` kernel launched next folds them in a fixed
+// order (deterministic).  No in-kernel cross-block handshake: on gfx950 an
+// agent-scope fence writes back and invalidates the XCD's L2, which, issued
+// once per block, ruined the bandwidth of every kernel that reduced.
 template <int NV>
-__device__ bool grid_sum(double (&v)[NV], double* __restrict__ partials,
-                         uint32_t* ticket) {
-  __shared__ bool is_last;
+__device__ void block_partials(double (&v)[NV], double* __restrict__ partials) {
   block_sum<NV>(v);
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0)
 #pragma unroll
-    for (int k = 0; k < NV; ++k)
-      __hip_atomic_store(&partials[k * gridDim.x + blockIdx.x], v[k], __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    uint32_t t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    is_last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
-#pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    double s = 0.0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x)
-      s += __hip_atomic_load(&partials[k * gridDim.x + b], __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
-    v[k] = s;
-  }
-  block_sum<NV>(v);
-  if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return true;
+    for (int k = 0; k < NV; ++k) partials[k * gridDim.x + blockIdx.x] = v[k];
 }
 
 // Contiguous row range of a block, with blocks of one XCD covering one
@@ -194,14 +175,87 @@ spmv_nb3(int64_t n_nodes, const int32_t* __restrict__ ptr,
   }
   if (!state) return;
   double v1[1] = {dot};
-  if (grid_sum<1>(v1, partials, &state->ticket) && threadIdx.x == 0) {
-    state->pAp = v1[0];
-    if (!(v1[0] > 0.0) || !isfinite(v1[0])) {
-      state->halt = kBreakdown;
-    } else {
-      state->alpha = state->rz / v1[0];
+  block_partials<1>(v1, partials);
+}
+
+// NB3 SpMV, one node row per 64-lane wave.  The row's column indices (up to
+// 128 blocks) are loaded once, coalesced, and handed to the lanes that need
+// them with cross-lane permutes, so the x gathers do not wait on a
+// per-value index load; each lane then issues the value and x loads of up
+// to 4 slots (256 values per pass) before the first FMA.  Rows longer than
+// 128 blocks fall back to direct index loads.
+template <class VT, class XT, class AT>
+__global__ void __launch_bounds__(kBlock)
+spmv_nb3_wave(int64_t n_nodes, const int32_t* __restrict__ ptr,
+              const int32_t* __restrict__ idx, const VT* __restrict__ val,
+              const XT* __restrict__ x, XT* __restrict__ y,
+              const uint8_t* __restrict__ fixed, PcgState* state,
+              double* __restrict__ partials) {
+  if (state && state->halt) return;
+  constexpr int kW = 64;
+  const int lane = threadIdx.x & (kW - 1);
+  const int wave = threadIdx.x / kW;
+  constexpr int kWaves = kBlock / kW;
+  int64_t r0, r1;
+  block_rows(n_nodes, r0, r1);
+  double dot = 0.0;
+  for (int64_t i = r0 + wave; i < r1; i += kWaves) {
+    const int32_t base = ptr[i];
+    const int32_t len = ptr[i + 1] - base;
+    const int32_t l3 = 3 * len, nv = 9 * len;
+    const VT* v = val + int64_t(9) * base;
+    const int32_t* col = idx + base;
+    const int c_lo = lane < len ? col[lane] : 0;
+    const int c_hi = lane + kW < len ? col[lane + kW] : 0;
+    const bool direct = len > 2 * kW;
+    AT a0 = 0, a1 = 0, a2 = 0;
+    for (int32_t f0 = 0; f0 < nv; f0 += 4 * kW) {
+      VT vv[4];
+      XT xv[4];
+      int cc[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int32_t f = f0 + u * kW + lane;
+        const bool ok = f < nv;
+        const int c = (f >= l3) + (f >= 2 * l3);
+        const int32_t q = f - c * l3;
+        const int32_t kq = q / 3;
+        const int cp = q - 3 * kq;
+        // the source lane (kq mod 64) holds col[kq] in c_lo (kq < 64) or
+        // c_hi (64 <= kq < 128): fetch both, select on the requesting lane
+        const int from_lo = __shfl(c_lo, kq & (kW - 1), kW);
+        const int from_hi = __shfl(c_hi, kq & (kW - 1), kW);
+        int cj = kq < kW ? from_lo : from_hi;
+        if (direct && ok) cj = col[kq];
+        cc[u] = ok ? c : 3;
+        vv[u] = ok ? ld(v + f) : VT(0);
+        xv[u] = ok ? x[3 * int64_t(cj) + cp] : XT(0);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const AT prod = AT(vv[u]) * AT(xv[u]);
+        a0 += cc[u] == 0 ? prod : AT(0);
+        a1 += cc[u] == 1 ? prod : AT(0);
+        a2 += cc[u] == 2 ? prod : AT(0);
+      }
+    }
+    a0 = group_sum<kW>(a0);
+    a1 = group_sum<kW>(a1);
+    a2 = group_sum<kW>(a2);
+    if (lane == 0) {
+      AT o[3] = {a0, a1, a2};
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int64_t d = 3 * i + c;
+        if (fixed && fixed[d]) o[c] = AT(0);
+        y[d] = XT(o[c]);
+        if (state) dot += double(x[d]) * double(XT(o[c]));
+      }
     }
   }
+  if (!state) return;
+  double v1[1] = {dot};
+  block_partials<1>(v1, partials);
 }
 
 template <class VT, class XT, class AT, int L>
@@ -231,14 +285,7 @@ spmv_csr(int64_t n_rows, const int32_t* __restrict__ ptr,
   }
   if (!state) return;
   double v1[1] = {dot};
-  if (grid_sum<1>(v1, partials, &state->ticket) && threadIdx.x == 0) {
-    state->pAp = v1[0];
-    if (!(v1[0] > 0.0) || !isfinite(v1[0])) {
-      state->halt = kBreakdown;
-    } else {
-      state->alpha = state->rz / v1[0];
-    }
-  }
+  block_partials<1>(v1, partials);
 }
 
 // ------------------------------------------------------------- preconditioners
@@ -376,22 +423,7 @@ cg_update(int64_t n_units, double* __restrict__ x, double* __restrict__ r,
     for (int c = 0; c < W; ++c) rz += r[W * u + c] * z[c];
   }
   double v[2] = {rr, rz};
-  if (grid_sum<2>(v, partials, &state->ticket) && threadIdx.x == 0) {
-    state->iter += 1;
-    if (!isfinite(v[0]) || !isfinite(v[1])) {
-      state->halt = kBreakdown;
-    } else if (v[0] <= state->stop2) {
-      state->rr = v[0];
-      state->halt = kConverged;
-    } else if (state->iter >= state->maxit) {
-      state->rr = v[0];
-      state->halt = kMaxit;
-    } else {
-      state->beta = v[1] / state->rz;
-      state->rz = v[1];
-      state->rr = v[0];
-    }
-  }
+  block_partials<2>(v, partials);
 }
 
 template <int PC>
@@ -437,7 +469,9 @@ residual(int64_t n, const double* __restrict__ b, const double* __restrict__ Ax,
     rr += v * v;
   }
   double v1[1] = {rr};
-  if (grid_sum<1>(v1, partials, &state->ticket) && threadIdx.x == 0) *out_norm2 = v1[0];
+  block_partials<1>(v1, partials);
+  (void)state;
+  (void)out_norm2;
 }
 
 // p = z = M^-1 r, state->rz = r.z, state->rr = r.r
@@ -459,10 +493,8 @@ cg_start(int64_t n_units, const double* __restrict__ r, double* __restrict__ p,
     }
   }
   double v[2] = {rr, rz};
-  if (grid_sum<2>(v, partials, &state->ticket) && threadIdx.x == 0) {
-    state->rr = v[0];
-    state->rz = v[1];
-  }
+  block_partials<2>(v, partials);
+  (void)state;
 }
 
 // ------------------------------------------------------------- mixed precision
@@ -541,14 +573,7 @@ spmv_nb3_planar(int64_t n_nodes, const int32_t* __restrict__ ptr,
   }
   if (!state) return;
   double v1[1] = {dot};
-  if (grid_sum<1>(v1, partials, &state->ticket) && threadIdx.x == 0) {
-    state->pAp = v1[0];
-    if (!(v1[0] > 0.0) || !isfinite(v1[0])) {
-      state->halt = kBreakdown;
-    } else {
-      state->alpha = state->rz / v1[0];
-    }
-  }
+  block_partials<1>(v1, partials);
 }
 
 __global__ void scale_csr_to_f32(int64_t n, const int32_t* __restrict__ ptr,
@@ -575,17 +600,9 @@ mx_load_residual(int64_t n, const double* __restrict__ r, const double* __restri
     rr += double(v) * double(v);
   }
   double v1[1] = {rr};
-  if (grid_sum<1>(v1, partials, &state->ticket) && threadIdx.x == 0) {
-    if (start) {
-      state->rz = v1[0];
-    } else {
-      // keep the search direction: beta = new r.r / previous r.r
-      state->beta = v1[0] / state->rz;
-      state->rz = v1[0];
-    }
-    state->rr = v1[0];
-    state->refresh2 = v1[0] * refresh_factor2;
-  }
+  block_partials<1>(v1, partials);
+  (void)state;
+  (void)refresh_factor2;
 }
 
 // y += alpha p; r -= alpha Ap; reduce r.r (FP32 vectors, FP64 sums).
@@ -604,22 +621,7 @@ mx_update(int64_t n, float* __restrict__ y, float* __restrict__ rh,
     rr += double(rn) * double(rn);
   }
   double v1[1] = {rr};
-  if (grid_sum<1>(v1, partials, &state->ticket) && threadIdx.x == 0) {
-    state->iter += 1;
-    const double rn = v1[0];
-    if (!isfinite(rn)) {
-      state->halt = kBreakdown;
-    } else if (rn <= state->stop2 || rn <= state->refresh2 || state->iter >= state->maxit) {
-      // leave rz at the previous r.r: after the replacement the direction
-      // update uses beta = (S r_true).(S r_true) / previous r.r
-      state->rr = rn;
-      state->halt = kRefresh;
-    } else {
-      state->beta = rn / state->rz;
-      state->rz = rn;
-      state->rr = rn;
-    }
-  }
+  block_partials<1>(v1, partials);
 }
 
 __global__ void mx_direction(int64_t n, const float* __restrict__ rh, float* __restrict__ ph,
@@ -737,7 +739,29 @@ mf_gather(int64_t n_nodes, const int32_t* __restrict__ offs, const uint32_t* __r
   double dot = 0.0;
   for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    for (int32_t j = offs[i]; j < offs[i + 1]; ++j) {
+    const int32_t j1 = offs[i + 1];
+    int32_t j = offs[i];
+    // 4 items per step: all index loads, then all force loads, then sums
+    for (; j + 4 <= j1; j += 4) {
+      uint32_t it[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) it[u] = perm[j + u];
+      double f[4][3];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double* v = fe + int64_t(it[u]) * 3;
+        f[u][0] = v[0];
+        f[u][1] = v[1];
+        f[u][2] = v[2];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a0 += f[u][0];
+        a1 += f[u][1];
+        a2 += f[u][2];
+      }
+    }
+    for (; j < j1; ++j) {
       const double* v = fe + int64_t(perm[j]) * 3;
       a0 += v[0];
       a1 += v[1];
@@ -754,14 +778,7 @@ mf_gather(int64_t n_nodes, const int32_t* __restrict__ offs, const uint32_t* __r
   }
   if (!state) return;
   double v1[1] = {dot};
-  if (grid_sum<1>(v1, partials, &state->ticket) && threadIdx.x == 0) {
-    state->pAp = v1[0];
-    if (!(v1[0] > 0.0) || !isfinite(v1[0])) {
-      state->halt = kBreakdown;
-    } else {
-      state->alpha = state->rz / v1[0];
-    }
-  }
+  block_partials<1>(v1, partials);
 }
 
 // Diagonal 3x3 node blocks of the Tet10 elasticity matrix without
@@ -842,6 +859,96 @@ __global__ void blocks_to_precond(int64_t n_nodes, const double* __restrict__ bl
   }
 }
 
+// ------------------------------------------------------------- finalize
+
+enum Fin : int {
+  kFinAlpha = 0,      // pAp -> alpha (or breakdown)
+  kFinCgUpdate = 1,   // (r.r, r.z) -> convergence test, beta
+  kFinStore = 2,      // sum -> *out
+  kFinStart = 3,      // (r.r, r.z) -> state
+  kFinMxStart = 4,    // r.r of a fresh FP32 residual, new direction
+  kFinMxReplace = 5,  // r.r of a replaced FP32 residual, kept direction
+  kFinMxUpdate = 6,   // recursive FP32 r.r -> beta / replacement request
+};
+
+// One block folds the per-block partials of the previous kernel (G of
+// them per value, fixed order) and updates the CG scalars.
+__global__ void __launch_bounds__(kBlock)
+finalize(int mode, int G, const double* __restrict__ partials, PcgState* state,
+         double* __restrict__ out, double aux) {
+  const bool gated = (mode == kFinAlpha || mode == kFinCgUpdate || mode == kFinMxUpdate);
+  if (gated && state->halt) return;
+  const int nv = (mode == kFinCgUpdate || mode == kFinStart) ? 2 : 1;
+  double v[2] = {0.0, 0.0};
+  for (int k = 0; k < nv; ++k)
+    for (int b = threadIdx.x; b < G; b += blockDim.x) v[k] += partials[k * G + b];
+  block_sum<2>(v);
+  if (threadIdx.x != 0) return;
+  switch (mode) {
+    case kFinAlpha:
+      state->pAp = v[0];
+      if (!(v[0] > 0.0) || !isfinite(v[0]))
+        state->halt = kBreakdown;
+      else
+        state->alpha = state->rz / v[0];
+      break;
+    case kFinCgUpdate:
+      state->iter += 1;
+      if (!isfinite(v[0]) || !isfinite(v[1])) {
+        state->halt = kBreakdown;
+      } else if (v[0] <= state->stop2) {
+        state->rr = v[0];
+        state->halt = kConverged;
+      } else if (state->iter >= state->maxit) {
+        state->rr = v[0];
+        state->halt = kMaxit;
+      } else {
+        state->beta = v[1] / state->rz;
+        state->rz = v[1];
+        state->rr = v[0];
+      }
+      break;
+    case kFinStore:
+      *out = v[0];
+      break;
+    case kFinStart:
+      state->rr = v[0];
+      state->rz = v[1];
+      break;
+    case kFinMxStart:
+      state->rz = v[0];
+      state->rr = v[0];
+      state->refresh2 = v[0] * aux;
+      break;
+    case kFinMxReplace:
+      // keep the search direction: beta = new r.r / previous r.r
+      state->beta = v[0] / state->rz;
+      state->rz = v[0];
+      state->rr = v[0];
+      state->refresh2 = v[0] * aux;
+      break;
+    case kFinMxUpdate: {
+      state->iter += 1;
+      const double rn = v[0];
+      if (!isfinite(rn)) {
+        state->halt = kBreakdown;
+      } else if (rn <= state->stop2 || rn <= state->refresh2 || state->iter >= state->maxit) {
+        // leave rz at the previous r.r: after the replacement the direction
+        // update uses beta = (S r_true).(S r_true) / previous r.r
+        state->rr = rn;
+        state->halt = kRefresh;
+      } else {
+        state->beta = rn / state->rz;
+        state->rz = rn;
+        state->rr = rn;
+      }
+      break;
+    }
+    default:
+      break;
+  }
+}
+
 // ------------------------------------------------------------- host side
 
 // Optional timing of the CG product kernels: HIP events around every 8th
@@ -907,8 +1014,8 @@ template <class VT, class XT, class AT>
 int launch_spmv(const cf_matrix* A, const VT* vals, const XT* x, XT* y, const uint8_t* fixed,
                 PcgState* state, double* partials, int grid, hipStream_t s) {
   if (A->format == CF_FMT_NB3) {
-    spmv_nb3<VT, XT, AT, kLanesNb3><<<grid, kBlock, 0, s>>>(A->n_rows, A->indptr, A->indices,
-                                                            vals, x, y, fixed, state, partials);
+    spmv_nb3_wave<VT, XT, AT><<<grid, kBlock, 0, s>>>(A->n_rows, A->indptr, A->indices, vals, x,
+                                                      y, fixed, state, partials);
   } else {
     spmv_csr<VT, XT, AT, kLanesCsr><<<grid, kBlock, 0, s>>>(A->n_rows, A->indptr, A->indices,
                                                             vals, x, y, fixed, state, partials);
@@ -978,6 +1085,13 @@ int read_state(const PcgState* d, PcgState* h, hipStream_t s) {
   return CF_OK;
 }
 
+// Fold the partials of the kernel just launched (grid pcg_grid()).
+int fin(int mode, const Layout& L, hipStream_t s, double* out = nullptr, double aux = 0.0) {
+  finalize<<<1, kBlock, 0, s>>>(mode, pcg_grid(), L.partials, L.state, out, aux);
+  CF_LAUNCHED();
+  return CF_OK;
+}
+
 }  // namespace
 
 extern "C" int cf_spmv(const cf_matrix* A, const double* x, double* y, const uint8_t* fixed,
@@ -1018,12 +1132,15 @@ int pcg_fp64_core(Apply&& apply, int64_t n, int64_t units, const double* b, doub
   if ((rc = apply(L.p, L.Ap, nullptr, nullptr))) return rc;
   residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->pad);
   CF_LAUNCHED();
+  if ((rc = fin(kFinStore, L, s, &L.state->pad))) return rc;
   // r = b - A x  (full x, warm start included)
   if ((rc = apply(x, L.Ap, nullptr, nullptr))) return rc;
   residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->rr);
   CF_LAUNCHED();
+  if ((rc = fin(kFinStore, L, s, &L.state->rr))) return rc;
   cg_start<PC><<<G, kBlock, 0, s>>>(units, L.r, L.p, L.dinv, L.state, L.partials);
   CF_LAUNCHED();
+  if ((rc = fin(kFinStart, L, s))) return rc;
   if ((rc = read_state(L.state, &h, s))) return rc;
   const double bb = h.pad;
   st->b_norm = std::sqrt(bb);
@@ -1056,8 +1173,10 @@ int pcg_fp64_core(Apply&& apply, int64_t n, int64_t units, const double* b, doub
       prof_begin(0, h.iter + t, s, &tok);
       if ((rc = apply(L.p, L.Ap, L.state, L.partials))) return rc;
       prof_end(tok, s);
+      if ((rc = fin(kFinAlpha, L, s))) return rc;
       cg_update<PC><<<G, kBlock, 0, s>>>(units, x, L.r, L.p, L.Ap, L.dinv, L.state, L.partials);
       CF_LAUNCHED();
+      if ((rc = fin(kFinCgUpdate, L, s))) return rc;
       cg_direction<PC><<<G, kBlock, 0, s>>>(units, L.r, L.p, L.dinv, L.state);
       CF_LAUNCHED();
     }
@@ -1082,6 +1201,7 @@ int pcg_fp64_core(Apply&& apply, int64_t n, int64_t units, const double* b, doub
   if ((rc = apply(x, L.Ap, nullptr, nullptr))) return rc;
   residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->pad);
   CF_LAUNCHED();
+  if ((rc = fin(kFinStore, L, s, &L.state->pad))) return rc;
   PcgState fin{};
   if ((rc = read_state(L.state, &fin, s))) return rc;
   st->rel_residual = std::sqrt(fin.pad / bb);
@@ -1142,12 +1262,14 @@ int pcg_mixed(const cf_matrix* A, const double* b, double* x, const uint8_t* fix
     return rc;
   residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->pad);
   CF_LAUNCHED();
+  if ((rc = fin(kFinStore, L, s, &L.state->pad))) return rc;
   // FP64 residual of the initial guess
   if ((rc = launch_spmv<double, double, double>(A, A->values, x, L.Ap, fixed, nullptr, nullptr,
                                                 G, s)))
     return rc;
   residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->rr);
   CF_LAUNCHED();
+  if ((rc = fin(kFinStore, L, s, &L.state->rr))) return rc;
   if ((rc = read_state(L.state, &h, s))) return rc;
   const double bb = h.pad;
   st->b_norm = std::sqrt(bb);
@@ -1174,6 +1296,7 @@ int pcg_mixed(const cf_matrix* A, const double* b, double* x, const uint8_t* fix
   mx_load_residual<<<G, kBlock, 0, s>>>(n, L.r, L.s, L.rh, L.ph, 1, L.state, L.partials,
                                         kRefresh2);
   CF_LAUNCHED();
+  if ((rc = fin(kFinMxStart, L, s, nullptr, kRefresh2))) return rc;
   if ((rc = read_state(L.state, &h, s))) return rc;
   int iters = 0, updates = 0, stall = 0;
   double best_rr = true_rr;
@@ -1202,8 +1325,10 @@ int pcg_mixed(const cf_matrix* A, const double* b, double* x, const uint8_t* fix
           return rc;
         }
         prof_end(tok, s);
+        if ((rc = fin(kFinAlpha, L, s))) return rc;
         mx_update<<<G, kBlock, 0, s>>>(n, L.yh, L.rh, L.ph, L.Aph, L.state, L.partials);
         CF_LAUNCHED();
+        if ((rc = fin(kFinMxUpdate, L, s))) return rc;
         mx_direction<<<G, kBlock, 0, s>>>(n, L.rh, L.ph, L.state);
         CF_LAUNCHED();
       }
@@ -1225,6 +1350,7 @@ int pcg_mixed(const cf_matrix* A, const double* b, double* x, const uint8_t* fix
       return rc;
     residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->pad);
     CF_LAUNCHED();
+    if ((rc = fin(kFinStore, L, s, &L.state->pad))) return rc;
     PcgState t{};
     if ((rc = read_state(L.state, &t, s))) return rc;
     true_rr = t.pad;
@@ -1245,12 +1371,14 @@ int pcg_mixed(const cf_matrix* A, const double* b, double* x, const uint8_t* fix
     mx_load_residual<<<G, kBlock, 0, s>>>(n, L.r, L.s, L.rh, L.ph, 0, L.state, L.partials,
                                           kRefresh2);
     CF_LAUNCHED();
+    if ((rc = fin(kFinMxReplace, L, s, nullptr, kRefresh2))) return rc;
     if ((rc = read_state(L.state, &h, s))) return rc;
     if (h.rr > kRestart2 * rec_rr) {
       // restart: p = r (mx_load_residual with start = 1 resets p and rz)
       mx_load_residual<<<G, kBlock, 0, s>>>(n, L.r, L.s, L.rh, L.ph, 1, L.state, L.partials,
                                             kRefresh2);
       CF_LAUNCHED();
+      if ((rc = fin(kFinMxStart, L, s, nullptr, kRefresh2))) return rc;
     } else {
       mx_direction_forced<<<G, kBlock, 0, s>>>(n, L.rh, L.ph, L.state);
       CF_LAUNCHED();

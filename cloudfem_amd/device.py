@@ -117,7 +117,7 @@ class DeviceMesh:
         self.n_nodes = int(x.numel())
         self.n_elem, self.k = int(conn.shape[0]), int(conn.shape[1])
         self.x, self.y, self.z, self.conn = x, y, z, conn
-        self._geo = self._pattern = self._incidence = None
+        self._geo = self._pattern = self._incidence = self._mf = None
         return self
 
     def add_face_rhs_device(self, op, faces, coef, rhs, incidence=None):
@@ -157,6 +157,25 @@ class DeviceMesh:
         if self._incidence is None:
             self._incidence = build_incidence(self.conn.reshape(-1), self.n_nodes, self.dev)
         return self._incidence
+
+    def matfree_arrays(self):
+        """(conn, geo, offs, perm) in node-local element order for the
+        matrix-free operator: elements stably sorted by their first corner
+        (one radix sort), so the elements touching a run of consecutive nodes
+        are themselves near each other and the force gather hits in L2.  The
+        operator is a sum over elements, so the order changes only rounding."""
+        if getattr(self, "_mf", None) is None:
+            _, order = build_incidence(self.conn[:, 0].contiguous(), self.n_nodes, self.dev)
+            order = order[:self.n_elem].long()
+            # the sort returns element ids; check before they index anything
+            lo, hi = int(order.min()), int(order.max())
+            if order.numel() != self.n_elem or lo < 0 or hi >= self.n_elem:
+                raise N.CFError(f"element order out of range [{lo}, {hi}] for {self.n_elem}")
+            conn = self.conn.index_select(0, order).contiguous()
+            geo = self.geometry().index_select(0, order).contiguous()
+            offs, perm = build_incidence(conn.reshape(-1), self.n_nodes, self.dev)
+            self._mf = (conn, geo, offs, perm)
+        return self._mf
 
     # ----------------------------------------------------------- assembly
 
