@@ -62,7 +62,7 @@ EXPORTED = (
     "cf_pcg_workspace_bytes", "cf_pcg_solve", "cf_dirichlet_workspace_bytes",
     "cf_dirichlet_count", "cf_dirichlet_fill", "cf_axpby", "cf_copy_masked",
     "cf_element_stress", "cf_spmv_profile", "cf_adjoint_dk", "cf_mf_t10_workspace_bytes",
-    "cf_mf_t10_apply", "cf_pcg_solve_mf_t10",
+    "cf_mf_t10_apply", "cf_pcg_solve_mf_t10", "cf_invert_permutation",
 )
 
 
@@ -124,10 +124,11 @@ _SIGS = {
     "cf_axpby": ([_I64, _D, _P, _D, _P, _P], _I32),
     "cf_copy_masked": ([_I64, _P, _P, _P, _P], _I32),
     "cf_mf_t10_workspace_bytes": ([_I64, _I64, _I32, _SZ], _I32),
-    "cf_mf_t10_apply": ([_I64, _I64, _P, _P, _D, _D, _P, _P, _P, _P, _P, _P, ctypes.c_size_t,
-                         _P], _I32),
-    "cf_pcg_solve_mf_t10": ([_I64, _I64, _P, _P, _D, _D, _P, _P, _P, _P, _P, _I32, _D, _I32, _P,
-                             ctypes.c_size_t, ctypes.POINTER(CfPcgStats), _P], _I32),
+    "cf_mf_t10_apply": ([_I64, _I64, _P, _P, _D, _D, _P, _P, _P, _P, _P, _P, _P,
+                         ctypes.c_size_t, _P], _I32),
+    "cf_pcg_solve_mf_t10": ([_I64, _I64, _P, _P, _D, _D, _P, _P, _P, _P, _P, _P, _I32, _D, _I32,
+                             _P, ctypes.c_size_t, ctypes.POINTER(CfPcgStats), _P], _I32),
+    "cf_invert_permutation": ([_I64, _P, _P, _P], _I32),
     "cf_adjoint_dk": ([_P, _I64, _I32, _P, _P, _P, _P, _P], _I32),
     "cf_spmv_profile": ([_I32, ctypes.POINTER(_D), ctypes.POINTER(_I64), ctypes.POINTER(_D),
                          ctypes.POINTER(_I64)], _I32),

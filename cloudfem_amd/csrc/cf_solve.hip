@@ -24,7 +24,11 @@
 // matrix ("reliable updates", Clark et al., Comput. Phys. Commun. 181 (2010)
 // 1517), so the stopping test is applied to the true FP64 residual.
 #include <cmath>
+#include <cstring>
 #include <vector>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include "cf_common.hpp"
 #include "cf_tables.hpp"
@@ -666,7 +670,8 @@ __constant__ cf::Tet10Tables kT10m = cf::build_tet10_tables();
 // into the nodes by mf_gather in fixed incidence order (no atomics).
 __global__ void __launch_bounds__(kBlock)
 mf_t10_elem(int64_t m, const int32_t* __restrict__ conn, const double* __restrict__ geo,
-            double lam, double mu, const double* __restrict__ x, double* __restrict__ fe,
+            double lam, double mu, const double* __restrict__ x,
+            const uint32_t* __restrict__ slot, double* __restrict__ fe,
             const PcgState* state) {
   if (state && state->halt) return;
   for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < m;
@@ -718,12 +723,78 @@ mf_t10_elem(int64_t m, const int32_t* __restrict__ conn, const double* __restric
           f[a][c] += kT10m.D[q][a][0] * P[c][0] + kT10m.D[q][a][1] * P[c][1] +
                      kT10m.D[q][a][2] * P[c][2];
     }
-    double* o = fe + e * 30;
+    if (slot) {
+      // write each nodal force into its position in node-sorted incidence
+      // order, so the node sums read a contiguous range
 #pragma unroll
-    for (int a = 0; a < 10; ++a)
+      for (int a = 0; a < 10; ++a) {
+        double* o = fe + int64_t(slot[e * 10 + a]) * 3;
+        o[0] = f[a][0];
+        o[1] = f[a][1];
+        o[2] = f[a][2];
+      }
+    } else {
+      double* o = fe + e * 30;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) o[3 * a + c] = f[a][c];
+      for (int a = 0; a < 10; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) o[3 * a + c] = f[a][c];
+    }
   }
+}
+
+// y_i = sum of the node's contiguous force slots fe[offs[i] .. offs[i+1])
+// (written in node-sorted order by mf_t10_elem), fixed rows zeroed; with
+// `state`: p.Ap partials.
+__global__ void __launch_bounds__(kBlock)
+mf_gather_sorted(int64_t n_nodes, const int32_t* __restrict__ offs,
+                 const double* __restrict__ fe, const uint8_t* __restrict__ fixed,
+                 const double* __restrict__ x, double* __restrict__ y, PcgState* state,
+                 double* __restrict__ partials) {
+  if (state && state->halt) return;
+  int64_t r0, r1;
+  block_rows(n_nodes, r0, r1);
+  double dot = 0.0;
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    const double* v = fe + int64_t(offs[i]) * 3;
+    const double* end = fe + int64_t(offs[i + 1]) * 3;
+    for (; v + 12 <= end; v += 12) {
+      double t[12];
+#pragma unroll
+      for (int u = 0; u < 12; ++u) t[u] = v[u];
+#pragma unroll
+      for (int u = 0; u < 12; u += 3) {
+        a0 += t[u];
+        a1 += t[u + 1];
+        a2 += t[u + 2];
+      }
+    }
+    for (; v < end; v += 3) {
+      a0 += v[0];
+      a1 += v[1];
+      a2 += v[2];
+    }
+    double o[3] = {a0, a1, a2};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t d = 3 * i + c;
+      if (fixed && fixed[d]) o[c] = 0.0;
+      y[d] = o[c];
+      if (state) dot += x[d] * o[c];
+    }
+  }
+  if (!state) return;
+  double v1[1] = {dot};
+  block_partials<1>(v1, partials);
+}
+
+// inv[perm[j]] = j
+__global__ void invert_permutation(int64_t n, const uint32_t* __restrict__ perm,
+                                   uint32_t* __restrict__ inv) {
+  for (int64_t j = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; j < n;
+       j += int64_t(gridDim.x) * blockDim.x)
+    inv[perm[j]] = uint32_t(j);
 }
 
 // y_i = sum over the node's incidence items (e, a) of fe[e][a] (fixed
@@ -857,6 +928,140 @@ __global__ void blocks_to_precond(int64_t n_nodes, const double* __restrict__ bl
     for (int c = 0; c < 3; ++c)
       for (int cp = 0; cp < 3; ++cp) dinv[9 * i + 3 * c + cp] = (fx[c] || fx[cp]) ? 0.0 : o[c][cp];
   }
+}
+
+// ------------------------------------------------------------- sliced NB3
+
+// Solver-internal copy of an NB3 matrix in a sliced layout (SELL-C-sigma over
+// node rows, C = 64, sigma = 2048): node rows are stably sorted by
+// decreasing length inside windows of 2048 rows, cut into slices of 64 rows
+// (one per lane of a wave), and each slice stores its k-th blocks
+// column-major: cols[(koff[s] + k) * 64 + lane], values
+// [((koff[s] + k) * 9 + q) * 64 + lane] for q = 3c + c'.  Every load of the
+// product is then a coalesced 64-lane access and each lane owns its three
+// outputs, so no cross-lane reduction is needed; padding blocks (rows
+// shorter than the slice's longest, 0.8 % on Tet10 box meshes) hold zeros.
+constexpr int kSlice = 64;
+constexpr int kSigma = kSlice * 32;
+
+// Sort key: window index above, descending row length below (exact, so the
+// sorted slice maxima never exceed the unsorted ones the workspace is sized by).
+__global__ void sell_keys(int64_t n, const int32_t* __restrict__ ptr, uint64_t* __restrict__ keys,
+                          uint32_t* __restrict__ ids) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const uint32_t len = uint32_t(ptr[i + 1] - ptr[i]);
+    keys[i] = (uint64_t(i / kSigma) << 32) | uint64_t(0xFFFFFFFFu - len);
+    ids[i] = uint32_t(i);
+  }
+}
+
+// L[s] = longest row of slice s (rows[] = sorted order, or identity if null)
+__global__ void sell_slice_len(int64_t n, int64_t n_slices, const int32_t* __restrict__ ptr,
+                               const uint32_t* __restrict__ rows, int32_t* __restrict__ L,
+                               unsigned long long* __restrict__ total) {
+  for (int64_t sl = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; sl < n_slices;
+       sl += int64_t(gridDim.x) * blockDim.x) {
+    int32_t mx = 0;
+    for (int t = 0; t < kSlice; ++t) {
+      const int64_t lin = sl * kSlice + t;
+      if (lin >= n) break;
+      const int64_t r = rows ? int64_t(rows[lin]) : lin;
+      const int32_t len = ptr[r + 1] - ptr[r];
+      mx = len > mx ? len : mx;
+    }
+    if (L) L[sl] = mx;
+    if (total) atomicAdd(total, (unsigned long long)mx);
+  }
+}
+
+template <class OT>
+__global__ void sell_fill(int64_t n, int64_t n_slices, const int32_t* __restrict__ ptr,
+                          const int32_t* __restrict__ idx, const double* __restrict__ val,
+                          const uint32_t* __restrict__ rows, const int32_t* __restrict__ koff,
+                          const double* __restrict__ scale, int32_t* __restrict__ scol,
+                          OT* __restrict__ sval) {
+  for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n_slices * kSlice;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t sl = t / kSlice;
+    const int lane = int(t - sl * kSlice);
+    const int64_t r = t < n ? int64_t(rows[t]) : -1;
+    const int32_t k0 = koff[sl], L = koff[sl + 1] - k0;
+    const int32_t base = r >= 0 ? ptr[r] : 0;
+    const int32_t len = r >= 0 ? ptr[r + 1] - base : 0;
+    for (int32_t k = 0; k < L; ++k) {
+      const int64_t g = int64_t(k0) + k;
+      int32_t col = r >= 0 ? int32_t(r) : 0;
+      double v[9] = {};
+      if (k < len) {
+        col = idx[base + k];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int cp = 0; cp < 3; ++cp) {
+            double a = val[int64_t(9) * base + c * 3 * len + 3 * k + cp];
+            if (scale) a = scale[3 * r + c] * a * scale[3 * int64_t(col) + cp];
+            v[3 * c + cp] = a;
+          }
+      }
+      scol[g * kSlice + lane] = col;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) sval[(g * 9 + q) * kSlice + lane] = OT(v[q]);
+    }
+  }
+}
+
+template <class VT, class XT, class AT>
+__device__ __forceinline__ void sell_block(const int32_t* __restrict__ scol,
+                                           const VT* __restrict__ sval, const XT* __restrict__ x,
+                                           int64_t g, int lane, AT& a0, AT& a1, AT& a2) {
+  const int64_t c3 = 3 * int64_t(scol[g * kSlice + lane]);
+  const AT x0 = AT(x[c3]), x1 = AT(x[c3 + 1]), x2 = AT(x[c3 + 2]);
+  const VT* v = sval + g * 9 * kSlice + lane;
+  a0 += AT(ld(v)) * x0 + AT(ld(v + kSlice)) * x1 + AT(ld(v + 2 * kSlice)) * x2;
+  a1 += AT(ld(v + 3 * kSlice)) * x0 + AT(ld(v + 4 * kSlice)) * x1 + AT(ld(v + 5 * kSlice)) * x2;
+  a2 += AT(ld(v + 6 * kSlice)) * x0 + AT(ld(v + 7 * kSlice)) * x1 + AT(ld(v + 8 * kSlice)) * x2;
+}
+
+// y = A x on the sliced copy; one slice per wave, one node row per lane.
+template <class VT, class XT, class AT>
+__global__ void __launch_bounds__(kBlock)
+spmv_sell(int64_t n, int64_t n_slices, const uint32_t* __restrict__ rows,
+          const int32_t* __restrict__ koff, const int32_t* __restrict__ scol,
+          const VT* __restrict__ sval, const XT* __restrict__ x, XT* __restrict__ y,
+          const uint8_t* __restrict__ fixed, PcgState* state, double* __restrict__ partials) {
+  if (state && state->halt) return;
+  const int lane = threadIdx.x & (kSlice - 1);
+  const int wave = threadIdx.x / kSlice;
+  constexpr int kWavesPerBlock = kBlock / kSlice;
+  int64_t s0, s1;
+  block_rows(n_slices, s0, s1);
+  double dot = 0.0;
+  for (int64_t sl = s0 + wave; sl < s1; sl += kWavesPerBlock) {
+    const int64_t lin = sl * kSlice + lane;
+    const int32_t k0 = koff[sl], L = koff[sl + 1] - k0;
+    AT a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
+    int32_t k = 0;
+    for (; k + 2 <= L; k += 2) {
+      sell_block<VT, XT, AT>(scol, sval, x, int64_t(k0) + k, lane, a0, a1, a2);
+      sell_block<VT, XT, AT>(scol, sval, x, int64_t(k0) + k + 1, lane, b0, b1, b2);
+    }
+    if (k < L) sell_block<VT, XT, AT>(scol, sval, x, int64_t(k0) + k, lane, a0, a1, a2);
+    if (lin < n) {
+      const int64_t r = rows[lin];
+      AT o[3] = {a0 + b0, a1 + b1, a2 + b2};
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int64_t d = 3 * r + c;
+        if (fixed && fixed[d]) o[c] = AT(0);
+        y[d] = XT(o[c]);
+        if (state) dot += double(x[d]) * double(XT(o[c]));
+      }
+    }
+  }
+  if (!state) return;
+  double v1[1] = {dot};
+  block_partials<1>(v1, partials);
 }
 
 // ------------------------------------------------------------- finalize
@@ -1039,6 +1244,19 @@ struct Layout {
   float* Aph;
   float* yh;
   float* vals32;
+  // sliced copy of an NB3 matrix (solver-internal product layout)
+  int64_t n_slices;
+  uint32_t* rows;
+  int32_t* koff;
+  int32_t* slice_len;
+  int32_t* scol;
+  double* sval64;
+  float* sval32;
+  uint64_t* keys_in;
+  uint64_t* keys_out;
+  uint32_t* ids_in;
+  void* tmp;
+  size_t tmp_bytes;
   size_t total;
 };
 
@@ -1047,9 +1265,32 @@ int pcg_grid() {
   return g - g % kXcd;
 }
 
-Layout layout(const cf_matrix* A, int precision, int precond, void* base) {
+int bits_for(int64_t n) {
+  int b = 1;
+  while ((int64_t(1) << b) < n) ++b;
+  return b;
+}
+
+// Upper bound on the number of 64-row block columns of the sliced copy: the
+// slice maxima of the unsorted rows (sorting inside windows only lowers it).
+int sell_bound(const cf_matrix* A, int64_t* lsum) {
+  std::vector<int32_t> ptr(size_t(A->n_rows) + 1);
+  CF_HIP(hipMemcpy(ptr.data(), A->indptr, ptr.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  int64_t total = 0;
+  for (int64_t s0 = 0; s0 < A->n_rows; s0 += kSlice) {
+    int32_t mx = 0;
+    for (int64_t r = s0; r < s0 + kSlice && r < A->n_rows; ++r)
+      mx = std::max(mx, ptr[r + 1] - ptr[r]);
+    total += mx;
+  }
+  *lsum = total;
+  return CF_OK;
+}
+
+Layout layout(const cf_matrix* A, int precision, int precond, void* base, int64_t lsum) {
   const int64_t n = n_dofs(A);
-  const int64_t nval = A->format == CF_FMT_NB3 ? 9 * A->nnz : A->nnz;
+  const bool nb3 = A->format == CF_FMT_NB3;
+  const int64_t nval = nb3 ? 9 * A->nnz : A->nnz;
   cf::Carve c(base);
   Layout L{};
   L.r = c.take<double>(n);
@@ -1065,10 +1306,61 @@ Layout layout(const cf_matrix* A, int precision, int precond, void* base) {
     L.ph = c.take<float>(n);
     L.Aph = c.take<float>(n);
     L.yh = c.take<float>(n);
-    L.vals32 = c.take<float>(nval);
+    if (!nb3) L.vals32 = c.take<float>(nval);
+  }
+  if (nb3) {
+    const int64_t N = A->n_rows;
+    L.n_slices = (N + kSlice - 1) / kSlice;
+    L.rows = c.take<uint32_t>(N);
+    L.koff = c.take<int32_t>(L.n_slices + 1);
+    L.slice_len = c.take<int32_t>(L.n_slices + 1);
+    L.scol = c.take<int32_t>(size_t(kSlice) * lsum);
+    if (precision == CF_PREC_MIXED)
+      L.sval32 = c.take<float>(size_t(9 * kSlice) * lsum);
+    else
+      L.sval64 = c.take<double>(size_t(9 * kSlice) * lsum);
+    L.keys_in = c.take<uint64_t>(N);
+    L.keys_out = c.take<uint64_t>(N);
+    L.ids_in = c.take<uint32_t>(N);
+    size_t sort_bytes = 0, scan_bytes = 0;
+    const int end_bit = bits_for((N / kSigma + 1)) + 32;
+    rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint64_t*)nullptr, (uint64_t*)nullptr,
+                              (uint32_t*)nullptr, (uint32_t*)nullptr, size_t(N), 0, end_bit);
+    rocprim::exclusive_scan(nullptr, scan_bytes, (int32_t*)nullptr, (int32_t*)nullptr, 0,
+                            size_t(L.n_slices + 1), rocprim::plus<int32_t>());
+    L.tmp_bytes = std::max(sort_bytes, scan_bytes);
+    L.tmp = c.take<char>(L.tmp_bytes);
   }
   L.total = c.used;
   return L;
+}
+
+// Build the sliced copy (FP64, or FP32 of S A S when `scale` is given).
+int build_sell(const cf_matrix* A, const Layout& L, const double* scale, hipStream_t s) {
+  const int64_t N = A->n_rows;
+  const int grid = cf::stride_grid(N, kBlock);
+  sell_keys<<<grid, kBlock, 0, s>>>(N, A->indptr, L.keys_in, L.ids_in);
+  CF_LAUNCHED();
+  size_t tb = L.tmp_bytes;
+  const int end_bit = bits_for((N / kSigma + 1)) + 32;
+  CF_HIP(rocprim::radix_sort_pairs(L.tmp, tb, L.keys_in, L.keys_out, L.ids_in, L.rows, size_t(N),
+                                   0, end_bit, s));
+  CF_HIP(hipMemsetAsync(L.slice_len + L.n_slices, 0, sizeof(int32_t), s));
+  sell_slice_len<<<cf::stride_grid(L.n_slices, kBlock), kBlock, 0, s>>>(
+      N, L.n_slices, A->indptr, L.rows, L.slice_len, nullptr);
+  CF_LAUNCHED();
+  tb = L.tmp_bytes;
+  CF_HIP(rocprim::exclusive_scan(L.tmp, tb, L.slice_len, L.koff, 0, size_t(L.n_slices + 1),
+                                 rocprim::plus<int32_t>(), s));
+  const int fgrid = cf::stride_grid(L.n_slices * kSlice, kBlock);
+  if (scale)
+    sell_fill<float><<<fgrid, kBlock, 0, s>>>(N, L.n_slices, A->indptr, A->indices, A->values,
+                                              L.rows, L.koff, scale, L.scol, L.sval32);
+  else
+    sell_fill<double><<<fgrid, kBlock, 0, s>>>(N, L.n_slices, A->indptr, A->indices, A->values,
+                                               L.rows, L.koff, nullptr, L.scol, L.sval64);
+  CF_LAUNCHED();
+  return CF_OK;
 }
 
 int check_matrix(const cf_matrix* A) {
@@ -1109,7 +1401,9 @@ extern "C" int cf_pcg_workspace_bytes(const cf_matrix* A, int32_t precision, int
   int rc = check_matrix(A);
   if (rc) return rc;
   CF_REQUIRE(bytes, "cf_pcg_workspace_bytes: null output");
-  *bytes = layout(A, precision, precond, nullptr).total;
+  int64_t lsum = 0;
+  if (A->format == CF_FMT_NB3 && (rc = sell_bound(A, &lsum))) return rc;
+  *bytes = layout(A, precision, precond, nullptr, lsum).total;
   return CF_OK;
 }
 
@@ -1223,7 +1517,15 @@ int pcg_fp64(const cf_matrix* A, const double* b, double* x, const uint8_t* fixe
   const int64_t n = n_dofs(A);
   const int64_t units = PC == CF_PC_BLOCK3 ? A->n_rows : n;
   const int G = pcg_grid();
+  int rc;
+  if (A->format == CF_FMT_NB3 && (rc = build_sell(A, L, nullptr, s))) return rc;
   auto apply = [&](const double* xin, double* y, PcgState* state, double* partials) {
+    if (A->format == CF_FMT_NB3) {
+      spmv_sell<double, double, double><<<G, kBlock, 0, s>>>(
+          A->n_rows, L.n_slices, L.rows, L.koff, L.scol, L.sval64, xin, y, fixed, state, partials);
+      CF_LAUNCHED();
+      return CF_OK;
+    }
     return launch_spmv<double, double, double>(A, A->values, xin, y, fixed, state, partials, G,
                                                s);
   };
@@ -1246,8 +1548,7 @@ int pcg_mixed(const cf_matrix* A, const double* b, double* x, const uint8_t* fix
   make_scale<<<G, kBlock, 0, s>>>(n, L.dinv, L.s);
   CF_LAUNCHED();
   if (A->format == CF_FMT_NB3) {
-    scale_nb3_to_f32<<<G, kBlock, 0, s>>>(A->n_rows, A->indptr, A->indices, A->values, L.s,
-                                          L.vals32);
+    if ((rc = build_sell(A, L, L.s, s))) return rc;
   } else {
     scale_csr_to_f32<<<G, kBlock, 0, s>>>(A->n_rows, A->indptr, A->indices, A->values, L.s,
                                           L.vals32);
@@ -1316,9 +1617,10 @@ int pcg_mixed(const cf_matrix* A, const double* b, double* x, const uint8_t* fix
         int tok;
         prof_begin(1, h.iter + t, s, &tok);
         if (A->format == CF_FMT_NB3) {
-          spmv_nb3_planar<kLanesPlanar><<<G, kBlock, 0, s>>>(A->n_rows, A->indptr, A->indices,
-                                                             L.vals32, L.ph, L.Aph, L.state,
-                                                             L.partials);
+          spmv_sell<float, float, float><<<G, kBlock, 0, s>>>(A->n_rows, L.n_slices, L.rows,
+                                                               L.koff, L.scol, L.sval32, L.ph,
+                                                               L.Aph, nullptr, L.state,
+                                                               L.partials);
           CF_LAUNCHED();
         } else if ((rc = launch_spmv<float, float, float>(&A32, L.vals32, L.ph, L.Aph, nullptr,
                                                           L.state, L.partials, G, s))) {
@@ -1416,7 +1718,9 @@ extern "C" int cf_pcg_solve(const cf_matrix* A, const double* b, double* x,
   CF_REQUIRE(!(precond == CF_PC_BLOCK3 && precision == CF_PREC_MIXED),
              "cf_pcg_solve: mixed precision uses Jacobi scaling");
   CF_REQUIRE(tol > 0.0 && maxit > 0, "cf_pcg_solve: tol and maxit must be positive");
-  Layout L = layout(A, precision, precond, ws);
+  int64_t lsum = 0;
+  if (A->format == CF_FMT_NB3 && (rc = sell_bound(A, &lsum))) return rc;
+  Layout L = layout(A, precision, precond, ws, lsum);
   if (ws_bytes < L.total) {
     cf::set_error("cf_pcg_solve: workspace %zu < %zu bytes", ws_bytes, L.total);
     return CF_ERR_WORKSPACE;
@@ -1475,6 +1779,7 @@ struct MfArgs {
   double lam, mu;
   const int32_t* offs;
   const uint32_t* perm;
+  const uint32_t* slot;  // optional: inverse of perm (node-sorted force slots)
 };
 
 int mf_check(const MfArgs& a) {
@@ -1503,10 +1808,14 @@ size_t mf_layout(int64_t n_nodes, int64_t n_elem, int precond, void* base, Layou
 int mf_apply(const MfArgs& a, const double* xin, double* y, const uint8_t* fixed, double* fe,
              PcgState* state, double* partials, hipStream_t s) {
   const int G = pcg_grid();
-  mf_t10_elem<<<cf::stride_grid(a.n_elem, kBlock), kBlock, 0, s>>>(a.n_elem, a.conn, a.geo,
-                                                                    a.lam, a.mu, xin, fe, state);
+  mf_t10_elem<<<cf::stride_grid(a.n_elem, kBlock), kBlock, 0, s>>>(
+      a.n_elem, a.conn, a.geo, a.lam, a.mu, xin, a.slot, fe, state);
   CF_LAUNCHED();
-  mf_gather<<<G, kBlock, 0, s>>>(a.n_nodes, a.offs, a.perm, fe, fixed, xin, y, state, partials);
+  if (a.slot)
+    mf_gather_sorted<<<G, kBlock, 0, s>>>(a.n_nodes, a.offs, fe, fixed, xin, y, state, partials);
+  else
+    mf_gather<<<G, kBlock, 0, s>>>(a.n_nodes, a.offs, a.perm, fe, fixed, xin, y, state,
+                                   partials);
   CF_LAUNCHED();
   return CF_OK;
 }
@@ -1524,9 +1833,10 @@ extern "C" int cf_mf_t10_workspace_bytes(int64_t n_nodes, int64_t n_elem, int32_
 
 extern "C" int cf_mf_t10_apply(int64_t n_nodes, int64_t n_elem, const int32_t* conn,
                                const double* geo, double lam, double mu, const int32_t* offs,
-                               const uint32_t* perm, const double* x, double* y,
-                               const uint8_t* fixed, void* ws, size_t ws_bytes, void* stream) {
-  MfArgs a{n_nodes, n_elem, conn, geo, lam, mu, offs, perm};
+                               const uint32_t* perm, const uint32_t* slot, const double* x,
+                               double* y, const uint8_t* fixed, void* ws, size_t ws_bytes,
+                               void* stream) {
+  MfArgs a{n_nodes, n_elem, conn, geo, lam, mu, offs, perm, slot};
   int rc = mf_check(a);
   if (rc) return rc;
   CF_REQUIRE(x && y && ws, "cf_mf_t10_apply: null argument");
@@ -1542,11 +1852,12 @@ extern "C" int cf_mf_t10_apply(int64_t n_nodes, int64_t n_elem, const int32_t* c
 
 extern "C" int cf_pcg_solve_mf_t10(int64_t n_nodes, int64_t n_elem, const int32_t* conn,
                                    const double* geo, double lam, double mu,
-                                   const int32_t* offs, const uint32_t* perm, const double* b,
-                                   double* x, const uint8_t* fixed, int32_t precond, double tol,
+                                   const int32_t* offs, const uint32_t* perm,
+                                   const uint32_t* slot, const double* b, double* x,
+                                   const uint8_t* fixed, int32_t precond, double tol,
                                    int32_t maxit, void* ws, size_t ws_bytes,
                                    cf_pcg_stats* stats_host, void* stream) {
-  MfArgs a{n_nodes, n_elem, conn, geo, lam, mu, offs, perm};
+  MfArgs a{n_nodes, n_elem, conn, geo, lam, mu, offs, perm, slot};
   int rc = mf_check(a);
   if (rc) return rc;
   CF_REQUIRE(b && x && ws && stats_host, "cf_pcg_solve_mf_t10: null argument");
@@ -1587,4 +1898,14 @@ extern "C" int cf_pcg_solve_mf_t10(int64_t n_nodes, int64_t n_elem, const int32_
     return pcg_fp64_core<CF_PC_BLOCK3>(apply, n, n_nodes, b, x, fixed, tol, maxit, L, stats_host,
                                        s);
   return pcg_fp64_core<CF_PC_JACOBI>(apply, n, n, b, x, fixed, tol, maxit, L, stats_host, s);
+}
+
+extern "C" int cf_invert_permutation(int64_t n, const uint32_t* perm, uint32_t* inv,
+                                     void* stream) {
+  CF_REQUIRE(n >= 0 && (n == 0 || (perm && inv)), "cf_invert_permutation: bad arguments");
+  if (n == 0) return CF_OK;
+  invert_permutation<<<cf::stride_grid(n, kBlock), kBlock, 0, cf::as_stream(stream)>>>(n, perm,
+                                                                                         inv);
+  CF_LAUNCHED();
+  return CF_OK;
 }

@@ -159,7 +159,7 @@ class DeviceMesh:
         return self._incidence
 
     def matfree_arrays(self):
-        """(conn, geo, offs, perm) in node-local element order for the
+        """(conn, geo, offs, perm, slot) in node-local element order for the
         matrix-free operator: elements stably sorted by their first corner
         (one radix sort), so the elements touching a run of consecutive nodes
         are themselves near each other and the force gather hits in L2.  The
@@ -174,7 +174,10 @@ class DeviceMesh:
             conn = self.conn.index_select(0, order).contiguous()
             geo = self.geometry().index_select(0, order).contiguous()
             offs, perm = build_incidence(conn.reshape(-1), self.n_nodes, self.dev)
-            self._mf = (conn, geo, offs, perm)
+            slot = torch.empty_like(perm)
+            N.check(N.load().cf_invert_permutation(int(conn.numel()), ptr(perm), ptr(slot),
+                                                   stream_handle()), "cf_invert_permutation")
+            self._mf = (conn, geo, offs, perm, slot)
         return self._mf
 
     # ----------------------------------------------------------- assembly

@@ -119,14 +119,15 @@ def pcg_matfree_t10(dm, lam, mu, b, x, fixed=None, precond=None, tol=None, maxit
     maxit = int(maxit if maxit is not None else o["maxit"])
     pc = {"jacobi": N.PC_JACOBI, "block3": N.PC_BLOCK3}[precond]
     lib = N.load()
-    conn, geo, offs, perm = dm.matfree_arrays()
+    conn, geo, offs, perm, slot = dm.matfree_arrays()
     nb = ctypes.c_size_t()
     N.check(lib.cf_mf_t10_workspace_bytes(dm.n_nodes, dm.n_elem, pc, ctypes.byref(nb)),
             "cf_mf_t10_workspace_bytes")
     ws = workspace(nb.value, b.device)
     st = N.CfPcgStats()
     rc = lib.cf_pcg_solve_mf_t10(dm.n_nodes, dm.n_elem, ptr(conn), ptr(geo),
-                                 float(lam), float(mu), ptr(offs), ptr(perm), ptr(b), ptr(x),
+                                 float(lam), float(mu), ptr(offs), ptr(perm), ptr(slot),
+                                 ptr(b), ptr(x),
                                  ptr(fixed), pc, tol, maxit, ptr(ws), nb.value,
                                  ctypes.byref(st), stream_handle())
     stats = {"iterations": st.iterations, "residual_updates": 0,
@@ -140,14 +141,15 @@ def pcg_matfree_t10(dm, lam, mu, b, x, fixed=None, precond=None, tol=None, maxit
 def matfree_t10_apply(dm, lam, mu, x, fixed=None):
     """y = K x with the matrix-free Tet10 elasticity operator."""
     lib = N.load()
-    conn, geo, offs, perm = dm.matfree_arrays()
+    conn, geo, offs, perm, slot = dm.matfree_arrays()
     nb = ctypes.c_size_t()
     N.check(lib.cf_mf_t10_workspace_bytes(dm.n_nodes, dm.n_elem, N.PC_JACOBI, ctypes.byref(nb)),
             "cf_mf_t10_workspace_bytes")
     ws = workspace(nb.value, x.device)
     y = torch.empty_like(x)
     N.check(lib.cf_mf_t10_apply(dm.n_nodes, dm.n_elem, ptr(conn), ptr(geo),
-                                float(lam), float(mu), ptr(offs), ptr(perm), ptr(x), ptr(y),
+                                float(lam), float(mu), ptr(offs), ptr(perm), ptr(slot),
+                                ptr(x), ptr(y),
                                 ptr(fixed), ptr(ws), nb.value, stream_handle()),
             "cf_mf_t10_apply")
     return y

@@ -283,21 +283,30 @@ int cf_pcg_solve(const cf_matrix* A, const double* b, double* x,
  *      cf_incidence_build of the Tet10 connectivity) in fixed order.  Same
  *      operator as CF_OP_T10_ELAST (isotropic lam, mu; affine elements), so
  *      it replaces the stiffness in the solve of elastic_tet10.py:214-229
- *      without the 12 B/value matrix traffic.  Workspace: 30 doubles per
- *      element + 9 per node + the PCG vectors.                                */
+ *      without the 12 B/value matrix traffic.  slot (nullable) = the inverse
+ *      of perm (cf_invert_permutation): with it every element writes its
+ *      nodal forces straight into node-sorted order and the node sums read
+ *      contiguous memory.  Workspace: 30 doubles per element + 9 per node +
+ *      the PCG vectors.                                                       */
 int cf_mf_t10_workspace_bytes(int64_t n_nodes, int64_t n_elem, int32_t precond,
                               size_t* bytes);
 int cf_mf_t10_apply(int64_t n_nodes, int64_t n_elem, const int32_t* conn,
                     const double* geo, double lam, double mu,
-                    const int32_t* offs, const uint32_t* perm, const double* x,
-                    double* y, const uint8_t* fixed, void* ws, size_t ws_bytes,
+                    const int32_t* offs, const uint32_t* perm,
+                    const uint32_t* slot, const double* x, double* y,
+                    const uint8_t* fixed, void* ws, size_t ws_bytes,
                     void* stream);
 int cf_pcg_solve_mf_t10(int64_t n_nodes, int64_t n_elem, const int32_t* conn,
                         const double* geo, double lam, double mu,
                         const int32_t* offs, const uint32_t* perm,
-                        const double* b, double* x, const uint8_t* fixed,
+                        const uint32_t* slot, const double* b, double* x,
+                        const uint8_t* fixed,
                         int32_t precond, double tol, int32_t maxit, void* ws,
                         size_t ws_bytes, cf_pcg_stats* stats_host, void* stream);
+
+/* ---- inv[perm[j]] = j for a permutation of 0..n-1.                     */
+int cf_invert_permutation(int64_t n, const uint32_t* perm, uint32_t* inv,
+                          void* stream);
 
 /* ---- timing of the CG product kernel (no reference counterpart; used by
  *      the benchmark to report achieved bandwidth).  Returns the average
