@@ -177,7 +177,7 @@ def main():
     bytes_spmv64 = 76 * nnz_b + 4 * (n_nodes + 1) + 16 * n_dof
     results["fp64"] = {"ms_per_step": ms, "iterations": st["iterations"],
                        "rel_residual": st["rel_residual"], "precond": st["precond"],
-                       "op": "spmv_nb3<double,double,double,32>", "op_avg_ms": prof[0],
+                       "op": "spmv_sell<double,double,double>", "op_avg_ms": prof[0],
                        "op_launches_sampled": prof[1], "op_bytes": bytes_spmv64}
     maxit["mixed"] = 4 * st["iterations"]   # bound the mixed attempt
     ms, (x_mf, st, _), prof = timed("matfree", args.precond64)
@@ -195,7 +195,7 @@ def main():
         results["mixed"] = {"ms_per_step": ms, "iterations": st["iterations"],
                             "residual_updates": st["residual_updates"],
                             "rel_residual": st["rel_residual"],
-                            "op": "spmv_nb3_planar<16> (FP32)", "op_avg_ms": prof[2],
+                            "op": "spmv_sell<float,float,float>", "op_avg_ms": prof[2],
                             "op_launches_sampled": prof[3],
                             "op_bytes": 40 * nnz_b + 4 * (n_nodes + 1) + 8 * n_dof,
                             "rel_diff_vs_fp64": float(torch.linalg.vector_norm(x_mx - x_64)

@@ -125,6 +125,14 @@ __device__ __forceinline__ void block_rows(int64_t n_rows, int64_t& r0, int64_t&
   r1 = r0 + chunk < n_rows ? r0 + chunk : n_rows;
 }
 
+// Tiled launches: grid = tile_grid(tiles) blocks (a multiple of kXcd), block b
+// takes tile tile_index(b), so consecutive tiles share an XCD's L2 and every
+// block gets at most one tile (no tail round on a few blocks).
+__device__ __forceinline__ int64_t tile_index(int64_t b) {
+  const int64_t G = gridDim.x;
+  return (b % kXcd) * (G / kXcd) + b / kXcd;
+}
+
 // ------------------------------------------------------------- SpMV
 
 template <class T>
@@ -510,76 +518,6 @@ __global__ void make_scale(int64_t n, const double* __restrict__ dinv, double* _
     s[i] = dinv[i] > 0.0 ? sqrt(dinv[i]) : 0.0;
 }
 
-// FP32 values of S A S in "row-planar" NB3 order: node row i keeps its 9*len
-// values at the same offset 9*ptr[i], as 9 planes of len entries, plane
-// q = 3c + c' holding block(i, j_k)[c][c'] for k = 0..len-1.  A lane that
-// owns block k then reads each plane at consecutive addresses.
-__global__ void scale_nb3_to_f32(int64_t n_nodes, const int32_t* __restrict__ ptr,
-                                 const int32_t* __restrict__ idx,
-                                 const double* __restrict__ val, const double* __restrict__ s,
-                                 float* __restrict__ out) {
-  // one node row per 64-lane wave
-  const int lane = threadIdx.x & 63;
-  const int64_t wave = (blockIdx.x * int64_t(blockDim.x) + threadIdx.x) >> 6;
-  const int64_t nwaves = (int64_t(gridDim.x) * blockDim.x) >> 6;
-  for (int64_t i = wave; i < n_nodes; i += nwaves) {
-    const int32_t base = ptr[i], len = ptr[i + 1] - base, l3 = 3 * len;
-    for (int32_t f = lane; f < 9 * len; f += 64) {
-      const int c = (f >= l3) + (f >= 2 * l3);
-      const int32_t q = f - c * l3, kq = q / 3;
-      const int cp = q - 3 * kq;
-      const int64_t t = int64_t(9) * base + f;
-      out[int64_t(9) * base + (3 * c + cp) * len + kq] =
-          float(s[3 * i + c] * val[t] * s[3 * int64_t(idx[base + kq]) + cp]);
-    }
-  }
-}
-
-// CG product on the row-planar FP32 copy (no mask: S zeroes fixed rows and
-// columns), lane per 3x3 block, L lanes per node row.
-template <int L>
-__global__ void __launch_bounds__(kBlock)
-spmv_nb3_planar(int64_t n_nodes, const int32_t* __restrict__ ptr,
-                const int32_t* __restrict__ idx, const float* __restrict__ val,
-                const float* __restrict__ x, float* __restrict__ y, PcgState* state,
-                double* __restrict__ partials) {
-  if (state && state->halt) return;
-  const int lane = threadIdx.x % L;
-  const int group = threadIdx.x / L;
-  constexpr int kGroups = kBlock / L;
-  int64_t r0, r1;
-  block_rows(n_nodes, r0, r1);
-  double dot = 0.0;
-  for (int64_t i = r0 + group; i < r1; i += kGroups) {
-    const int32_t base = ptr[i];
-    const int32_t len = ptr[i + 1] - base;
-    const float* v = val + int64_t(9) * base;
-    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-    for (int32_t k = lane; k < len; k += L) {
-      const int64_t c3 = 3 * int64_t(idx[base + k]);
-      const float x0 = x[c3], x1 = x[c3 + 1], x2 = x[c3 + 2];
-      const float* b = v + k;
-      a0 += ld(b) * x0 + ld(b + len) * x1 + ld(b + 2 * len) * x2;
-      a1 += ld(b + 3 * len) * x0 + ld(b + 4 * len) * x1 + ld(b + 5 * len) * x2;
-      a2 += ld(b + 6 * len) * x0 + ld(b + 7 * len) * x1 + ld(b + 8 * len) * x2;
-    }
-    a0 = group_sum<L>(a0);
-    a1 = group_sum<L>(a1);
-    a2 = group_sum<L>(a2);
-    if (lane == 0) {
-      y[3 * i] = a0;
-      y[3 * i + 1] = a1;
-      y[3 * i + 2] = a2;
-      if (state)
-        dot += double(x[3 * i]) * double(a0) + double(x[3 * i + 1]) * double(a1) +
-               double(x[3 * i + 2]) * double(a2);
-    }
-  }
-  if (!state) return;
-  double v1[1] = {dot};
-  block_partials<1>(v1, partials);
-}
-
 __global__ void scale_csr_to_f32(int64_t n, const int32_t* __restrict__ ptr,
                                  const int32_t* __restrict__ idx,
                                  const double* __restrict__ val, const double* __restrict__ s,
@@ -752,10 +690,9 @@ mf_gather_sorted(int64_t n_nodes, const int32_t* __restrict__ offs,
                  const double* __restrict__ x, double* __restrict__ y, PcgState* state,
                  double* __restrict__ partials) {
   if (state && state->halt) return;
-  int64_t r0, r1;
-  block_rows(n_nodes, r0, r1);
   double dot = 0.0;
-  for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+  const int64_t i = tile_index(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n_nodes) {
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     const double* v = fe + int64_t(offs[i]) * 3;
     const double* end = fe + int64_t(offs[i + 1]) * 3;
@@ -933,19 +870,25 @@ __global__ void blocks_to_precond(int64_t n_nodes, const double* __restrict__ bl
 // ------------------------------------------------------------- sliced NB3
 
 // Solver-internal copy of an NB3 matrix in a sliced layout (SELL-C-sigma over
-// node rows, C = 64, sigma = 2048): node rows are stably sorted by
-// decreasing length inside windows of 2048 rows, cut into slices of 64 rows
-// (one per lane of a wave), and each slice stores its k-th blocks
-// column-major: cols[(koff[s] + k) * 64 + lane], values
-// [((koff[s] + k) * 9 + q) * 64 + lane] for q = 3c + c'.  Every load of the
-// product is then a coalesced 64-lane access and each lane owns its three
-// outputs, so no cross-lane reduction is needed; padding blocks (rows
-// shorter than the slice's longest, 0.8 % on Tet10 box meshes) hold zeros.
-constexpr int kSlice = 64;
-constexpr int kSigma = kSlice * 32;
+// node rows with C = 16 rows per slice and sigma = 2048): node rows are stably
+// sorted by decreasing length inside windows of 2048 rows and cut into slices
+// of 16.  A wave owns one slice; its 64 lanes are 4 phases x 16 rows, phase h
+// taking blocks k = 4g + h of its row, so a slice of longest row L costs
+// ceil(L / 4) steps g.  Step g of slice s stores, for lane = 16h + row:
+// the block column at cols[(koff[s] + g) * 64 + lane] and the 9 block values
+// at vals[((koff[s] + g) * 9 + q) * 64 + lane], q = 3c + c'.  Every load of
+// the product is a coalesced 64-lane access; the 4 phases of a row are summed
+// with two lane swaps at the end; padding (rows shorter than the slice's
+// longest, about 5 % on Tet10 box meshes) holds zero values and column 0.
+// Four phases give 4x more waves than one-row-per-lane slices, which keeps
+// every SIMD busy on meshes of a few 10^5 nodes.
+constexpr int kSliceRows = 16;
+constexpr int kPhases = 64 / kSliceRows;
+constexpr int kSigma = 2048;
+constexpr int kSellUnroll = 4;
 
 // Sort key: window index above, descending row length below (exact, so the
-// sorted slice maxima never exceed the unsorted ones the workspace is sized by).
+// sorted slice lengths never exceed the unsorted ones the workspace is sized by).
 __global__ void sell_keys(int64_t n, const int32_t* __restrict__ ptr, uint64_t* __restrict__ keys,
                           uint32_t* __restrict__ ids) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
@@ -956,42 +899,44 @@ __global__ void sell_keys(int64_t n, const int32_t* __restrict__ ptr, uint64_t* 
   }
 }
 
-// L[s] = longest row of slice s (rows[] = sorted order, or identity if null)
+// steps[s] = ceil(longest row of slice s / kPhases), rows[] in sorted order
 __global__ void sell_slice_len(int64_t n, int64_t n_slices, const int32_t* __restrict__ ptr,
-                               const uint32_t* __restrict__ rows, int32_t* __restrict__ L,
-                               unsigned long long* __restrict__ total) {
+                               const uint32_t* __restrict__ rows, int32_t* __restrict__ steps) {
   for (int64_t sl = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; sl < n_slices;
        sl += int64_t(gridDim.x) * blockDim.x) {
     int32_t mx = 0;
-    for (int t = 0; t < kSlice; ++t) {
-      const int64_t lin = sl * kSlice + t;
+    for (int t = 0; t < kSliceRows; ++t) {
+      const int64_t lin = sl * kSliceRows + t;
       if (lin >= n) break;
-      const int64_t r = rows ? int64_t(rows[lin]) : lin;
+      const int64_t r = rows[lin];
       const int32_t len = ptr[r + 1] - ptr[r];
       mx = len > mx ? len : mx;
     }
-    if (L) L[sl] = mx;
-    if (total) atomicAdd(total, (unsigned long long)mx);
+    steps[sl] = (mx + kPhases - 1) / kPhases;
   }
 }
 
+// One thread per (slice, lane): fills the lane's column of every step.
 template <class OT>
 __global__ void sell_fill(int64_t n, int64_t n_slices, const int32_t* __restrict__ ptr,
                           const int32_t* __restrict__ idx, const double* __restrict__ val,
                           const uint32_t* __restrict__ rows, const int32_t* __restrict__ koff,
                           const double* __restrict__ scale, int32_t* __restrict__ scol,
                           OT* __restrict__ sval) {
-  for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n_slices * kSlice;
+  for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n_slices * 64;
        t += int64_t(gridDim.x) * blockDim.x) {
-    const int64_t sl = t / kSlice;
-    const int lane = int(t - sl * kSlice);
-    const int64_t r = t < n ? int64_t(rows[t]) : -1;
-    const int32_t k0 = koff[sl], L = koff[sl + 1] - k0;
+    const int64_t sl = t / 64;
+    const int lane = int(t - sl * 64);
+    const int h = lane / kSliceRows;
+    const int64_t lin = sl * kSliceRows + (lane % kSliceRows);
+    const int64_t r = lin < n ? int64_t(rows[lin]) : -1;
+    const int32_t g0 = koff[sl], steps = koff[sl + 1] - g0;
     const int32_t base = r >= 0 ? ptr[r] : 0;
     const int32_t len = r >= 0 ? ptr[r + 1] - base : 0;
-    for (int32_t k = 0; k < L; ++k) {
-      const int64_t g = int64_t(k0) + k;
-      int32_t col = r >= 0 ? int32_t(r) : 0;
+    for (int32_t g = 0; g < steps; ++g) {
+      const int32_t k = g * kPhases + h;
+      const int64_t at = int64_t(g0) + g;
+      int32_t col = 0;
       double v[9] = {};
       if (k < len) {
         col = idx[base + k];
@@ -1004,26 +949,17 @@ __global__ void sell_fill(int64_t n, int64_t n_slices, const int32_t* __restrict
             v[3 * c + cp] = a;
           }
       }
-      scol[g * kSlice + lane] = col;
+      scol[at * 64 + lane] = col;
 #pragma unroll
-      for (int q = 0; q < 9; ++q) sval[(g * 9 + q) * kSlice + lane] = OT(v[q]);
+      for (int q = 0; q < 9; ++q) sval[(at * 9 + q) * 64 + lane] = OT(v[q]);
     }
   }
 }
 
-template <class VT, class XT, class AT>
-__device__ __forceinline__ void sell_block(const int32_t* __restrict__ scol,
-                                           const VT* __restrict__ sval, const XT* __restrict__ x,
-                                           int64_t g, int lane, AT& a0, AT& a1, AT& a2) {
-  const int64_t c3 = 3 * int64_t(scol[g * kSlice + lane]);
-  const AT x0 = AT(x[c3]), x1 = AT(x[c3 + 1]), x2 = AT(x[c3 + 2]);
-  const VT* v = sval + g * 9 * kSlice + lane;
-  a0 += AT(ld(v)) * x0 + AT(ld(v + kSlice)) * x1 + AT(ld(v + 2 * kSlice)) * x2;
-  a1 += AT(ld(v + 3 * kSlice)) * x0 + AT(ld(v + 4 * kSlice)) * x1 + AT(ld(v + 5 * kSlice)) * x2;
-  a2 += AT(ld(v + 6 * kSlice)) * x0 + AT(ld(v + 7 * kSlice)) * x1 + AT(ld(v + 8 * kSlice)) * x2;
-}
-
-// y = A x on the sliced copy; one slice per wave, one node row per lane.
+// y = A x on the sliced copy: one slice per wave, kSellUnroll steps per
+// iteration with the next steps' column indices loaded one iteration ahead,
+// so the x gathers of a step never wait on its own index load.  Grid:
+// sell_grid(n_slices) blocks of 4 waves, block b -> slices of tile(b).
 template <class VT, class XT, class AT>
 __global__ void __launch_bounds__(kBlock)
 spmv_sell(int64_t n, int64_t n_slices, const uint32_t* __restrict__ rows,
@@ -1031,31 +967,60 @@ spmv_sell(int64_t n, int64_t n_slices, const uint32_t* __restrict__ rows,
           const VT* __restrict__ sval, const XT* __restrict__ x, XT* __restrict__ y,
           const uint8_t* __restrict__ fixed, PcgState* state, double* __restrict__ partials) {
   if (state && state->halt) return;
-  const int lane = threadIdx.x & (kSlice - 1);
-  const int wave = threadIdx.x / kSlice;
-  constexpr int kWavesPerBlock = kBlock / kSlice;
-  int64_t s0, s1;
-  block_rows(n_slices, s0, s1);
+  constexpr int U = kSellUnroll;
+  const int lane = threadIdx.x & 63;
+  const int h = lane / kSliceRows;
+  const int64_t sl = tile_index(blockIdx.x) * (kBlock / 64) + threadIdx.x / 64;
   double dot = 0.0;
-  for (int64_t sl = s0 + wave; sl < s1; sl += kWavesPerBlock) {
-    const int64_t lin = sl * kSlice + lane;
-    const int32_t k0 = koff[sl], L = koff[sl + 1] - k0;
-    AT a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
-    int32_t k = 0;
-    for (; k + 2 <= L; k += 2) {
-      sell_block<VT, XT, AT>(scol, sval, x, int64_t(k0) + k, lane, a0, a1, a2);
-      sell_block<VT, XT, AT>(scol, sval, x, int64_t(k0) + k + 1, lane, b0, b1, b2);
-    }
-    if (k < L) sell_block<VT, XT, AT>(scol, sval, x, int64_t(k0) + k, lane, a0, a1, a2);
-    if (lin < n) {
-      const int64_t r = rows[lin];
-      AT o[3] = {a0 + b0, a1 + b1, a2 + b2};
+  if (sl < n_slices) {
+    const int64_t lin = sl * kSliceRows + (lane % kSliceRows);
+    const int32_t g0 = koff[sl], g1 = koff[sl + 1];
+    AT a0 = 0, a1 = 0, a2 = 0;
+    int32_t c[U];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const int64_t d = 3 * r + c;
-        if (fixed && fixed[d]) o[c] = AT(0);
-        y[d] = XT(o[c]);
-        if (state) dot += double(x[d]) * double(XT(o[c]));
+    for (int u = 0; u < U; ++u) c[u] = g0 + u < g1 ? scol[int64_t(g0 + u) * 64 + lane] : 0;
+    for (int32_t g = g0; g < g1; g += U) {
+      AT xv[U][3], v[U][9];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t c3 = 3 * int64_t(c[u]);
+        const bool ok = g + u < g1;
+        xv[u][0] = ok ? AT(x[c3]) : AT(0);
+        xv[u][1] = ok ? AT(x[c3 + 1]) : AT(0);
+        xv[u][2] = ok ? AT(x[c3 + 2]) : AT(0);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const VT* p = sval + int64_t(g + u) * 9 * 64 + lane;
+        const bool ok = g + u < g1;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) v[u][q] = ok ? AT(ld(p + q * 64)) : AT(0);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        c[u] = g + U + u < g1 ? scol[int64_t(g + U + u) * 64 + lane] : 0;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        a0 += v[u][0] * xv[u][0] + v[u][1] * xv[u][1] + v[u][2] * xv[u][2];
+        a1 += v[u][3] * xv[u][0] + v[u][4] * xv[u][1] + v[u][5] * xv[u][2];
+        a2 += v[u][6] * xv[u][0] + v[u][7] * xv[u][1] + v[u][8] * xv[u][2];
+      }
+    }
+#pragma unroll
+    for (int m = kSliceRows; m < 64; m <<= 1) {
+      a0 += __shfl_xor(a0, m, 64);
+      a1 += __shfl_xor(a1, m, 64);
+      a2 += __shfl_xor(a2, m, 64);
+    }
+    if (h == 0 && lin < n) {
+      const int64_t r = rows[lin];
+      AT o[3] = {a0, a1, a2};
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) {
+        const int64_t d = 3 * r + cc;
+        if (fixed && fixed[d]) o[cc] = AT(0);
+        y[d] = XT(o[cc]);
+        if (state) dot += double(x[d]) * double(XT(o[cc]));
       }
     }
   }
@@ -1213,7 +1178,6 @@ int n_dofs(const cf_matrix* A) { return A->format == CF_FMT_NB3 ? 3 * A->n_rows 
 // blocks (90..585 values), P1 scalar rows 5..15 entries.
 constexpr int kLanesNb3 = 32;
 constexpr int kLanesCsr = 8;
-constexpr int kLanesPlanar = 16;
 
 template <class VT, class XT, class AT>
 int launch_spmv(const cf_matrix* A, const VT* vals, const XT* x, XT* y, const uint8_t* fixed,
@@ -1265,23 +1229,32 @@ int pcg_grid() {
   return g - g % kXcd;
 }
 
+// blocks for `tiles` tiles (see tile_index): a multiple of kXcd
+int64_t tile_grid(int64_t tiles) {
+  const int64_t g = tiles < 1 ? 1 : tiles;
+  return (g + kXcd - 1) / kXcd * kXcd;
+}
+
+int64_t sell_grid(int64_t n_slices) { return tile_grid((n_slices + 3) / 4); }
+
 int bits_for(int64_t n) {
   int b = 1;
   while ((int64_t(1) << b) < n) ++b;
   return b;
 }
 
-// Upper bound on the number of 64-row block columns of the sliced copy: the
-// slice maxima of the unsorted rows (sorting inside windows only lowers it).
+// Upper bound on the number of steps of the sliced copy: the slice steps of
+// the unsorted rows (sorting inside windows only lowers each slice maximum's
+// rank, so the sorted total never exceeds this).
 int sell_bound(const cf_matrix* A, int64_t* lsum) {
   std::vector<int32_t> ptr(size_t(A->n_rows) + 1);
   CF_HIP(hipMemcpy(ptr.data(), A->indptr, ptr.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
   int64_t total = 0;
-  for (int64_t s0 = 0; s0 < A->n_rows; s0 += kSlice) {
+  for (int64_t s0 = 0; s0 < A->n_rows; s0 += kSliceRows) {
     int32_t mx = 0;
-    for (int64_t r = s0; r < s0 + kSlice && r < A->n_rows; ++r)
+    for (int64_t r = s0; r < s0 + kSliceRows && r < A->n_rows; ++r)
       mx = std::max(mx, ptr[r + 1] - ptr[r]);
-    total += mx;
+    total += (mx + kPhases - 1) / kPhases;
   }
   *lsum = total;
   return CF_OK;
@@ -1297,7 +1270,10 @@ Layout layout(const cf_matrix* A, int precision, int precond, void* base, int64_
   L.p = c.take<double>(n);
   L.Ap = c.take<double>(n);
   L.dinv = c.take<double>(precond == CF_PC_BLOCK3 ? 3 * n : n);
-  L.partials = c.take<double>(4 * size_t(pcg_grid()));
+  const int64_t parts =
+      nb3 ? std::max<int64_t>(pcg_grid(), sell_grid((A->n_rows + kSliceRows - 1) / kSliceRows))
+          : pcg_grid();
+  L.partials = c.take<double>(4 * size_t(parts));
   L.state = c.take<PcgState>(1);
   L.bad = c.take<int32_t>(1);
   if (precision == CF_PREC_MIXED) {
@@ -1310,15 +1286,15 @@ Layout layout(const cf_matrix* A, int precision, int precond, void* base, int64_
   }
   if (nb3) {
     const int64_t N = A->n_rows;
-    L.n_slices = (N + kSlice - 1) / kSlice;
+    L.n_slices = (N + kSliceRows - 1) / kSliceRows;
     L.rows = c.take<uint32_t>(N);
     L.koff = c.take<int32_t>(L.n_slices + 1);
     L.slice_len = c.take<int32_t>(L.n_slices + 1);
-    L.scol = c.take<int32_t>(size_t(kSlice) * lsum);
+    L.scol = c.take<int32_t>(size_t(64) * lsum);
     if (precision == CF_PREC_MIXED)
-      L.sval32 = c.take<float>(size_t(9 * kSlice) * lsum);
+      L.sval32 = c.take<float>(size_t(9 * 64) * lsum);
     else
-      L.sval64 = c.take<double>(size_t(9 * kSlice) * lsum);
+      L.sval64 = c.take<double>(size_t(9 * 64) * lsum);
     L.keys_in = c.take<uint64_t>(N);
     L.keys_out = c.take<uint64_t>(N);
     L.ids_in = c.take<uint32_t>(N);
@@ -1346,13 +1322,13 @@ int build_sell(const cf_matrix* A, const Layout& L, const double* scale, hipStre
   CF_HIP(rocprim::radix_sort_pairs(L.tmp, tb, L.keys_in, L.keys_out, L.ids_in, L.rows, size_t(N),
                                    0, end_bit, s));
   CF_HIP(hipMemsetAsync(L.slice_len + L.n_slices, 0, sizeof(int32_t), s));
-  sell_slice_len<<<cf::stride_grid(L.n_slices, kBlock), kBlock, 0, s>>>(
-      N, L.n_slices, A->indptr, L.rows, L.slice_len, nullptr);
+  sell_slice_len<<<cf::stride_grid(L.n_slices, kBlock), kBlock, 0, s>>>(N, L.n_slices, A->indptr,
+                                                                         L.rows, L.slice_len);
   CF_LAUNCHED();
   tb = L.tmp_bytes;
   CF_HIP(rocprim::exclusive_scan(L.tmp, tb, L.slice_len, L.koff, 0, size_t(L.n_slices + 1),
                                  rocprim::plus<int32_t>(), s));
-  const int fgrid = cf::stride_grid(L.n_slices * kSlice, kBlock);
+  const int fgrid = cf::stride_grid(L.n_slices * 64, kBlock);
   if (scale)
     sell_fill<float><<<fgrid, kBlock, 0, s>>>(N, L.n_slices, A->indptr, A->indices, A->values,
                                               L.rows, L.koff, scale, L.scol, L.sval32);
@@ -1378,8 +1354,11 @@ int read_state(const PcgState* d, PcgState* h, hipStream_t s) {
 }
 
 // Fold the partials of the kernel just launched (grid pcg_grid()).
-int fin(int mode, const Layout& L, hipStream_t s, double* out = nullptr, double aux = 0.0) {
-  finalize<<<1, kBlock, 0, s>>>(mode, pcg_grid(), L.partials, L.state, out, aux);
+// `parts`: number of block partials the preceding kernel wrote (its grid)
+int fin(int mode, const Layout& L, hipStream_t s, double* out = nullptr, double aux = 0.0,
+        int parts = 0) {
+  finalize<<<1, kBlock, 0, s>>>(mode, parts > 0 ? parts : pcg_grid(), L.partials, L.state, out,
+                                aux);
   CF_LAUNCHED();
   return CF_OK;
 }
@@ -1411,12 +1390,12 @@ namespace {
 
 // FP64 PCG on the masked free system.  `apply(xin, y, state, partials)`
 // launches y = A xin with fixed rows zeroed; when `state` is set it also
-// reduces p.Ap and sets alpha (the CG product).  `dinv` holds the
+// writes `apply_parts` block partials of p.Ap (the CG product).  `dinv` holds the
 // preconditioner (Jacobi 1/A_ii, or inverse 3x3 node blocks).
 template <int PC, class Apply>
-int pcg_fp64_core(Apply&& apply, int64_t n, int64_t units, const double* b, double* x,
-                  const uint8_t* fixed, double tol, int maxit, const Layout& L, cf_pcg_stats* st,
-                  hipStream_t s) {
+int pcg_fp64_core(Apply&& apply, int apply_parts, int64_t n, int64_t units, const double* b,
+                  double* x, const uint8_t* fixed, double tol, int maxit, const Layout& L,
+                  cf_pcg_stats* st, hipStream_t s) {
   const int G = pcg_grid();
   int rc;
   PcgState h{};
@@ -1467,7 +1446,7 @@ int pcg_fp64_core(Apply&& apply, int64_t n, int64_t units, const double* b, doub
       prof_begin(0, h.iter + t, s, &tok);
       if ((rc = apply(L.p, L.Ap, L.state, L.partials))) return rc;
       prof_end(tok, s);
-      if ((rc = fin(kFinAlpha, L, s))) return rc;
+      if ((rc = fin(kFinAlpha, L, s, nullptr, 0.0, apply_parts))) return rc;
       cg_update<PC><<<G, kBlock, 0, s>>>(units, x, L.r, L.p, L.Ap, L.dinv, L.state, L.partials);
       CF_LAUNCHED();
       if ((rc = fin(kFinCgUpdate, L, s))) return rc;
@@ -1519,9 +1498,10 @@ int pcg_fp64(const cf_matrix* A, const double* b, double* x, const uint8_t* fixe
   const int G = pcg_grid();
   int rc;
   if (A->format == CF_FMT_NB3 && (rc = build_sell(A, L, nullptr, s))) return rc;
+  const int GS = A->format == CF_FMT_NB3 ? int(sell_grid(L.n_slices)) : G;
   auto apply = [&](const double* xin, double* y, PcgState* state, double* partials) {
     if (A->format == CF_FMT_NB3) {
-      spmv_sell<double, double, double><<<G, kBlock, 0, s>>>(
+      spmv_sell<double, double, double><<<GS, kBlock, 0, s>>>(
           A->n_rows, L.n_slices, L.rows, L.koff, L.scol, L.sval64, xin, y, fixed, state, partials);
       CF_LAUNCHED();
       return CF_OK;
@@ -1529,13 +1509,14 @@ int pcg_fp64(const cf_matrix* A, const double* b, double* x, const uint8_t* fixe
     return launch_spmv<double, double, double>(A, A->values, xin, y, fixed, state, partials, G,
                                                s);
   };
-  return pcg_fp64_core<PC>(apply, n, units, b, x, fixed, tol, maxit, L, st, s);
+  return pcg_fp64_core<PC>(apply, GS, n, units, b, x, fixed, tol, maxit, L, st, s);
 }
 
 int pcg_mixed(const cf_matrix* A, const double* b, double* x, const uint8_t* fixed, double tol,
               int maxit, const Layout& L, cf_pcg_stats* st, hipStream_t s) {
   const int64_t n = n_dofs(A);
   const int G = pcg_grid();
+  const int GS = A->format == CF_FMT_NB3 ? int(sell_grid(L.n_slices)) : G;
   // Replace the residual once the FP32 recursion has dropped it by 10x
   // since the last replacement; restart the direction when the replaced
   // residual is more than 2x the recursive one (the FP32 recursion has
@@ -1617,17 +1598,17 @@ int pcg_mixed(const cf_matrix* A, const double* b, double* x, const uint8_t* fix
         int tok;
         prof_begin(1, h.iter + t, s, &tok);
         if (A->format == CF_FMT_NB3) {
-          spmv_sell<float, float, float><<<G, kBlock, 0, s>>>(A->n_rows, L.n_slices, L.rows,
-                                                               L.koff, L.scol, L.sval32, L.ph,
-                                                               L.Aph, nullptr, L.state,
-                                                               L.partials);
+          spmv_sell<float, float, float><<<GS, kBlock, 0, s>>>(A->n_rows, L.n_slices, L.rows,
+                                                                L.koff, L.scol, L.sval32, L.ph,
+                                                                L.Aph, nullptr, L.state,
+                                                                L.partials);
           CF_LAUNCHED();
         } else if ((rc = launch_spmv<float, float, float>(&A32, L.vals32, L.ph, L.Aph, nullptr,
                                                           L.state, L.partials, G, s))) {
           return rc;
         }
         prof_end(tok, s);
-        if ((rc = fin(kFinAlpha, L, s))) return rc;
+        if ((rc = fin(kFinAlpha, L, s, nullptr, 0.0, GS))) return rc;
         mx_update<<<G, kBlock, 0, s>>>(n, L.yh, L.rh, L.ph, L.Aph, L.state, L.partials);
         CF_LAUNCHED();
         if ((rc = fin(kFinMxUpdate, L, s))) return rc;
@@ -1788,6 +1769,9 @@ int mf_check(const MfArgs& a) {
   return CF_OK;
 }
 
+// one node per thread, kBlock nodes per block
+int64_t mf_gather_grid(int64_t n_nodes) { return tile_grid((n_nodes + kBlock - 1) / kBlock); }
+
 size_t mf_layout(int64_t n_nodes, int64_t n_elem, int precond, void* base, Layout* L,
                  double** fe, double** blocks) {
   cf::Carve c(base);
@@ -1796,7 +1780,7 @@ size_t mf_layout(int64_t n_nodes, int64_t n_elem, int precond, void* base, Layou
   L->p = c.take<double>(n);
   L->Ap = c.take<double>(n);
   L->dinv = c.take<double>(precond == CF_PC_BLOCK3 ? 3 * n : n);
-  L->partials = c.take<double>(4 * size_t(pcg_grid()));
+  L->partials = c.take<double>(4 * size_t(std::max<int64_t>(pcg_grid(), mf_gather_grid(n_nodes))));
   L->state = c.take<PcgState>(1);
   L->bad = c.take<int32_t>(1);
   *fe = c.take<double>(30 * n_elem);
@@ -1812,7 +1796,9 @@ int mf_apply(const MfArgs& a, const double* xin, double* y, const uint8_t* fixed
       a.n_elem, a.conn, a.geo, a.lam, a.mu, xin, a.slot, fe, state);
   CF_LAUNCHED();
   if (a.slot)
-    mf_gather_sorted<<<G, kBlock, 0, s>>>(a.n_nodes, a.offs, fe, fixed, xin, y, state, partials);
+    mf_gather_sorted<<<int(mf_gather_grid(a.n_nodes)), kBlock, 0, s>>>(a.n_nodes, a.offs, fe,
+                                                                      fixed, xin, y, state,
+                                                                      partials);
   else
     mf_gather<<<G, kBlock, 0, s>>>(a.n_nodes, a.offs, a.perm, fe, fixed, xin, y, state,
                                    partials);
@@ -1889,15 +1875,16 @@ extern "C" int cf_pcg_solve_mf_t10(int64_t n_nodes, int64_t n_elem, const int32_
     cf::set_error("cf_pcg_solve_mf_t10: %d free diagonal entries/blocks are not positive", bad);
     return CF_ERR_BREAKDOWN;
   }
-  (void)G;
   auto apply = [&](const double* xin, double* y, PcgState* state, double* partials) {
     return mf_apply(a, xin, y, fixed, fe, state, partials, s);
   };
   const int64_t n = 3 * n_nodes;
+  const int parts = a.slot ? int(mf_gather_grid(n_nodes)) : G;
   if (precond == CF_PC_BLOCK3)
-    return pcg_fp64_core<CF_PC_BLOCK3>(apply, n, n_nodes, b, x, fixed, tol, maxit, L, stats_host,
-                                       s);
-  return pcg_fp64_core<CF_PC_JACOBI>(apply, n, n, b, x, fixed, tol, maxit, L, stats_host, s);
+    return pcg_fp64_core<CF_PC_BLOCK3>(apply, parts, n, n_nodes, b, x, fixed, tol, maxit, L,
+                                       stats_host, s);
+  return pcg_fp64_core<CF_PC_JACOBI>(apply, parts, n, n, b, x, fixed, tol, maxit, L, stats_host,
+                                     s);
 }
 
 extern "C" int cf_invert_permutation(int64_t n, const uint32_t* perm, uint32_t* inv,
