@@ -10,13 +10,14 @@ timed region and resident in HBM when it starts.
 One step = the whole drop-in path on device-resident inputs: element
 geometry, node-pair CSR pattern (radix sort), stiffness values, traction
 load, Dirichlet-masked PCG to ||b_f - A_ff x_f|| / ||b_f|| <= 1e-10 (FP64
-residual).  Three solver paths are timed on the same mesh, each after the
+residual).  Four solver paths are timed on the same mesh, each after the
 same full CSR assembly: plain FP64 PCG on the assembled NB3 matrix (`fp64`),
 FP64 PCG with the matrix-free Tet10 operator (`matfree`, Jacobi from the
-same diagonal), and mixed precision (`mixed`: FP32 inner CG on the
-Jacobi-scaled matrix + FP64 residual replacement, capped at 4x the FP64
-iteration count).  `value` is the fastest path that reaches the FP64
-residual bar; `config.solver` names it.  DOF counts all nodal DOF (3N).
+same diagonal), and mixed precision (`mixed`, `mixed16`: FP64 flexible CG
+on the assembled matrix, preconditioned by an FP32 Chebyshev polynomial of
+the Jacobi-scaled matrix whose copy is stored in FP32 / FP16).  `value` is
+the fastest path that reaches the FP64 residual bar; `config.solver` names
+it.  DOF counts all nodal DOF (3N).
 
 The roofline object describes the dominant kernel (the operator inside PCG)
 of that path: its average duration is measured with HIP events on sampled
@@ -132,7 +133,7 @@ def main():
     lam, mu = lame(E_MOD, NU)
     torch.cuda.synchronize()
 
-    maxit = {"fp64": 200000, "mixed": 200000}
+    maxit = {"fp64": 200000, "mixed": 200000, "mixed16": 200000}
     offs_perm = {}
 
     def step(path, precond):
@@ -179,7 +180,7 @@ def main():
                        "rel_residual": st["rel_residual"], "precond": st["precond"],
                        "op": "spmv_sell<double,double,double>", "op_avg_ms": prof[0],
                        "op_launches_sampled": prof[1], "op_bytes": bytes_spmv64}
-    maxit["mixed"] = 4 * st["iterations"]   # bound the mixed attempt
+    maxit["mixed"] = maxit["mixed16"] = 4 * st["iterations"]   # bound the mixed attempts
     ms, (x_mf, st, _), prof = timed("matfree", args.precond64)
     # conn 40 + geo 80 + element-force write/read 2 x 240 + incidence 40 B per element,
     # offsets 4 B per node, x read once + y written once (16 B per DOF)
@@ -190,18 +191,24 @@ def main():
                           "op_launches_sampled": prof[1], "op_bytes": bytes_mf,
                           "rel_diff_vs_fp64": float(torch.linalg.vector_norm(x_mf - x_64)
                                                     / torch.linalg.vector_norm(x_64))}
-    try:
-        ms, (x_mx, st, _), prof = timed("mixed", "jacobi")
-        results["mixed"] = {"ms_per_step": ms, "iterations": st["iterations"],
-                            "residual_updates": st["residual_updates"],
-                            "rel_residual": st["rel_residual"],
-                            "op": "spmv_sell<float,float,float>", "op_avg_ms": prof[2],
-                            "op_launches_sampled": prof[3],
-                            "op_bytes": 40 * nnz_b + 4 * (n_nodes + 1) + 8 * n_dof,
-                            "rel_diff_vs_fp64": float(torch.linalg.vector_norm(x_mx - x_64)
-                                                      / torch.linalg.vector_norm(x_64))}
-    except N.SolverNotConverged as exc:
-        results["mixed"] = {"converged": False, "error": str(exc)[-300:]}
+    for path, vbytes in (("mixed", 36), ("mixed16", 18)):
+        try:
+            ms, (x_mx, st, _), prof = timed(path, "jacobi")
+        except (N.SolverNotConverged, N.SolverBreakdown) as exc:
+            results[path] = {"converged": False, "error": str(exc)[-300:]}
+            continue
+        # Chebyshev step: stored values + 4 B column per block, 4 B row id per
+        # node, the FP32 direction read once, w and z read+written, d' written
+        bytes_cheb = (vbytes + 4) * nnz_b + 4 * n_nodes + 24 * n_dof
+        results[path] = {"ms_per_step": ms, "iterations": st["iterations"],
+                         "low_precision_products": st["residual_updates"],
+                         "rel_residual": st["rel_residual"],
+                         "op": "cheb_step<%s>" % ("float" if vbytes == 36 else "_Float16"),
+                         "op_avg_ms": prof[2], "op_launches_sampled": prof[3],
+                         "op_bytes": bytes_cheb,
+                         "outer_spmv_avg_ms": prof[0],
+                         "rel_diff_vs_fp64": float(torch.linalg.vector_norm(x_mx - x_64)
+                                                   / torch.linalg.vector_norm(x_64))}
     for r in results.values():
         if "ms_per_step" in r:
             r["value"] = n_dof / (r["ms_per_step"] / 1e3)
@@ -226,15 +233,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if best == "mixed" else "f64",
+        "dtype": "f32+f64" if best.startswith("mixed") else "f64",
         "data": "synthetic",
         "config": {"workload": f"{args.workload} Tet10 cantilever 0.4x0.05x0.05 m, "
                                f"{mesh.n_elements} elements, {n_dof} DOF, E=210GPa nu=0.3, "
                                "clamped x=0, 1 MPa -z traction at x=0.4",
                    "n_dof": n_dof, "nnz_blocks": nnz_b, "tol": TOL, "solver": best},
         "paths": results,
-        "speedup_mixed_vs_fp64": (results["fp64"]["ms_per_step"] / results["mixed"]["ms_per_step"]
-                                  if "ms_per_step" in results["mixed"] else None),
+        "speedup_mixed_vs_fp64": {
+            k: results["fp64"]["ms_per_step"] / results[k]["ms_per_step"]
+            for k in ("mixed", "mixed16") if "ms_per_step" in results[k]},
         "tip_uz": tip,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,

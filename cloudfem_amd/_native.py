@@ -50,6 +50,7 @@ FMT_CSR = 0
 FMT_NB3 = 1
 PREC_FP64 = 0
 PREC_MIXED = 1
+PREC_MIXED16 = 2
 PC_JACOBI = 0
 PC_BLOCK3 = 1
 

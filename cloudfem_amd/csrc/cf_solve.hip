@@ -113,25 +113,23 @@ __device__ void block_partials(double (&v)[NV], double* __restrict__ partials) {
     for (int k = 0; k < NV; ++k) partials[k * gridDim.x + blockIdx.x] = v[k];
 }
 
-// Contiguous row range of a block, with blocks of one XCD covering one
-// contiguous eighth of the rows (the hardware deals workgroups round-robin
-// to the XCDs), so each XCD's L2 holds the part of x its rows gather.
+// Contiguous row range of a block.  Blocks are NOT regrouped per XCD: the
+// hardware deals workgroups round-robin to the 8 XCDs, and keeping that order
+// makes the blocks resident at any moment stream one contiguous region of the
+// matrix.  Regrouping so that each XCD walks its own contiguous eighth (to
+// keep its part of x in its L2) measured 1.6x slower on the Tet10 SpMV
+// (341 vs 212 us at 539,601 node rows): HBM streaming locality matters more
+// here than L2 reuse of x, which the 256 MB last-level cache serves anyway.
 __device__ __forceinline__ void block_rows(int64_t n_rows, int64_t& r0, int64_t& r1) {
   const int64_t G = gridDim.x;
-  int64_t b = blockIdx.x;
-  if (G % kXcd == 0) b = (b % kXcd) * (G / kXcd) + b / kXcd;
   const int64_t chunk = (n_rows + G - 1) / G;
-  r0 = b * chunk;
+  r0 = blockIdx.x * chunk;
   r1 = r0 + chunk < n_rows ? r0 + chunk : n_rows;
 }
 
-// Tiled launches: grid = tile_grid(tiles) blocks (a multiple of kXcd), block b
-// takes tile tile_index(b), so consecutive tiles share an XCD's L2 and every
-// block gets at most one tile (no tail round on a few blocks).
-__device__ __forceinline__ int64_t tile_index(int64_t b) {
-  const int64_t G = gridDim.x;
-  return (b % kXcd) * (G / kXcd) + b / kXcd;
-}
+// Tiled launches: grid = tile_grid(tiles) blocks, block b takes tile b, so
+// every block gets at most one tile (no tail round on a few blocks).
+__device__ __forceinline__ int64_t tile_index(int64_t b) { return b; }
 
 // ------------------------------------------------------------- SpMV
 
@@ -594,6 +592,132 @@ __global__ void mx_accumulate(int64_t n, double* __restrict__ x, float* __restri
   }
 }
 
+// ----------------------------------------- mixed precision: FP64 flexible CG
+// with a Chebyshev polynomial of the Jacobi-scaled matrix S A S (FP32
+// vectors, FP32 or FP16 stored values) as the preconditioner:
+//   z = S p(S A S) S r,  p = degree-(K-1) Chebyshev iteration on [b/ratio, b].
+// The outer recurrence, its residual and the solution stay FP64.
+
+// x += alpha p; r -= alpha Ap (skipped when !apply_step); r.r partials; then
+// the first Chebyshev step, which needs no product: w = fp32(S r),
+// d = w / theta, z = d.
+__global__ void __launch_bounds__(kBlock)
+fcg_update(int64_t n, double* __restrict__ x, double* __restrict__ r,
+           const double* __restrict__ p, const double* __restrict__ Ap,
+           const double* __restrict__ sc, float* __restrict__ w, float* __restrict__ d,
+           float* __restrict__ z, float inv_theta, int apply_step, PcgState* state,
+           double* __restrict__ partials) {
+  if (apply_step && state->halt) return;
+  const double alpha = apply_step ? state->alpha : 0.0;
+  double rr = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    double rn = r[i];
+    if (apply_step) {
+      x[i] += alpha * p[i];
+      rn -= alpha * Ap[i];
+      r[i] = rn;
+    }
+    rr += rn * rn;
+    const float wv = float(sc[i] * rn);
+    w[i] = wv;
+    const float dv = wv * inv_theta;
+    d[i] = dv;
+    z[i] = dv;
+  }
+  double v1[1] = {rr};
+  block_partials<1>(v1, partials);
+}
+
+// z = S zh (FP64); partials of r.z and r.z_old
+__global__ void __launch_bounds__(kBlock)
+fcg_zdots(int64_t n, const double* __restrict__ r, const double* __restrict__ sc,
+          const float* __restrict__ zh, double* __restrict__ z, const double* __restrict__ zold,
+          const PcgState* state, double* __restrict__ partials) {
+  if (state->halt) return;
+  double rz = 0.0, rzo = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const double zv = sc[i] * double(zh[i]);
+    z[i] = zv;
+    rz += r[i] * zv;
+    rzo += r[i] * zold[i];
+  }
+  double v[2] = {rz, rzo};
+  block_partials<2>(v, partials);
+}
+
+// p = z + beta p
+__global__ void __launch_bounds__(kBlock)
+fcg_direction(int64_t n, const double* __restrict__ z, double* __restrict__ p,
+              const PcgState* state) {
+  if (state->halt) return;
+  const double beta = state->beta;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    p[i] = z[i] + beta * p[i];
+}
+
+// Power iteration on S A S for the top of the spectrum: v0 = S 1 spread by a
+// fixed hash, then v = y / ||y|| with y = S A S v.
+__global__ void power_start(int64_t n, const double* __restrict__ sc, float* __restrict__ v) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const uint32_t hsh = uint32_t(i) * 2654435761u;
+    v[i] = sc[i] > 0.0 ? 1.0f + float(hsh >> 22) * (1.0f / 1024.0f) : 0.0f;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+norm2_partials(int64_t n, const float* __restrict__ y, double* __restrict__ partials) {
+  double a = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    a += double(y[i]) * double(y[i]);
+  double v1[1] = {a};
+  block_partials<1>(v1, partials);
+}
+
+__global__ void power_scale(int64_t n, const float* __restrict__ y, float* __restrict__ v,
+                            const PcgState* state) {
+  const float inv = state->pad > 0.0 ? float(1.0 / sqrt(state->pad)) : 0.0f;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    v[i] = y[i] * inv;
+}
+
+// Gershgorin bound of S A S: per DOF row sum_j |S_i A_ij S_j|, block maxima
+// in partials (folded by finalize kFinMax).
+__global__ void __launch_bounds__(kBlock)
+gershgorin_nb3(int64_t n_nodes, const int32_t* __restrict__ ptr, const int32_t* __restrict__ idx,
+               const double* __restrict__ val, const double* __restrict__ sc,
+               double* __restrict__ partials) {
+  double mx = 0.0;
+  for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < 3 * n_nodes;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t i = t / 3;
+    const int c = int(t - 3 * i);
+    const int32_t base = ptr[i], len = ptr[i + 1] - base;
+    const double si = sc[t];
+    double acc = 0.0;
+    for (int32_t k = 0; k < len; ++k) {
+      const int64_t j3 = 3 * int64_t(idx[base + k]);
+      const double* v = val + int64_t(9) * base + c * 3 * len + 3 * k;
+      acc += fabs(v[0] * sc[j3]) + fabs(v[1] * sc[j3 + 1]) + fabs(v[2] * sc[j3 + 2]);
+    }
+    mx = fmax(mx, si * acc);
+  }
+  __shared__ double red[kBlock / 64];
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int k = 1; k < kBlock / 64; ++k) m = fmax(m, red[k]);
+    partials[blockIdx.x] = m;
+  }
+}
+
 // ------------------------------------------------------------- matrix-free Tet10
 
 // Reference tables for the matrix-free operator (same values as the
@@ -956,10 +1080,66 @@ __global__ void sell_fill(int64_t n, int64_t n_slices, const int32_t* __restrict
   }
 }
 
-// y = A x on the sliced copy: one slice per wave, kSellUnroll steps per
+// Row products of one slice: one slice per wave, kSellUnroll steps per
 // iteration with the next steps' column indices loaded one iteration ahead,
-// so the x gathers of a step never wait on its own index load.  Grid:
-// sell_grid(n_slices) blocks of 4 waves, block b -> slices of tile(b).
+// so the x gathers of a step never wait on its own index load.  Returns the
+// row (or -1) whose 3 sums `o` this lane holds after the phase reduction.
+template <class VT, class XT, class AT>
+__device__ __forceinline__ int64_t sell_rows(int64_t n, int64_t sl,
+                                             const uint32_t* __restrict__ rows,
+                                             const int32_t* __restrict__ koff,
+                                             const int32_t* __restrict__ scol,
+                                             const VT* __restrict__ sval,
+                                             const XT* __restrict__ x, AT (&o)[3]) {
+  constexpr int U = kSellUnroll;
+  const int lane = threadIdx.x & 63;
+  const int64_t lin = sl * kSliceRows + (lane % kSliceRows);
+  const int32_t g0 = koff[sl], g1 = koff[sl + 1];
+  AT a0 = 0, a1 = 0, a2 = 0;
+  int32_t c[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) c[u] = g0 + u < g1 ? scol[int64_t(g0 + u) * 64 + lane] : 0;
+  for (int32_t g = g0; g < g1; g += U) {
+    AT xv[U][3], v[U][9];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t c3 = 3 * int64_t(c[u]);
+      const bool ok = g + u < g1;
+      xv[u][0] = ok ? AT(x[c3]) : AT(0);
+      xv[u][1] = ok ? AT(x[c3 + 1]) : AT(0);
+      xv[u][2] = ok ? AT(x[c3 + 2]) : AT(0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const VT* p = sval + int64_t(g + u) * 9 * 64 + lane;
+      const bool ok = g + u < g1;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) v[u][q] = ok ? AT(ld(p + q * 64)) : AT(0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      c[u] = g + U + u < g1 ? scol[int64_t(g + U + u) * 64 + lane] : 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      a0 += v[u][0] * xv[u][0] + v[u][1] * xv[u][1] + v[u][2] * xv[u][2];
+      a1 += v[u][3] * xv[u][0] + v[u][4] * xv[u][1] + v[u][5] * xv[u][2];
+      a2 += v[u][6] * xv[u][0] + v[u][7] * xv[u][1] + v[u][8] * xv[u][2];
+    }
+  }
+#pragma unroll
+  for (int m = kSliceRows; m < 64; m <<= 1) {
+    a0 += __shfl_xor(a0, m, 64);
+    a1 += __shfl_xor(a1, m, 64);
+    a2 += __shfl_xor(a2, m, 64);
+  }
+  o[0] = a0;
+  o[1] = a1;
+  o[2] = a2;
+  return (lane < kSliceRows && lin < n) ? int64_t(rows[lin]) : -1;
+}
+
+// y = A x on the sliced copy, fixed rows zeroed; with `state`: p.Ap
+// partials.  Grid: sell_grid(n_slices) blocks of 4 waves.
 template <class VT, class XT, class AT>
 __global__ void __launch_bounds__(kBlock)
 spmv_sell(int64_t n, int64_t n_slices, const uint32_t* __restrict__ rows,
@@ -967,54 +1147,12 @@ spmv_sell(int64_t n, int64_t n_slices, const uint32_t* __restrict__ rows,
           const VT* __restrict__ sval, const XT* __restrict__ x, XT* __restrict__ y,
           const uint8_t* __restrict__ fixed, PcgState* state, double* __restrict__ partials) {
   if (state && state->halt) return;
-  constexpr int U = kSellUnroll;
-  const int lane = threadIdx.x & 63;
-  const int h = lane / kSliceRows;
   const int64_t sl = tile_index(blockIdx.x) * (kBlock / 64) + threadIdx.x / 64;
   double dot = 0.0;
   if (sl < n_slices) {
-    const int64_t lin = sl * kSliceRows + (lane % kSliceRows);
-    const int32_t g0 = koff[sl], g1 = koff[sl + 1];
-    AT a0 = 0, a1 = 0, a2 = 0;
-    int32_t c[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) c[u] = g0 + u < g1 ? scol[int64_t(g0 + u) * 64 + lane] : 0;
-    for (int32_t g = g0; g < g1; g += U) {
-      AT xv[U][3], v[U][9];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int64_t c3 = 3 * int64_t(c[u]);
-        const bool ok = g + u < g1;
-        xv[u][0] = ok ? AT(x[c3]) : AT(0);
-        xv[u][1] = ok ? AT(x[c3 + 1]) : AT(0);
-        xv[u][2] = ok ? AT(x[c3 + 2]) : AT(0);
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const VT* p = sval + int64_t(g + u) * 9 * 64 + lane;
-        const bool ok = g + u < g1;
-#pragma unroll
-        for (int q = 0; q < 9; ++q) v[u][q] = ok ? AT(ld(p + q * 64)) : AT(0);
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-        c[u] = g + U + u < g1 ? scol[int64_t(g + U + u) * 64 + lane] : 0;
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        a0 += v[u][0] * xv[u][0] + v[u][1] * xv[u][1] + v[u][2] * xv[u][2];
-        a1 += v[u][3] * xv[u][0] + v[u][4] * xv[u][1] + v[u][5] * xv[u][2];
-        a2 += v[u][6] * xv[u][0] + v[u][7] * xv[u][1] + v[u][8] * xv[u][2];
-      }
-    }
-#pragma unroll
-    for (int m = kSliceRows; m < 64; m <<= 1) {
-      a0 += __shfl_xor(a0, m, 64);
-      a1 += __shfl_xor(a1, m, 64);
-      a2 += __shfl_xor(a2, m, 64);
-    }
-    if (h == 0 && lin < n) {
-      const int64_t r = rows[lin];
-      AT o[3] = {a0, a1, a2};
+    AT o[3];
+    const int64_t r = sell_rows<VT, XT, AT>(n, sl, rows, koff, scol, sval, x, o);
+    if (r >= 0) {
 #pragma unroll
       for (int cc = 0; cc < 3; ++cc) {
         const int64_t d = 3 * r + cc;
@@ -1029,6 +1167,33 @@ spmv_sell(int64_t n, int64_t n_slices, const uint32_t* __restrict__ rows,
   block_partials<1>(v1, partials);
 }
 
+// One step of the Chebyshev iteration on S A S (Jacobi-scaled, FP32 vectors,
+// values stored as VT):  w -= S A S d;  d' = cd d + cw w;  z += d'.
+// S zeroes fixed rows and columns, so no mask is needed.
+template <class VT>
+__global__ void __launch_bounds__(kBlock)
+cheb_step(int64_t n, int64_t n_slices, const uint32_t* __restrict__ rows,
+          const int32_t* __restrict__ koff, const int32_t* __restrict__ scol,
+          const VT* __restrict__ sval, const float* __restrict__ d, float* __restrict__ dn,
+          float* __restrict__ w, float* __restrict__ z, float cd, float cw,
+          const PcgState* state) {
+  if (state->halt) return;
+  const int64_t sl = tile_index(blockIdx.x) * (kBlock / 64) + threadIdx.x / 64;
+  if (sl >= n_slices) return;
+  float o[3];
+  const int64_t r = sell_rows<VT, float, float>(n, sl, rows, koff, scol, sval, d, o);
+  if (r < 0) return;
+#pragma unroll
+  for (int cc = 0; cc < 3; ++cc) {
+    const int64_t i = 3 * r + cc;
+    const float wn = w[i] - o[cc];
+    w[i] = wn;
+    const float dd = cd * d[i] + cw * wn;
+    dn[i] = dd;
+    z[i] += dd;
+  }
+}
+
 // ------------------------------------------------------------- finalize
 
 enum Fin : int {
@@ -1039,6 +1204,9 @@ enum Fin : int {
   kFinMxStart = 4,    // r.r of a fresh FP32 residual, new direction
   kFinMxReplace = 5,  // r.r of a replaced FP32 residual, kept direction
   kFinMxUpdate = 6,   // recursive FP32 r.r -> beta / replacement request
+  kFinFcgCheck = 7,   // r.r after the update -> convergence test
+  kFinFcgBeta = 8,    // (r.z, r.z_old) -> flexible (Polak-Ribiere) beta
+  kFinMax = 9,        // max of the partials -> *out
 };
 
 // One block folds the per-block partials of the previous kernel (G of
@@ -1046,9 +1214,24 @@ enum Fin : int {
 __global__ void __launch_bounds__(kBlock)
 finalize(int mode, int G, const double* __restrict__ partials, PcgState* state,
          double* __restrict__ out, double aux) {
-  const bool gated = (mode == kFinAlpha || mode == kFinCgUpdate || mode == kFinMxUpdate);
+  const bool gated = (mode == kFinAlpha || mode == kFinCgUpdate || mode == kFinMxUpdate ||
+                      mode == kFinFcgCheck || mode == kFinFcgBeta);
   if (gated && state->halt) return;
-  const int nv = (mode == kFinCgUpdate || mode == kFinStart) ? 2 : 1;
+  if (mode == kFinMax) {
+    __shared__ double red[kBlock / 64];
+    double m = 0.0;
+    for (int b = threadIdx.x; b < G; b += blockDim.x) m = fmax(m, partials[b]);
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double mm = red[0];
+      for (int k = 1; k < kBlock / 64; ++k) mm = fmax(mm, red[k]);
+      *out = mm;
+    }
+    return;
+  }
+  const int nv = (mode == kFinCgUpdate || mode == kFinStart || mode == kFinFcgBeta) ? 2 : 1;
   double v[2] = {0.0, 0.0};
   for (int k = 0; k < nv; ++k)
     for (int b = threadIdx.x; b < G; b += blockDim.x) v[k] += partials[k * G + b];
@@ -1114,6 +1297,25 @@ finalize(int mode, int G, const double* __restrict__ partials, PcgState* state,
       }
       break;
     }
+    case kFinFcgCheck:
+      state->iter += 1;
+      state->rr = v[0];
+      if (!isfinite(v[0]))
+        state->halt = kBreakdown;
+      else if (v[0] <= state->stop2)
+        state->halt = kConverged;
+      else if (state->iter >= state->maxit)
+        state->halt = kMaxit;
+      break;
+    case kFinFcgBeta:
+      // v[0] = r.z (new), v[1] = r.z_old; rz holds the previous r.z
+      if (!(v[0] > 0.0) || !isfinite(v[0]) || !isfinite(v[1])) {
+        state->halt = kBreakdown;
+      } else {
+        state->beta = (v[0] - v[1]) / state->rz;
+        state->rz = v[0];
+      }
+      break;
     default:
       break;
   }
@@ -1216,6 +1418,9 @@ struct Layout {
   int32_t* scol;
   double* sval64;
   float* sval32;
+  _Float16* sval16;
+  double* z0;  // flexible-CG preconditioned residuals (current / previous)
+  double* z1;
   uint64_t* keys_in;
   uint64_t* keys_out;
   uint32_t* ids_in;
@@ -1229,11 +1434,8 @@ int pcg_grid() {
   return g - g % kXcd;
 }
 
-// blocks for `tiles` tiles (see tile_index): a multiple of kXcd
-int64_t tile_grid(int64_t tiles) {
-  const int64_t g = tiles < 1 ? 1 : tiles;
-  return (g + kXcd - 1) / kXcd * kXcd;
-}
+// blocks for `tiles` tiles (see tile_index)
+int64_t tile_grid(int64_t tiles) { return tiles < 1 ? 1 : tiles; }
 
 int64_t sell_grid(int64_t n_slices) { return tile_grid((n_slices + 3) / 4); }
 
@@ -1276,13 +1478,18 @@ Layout layout(const cf_matrix* A, int precision, int precond, void* base, int64_
   L.partials = c.take<double>(4 * size_t(parts));
   L.state = c.take<PcgState>(1);
   L.bad = c.take<int32_t>(1);
-  if (precision == CF_PREC_MIXED) {
+  const bool mixed = precision != CF_PREC_FP64;
+  if (mixed) {
     L.s = c.take<double>(n);
     L.rh = c.take<float>(n);
     L.ph = c.take<float>(n);
     L.Aph = c.take<float>(n);
     L.yh = c.take<float>(n);
     if (!nb3) L.vals32 = c.take<float>(nval);
+  }
+  if (mixed && nb3) {
+    L.z0 = c.take<double>(n);
+    L.z1 = c.take<double>(n);
   }
   if (nb3) {
     const int64_t N = A->n_rows;
@@ -1291,10 +1498,9 @@ Layout layout(const cf_matrix* A, int precision, int precond, void* base, int64_
     L.koff = c.take<int32_t>(L.n_slices + 1);
     L.slice_len = c.take<int32_t>(L.n_slices + 1);
     L.scol = c.take<int32_t>(size_t(64) * lsum);
-    if (precision == CF_PREC_MIXED)
-      L.sval32 = c.take<float>(size_t(9 * 64) * lsum);
-    else
-      L.sval64 = c.take<double>(size_t(9 * 64) * lsum);
+    L.sval64 = c.take<double>(size_t(9 * 64) * lsum);
+    if (precision == CF_PREC_MIXED) L.sval32 = c.take<float>(size_t(9 * 64) * lsum);
+    if (precision == CF_PREC_MIXED16) L.sval16 = c.take<_Float16>(size_t(9 * 64) * lsum);
     L.keys_in = c.take<uint64_t>(N);
     L.keys_out = c.take<uint64_t>(N);
     L.ids_in = c.take<uint32_t>(N);
@@ -1311,8 +1517,8 @@ Layout layout(const cf_matrix* A, int precision, int precond, void* base, int64_
   return L;
 }
 
-// Build the sliced copy (FP64, or FP32 of S A S when `scale` is given).
-int build_sell(const cf_matrix* A, const Layout& L, const double* scale, hipStream_t s) {
+// Slice structure of the sliced copy: window sort, slice steps, offsets.
+int build_sell(const cf_matrix* A, const Layout& L, hipStream_t s) {
   const int64_t N = A->n_rows;
   const int grid = cf::stride_grid(N, kBlock);
   sell_keys<<<grid, kBlock, 0, s>>>(N, A->indptr, L.keys_in, L.ids_in);
@@ -1328,13 +1534,15 @@ int build_sell(const cf_matrix* A, const Layout& L, const double* scale, hipStre
   tb = L.tmp_bytes;
   CF_HIP(rocprim::exclusive_scan(L.tmp, tb, L.slice_len, L.koff, 0, size_t(L.n_slices + 1),
                                  rocprim::plus<int32_t>(), s));
+  return CF_OK;
+}
+
+// Values of the sliced copy: A (scale null) or S A S, stored as OT.
+template <class OT>
+int fill_sell(const cf_matrix* A, const Layout& L, const double* scale, OT* out, hipStream_t s) {
   const int fgrid = cf::stride_grid(L.n_slices * 64, kBlock);
-  if (scale)
-    sell_fill<float><<<fgrid, kBlock, 0, s>>>(N, L.n_slices, A->indptr, A->indices, A->values,
-                                              L.rows, L.koff, scale, L.scol, L.sval32);
-  else
-    sell_fill<double><<<fgrid, kBlock, 0, s>>>(N, L.n_slices, A->indptr, A->indices, A->values,
-                                               L.rows, L.koff, nullptr, L.scol, L.sval64);
+  sell_fill<OT><<<fgrid, kBlock, 0, s>>>(A->n_rows, L.n_slices, A->indptr, A->indices, A->values,
+                                         L.rows, L.koff, scale, L.scol, out);
   CF_LAUNCHED();
   return CF_OK;
 }
@@ -1497,7 +1705,9 @@ int pcg_fp64(const cf_matrix* A, const double* b, double* x, const uint8_t* fixe
   const int64_t units = PC == CF_PC_BLOCK3 ? A->n_rows : n;
   const int G = pcg_grid();
   int rc;
-  if (A->format == CF_FMT_NB3 && (rc = build_sell(A, L, nullptr, s))) return rc;
+  if (A->format == CF_FMT_NB3 &&
+      ((rc = build_sell(A, L, s)) || (rc = fill_sell<double>(A, L, nullptr, L.sval64, s))))
+    return rc;
   const int GS = A->format == CF_FMT_NB3 ? int(sell_grid(L.n_slices)) : G;
   auto apply = [&](const double* xin, double* y, PcgState* state, double* partials) {
     if (A->format == CF_FMT_NB3) {
@@ -1512,11 +1722,193 @@ int pcg_fp64(const cf_matrix* A, const double* b, double* x, const uint8_t* fixe
   return pcg_fp64_core<PC>(apply, GS, n, units, b, x, fixed, tol, maxit, L, st, s);
 }
 
+// NB3 systems: FP64 flexible CG, preconditioned by S p(S A S) S with p the
+// Chebyshev iteration of degree kChebSteps on [b / kChebRatio, b] run in FP32
+// on a sliced copy of S A S stored as VT (float or _Float16).  b is the
+// smaller of the Gershgorin bound and 1.25x a 40-step power estimate; p(t) > 0
+// for every t < b + b / kChebRatio, so the preconditioner is SPD even where
+// rounding of the stored values shifts the lowest eigenvalues.  Polak-Ribiere
+// beta keeps the outer iteration robust to the FP32 rounding inside p.
+constexpr int kChebSteps = 8;
+constexpr double kChebRatio = 100.0;
+constexpr int kPowerSteps = 40;
+
+template <class VT>
+int pcg_fcg_cheb(const cf_matrix* A, VT* sval, const double* b, double* x, const uint8_t* fixed,
+                 double tol, int maxit, const Layout& L, cf_pcg_stats* st, hipStream_t s) {
+  const int64_t n = n_dofs(A);
+  const int G = pcg_grid();
+  const int GS = int(sell_grid(L.n_slices));
+  int rc;
+  PcgState h{};
+  make_scale<<<G, kBlock, 0, s>>>(n, L.dinv, L.s);
+  CF_LAUNCHED();
+  if ((rc = build_sell(A, L, s)) || (rc = fill_sell<double>(A, L, nullptr, L.sval64, s)) ||
+      (rc = fill_sell<VT>(A, L, L.s, sval, s)))
+    return rc;
+  auto spmv64 = [&](const double* xin, double* y, PcgState* state) {
+    spmv_sell<double, double, double><<<GS, kBlock, 0, s>>>(
+        A->n_rows, L.n_slices, L.rows, L.koff, L.scol, L.sval64, xin, y, fixed, state,
+        L.partials);
+    CF_LAUNCHED();
+    return CF_OK;
+  };
+  // spectrum bound of S A S
+  gershgorin_nb3<<<G, kBlock, 0, s>>>(A->n_rows, A->indptr, A->indices, A->values, L.s,
+                                      L.partials);
+  CF_LAUNCHED();
+  if ((rc = fin(kFinMax, L, s, &L.state->pad, 0.0, G))) return rc;
+  if ((rc = read_state(L.state, &h, s))) return rc;
+  const double gersh = h.pad;
+  power_start<<<G, kBlock, 0, s>>>(n, L.s, L.ph);
+  CF_LAUNCHED();
+  for (int k = 0; k < kPowerSteps; ++k) {
+    spmv_sell<VT, float, float><<<GS, kBlock, 0, s>>>(A->n_rows, L.n_slices, L.rows, L.koff,
+                                                       L.scol, sval, L.ph, L.Aph, nullptr,
+                                                       nullptr, nullptr);
+    CF_LAUNCHED();
+    norm2_partials<<<G, kBlock, 0, s>>>(n, L.Aph, L.partials);
+    CF_LAUNCHED();
+    if ((rc = fin(kFinStore, L, s, &L.state->pad))) return rc;
+    power_scale<<<G, kBlock, 0, s>>>(n, L.Aph, L.ph, L.state);
+    CF_LAUNCHED();
+  }
+  if ((rc = read_state(L.state, &h, s))) return rc;
+  const double est = std::sqrt(h.pad);
+  const double bmax = std::min(1.25 * est, gersh);
+  if (!(bmax > 0.0) || !std::isfinite(bmax)) {
+    cf::set_error("cf_pcg_solve: no spectrum bound for the Chebyshev preconditioner "
+                  "(power estimate %.3e, Gershgorin %.3e)", est, gersh);
+    return CF_ERR_BREAKDOWN;
+  }
+  const double amin = bmax / kChebRatio;
+  const double theta = 0.5 * (bmax + amin), delta = 0.5 * (bmax - amin), sigma = theta / delta;
+  float cd[kChebSteps], cw[kChebSteps];
+  double rho0 = 1.0 / sigma;
+  for (int k = 1; k < kChebSteps; ++k) {
+    const double rho1 = 1.0 / (2.0 * sigma - rho0);
+    cd[k] = float(rho1 * rho0);
+    cw[k] = float(2.0 * rho1 / delta);
+    rho0 = rho1;
+  }
+  const float inv_theta = float(1.0 / theta);
+  float* D[2] = {L.ph, L.Aph};
+  double* Z[2] = {L.z0, L.z1};
+  // w = rh, zh = yh; z = S p(S A S) S r into zc, r.z against zo
+  auto precondition = [&](int apply_step, double* zc, const double* zo, int sample) {
+    fcg_update<<<G, kBlock, 0, s>>>(n, x, L.r, L.p, L.Ap, L.s, L.rh, D[0], L.yh, inv_theta,
+                                    apply_step, L.state, L.partials);
+    CF_LAUNCHED();
+    int r2;
+    if (apply_step && (r2 = fin(kFinFcgCheck, L, s))) return r2;
+    for (int k = 1; k < kChebSteps; ++k) {
+      int tok = -1;
+      if (k == 1) prof_begin(1, sample, s, &tok);
+      cheb_step<VT><<<GS, kBlock, 0, s>>>(A->n_rows, L.n_slices, L.rows, L.koff, L.scol, sval,
+                                          D[(k - 1) & 1], D[k & 1], L.rh, L.yh, cd[k], cw[k],
+                                          L.state);
+      CF_LAUNCHED();
+      if (k == 1) prof_end(tok, s);
+    }
+    fcg_zdots<<<G, kBlock, 0, s>>>(n, L.r, L.s, L.yh, zc, zo, L.state, L.partials);
+    CF_LAUNCHED();
+    if ((r2 = fin(kFinFcgBeta, L, s))) return r2;
+    fcg_direction<<<G, kBlock, 0, s>>>(n, zc, L.p, L.state);
+    CF_LAUNCHED();
+    return CF_OK;
+  };
+  // reference rhs b_f - A_fF x_F, then r = b - A x
+  split_vector<<<G, kBlock, 0, s>>>(n, x, fixed, 1, L.p);
+  CF_LAUNCHED();
+  if ((rc = spmv64(L.p, L.Ap, nullptr))) return rc;
+  residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->pad);
+  CF_LAUNCHED();
+  if ((rc = fin(kFinStore, L, s, &L.state->pad))) return rc;
+  if ((rc = spmv64(x, L.Ap, nullptr))) return rc;
+  residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->rr);
+  CF_LAUNCHED();
+  if ((rc = fin(kFinStore, L, s, &L.state->rr))) return rc;
+  if ((rc = read_state(L.state, &h, s))) return rc;
+  const double bb = h.pad;
+  st->b_norm = std::sqrt(bb);
+  st->iterations = 0;
+  st->residual_updates = 0;
+  if (!std::isfinite(bb) || !std::isfinite(h.rr)) {
+    cf::set_error("cf_pcg_solve: non-finite right-hand side");
+    return CF_ERR_BREAKDOWN;
+  }
+  if (bb == 0.0) {
+    split_vector<<<G, kBlock, 0, s>>>(n, x, fixed, 1, x);
+    CF_LAUNCHED();
+    CF_HIP(hipStreamSynchronize(s));
+    st->rel_residual = 0.0;
+    return CF_OK;
+  }
+  h.stop2 = tol * tol * bb;
+  h.maxit = maxit;
+  h.iter = 0;
+  h.rz = 1.0;
+  h.halt = h.rr <= h.stop2 ? kConverged : kRunning;
+  CF_HIP(hipMemcpyAsync(L.state, &h, sizeof(PcgState), hipMemcpyHostToDevice, s));
+  // first direction: z_old = 0 and p = 0, so p = z and rz = r.z
+  CF_HIP(hipMemsetAsync(Z[0], 0, size_t(n) * sizeof(double), s));
+  CF_HIP(hipMemsetAsync(L.p, 0, size_t(n) * sizeof(double), s));
+  if (h.halt == kRunning && (rc = precondition(0, Z[1], Z[0], 1))) return rc;
+  int64_t enq = 0;
+  int batch = 8;
+  double prev_rr = h.rr;
+  int prev_it = 0;
+  while (h.halt == kRunning) {
+    for (int t = 0; t < batch; ++t, ++enq) {
+      int tok;
+      prof_begin(0, int(enq), s, &tok);
+      if ((rc = spmv64(L.p, L.Ap, L.state))) return rc;
+      prof_end(tok, s);
+      if ((rc = fin(kFinAlpha, L, s, nullptr, 0.0, GS))) return rc;
+      if ((rc = precondition(1, Z[enq & 1], Z[(enq + 1) & 1], int(enq)))) return rc;
+    }
+    if ((rc = read_state(L.state, &h, s))) return rc;
+    prof_collect();
+    const int done_it = h.iter - prev_it;
+    if (h.halt == kRunning && done_it > 0 && h.rr < prev_rr) {
+      const double rate = std::log(h.rr / prev_rr) / done_it;
+      const double need = std::log(h.stop2 / h.rr) / rate;
+      int nb = int(need * 0.9) + 2;
+      batch = nb < 4 ? 4 : (nb > 128 ? 128 : nb);
+    } else {
+      batch = batch * 2 > 128 ? 128 : batch * 2;
+    }
+    prev_rr = h.rr;
+    prev_it = h.iter;
+  }
+  st->iterations = h.iter;
+  st->residual_updates = (kChebSteps - 1) * (h.iter + 1);  // low-precision products
+  if ((rc = spmv64(x, L.Ap, nullptr))) return rc;
+  residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->pad);
+  CF_LAUNCHED();
+  if ((rc = fin(kFinStore, L, s, &L.state->pad))) return rc;
+  PcgState fin_state{};
+  if ((rc = read_state(L.state, &fin_state, s))) return rc;
+  st->rel_residual = std::sqrt(fin_state.pad / bb);
+  if (h.halt == kBreakdown) {
+    cf::set_error("cf_pcg_solve: flexible CG breakdown after %d iterations (NaN/Inf, "
+                  "p.Ap <= 0 or r.z <= 0)", h.iter);
+    return CF_ERR_BREAKDOWN;
+  }
+  if (h.halt == kMaxit) {
+    cf::set_error("cf_pcg_solve: not converged after %d iterations (rel. residual %.3e)",
+                  h.iter, st->rel_residual);
+    return CF_ERR_NOT_CONVERGED;
+  }
+  return CF_OK;
+}
+
+// Scalar CSR systems (P1 heat, capacity-dominated theta-method steps): FP32
+// inner CG on S A S with FP64 residual replacement.
 int pcg_mixed(const cf_matrix* A, const double* b, double* x, const uint8_t* fixed, double tol,
               int maxit, const Layout& L, cf_pcg_stats* st, hipStream_t s) {
   const int64_t n = n_dofs(A);
   const int G = pcg_grid();
-  const int GS = A->format == CF_FMT_NB3 ? int(sell_grid(L.n_slices)) : G;
   // Replace the residual once the FP32 recursion has dropped it by 10x
   // since the last replacement; restart the direction when the replaced
   // residual is more than 2x the recursive one (the FP32 recursion has
@@ -1528,12 +1920,8 @@ int pcg_mixed(const cf_matrix* A, const double* b, double* x, const uint8_t* fix
   PcgState h{};
   make_scale<<<G, kBlock, 0, s>>>(n, L.dinv, L.s);
   CF_LAUNCHED();
-  if (A->format == CF_FMT_NB3) {
-    if ((rc = build_sell(A, L, L.s, s))) return rc;
-  } else {
-    scale_csr_to_f32<<<G, kBlock, 0, s>>>(A->n_rows, A->indptr, A->indices, A->values, L.s,
-                                          L.vals32);
-  }
+  scale_csr_to_f32<<<G, kBlock, 0, s>>>(A->n_rows, A->indptr, A->indices, A->values, L.s,
+                                        L.vals32);
   CF_LAUNCHED();
   cf_matrix A32 = *A;
   // reference norm ||b_f - A_fF x_F||
@@ -1597,18 +1985,11 @@ int pcg_mixed(const cf_matrix* A, const double* b, double* x, const uint8_t* fix
       for (int t = 0; t < batch; ++t) {
         int tok;
         prof_begin(1, h.iter + t, s, &tok);
-        if (A->format == CF_FMT_NB3) {
-          spmv_sell<float, float, float><<<GS, kBlock, 0, s>>>(A->n_rows, L.n_slices, L.rows,
-                                                                L.koff, L.scol, L.sval32, L.ph,
-                                                                L.Aph, nullptr, L.state,
-                                                                L.partials);
-          CF_LAUNCHED();
-        } else if ((rc = launch_spmv<float, float, float>(&A32, L.vals32, L.ph, L.Aph, nullptr,
-                                                          L.state, L.partials, G, s))) {
+        if ((rc = launch_spmv<float, float, float>(&A32, L.vals32, L.ph, L.Aph, nullptr, L.state,
+                                                   L.partials, G, s)))
           return rc;
-        }
         prof_end(tok, s);
-        if ((rc = fin(kFinAlpha, L, s, nullptr, 0.0, GS))) return rc;
+        if ((rc = fin(kFinAlpha, L, s))) return rc;
         mx_update<<<G, kBlock, 0, s>>>(n, L.yh, L.rh, L.ph, L.Aph, L.state, L.partials);
         CF_LAUNCHED();
         if ((rc = fin(kFinMxUpdate, L, s))) return rc;
@@ -1690,13 +2071,14 @@ extern "C" int cf_pcg_solve(const cf_matrix* A, const double* b, double* x,
   int rc = check_matrix(A);
   if (rc) return rc;
   CF_REQUIRE(b && x && ws && stats_host, "cf_pcg_solve: null argument");
-  CF_REQUIRE(precision == CF_PREC_FP64 || precision == CF_PREC_MIXED,
+  CF_REQUIRE(precision == CF_PREC_FP64 || precision == CF_PREC_MIXED ||
+                 precision == CF_PREC_MIXED16,
              "cf_pcg_solve: bad precision %d", precision);
   CF_REQUIRE(precond == CF_PC_JACOBI || precond == CF_PC_BLOCK3, "cf_pcg_solve: bad precond %d",
              precond);
   CF_REQUIRE(!(precond == CF_PC_BLOCK3 && A->format != CF_FMT_NB3),
              "cf_pcg_solve: block3 preconditioner needs an NB3 matrix");
-  CF_REQUIRE(!(precond == CF_PC_BLOCK3 && precision == CF_PREC_MIXED),
+  CF_REQUIRE(!(precond == CF_PC_BLOCK3 && precision != CF_PREC_FP64),
              "cf_pcg_solve: mixed precision uses Jacobi scaling");
   CF_REQUIRE(tol > 0.0 && maxit > 0, "cf_pcg_solve: tol and maxit must be positive");
   int64_t lsum = 0;
@@ -1728,7 +2110,12 @@ extern "C" int cf_pcg_solve(const cf_matrix* A, const double* b, double* x,
                   bad);
     return CF_ERR_BREAKDOWN;
   }
-  if (precision == CF_PREC_MIXED)
+  if (precision != CF_PREC_FP64 && A->format == CF_FMT_NB3) {
+    if (precision == CF_PREC_MIXED16)
+      return pcg_fcg_cheb<_Float16>(A, L.sval16, b, x, fixed, tol, maxit, L, stats_host, s);
+    return pcg_fcg_cheb<float>(A, L.sval32, b, x, fixed, tol, maxit, L, stats_host, s);
+  }
+  if (precision != CF_PREC_FP64)
     return pcg_mixed(A, b, x, fixed, tol, maxit, L, stats_host, s);
   if (precond == CF_PC_BLOCK3)
     return pcg_fp64<CF_PC_BLOCK3>(A, b, x, fixed, tol, maxit, L, stats_host, s);

@@ -7,9 +7,13 @@ configuration), and the defaults are chosen so that solutions agree with a
 direct solve to well within the 1e-10 relative bar on the validation
 problems:
 
-  precision  "fp64" (default) or "mixed" (FP32 inner CG + FP64 residual
-             replacement; error against FP64 is bounded by the same residual
-             test, see cf_pcg_solve)
+  precision  "fp64" (default), "mixed" or "mixed16".  For 3x3-block
+             (vector) systems: FP64 flexible CG preconditioned by an FP32
+             Chebyshev polynomial of the Jacobi-scaled matrix, whose copy is
+             stored in FP32 ("mixed") or FP16 ("mixed16"); for scalar systems
+             both mean FP32 inner CG with FP64 residual replacement.  The stop
+             test is the FP64 residual in every case, so the error against
+             FP64 is bounded by the same tolerance (cf_pcg_solve)
   precond    "jacobi" (default) or "block3" (3x3 node blocks, vector
              problems, FP64 only)
   tol        relative residual ||b_f - A_ff x_f|| / ||b_f|| (default 1e-12)
@@ -89,9 +93,9 @@ def pcg(A, b, x, fixed=None, precision=None, precond=None, tol=None, maxit=None)
     precond = precond or o["precond"]
     tol = float(tol if tol is not None else o["tol"])
     maxit = int(maxit if maxit is not None else o["maxit"])
-    prec = {"fp64": N.PREC_FP64, "mixed": N.PREC_MIXED}[precision]
+    prec = {"fp64": N.PREC_FP64, "mixed": N.PREC_MIXED, "mixed16": N.PREC_MIXED16}[precision]
     pc = {"jacobi": N.PC_JACOBI, "block3": N.PC_BLOCK3}[precond]
-    if pc == N.PC_BLOCK3 and (A.fmt != N.FMT_NB3 or prec == N.PREC_MIXED):
+    if pc == N.PC_BLOCK3 and (A.fmt != N.FMT_NB3 or prec != N.PREC_FP64):
         pc = N.PC_JACOBI
     lib = N.load()
     As = A.cstruct()

@@ -80,8 +80,13 @@ extern "C" {
 #define CF_FMT_CSR 0    /* scalar CSR: rows = DOFs                           */
 #define CF_FMT_NB3 1    /* 3x3 node blocks in NB3 order: rows = nodes        */
 #define CF_PREC_FP64 0  /* plain FP64 PCG                                    */
-#define CF_PREC_MIXED 1 /* FP32 inner CG on the Jacobi-scaled matrix, FP64
-                           residual replacement + solution accumulation      */
+#define CF_PREC_MIXED 1 /* NB3: FP64 flexible CG preconditioned by an FP32
+                           Chebyshev polynomial of the Jacobi-scaled matrix
+                           (FP32 stored values).  CSR: FP32 inner CG on the
+                           Jacobi-scaled matrix with FP64 residual
+                           replacement + solution accumulation              */
+#define CF_PREC_MIXED16 2 /* as CF_PREC_MIXED, the polynomial's matrix copy
+                           stored in FP16 (FP32 arithmetic)                  */
 #define CF_PC_JACOBI 0
 #define CF_PC_BLOCK3 1  /* 3x3 node-block Jacobi (NB3 only, FP64 only)       */
 
