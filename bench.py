@@ -55,6 +55,11 @@ def parse():
     ap.add_argument("--workload", default="tet10-160x20x20", choices=sorted(WORKLOADS))
     ap.add_argument("--precond64", default="jacobi", choices=["jacobi", "block3"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--paths", default="fp64,matfree,mixed,mixed16",
+                    help="comma-separated solver paths to time (fp64 always runs first)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="before the workload, copy a 256 MiB device buffer 3 times "
+                         "(reference kernel for HBM byte counters)")
     ap.add_argument("--cpu-only", action="store_true", help=argparse.SUPPRESS)
     return ap.parse_args()
 
@@ -80,6 +85,22 @@ def cpu_baseline():
                        f"at {CPU_SAMPLE[0]}x{CPU_SAMPLE[1]}x{CPU_SAMPLE[2]} cells ({n} DOF): "
                        f"assemble {t2 - t1:.2f} s, BC+solve {t3 - t2:.2f} s; SuperLU fill "
                        "grows superlinearly, so the full-size rate is lower")}
+
+
+def measured_traffic(workload, path):
+    """HBM bytes per launch of the path's operator from the committed counter
+    summary (two rocprofv3 --pmc passes, FETCH_SIZE and WRITE_SIZE, scaled by
+    a 256 MiB copy), or None when none was taken for this workload."""
+    f = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload}.json")
+    if not os.path.exists(f):
+        return None
+    with open(f) as fh:
+        d = json.load(fh)
+    if path == "matfree":
+        return d.get("matfree_operator_traffic_bytes")
+    kern = {"fp64": "spmv_sell<double, double, double>", "mixed": "cheb_step<float>",
+            "mixed16": "cheb_step<_Float16>"}.get(path)
+    return d["kernels"].get(kern, {}).get("traffic_bytes")
 
 
 def run_cpu_baseline_subprocess():
@@ -133,6 +154,14 @@ def main():
     lam, mu = lame(E_MOD, NU)
     torch.cuda.synchronize()
 
+    if args.copy_probe:
+        src = torch.ones(32 << 20, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    paths = set(args.paths.split(","))
     maxit = {"fp64": 200000, "mixed": 200000, "mixed16": 200000}
     offs_perm = {}
 
@@ -181,17 +210,20 @@ def main():
                        "op": "spmv_sell<double,double,double>", "op_avg_ms": prof[0],
                        "op_launches_sampled": prof[1], "op_bytes": bytes_spmv64}
     maxit["mixed"] = maxit["mixed16"] = 4 * st["iterations"]   # bound the mixed attempts
-    ms, (x_mf, st, _), prof = timed("matfree", args.precond64)
-    # conn 40 + geo 80 + element-force write/read 2 x 240 + incidence 40 B per element,
-    # offsets 4 B per node, x read once + y written once (16 B per DOF)
-    bytes_mf = 640 * n_elem + 4 * (n_nodes + 1) + 16 * n_dof
-    results["matfree"] = {"ms_per_step": ms, "iterations": st["iterations"],
-                          "rel_residual": st["rel_residual"], "precond": st["precond"],
-                          "op": "mf_t10_elem + mf_gather", "op_avg_ms": prof[0],
-                          "op_launches_sampled": prof[1], "op_bytes": bytes_mf,
-                          "rel_diff_vs_fp64": float(torch.linalg.vector_norm(x_mf - x_64)
-                                                    / torch.linalg.vector_norm(x_64))}
+    if "matfree" in paths:
+        ms, (x_mf, st, _), prof = timed("matfree", args.precond64)
+        # conn 40 + geo 80 + element-force write/read 2 x 240 + incidence 40 B per element,
+        # offsets 4 B per node, x read once + y written once (16 B per DOF)
+        bytes_mf = 640 * n_elem + 4 * (n_nodes + 1) + 16 * n_dof
+        results["matfree"] = {"ms_per_step": ms, "iterations": st["iterations"],
+                              "rel_residual": st["rel_residual"], "precond": st["precond"],
+                              "op": "mf_t10_elem + mf_gather", "op_avg_ms": prof[0],
+                              "op_launches_sampled": prof[1], "op_bytes": bytes_mf,
+                              "rel_diff_vs_fp64": float(torch.linalg.vector_norm(x_mf - x_64)
+                                                        / torch.linalg.vector_norm(x_64))}
     for path, vbytes in (("mixed", 36), ("mixed16", 18)):
+        if path not in paths:
+            continue
         try:
             ms, (x_mx, st, _), prof = timed(path, "jacobi")
         except (N.SolverNotConverged, N.SolverBreakdown) as exc:
@@ -222,6 +254,7 @@ def main():
     tip = float(u[2::3][mesh.boundaries["xmax"]].mean())
     rb = results[best]
     achieved = rb["op_gbs"]
+    traffic = measured_traffic(args.workload, best)
     line = {
         "metric": "Tet10 elasticity DOF/s (assemble+PCG to 1e-10), mixed vs FP64 on 1 GPU",
         "value": value,
@@ -242,10 +275,11 @@ def main():
         "paths": results,
         "speedup_mixed_vs_fp64": {
             k: results["fp64"]["ms_per_step"] / results[k]["ms_per_step"]
-            for k in ("mixed", "mixed16") if "ms_per_step" in results[k]},
+            for k in ("mixed", "mixed16") if "ms_per_step" in results.get(k, {})},
         "tip_uz": tip,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                     "traffic": traffic,
                      "kernel": rb["op"], "avg_ms": rb["op_avg_ms"],
                      "launches_sampled": rb["op_launches_sampled"],
                      "algorithmic_bytes": rb["op_bytes"]},
