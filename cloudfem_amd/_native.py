@@ -64,6 +64,7 @@ EXPORTED = (
     "cf_dirichlet_count", "cf_dirichlet_fill", "cf_axpby", "cf_copy_masked",
     "cf_element_stress", "cf_spmv_profile", "cf_adjoint_dk", "cf_mf_t10_workspace_bytes",
     "cf_mf_t10_apply", "cf_pcg_solve_mf_t10", "cf_invert_permutation",
+    "cf_batch_workspace_bytes", "cf_batch_autotune", "cf_batch_theta_run",
 )
 
 
@@ -89,6 +90,15 @@ class CfPcgStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32),
                 ("residual_updates", ctypes.c_int32),
                 ("rel_residual", ctypes.c_double), ("b_norm", ctypes.c_double)]
+
+
+class CfBatchStats(ctypes.Structure):
+    _fields_ = [("n_steps", ctypes.c_int32), ("lanes", ctypes.c_int32),
+                ("rows_per_tile", ctypes.c_int32), ("step_iterations", ctypes.c_int32),
+                ("graph_replays", ctypes.c_int32), ("extra_replays", ctypes.c_int32),
+                ("iterations", ctypes.c_int64), ("sweeps", ctypes.c_int64),
+                ("worst_step", ctypes.c_int32), ("failed", ctypes.c_int32),
+                ("max_rel_residual", ctypes.c_double)]
 
 
 _lib = None
@@ -136,6 +146,12 @@ _SIGS = {
     "cf_element_stress": ([_P, _I64, _I32, _P, _P, _P, _D, _D, _P, _P, _P], _I32),
     "cf_pcg_solve": ([ctypes.POINTER(CfMatrix), _P, _P, _P, _I32, _I32, _D, _I32, _P,
                       ctypes.c_size_t, ctypes.POINTER(CfPcgStats), _P], _I32),
+    "cf_batch_workspace_bytes": ([_I32, _P, _I64, _I32, _SZ], _I32),
+    "cf_batch_autotune": ([_I32, _P, _P, _P, _P, _I64, _I32, ctypes.POINTER(_I32),
+                           ctypes.POINTER(_I32), _P, _P, ctypes.c_size_t, _P], _I32),
+    "cf_batch_theta_run": ([_I32, _P, _P, _P, _I64, _P, _P, _P, _D, _P, _P, _I32, _I32, _D,
+                            _I32, _I32, _I32, _P, ctypes.c_size_t,
+                            ctypes.POINTER(CfBatchStats), _P], _I32),
 }
 
 

@@ -27,25 +27,36 @@ def _exterior(tris, n_nodes):
     return counts[inv.ravel()] == 1
 
 
+def _exterior_faces(mesh, local_faces, width):
+    """All exterior faces (occurrence order) of the mesh, memoised on the mesh
+    object for repeated calls with different boundary names."""
+    els = np.asarray(mesh.elements)
+    key = (width, id(mesh.elements), els.shape, mesh.n_nodes)
+    cache = getattr(mesh, "_cf_exterior", None)
+    if cache is not None and cache[0] == key:
+        return cache[1]
+    full = np.stack([els[:, list(f)] for f in local_faces], axis=1).reshape(-1, width)
+    ext = full[_exterior(full[:, :3], mesh.n_nodes)]
+    try:
+        mesh._cf_exterior = (key, ext)
+    except AttributeError:
+        pass
+    return ext
+
+
 def boundary_faces(mesh, name):
     """(F, 3) P1 faces of boundary `name` (KeyError if unknown)."""
     ids = np.asarray(mesh.boundaries[name], dtype=int)
-    els = np.asarray(mesh.elements)[:, :4]
-    n = mesh.n_nodes
-    tris = np.stack([els[:, list(f)] for f in P1_FACES], axis=1).reshape(-1, 3)
-    member = np.zeros(n, dtype=bool)
+    tris = _exterior_faces(mesh, P1_FACES, 3)
+    member = np.zeros(mesh.n_nodes, dtype=bool)
     member[ids] = True
-    keep = _exterior(tris, n) & member[tris].all(axis=1)
-    return tris[keep].astype(int).reshape(-1, 3)
+    return tris[member[tris].all(axis=1)].astype(int).reshape(-1, 3)
 
 
 def boundary_faces10(mesh, name):
     """(F, 6) Tet10 faces [c0, c1, c2, m01, m12, m20] of boundary `name`."""
     ids = np.asarray(mesh.boundaries[name], dtype=int)
-    els = np.asarray(mesh.elements)
-    n = mesh.n_nodes
-    full = np.stack([els[:, list(c + m)] for c, m in T10_FACES], axis=1).reshape(-1, 6)
-    member = np.zeros(n, dtype=bool)
+    full = _exterior_faces(mesh, [c + m for c, m in T10_FACES], 6)
+    member = np.zeros(mesh.n_nodes, dtype=bool)
     member[ids] = True
-    keep = _exterior(full[:, :3], n) & member[full[:, :3]].all(axis=1)
-    return full[keep].astype(int).reshape(-1, 6)
+    return full[member[full[:, :3]].all(axis=1)].astype(int).reshape(-1, 6)

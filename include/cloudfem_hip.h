@@ -321,6 +321,51 @@ int cf_invert_permutation(int64_t n, const uint32_t* perm, uint32_t* inv,
 int cf_spmv_profile(int32_t enable, double* avg_ms_fp64, int64_t* n_fp64,
                     double* avg_ms_fp32, int64_t* n_fp32);
 
+/* ---- batched theta-method heat (many meshes, one block-diagonal CSR).
+ *      Replaces the per-mesh loop of transient_heat_tet (tet_heat.py:319-396:
+ *      (C + th h K) T+ = (C - (1-th) h K) T + h F, factorized once) for a
+ *      batch of meshes without Dirichlet nodes: every launch covers all
+ *      systems, CG scalars and stop tests are per system (tol relative to
+ *      each system's step rhs), one time step is a replayed hipGraph.
+ *  sys_rows (host, n_sys + 1): first row of each system, sys_rows[n_sys] =
+ *  rows.  A_vals / R_vals: C + th h K and C - (1-th) h K on one pattern.
+ *  T (device, rows): initial state in, final state out.  hist (device,
+ *  nullable): (n_steps + 1) x rows history, row 0 = the initial state.
+ *  precision CF_PREC_FP64 (Jacobi CG) or CF_PREC_MIXED (FP32 CG on S A S
+ *  inside FP64 iterative refinement).  lanes (4, 8, 16) per row and
+ *  rows_per_tile come from cf_batch_autotune.                              */
+typedef struct cf_batch_stats {
+  int32_t n_steps;
+  int32_t lanes, rows_per_tile;  /* launch configuration used              */
+  int32_t step_iterations;  /* CG iterations (mixed: sweeps) in the step graph */
+  int32_t graph_replays;    /* hipGraph launches (step + iteration graphs)  */
+  int32_t extra_replays;    /* iteration-graph launches after a step graph  */
+  int64_t iterations;       /* CG iterations summed over systems and steps
+                               (mixed: FP32 inner iterations)              */
+  int64_t sweeps;           /* mixed: FP64 refinement sweeps, summed        */
+  int32_t worst_step;       /* most iterations (sweeps) one system needed   */
+  int32_t failed;           /* system-steps that did not converge           */
+  double max_rel_residual;  /* largest final ||r|| / ||b|| over systems     */
+} cf_batch_stats;
+
+int cf_batch_workspace_bytes(int32_t n_sys, const int32_t* sys_rows, int64_t nnz,
+                             int32_t precision, size_t* bytes);
+/* Times y = A x on the whole batch for the 9 (lanes, rows per tile)
+ * configurations (reps launches each) and returns the fastest; us_out
+ * (nullable, 9 floats) receives every timing, lanes-major.  No reference
+ * counterpart: it is the launch-shape tuner of the batched path.         */
+int cf_batch_autotune(int32_t n_sys, const int32_t* sys_rows, const int32_t* indptr,
+                      const int32_t* indices, const double* values, int64_t nnz,
+                      int32_t reps, int32_t* lanes_out, int32_t* rows_out,
+                      float* us_out, void* ws, size_t ws_bytes, void* stream);
+int cf_batch_theta_run(int32_t n_sys, const int32_t* sys_rows, const int32_t* indptr,
+                       const int32_t* indices, int64_t nnz, const double* A_vals,
+                       const double* R_vals, const double* F, double h, double* T,
+                       double* hist, int32_t n_steps, int32_t precision, double tol,
+                       int32_t maxit, int32_t lanes, int32_t rows_per_tile,
+                       void* ws, size_t ws_bytes, cf_batch_stats* stats,
+                       void* stream);
+
 #ifdef __cplusplus
 }
 #endif
