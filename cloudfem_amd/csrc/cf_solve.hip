@@ -54,6 +54,7 @@ struct PcgState {
   int32_t maxit;
   int32_t halt;   // Status
   uint32_t ticket;
+  double rzs[2];  // r.z by iteration parity (the fused CG kernels read one, write the other)
 };
 
 // ------------------------------------------------------------- reductions
@@ -443,6 +444,104 @@ cg_direction(int64_t n_units, const double* __restrict__ r, double* __restrict__
   if (state->halt) return;
   constexpr int W = (PC == CF_PC_BLOCK3) ? 3 : 1;
   const double beta = state->beta;
+  for (int64_t u = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; u < n_units;
+       u += int64_t(gridDim.x) * blockDim.x) {
+    double z[3];
+    precond<PC>(dinv, r, u, z);
+#pragma unroll
+    for (int c = 0; c < W; ++c) p[W * u + c] = z[c] + beta * p[W * u + c];
+  }
+}
+
+// Fused CG scalars.  Each block of the kernel after a reduction folds that
+// reduction's block partials itself (same order in every block, so every
+// block computes the same value) instead of waiting for a one-block
+// `finalize` launch; block 0 alone writes the scalars back.  r.z lives in
+// rzs[slot] / rzs[slot ^ 1] so no block reads a value another block of the
+// same kernel is overwriting.
+template <int NV>
+__device__ __forceinline__ void fold_partials(const double* __restrict__ parts, int n,
+                                              double (&v)[NV]) {
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = 0.0;
+  for (int b = threadIdx.x; b < n; b += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] += parts[k * n + b];
+  block_sum<NV>(v);
+}
+
+// alpha = rzs[slot] / p.Ap (p.Ap from the product's n_pa partials); then as
+// cg_update, writing (r.r, r.z) partials to pb.
+template <int PC>
+__global__ void __launch_bounds__(kBlock)
+cg_update_fused(int64_t n_units, double* __restrict__ x, double* __restrict__ r,
+                const double* __restrict__ p, const double* __restrict__ Ap,
+                const double* __restrict__ dinv, PcgState* state,
+                const double* __restrict__ pa, int n_pa, double* __restrict__ pb, int slot) {
+  if (state->halt) return;
+  double f[1];
+  fold_partials<1>(pa, n_pa, f);
+  const double pAp = f[0];
+  const bool ok = pAp > 0.0 && isfinite(pAp);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    state->pAp = pAp;
+    if (ok)
+      state->alpha = state->rzs[slot] / pAp;
+    else
+      state->halt = kBreakdown;
+  }
+  if (!ok) return;
+  constexpr int W = (PC == CF_PC_BLOCK3) ? 3 : 1;
+  const double alpha = state->rzs[slot] / pAp;
+  double rr = 0.0, rz = 0.0;
+  for (int64_t u = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; u < n_units;
+       u += int64_t(gridDim.x) * blockDim.x) {
+#pragma unroll
+    for (int c = 0; c < W; ++c) {
+      const int64_t d = W * u + c;
+      x[d] += alpha * p[d];
+      const double rn = r[d] - alpha * Ap[d];
+      r[d] = rn;
+      rr += rn * rn;
+    }
+    double z[3];
+    precond<PC>(dinv, r, u, z);
+#pragma unroll
+    for (int c = 0; c < W; ++c) rz += r[W * u + c] * z[c];
+  }
+  double v[2] = {rr, rz};
+  block_partials<2>(v, pb);
+}
+
+// (r.r, r.z) from cg_update_fused's partials -> stop test and beta; block 0
+// records iter / rr / rzs[slot ^ 1] / halt; p = z + beta p.
+template <int PC>
+__global__ void __launch_bounds__(kBlock)
+cg_direction_fused(int64_t n_units, const double* __restrict__ r, double* __restrict__ p,
+                   const double* __restrict__ dinv, PcgState* state,
+                   const double* __restrict__ pb, int n_pb, int slot) {
+  if (state->halt) return;
+  double v[2];
+  fold_partials<2>(pb, n_pb, v);
+  const double rr = v[0], rz = v[1];
+  const bool finite = isfinite(rr) && isfinite(rz);
+  const bool done = !finite || rr <= state->stop2;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const int32_t it = state->iter + 1;
+    state->iter = it;
+    state->rr = rr;
+    state->rzs[slot ^ 1] = rz;
+    state->rz = rz;
+    if (!finite)
+      state->halt = kBreakdown;
+    else if (rr <= state->stop2)
+      state->halt = kConverged;
+    else if (it >= state->maxit)
+      state->halt = kMaxit;
+  }
+  if (done) return;
+  constexpr int W = (PC == CF_PC_BLOCK3) ? 3 : 1;
+  const double beta = rz / state->rzs[slot];
   for (int64_t u = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; u < n_units;
        u += int64_t(gridDim.x) * blockDim.x) {
     double z[3];
@@ -1010,6 +1109,7 @@ constexpr int kSliceRows = 16;
 constexpr int kPhases = 64 / kSliceRows;
 constexpr int kSigma = 2048;
 constexpr int kSellUnroll = 4;
+constexpr int kSlicesPerWave = 4;  // consecutive slices of one wave (fewer partials)
 
 // Sort key: window index above, descending row length below (exact, so the
 // sorted slice lengths never exceed the unsorted ones the workspace is sized by).
@@ -1147,9 +1247,9 @@ spmv_sell(int64_t n, int64_t n_slices, const uint32_t* __restrict__ rows,
           const VT* __restrict__ sval, const XT* __restrict__ x, XT* __restrict__ y,
           const uint8_t* __restrict__ fixed, PcgState* state, double* __restrict__ partials) {
   if (state && state->halt) return;
-  const int64_t sl = tile_index(blockIdx.x) * (kBlock / 64) + threadIdx.x / 64;
+  const int64_t sl0 = (tile_index(blockIdx.x) * (kBlock / 64) + threadIdx.x / 64) * kSlicesPerWave;
   double dot = 0.0;
-  if (sl < n_slices) {
+  for (int64_t sl = sl0; sl < sl0 + kSlicesPerWave && sl < n_slices; ++sl) {
     AT o[3];
     const int64_t r = sell_rows<VT, XT, AT>(n, sl, rows, koff, scol, sval, x, o);
     if (r >= 0) {
@@ -1178,19 +1278,20 @@ cheb_step(int64_t n, int64_t n_slices, const uint32_t* __restrict__ rows,
           float* __restrict__ w, float* __restrict__ z, float cd, float cw,
           const PcgState* state) {
   if (state->halt) return;
-  const int64_t sl = tile_index(blockIdx.x) * (kBlock / 64) + threadIdx.x / 64;
-  if (sl >= n_slices) return;
-  float o[3];
-  const int64_t r = sell_rows<VT, float, float>(n, sl, rows, koff, scol, sval, d, o);
-  if (r < 0) return;
+  const int64_t sl0 = (tile_index(blockIdx.x) * (kBlock / 64) + threadIdx.x / 64) * kSlicesPerWave;
+  for (int64_t sl = sl0; sl < sl0 + kSlicesPerWave && sl < n_slices; ++sl) {
+    float o[3];
+    const int64_t r = sell_rows<VT, float, float>(n, sl, rows, koff, scol, sval, d, o);
+    if (r < 0) continue;
 #pragma unroll
-  for (int cc = 0; cc < 3; ++cc) {
-    const int64_t i = 3 * r + cc;
-    const float wn = w[i] - o[cc];
-    w[i] = wn;
-    const float dd = cd * d[i] + cw * wn;
-    dn[i] = dd;
-    z[i] += dd;
+    for (int cc = 0; cc < 3; ++cc) {
+      const int64_t i = 3 * r + cc;
+      const float wn = w[i] - o[cc];
+      w[i] = wn;
+      const float dd = cd * d[i] + cw * wn;
+      dn[i] = dd;
+      z[i] += dd;
+    }
   }
 }
 
@@ -1267,6 +1368,7 @@ finalize(int mode, int G, const double* __restrict__ partials, PcgState* state,
     case kFinStart:
       state->rr = v[0];
       state->rz = v[1];
+      state->rzs[0] = v[1];
       break;
     case kFinMxStart:
       state->rz = v[0];
@@ -1401,6 +1503,7 @@ struct Layout {
   double* Ap;
   double* dinv;
   double* partials;
+  double* partials_b;  // (r.r, r.z) partials of cg_update_fused
   PcgState* state;
   int32_t* bad;
   // mixed only
@@ -1437,7 +1540,17 @@ int pcg_grid() {
 // blocks for `tiles` tiles (see tile_index)
 int64_t tile_grid(int64_t tiles) { return tiles < 1 ? 1 : tiles; }
 
-int64_t sell_grid(int64_t n_slices) { return tile_grid((n_slices + 3) / 4); }
+int64_t sell_grid(int64_t n_slices) {
+  const int64_t per_block = (kBlock / 64) * kSlicesPerWave;
+  return tile_grid((n_slices + per_block - 1) / per_block);
+}
+
+// grid of the fused CG update / direction kernels: each of their blocks folds
+// the previous reduction's partials, so fewer, fuller blocks
+int pcg_vec_grid() {
+  int g = cf::cu_count() * 4;
+  return g - g % kXcd;
+}
 
 int bits_for(int64_t n) {
   int b = 1;
@@ -1476,6 +1589,7 @@ Layout layout(const cf_matrix* A, int precision, int precond, void* base, int64_
       nb3 ? std::max<int64_t>(pcg_grid(), sell_grid((A->n_rows + kSliceRows - 1) / kSliceRows))
           : pcg_grid();
   L.partials = c.take<double>(4 * size_t(parts));
+  L.partials_b = c.take<double>(2 * size_t(pcg_vec_grid()));
   L.state = c.take<PcgState>(1);
   L.bad = c.take<int32_t>(1);
   const bool mixed = precision != CF_PREC_FP64;
@@ -1648,17 +1762,22 @@ int pcg_fp64_core(Apply&& apply, int apply_parts, int64_t n, int64_t units, cons
   int batch = 8;
   double prev_rr = h.rr;
   int prev_it = 0;
+  const int GV = pcg_vec_grid();
+  int64_t enq = 0;
+  // three launches per iteration: product (+ p.Ap partials), update (folds
+  // p.Ap itself), direction (folds r.r, r.z itself)
   while (h.halt == kRunning) {
-    for (int t = 0; t < batch; ++t) {
+    for (int t = 0; t < batch; ++t, ++enq) {
       int tok;
       prof_begin(0, h.iter + t, s, &tok);
       if ((rc = apply(L.p, L.Ap, L.state, L.partials))) return rc;
       prof_end(tok, s);
-      if ((rc = fin(kFinAlpha, L, s, nullptr, 0.0, apply_parts))) return rc;
-      cg_update<PC><<<G, kBlock, 0, s>>>(units, x, L.r, L.p, L.Ap, L.dinv, L.state, L.partials);
+      const int slot = int(enq & 1);
+      cg_update_fused<PC><<<GV, kBlock, 0, s>>>(units, x, L.r, L.p, L.Ap, L.dinv, L.state,
+                                                L.partials, apply_parts, L.partials_b, slot);
       CF_LAUNCHED();
-      if ((rc = fin(kFinCgUpdate, L, s))) return rc;
-      cg_direction<PC><<<G, kBlock, 0, s>>>(units, L.r, L.p, L.dinv, L.state);
+      cg_direction_fused<PC><<<GV, kBlock, 0, s>>>(units, L.r, L.p, L.dinv, L.state,
+                                                   L.partials_b, GV, slot);
       CF_LAUNCHED();
     }
     if ((rc = read_state(L.state, &h, s))) return rc;
@@ -2168,6 +2287,7 @@ size_t mf_layout(int64_t n_nodes, int64_t n_elem, int precond, void* base, Layou
   L->Ap = c.take<double>(n);
   L->dinv = c.take<double>(precond == CF_PC_BLOCK3 ? 3 * n : n);
   L->partials = c.take<double>(4 * size_t(std::max<int64_t>(pcg_grid(), mf_gather_grid(n_nodes))));
+  L->partials_b = c.take<double>(2 * size_t(pcg_vec_grid()));
   L->state = c.take<PcgState>(1);
   L->bad = c.take<int32_t>(1);
   *fe = c.take<double>(30 * n_elem);
