@@ -57,7 +57,7 @@ def parse():
     ap.add_argument("--workload", default="tet10-160x20x20", choices=sorted(WORKLOADS))
     ap.add_argument("--precond64", default="jacobi", choices=["jacobi", "block3"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--paths", default="fp64,matfree,mixed,mixed16",
+    ap.add_argument("--paths", default="fp64,matfree,matfree_lds,mixed,mixed16",
                     help="comma-separated solver paths to time (fp64 always runs first)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="before the workload, copy a 256 MiB device buffer 3 times "
@@ -155,8 +155,9 @@ def measured_traffic(workload, path):
         return None
     with open(f) as fh:
         d = json.load(fh)
-    if path == "matfree":
-        return d.get("matfree_operator_traffic_bytes")
+    if path in ("matfree", "matfree_lds"):
+        return d.get("matfree_operator_traffic_bytes" if path == "matfree"
+                     else "matfree_lds_traffic_bytes")
     kern = {"fp64": "spmv_sell<double, double, double>", "mixed": "cheb_step<float>",
             "mixed16": "cheb_step<_Float16>"}.get(path)
     return d["kernels"].get(kern, {}).get("traffic_bytes")
@@ -224,7 +225,7 @@ def main():
         del src, dst
     paths = set(args.paths.split(","))
     maxit = {"fp64": 200000, "mixed": 200000, "mixed16": 200000}
-    offs_perm = {}
+    plan_counts = [0, 0, 0]   # pairs, shared pairs, shared nodes of the chunked operator
 
     def step(path, precond):
         dm = DeviceMesh.from_device(xyz[0], xyz[1], xyz[2], conn)
@@ -236,6 +237,10 @@ def main():
         if path == "matfree":
             stats = pcg_matfree_t10(dm, lam, mu, rhs, x, fixed, precond=precond, tol=TOL,
                                     maxit=maxit["fp64"])
+        elif path == "matfree_lds":
+            stats = pcg_matfree_t10(dm, lam, mu, rhs, x, fixed, precond=precond, tol=TOL,
+                                    maxit=maxit["fp64"], chunked=True)
+            plan_counts[:] = [int(c) for c in dm.matfree_plan()[1]]
         else:
             A = DeviceMatrix(N.FMT_NB3, dm.pattern(), vals)
             stats = pcg(A, rhs, x, fixed, precision=path, precond=precond, tol=TOL,
@@ -273,15 +278,31 @@ def main():
     maxit["mixed"] = maxit["mixed16"] = 4 * st["iterations"]   # bound the mixed attempts
     if "matfree" in paths:
         ms, (x_mf, st, _), prof = timed("matfree", args.precond64)
-        # conn 40 + geo 80 + element-force write/read 2 x 240 + incidence 40 B per element,
-        # offsets 4 B per node, x read once + y written once (16 B per DOF)
-        bytes_mf = 640 * n_elem + 4 * (n_nodes + 1) + 16 * n_dof
+        # two-pass operator: conn 40 + geometry 80 + slot 40 + force write and
+        # read 2 x 240 B per element, offsets, x and y
         results["matfree"] = {"ms_per_step": ms, "iterations": st["iterations"],
                               "rel_residual": st["rel_residual"], "precond": st["precond"],
-                              "op": "mf_t10_elem + mf_gather", "op_avg_ms": prof[0],
-                              "op_launches_sampled": prof[1], "op_bytes": bytes_mf,
+                              "op": "mf_t10_elem + mf_gather_sorted", "op_avg_ms": prof[0],
+                              "op_launches_sampled": prof[1],
+                              "op_bytes": 640 * n_elem + 4 * (n_nodes + 1) + 16 * n_dof,
                               "rel_diff_vs_fp64": float(torch.linalg.vector_norm(x_mf - x_64)
                                                         / torch.linalg.vector_norm(x_64))}
+    if "matfree_lds" in paths:
+        ms, (x_ml, st, _), prof = timed("matfree_lds", args.precond64)
+        # chunked operator, per element: conn 40 + geometry 80 + local slot ids 20 B;
+        # per (chunk, node) pair: node, slot offset, output id 12 B; per shared pair a
+        # 24 B partial written and read back; per shared node 8 B of list; x read
+        # once and y written once (16 B per DOF); chunk offsets
+        n_pairs, n_shp, n_shn = plan_counts
+        bytes_ml = (140 * n_elem + 12 * n_pairs + 48 * n_shp + 8 * n_shn + 16 * n_dof
+                    + 4 * ((n_elem + 255) // 256 + 1))
+        results["matfree_lds"] = {"ms_per_step": ms, "iterations": st["iterations"],
+                                  "rel_residual": st["rel_residual"], "precond": st["precond"],
+                                  "op": "mf_chunk_apply + mf_shared_gather",
+                                  "op_avg_ms": prof[0], "plan_counts": list(plan_counts),
+                                  "op_launches_sampled": prof[1], "op_bytes": bytes_ml,
+                                  "rel_diff_vs_fp64": float(torch.linalg.vector_norm(x_ml - x_64)
+                                                            / torch.linalg.vector_norm(x_64))}
     for path, vbytes in (("mixed", 36), ("mixed16", 18)):
         if path not in paths:
             continue

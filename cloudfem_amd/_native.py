@@ -65,6 +65,8 @@ EXPORTED = (
     "cf_element_stress", "cf_spmv_profile", "cf_adjoint_dk", "cf_mf_t10_workspace_bytes",
     "cf_mf_t10_apply", "cf_pcg_solve_mf_t10", "cf_invert_permutation",
     "cf_batch_workspace_bytes", "cf_batch_autotune", "cf_batch_theta_run",
+    "cf_mf_t10_plan_bytes", "cf_mf_t10_plan_build", "cf_mf_t10_plan_workspace_bytes",
+    "cf_mf_t10_plan_apply", "cf_pcg_solve_mf_t10_plan",
 )
 
 
@@ -147,6 +149,15 @@ _SIGS = {
     "cf_pcg_solve": ([ctypes.POINTER(CfMatrix), _P, _P, _P, _I32, _I32, _D, _I32, _P,
                       ctypes.c_size_t, ctypes.POINTER(CfPcgStats), _P], _I32),
     "cf_batch_workspace_bytes": ([_I32, _P, _I64, _I32, _SZ], _I32),
+    "cf_mf_t10_plan_bytes": ([_I64, _SZ], _I32),
+    "cf_mf_t10_plan_build": ([_I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_size_t, _P,
+                              _P], _I32),
+    "cf_mf_t10_plan_workspace_bytes": ([_I64, _I64, _P, _I32, _SZ], _I32),
+    "cf_mf_t10_plan_apply": ([_I64, _I64, _P, _P, _D, _D, _P, _P, _P, _P, ctypes.c_size_t, _P],
+                             _I32),
+    "cf_pcg_solve_mf_t10_plan": ([_I64, _I64, _P, _P, _P, _P, _P, _P, _D, _D, _P, _P, _P, _I32,
+                                  _D, _I32, _P, ctypes.c_size_t, ctypes.POINTER(CfPcgStats), _P],
+                                 _I32),
     "cf_batch_autotune": ([_I32, _P, _P, _P, _P, _I64, _I32, ctypes.POINTER(_I32),
                            ctypes.POINTER(_I32), _P, _P, ctypes.c_size_t, _P], _I32),
     "cf_batch_theta_run": ([_I32, _P, _P, _P, _I64, _P, _P, _P, _D, _P, _P, _I32, _I32, _D,

@@ -827,8 +827,71 @@ __constant__ cf::Tet10Tables kT10m = cf::build_tet10_tables();
 // the 5 Keast points: H_q = grad u = (sum_a x_a d_aq^T) Jinv,
 // sigma_q = lam tr(H) I + mu (H + H^T), f_a += V w_q sigma_q Jinv^T d_aq.
 // Exact for affine elements, same integrand and rule as
-// elastic_tet10.py:164-170, ~1.2k FMA per element.  fe[e][a][c] is summed
-// into the nodes by mf_gather in fixed incidence order (no atomics).
+// elastic_tet10.py:164-170, ~1.2k FMA per element; out(a, f_a) receives the
+// 10 nodal forces.  The tables stay in constant memory: every lane reads the
+// same entry, served as scalar operands.  (Compile-time literals instead
+// fold the zero gradient entries but cost more registers and moves: 50.7 vs
+// 46.5 us per application at 384,000 elements.)
+template <class Out>
+__device__ __forceinline__ void t10_forces(const int32_t* __restrict__ cn,
+                                           const double* __restrict__ g, double lam, double mu,
+                                           const double* __restrict__ x, Out&& out) {
+  double J[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) J[r][c] = g[3 * r + c];
+  const double V = g[9];
+  double xa[10][3];
+#pragma unroll
+  for (int a = 0; a < 10; ++a) {
+    const int64_t nd = cn[a];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) xa[a][c] = x[3 * nd + c];
+  }
+  double f[10][3];
+#pragma unroll
+  for (int a = 0; a < 10; ++a)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f[a][c] = 0.0;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    double U[3][3] = {};
+#pragma unroll
+    for (int a = 0; a < 10; ++a)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) U[c][k] += xa[a][c] * kT10m.D[q][a][k];
+    double H[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) H[c][i] = U[c][0] * J[0][i] + U[c][1] * J[1][i] + U[c][2] * J[2][i];
+    const double ltr = lam * (H[0][0] + H[1][1] + H[2][2]);
+    const double vw = V * kT10m.w[q];
+    double P[3][3];  // P[c][k] = V w_q sum_i sigma[c][i] Jinv[k][i]
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double sg[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) sg[i] = mu * (H[c][i] + H[i][c]) + (c == i ? ltr : 0.0);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) P[c][k] = vw * (sg[0] * J[k][0] + sg[1] * J[k][1] + sg[2] * J[k][2]);
+    }
+#pragma unroll
+    for (int a = 0; a < 10; ++a)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        f[a][c] += kT10m.D[q][a][0] * P[c][0] + kT10m.D[q][a][1] * P[c][1] +
+                   kT10m.D[q][a][2] * P[c][2];
+  }
+#pragma unroll
+  for (int a = 0; a < 10; ++a) out(a, f[a]);
+}
+
+// fe[e][a][c] (or the node-sorted slot of (e, a)) = forces of element e,
+// summed into the nodes by mf_gather / mf_gather_sorted in fixed order.
 __global__ void __launch_bounds__(kBlock)
 mf_t10_elem(int64_t m, const int32_t* __restrict__ conn, const double* __restrict__ geo,
             double lam, double mu, const double* __restrict__ x,
@@ -837,70 +900,14 @@ mf_t10_elem(int64_t m, const int32_t* __restrict__ conn, const double* __restric
   if (state && state->halt) return;
   for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < m;
        e += int64_t(gridDim.x) * blockDim.x) {
-    const double* g = geo + e * 10;
-    double J[3][3];
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) J[r][c] = g[3 * r + c];
-    const double V = g[9];
-    double xa[10][3];
-#pragma unroll
-    for (int a = 0; a < 10; ++a) {
-      const int64_t nd = conn[e * 10 + a];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) xa[a][c] = x[3 * nd + c];
-    }
-    double f[10][3] = {};
-#pragma unroll
-    for (int q = 0; q < 5; ++q) {
-      double U[3][3] = {};
-#pragma unroll
-      for (int a = 0; a < 10; ++a)
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-          for (int k = 0; k < 3; ++k) U[c][k] += xa[a][c] * kT10m.D[q][a][k];
-      double H[3][3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int i = 0; i < 3; ++i) H[c][i] = U[c][0] * J[0][i] + U[c][1] * J[1][i] + U[c][2] * J[2][i];
-      const double ltr = lam * (H[0][0] + H[1][1] + H[2][2]);
-      const double vw = V * kT10m.w[q];
-      double P[3][3];  // P[c][k] = V w_q sum_i sigma[c][i] Jinv[k][i]
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        double sg[3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) sg[i] = mu * (H[c][i] + H[i][c]) + (c == i ? ltr : 0.0);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) P[c][k] = vw * (sg[0] * J[k][0] + sg[1] * J[k][1] + sg[2] * J[k][2]);
-      }
-#pragma unroll
-      for (int a = 0; a < 10; ++a)
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-          f[a][c] += kT10m.D[q][a][0] * P[c][0] + kT10m.D[q][a][1] * P[c][1] +
-                     kT10m.D[q][a][2] * P[c][2];
-    }
-    if (slot) {
-      // write each nodal force into its position in node-sorted incidence
-      // order, so the node sums read a contiguous range
-#pragma unroll
-      for (int a = 0; a < 10; ++a) {
-        double* o = fe + int64_t(slot[e * 10 + a]) * 3;
-        o[0] = f[a][0];
-        o[1] = f[a][1];
-        o[2] = f[a][2];
-      }
-    } else {
-      double* o = fe + e * 30;
-#pragma unroll
-      for (int a = 0; a < 10; ++a)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) o[3 * a + c] = f[a][c];
-    }
+    // with slots, each nodal force goes to its position in node-sorted
+    // incidence order, so the node sums read a contiguous range
+    t10_forces(conn + e * 10, geo + e * 10, lam, mu, x, [&](int a, const double (&f)[3]) {
+      double* o = slot ? fe + int64_t(slot[e * 10 + a]) * 3 : fe + e * 30 + 3 * a;
+      o[0] = f[0];
+      o[1] = f[1];
+      o[2] = f[2];
+    });
   }
 }
 
@@ -939,6 +946,252 @@ mf_gather_sorted(int64_t n_nodes, const int32_t* __restrict__ offs,
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const int64_t d = 3 * i + c;
+      if (fixed && fixed[d]) o[c] = 0.0;
+      y[d] = o[c];
+      if (state) dot += x[d] * o[c];
+    }
+  }
+  if (!state) return;
+  double v1[1] = {dot};
+  block_partials<1>(v1, partials);
+}
+
+// ---------------------------------------- matrix-free Tet10, chunked plan
+// Elements are ordered along a Morton curve of their centroids and cut into
+// chunks of kChunk consecutive elements, one chunk per workgroup.  The
+// workgroup computes its elements' forces into LDS and sums them per node
+// in fixed order: a node whose incident elements all lie in the chunk gets
+// its y written directly; a node shared with other chunks gets one partial
+// (3 doubles) per chunk, summed in chunk order by mf_shared_gather.  Against
+// the two-pass operator this removes the 480 B per element force round trip
+// through HBM; results are deterministic.
+constexpr int kChunk = kBlock;   // elements per chunk = threads per block
+
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+// Morton key of each element's corner centroid (10 bits per axis)
+__global__ void mfp_morton(int64_t m, const int32_t* __restrict__ conn,
+                           const double* __restrict__ x, const double* __restrict__ y,
+                           const double* __restrict__ z, double lx, double ly, double lz,
+                           double sx, double sy, double sz, uint32_t* __restrict__ keys,
+                           uint32_t* __restrict__ ids) {
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < m;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    double c[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int64_t nd = conn[e * 10 + a];
+      c[0] += x[nd];
+      c[1] += y[nd];
+      c[2] += z[nd];
+    }
+    const double lo[3] = {lx, ly, lz}, sc[3] = {sx, sy, sz};
+    uint32_t q[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double t = (0.25 * c[k] - lo[k]) * sc[k];
+      t = t < 0.0 ? 0.0 : (t > 1023.0 ? 1023.0 : t);
+      q[k] = uint32_t(t);
+    }
+    keys[e] = spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
+    ids[e] = uint32_t(e);
+  }
+}
+
+__global__ void mfp_permute(int64_t m, const uint32_t* __restrict__ order,
+                            const int32_t* __restrict__ conn, const double* __restrict__ geo,
+                            int32_t* __restrict__ conn_c, double* __restrict__ geo_c) {
+  for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < m * 10;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t e = t / 10, a = t - 10 * e;
+    const int64_t src = int64_t(order[e]) * 10 + a;
+    conn_c[t] = conn[src];
+    geo_c[t] = geo[src];
+  }
+}
+
+// slot t = 10 e + a of the chunk order -> key (chunk, node), value t
+__global__ void mfp_pair_keys(int64_t m10, const int32_t* __restrict__ conn_c,
+                              uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < m10;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const uint64_t chunk = uint64_t(t / (10 * kChunk));
+    keys[t] = (chunk << 32) | uint64_t(uint32_t(conn_c[t]));
+    vals[t] = uint32_t(t);
+  }
+}
+
+// head flags of runs of equal keys (or of keys below `limit` when not null)
+__global__ void mfp_heads(int64_t n, const uint64_t* __restrict__ keys,
+                          const int64_t* __restrict__ limit, int32_t* __restrict__ heads) {
+  const int64_t lim = limit ? *limit : n;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const bool in = i < lim;
+    const uint64_t k = keys[i] >> (limit ? 32 : 0);
+    heads[i] = (in && (i == 0 || k != (keys[i - 1] >> (limit ? 32 : 0)))) ? 1 : 0;
+  }
+}
+
+// pairs (chunk, node) from the sorted slot keys and their run index
+__global__ void mfp_pairs(int64_t m10, int64_t n_chunks, const uint64_t* __restrict__ keys,
+                          const uint32_t* __restrict__ vals, const int32_t* __restrict__ scan,
+                          uint16_t* __restrict__ slot_list, int32_t* __restrict__ pair_node,
+                          int32_t* __restrict__ pair_slot_off, int32_t* __restrict__ chunk_off,
+                          int64_t* __restrict__ counts) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < m10;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const uint64_t k = keys[i];
+    const int64_t chunk = int64_t(k >> 32);
+    const int32_t p = scan[i] - 1;
+    slot_list[i] = uint16_t(int64_t(vals[i]) - chunk * (10 * kChunk));
+    if (i == 0 || k != keys[i - 1]) {
+      pair_node[p] = int32_t(uint32_t(k));
+      pair_slot_off[p] = int32_t(i);
+      if (i == 0 || chunk != int64_t(keys[i - 1] >> 32)) chunk_off[chunk] = p;
+    }
+    if (i == m10 - 1) {
+      counts[0] = p + 1;
+      pair_slot_off[p + 1] = int32_t(m10);
+      chunk_off[n_chunks] = p + 1;
+    }
+  }
+}
+
+// A pair is shared when its node has incident elements outside the chunk.
+// keys = (node, chunk) for shared pairs, all-ones otherwise; pair_out = -1.
+__global__ void mfp_shared_keys(int64_t m10, const int64_t* __restrict__ counts,
+                                const uint64_t* __restrict__ slot_keys,
+                                const int32_t* __restrict__ pair_node,
+                                const int32_t* __restrict__ pair_slot_off,
+                                const int32_t* __restrict__ offs, uint64_t* __restrict__ keys,
+                                uint32_t* __restrict__ vals, int32_t* __restrict__ pair_out) {
+  const int64_t n_pairs = counts[0];
+  for (int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; p < m10;
+       p += int64_t(gridDim.x) * blockDim.x) {
+    uint64_t key = ~uint64_t(0);
+    if (p < n_pairs) {
+      const int32_t node = pair_node[p];
+      const int32_t s0 = pair_slot_off[p];
+      const int32_t cnt = pair_slot_off[p + 1] - s0;
+      const int32_t tot = offs[node + 1] - offs[node];
+      if (cnt != tot) key = (uint64_t(uint32_t(node)) << 32) | (slot_keys[s0] >> 32);
+      pair_out[p] = -1;
+    }
+    keys[p] = key;
+    vals[p] = uint32_t(p);
+  }
+}
+
+// counts[1] = number of shared pairs (keys sorted, all-ones last)
+__global__ void mfp_count_shared(int64_t m10, const uint64_t* __restrict__ keys,
+                                 int64_t* __restrict__ counts) {
+  for (int64_t r = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; r < m10;
+       r += int64_t(gridDim.x) * blockDim.x) {
+    const bool in = keys[r] != ~uint64_t(0);
+    if (in && (r == m10 - 1 || keys[r + 1] == ~uint64_t(0))) counts[1] = r + 1;
+  }
+}
+
+__global__ void mfp_shared_fill(int64_t m10, const uint64_t* __restrict__ keys,
+                                const uint32_t* __restrict__ vals,
+                                const int32_t* __restrict__ scan,
+                                int64_t* __restrict__ counts, int32_t* __restrict__ pair_out,
+                                int32_t* __restrict__ shared_node,
+                                int32_t* __restrict__ shared_off) {
+  const int64_t n_sh = counts[1];
+  for (int64_t r = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; r < n_sh;
+       r += int64_t(gridDim.x) * blockDim.x) {
+    pair_out[vals[r]] = int32_t(r);
+    const int32_t j = scan[r] - 1;
+    const uint32_t node = uint32_t(keys[r] >> 32);
+    if (r == 0 || node != uint32_t(keys[r - 1] >> 32)) {
+      shared_node[j] = int32_t(node);
+      shared_off[j] = int32_t(r);
+    }
+    if (r == n_sh - 1) {
+      counts[2] = j + 1;
+      shared_off[j + 1] = int32_t(n_sh);
+    }
+  }
+}
+
+// y = K x for one chunk per block (see above); p.Ap partials of the direct
+// nodes in partials[blockIdx].
+__global__ void __launch_bounds__(kBlock)
+mf_chunk_apply(int64_t m, const int32_t* __restrict__ conn_c, const double* __restrict__ geo_c,
+               double lam, double mu, const double* __restrict__ x,
+               const uint16_t* __restrict__ slot_list, const int32_t* __restrict__ chunk_off,
+               const int32_t* __restrict__ pair_node, const int32_t* __restrict__ pair_slot_off,
+               const int32_t* __restrict__ pair_out, const uint8_t* __restrict__ fixed,
+               double* __restrict__ y, double* __restrict__ part_buf, PcgState* state,
+               double* __restrict__ partials) {
+  if (state && state->halt) return;
+  // forces of the chunk, component-major: fl[(3 a + c) * kChunk + e_local]
+  __shared__ double fl[kChunk * 30];
+  const int64_t e = int64_t(blockIdx.x) * kChunk + threadIdx.x;
+  if (e < m)
+    t10_forces(conn_c + e * 10, geo_c + e * 10, lam, mu, x, [&](int a, const double (&f)[3]) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) fl[(3 * a + c) * kChunk + threadIdx.x] = f[c];
+    });
+  __syncthreads();
+  double dot = 0.0;
+  const int32_t p1 = chunk_off[blockIdx.x + 1];
+  for (int32_t p = chunk_off[blockIdx.x] + threadIdx.x; p < p1; p += kBlock) {
+    double o[3] = {0.0, 0.0, 0.0};
+    const int32_t s1 = pair_slot_off[p + 1];
+    for (int32_t k = pair_slot_off[p]; k < s1; ++k) {
+      const int sl = slot_list[k];  // e_local * 10 + a
+      const int el = sl / 10, a = sl - 10 * el;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) o[c] += fl[(3 * a + c) * kChunk + el];
+    }
+    const int32_t out = pair_out[p];
+    if (out >= 0) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) part_buf[3 * int64_t(out) + c] = o[c];
+    } else {
+      const int64_t node = pair_node[p];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int64_t d = 3 * node + c;
+        if (fixed && fixed[d]) o[c] = 0.0;
+        y[d] = o[c];
+        if (state) dot += x[d] * o[c];
+      }
+    }
+  }
+  if (!state) return;
+  double v1[1] = {dot};
+  block_partials<1>(v1, partials);
+}
+
+// y of the shared nodes: sum of their chunk partials in chunk order
+__global__ void __launch_bounds__(kBlock)
+mf_shared_gather(int64_t n_sn, const int32_t* __restrict__ shared_node,
+                 const int32_t* __restrict__ shared_off, const double* __restrict__ part_buf,
+                 const uint8_t* __restrict__ fixed, const double* __restrict__ x,
+                 double* __restrict__ y, PcgState* state, double* __restrict__ partials) {
+  if (state && state->halt) return;
+  double dot = 0.0;
+  const int64_t j = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (j < n_sn) {
+    double o[3] = {0.0, 0.0, 0.0};
+    for (int32_t r = shared_off[j]; r < shared_off[j + 1]; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) o[c] += part_buf[3 * int64_t(r) + c];
+    const int64_t node = shared_node[j];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t d = 3 * node + c;
       if (fixed && fixed[d]) o[c] = 0.0;
       y[d] = o[c];
       if (state) dot += x[d] * o[c];
@@ -2280,22 +2533,30 @@ int mf_check(const MfArgs& a) {
 // one node per thread, kBlock nodes per block
 int64_t mf_gather_grid(int64_t n_nodes) { return tile_grid((n_nodes + kBlock - 1) / kBlock); }
 
-size_t mf_layout(int64_t n_nodes, int64_t n_elem, int precond, void* base, Layout* L,
-                 double** fe, double** blocks) {
+// fe_doubles: element forces (30 per element) or, for the chunked plan,
+// shared-node partials (3 per shared pair); parts: most block partials a
+// product writes.
+size_t mf_layout(int64_t n_nodes, int64_t fe_doubles, int64_t parts, int precond, void* base,
+                 Layout* L, double** fe, double** blocks) {
   cf::Carve c(base);
   const int64_t n = 3 * n_nodes;
   L->r = c.take<double>(n);
   L->p = c.take<double>(n);
   L->Ap = c.take<double>(n);
   L->dinv = c.take<double>(precond == CF_PC_BLOCK3 ? 3 * n : n);
-  L->partials = c.take<double>(4 * size_t(std::max<int64_t>(pcg_grid(), mf_gather_grid(n_nodes))));
+  L->partials = c.take<double>(4 * size_t(std::max<int64_t>(pcg_grid(), parts)));
   L->partials_b = c.take<double>(2 * size_t(pcg_vec_grid()));
   L->state = c.take<PcgState>(1);
   L->bad = c.take<int32_t>(1);
-  *fe = c.take<double>(30 * n_elem);
+  *fe = c.take<double>(std::max<int64_t>(fe_doubles, 1));
   *blocks = c.take<double>(9 * n_nodes);
   L->total = c.used;
   return c.used;
+}
+
+size_t mf_layout(int64_t n_nodes, int64_t n_elem, int precond, void* base, Layout* L,
+                 double** fe, double** blocks) {
+  return mf_layout(n_nodes, 30 * n_elem, mf_gather_grid(n_nodes), precond, base, L, fe, blocks);
 }
 
 int mf_apply(const MfArgs& a, const double* xin, double* y, const uint8_t* fixed, double* fe,
@@ -2312,6 +2573,143 @@ int mf_apply(const MfArgs& a, const double* xin, double* y, const uint8_t* fixed
     mf_gather<<<G, kBlock, 0, s>>>(a.n_nodes, a.offs, a.perm, fe, fixed, xin, y, state,
                                    partials);
   CF_LAUNCHED();
+  return CF_OK;
+}
+
+// ---- chunked plan (see mf_chunk_apply)
+
+struct MfPlan {
+  int64_t n_chunks, m10;
+  int32_t* conn_c;
+  double* geo_c;
+  uint16_t* slot_list;
+  int32_t* chunk_off;
+  int32_t* pair_node;
+  int32_t* pair_slot_off;
+  int32_t* pair_out;
+  int32_t* shared_node;
+  int32_t* shared_off;
+  int64_t* counts;  // n_pairs, n_shared_pairs, n_shared_nodes
+  uint64_t* keys_a;
+  uint64_t* keys_b;
+  uint32_t* vals_a;
+  uint32_t* vals_b;
+  int32_t* flags;
+  int32_t* scan;
+  void* tmp;
+  size_t tmp_bytes;
+  size_t total;
+};
+
+MfPlan mf_plan_layout(int64_t n_elem, void* base) {
+  cf::Carve c(base);
+  MfPlan P{};
+  P.n_chunks = (n_elem + kChunk - 1) / kChunk;
+  P.m10 = 10 * n_elem;
+  P.counts = c.take<int64_t>(4);
+  P.conn_c = c.take<int32_t>(P.m10);
+  P.geo_c = c.take<double>(P.m10);
+  P.slot_list = c.take<uint16_t>(P.m10);
+  P.chunk_off = c.take<int32_t>(P.n_chunks + 1);
+  P.pair_node = c.take<int32_t>(P.m10);
+  P.pair_slot_off = c.take<int32_t>(P.m10 + 1);
+  P.pair_out = c.take<int32_t>(P.m10);
+  P.shared_node = c.take<int32_t>(P.m10);
+  P.shared_off = c.take<int32_t>(P.m10 + 1);
+  P.keys_a = c.take<uint64_t>(P.m10);
+  P.keys_b = c.take<uint64_t>(P.m10);
+  P.vals_a = c.take<uint32_t>(P.m10);
+  P.vals_b = c.take<uint32_t>(P.m10);
+  P.flags = c.take<int32_t>(P.m10);
+  P.scan = c.take<int32_t>(P.m10);
+  size_t a = 0, b = 0, d = 0;
+  rocprim::radix_sort_pairs(nullptr, a, (uint64_t*)nullptr, (uint64_t*)nullptr,
+                            (uint32_t*)nullptr, (uint32_t*)nullptr, size_t(P.m10), 0, 64);
+  rocprim::radix_sort_pairs(nullptr, b, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                            (uint32_t*)nullptr, (uint32_t*)nullptr, size_t(n_elem), 0, 30);
+  rocprim::inclusive_scan(nullptr, d, (int32_t*)nullptr, (int32_t*)nullptr, size_t(P.m10),
+                          rocprim::plus<int32_t>());
+  P.tmp_bytes = std::max(a, std::max(b, d));
+  P.tmp = c.take<char>(P.tmp_bytes);
+  P.total = c.used;
+  return P;
+}
+
+int mf_plan_build(const MfArgs& A, const double* x, const double* y, const double* z,
+                  const double* bbox, const MfPlan& P, int64_t counts_host[3], hipStream_t s) {
+  const int64_t m = A.n_elem, m10 = P.m10;
+  const int gm = cf::stride_grid(m, kBlock), g10 = cf::stride_grid(m10, kBlock);
+  double sc[3];
+  for (int k = 0; k < 3; ++k) {
+    const double ext = bbox[3 + k] - bbox[k];
+    sc[k] = ext > 0.0 ? 1023.0 / ext : 0.0;
+  }
+  CF_HIP(hipMemsetAsync(P.counts, 0, 4 * sizeof(int64_t), s));
+  CF_HIP(hipMemsetAsync(P.shared_off, 0, sizeof(int32_t), s));
+  // 1. Morton order of the elements, permuted connectivity and geometry
+  mfp_morton<<<gm, kBlock, 0, s>>>(m, A.conn, x, y, z, bbox[0], bbox[1], bbox[2], sc[0], sc[1],
+                                   sc[2], P.vals_a, P.vals_b);
+  CF_LAUNCHED();
+  uint32_t* mkeys = reinterpret_cast<uint32_t*>(P.keys_a);
+  uint32_t* morder = reinterpret_cast<uint32_t*>(P.keys_b);
+  size_t tb = P.tmp_bytes;
+  CF_HIP(rocprim::radix_sort_pairs(P.tmp, tb, P.vals_a, mkeys, P.vals_b, morder, size_t(m), 0, 30,
+                                   s));
+  mfp_permute<<<g10, kBlock, 0, s>>>(m, morder, A.conn, A.geo, P.conn_c, P.geo_c);
+  CF_LAUNCHED();
+  // 2. (chunk, node) pairs of the slots
+  mfp_pair_keys<<<g10, kBlock, 0, s>>>(m10, P.conn_c, P.keys_a, P.vals_a);
+  CF_LAUNCHED();
+  int chunk_bits = 1;
+  while ((int64_t(1) << chunk_bits) < P.n_chunks + 1) ++chunk_bits;
+  tb = P.tmp_bytes;
+  CF_HIP(rocprim::radix_sort_pairs(P.tmp, tb, P.keys_a, P.keys_b, P.vals_a, P.vals_b,
+                                   size_t(m10), 0, 32 + chunk_bits, s));
+  mfp_heads<<<g10, kBlock, 0, s>>>(m10, P.keys_b, nullptr, P.flags);
+  CF_LAUNCHED();
+  tb = P.tmp_bytes;
+  CF_HIP(rocprim::inclusive_scan(P.tmp, tb, P.flags, P.scan, size_t(m10),
+                                 rocprim::plus<int32_t>(), s));
+  mfp_pairs<<<g10, kBlock, 0, s>>>(m10, P.n_chunks, P.keys_b, P.vals_b, P.scan, P.slot_list,
+                                   P.pair_node, P.pair_slot_off, P.chunk_off, P.counts);
+  CF_LAUNCHED();
+  // 3. shared pairs in (node, chunk) order -> partial ids, shared node lists
+  mfp_shared_keys<<<g10, kBlock, 0, s>>>(m10, P.counts, P.keys_b, P.pair_node, P.pair_slot_off,
+                                         A.offs, P.keys_a, P.vals_a, P.pair_out);
+  CF_LAUNCHED();
+  tb = P.tmp_bytes;
+  CF_HIP(rocprim::radix_sort_pairs(P.tmp, tb, P.keys_a, P.keys_b, P.vals_a, P.vals_b,
+                                   size_t(m10), 0, 64, s));
+  mfp_count_shared<<<g10, kBlock, 0, s>>>(m10, P.keys_b, P.counts);
+  CF_LAUNCHED();
+  mfp_heads<<<g10, kBlock, 0, s>>>(m10, P.keys_b, P.counts + 1, P.flags);
+  CF_LAUNCHED();
+  tb = P.tmp_bytes;
+  CF_HIP(rocprim::inclusive_scan(P.tmp, tb, P.flags, P.scan, size_t(m10),
+                                 rocprim::plus<int32_t>(), s));
+  mfp_shared_fill<<<g10, kBlock, 0, s>>>(m10, P.keys_b, P.vals_b, P.scan, P.counts, P.pair_out,
+                                         P.shared_node, P.shared_off);
+  CF_LAUNCHED();
+  CF_HIP(hipMemcpyAsync(counts_host, P.counts, 3 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  CF_HIP(hipStreamSynchronize(s));
+  return CF_OK;
+}
+
+int64_t mf_shared_grid(int64_t n_sn) { return tile_grid((n_sn + kBlock - 1) / kBlock); }
+
+int mf_plan_apply(const MfArgs& A, const MfPlan& P, const int64_t counts[3], const double* xin,
+                  double* y, const uint8_t* fixed, double* part_buf, PcgState* state,
+                  double* partials, hipStream_t s) {
+  mf_chunk_apply<<<int(P.n_chunks), kBlock, 0, s>>>(
+      A.n_elem, P.conn_c, P.geo_c, A.lam, A.mu, xin, P.slot_list, P.chunk_off, P.pair_node,
+      P.pair_slot_off, P.pair_out, fixed, y, part_buf, state, partials);
+  CF_LAUNCHED();
+  if (counts[2] > 0) {
+    mf_shared_gather<<<int(mf_shared_grid(counts[2])), kBlock, 0, s>>>(
+        counts[2], P.shared_node, P.shared_off, part_buf, fixed, xin, y, state,
+        partials ? partials + P.n_chunks : nullptr);
+    CF_LAUNCHED();
+  }
   return CF_OK;
 }
 
@@ -2394,6 +2792,118 @@ extern "C" int cf_pcg_solve_mf_t10(int64_t n_nodes, int64_t n_elem, const int32_
                                        stats_host, s);
   return pcg_fp64_core<CF_PC_JACOBI>(apply, parts, n, n, b, x, fixed, tol, maxit, L, stats_host,
                                      s);
+}
+
+extern "C" int cf_mf_t10_plan_bytes(int64_t n_elem, size_t* bytes) {
+  CF_REQUIRE(n_elem > 0 && bytes, "cf_mf_t10_plan_bytes: bad arguments");
+  CF_REQUIRE(10 * n_elem < (int64_t(1) << 31), "cf_mf_t10_plan_bytes: too many elements");
+  *bytes = mf_plan_layout(n_elem, nullptr).total;
+  return CF_OK;
+}
+
+extern "C" int cf_mf_t10_plan_build(int64_t n_nodes, int64_t n_elem, const int32_t* conn,
+                                    const double* geo, const double* x, const double* y,
+                                    const double* z, const int32_t* offs, const double* bbox,
+                                    void* plan, size_t plan_bytes, int64_t* counts,
+                                    void* stream) {
+  MfArgs a{n_nodes, n_elem, conn, geo, 0.0, 0.0, offs, nullptr, nullptr};
+  CF_REQUIRE(n_nodes > 0 && n_elem > 0 && conn && geo && offs && x && y && z && bbox && plan &&
+                 counts,
+             "cf_mf_t10_plan_build: bad arguments");
+  CF_REQUIRE(10 * n_elem < (int64_t(1) << 31), "cf_mf_t10_plan_build: too many elements");
+  MfPlan P = mf_plan_layout(n_elem, plan);
+  if (plan_bytes < P.total) {
+    cf::set_error("cf_mf_t10_plan_build: plan buffer %zu < %zu bytes", plan_bytes, P.total);
+    return CF_ERR_WORKSPACE;
+  }
+  return mf_plan_build(a, x, y, z, bbox, P, counts, cf::as_stream(stream));
+}
+
+extern "C" int cf_mf_t10_plan_workspace_bytes(int64_t n_nodes, int64_t n_elem,
+                                              const int64_t* counts, int32_t precond,
+                                              size_t* bytes) {
+  CF_REQUIRE(n_nodes > 0 && n_elem > 0 && counts && bytes,
+             "cf_mf_t10_plan_workspace_bytes: bad arguments");
+  Layout L{};
+  double *fe, *blocks;
+  const int64_t parts = (n_elem + kChunk - 1) / kChunk + mf_shared_grid(counts[2]);
+  *bytes = mf_layout(n_nodes, 3 * counts[1], parts, precond, nullptr, &L, &fe, &blocks);
+  return CF_OK;
+}
+
+extern "C" int cf_mf_t10_plan_apply(int64_t n_nodes, int64_t n_elem, const void* plan,
+                                    const int64_t* counts, double lam, double mu,
+                                    const double* x, double* y, const uint8_t* fixed, void* ws,
+                                    size_t ws_bytes, void* stream) {
+  CF_REQUIRE(n_nodes > 0 && n_elem > 0 && plan && counts && x && y && ws,
+             "cf_mf_t10_plan_apply: bad arguments");
+  MfArgs a{n_nodes, n_elem, nullptr, nullptr, lam, mu, nullptr, nullptr, nullptr};
+  MfPlan P = mf_plan_layout(n_elem, const_cast<void*>(plan));
+  Layout L{};
+  double *fe, *blocks;
+  const int64_t parts = P.n_chunks + mf_shared_grid(counts[2]);
+  const size_t need = mf_layout(n_nodes, 3 * counts[1], parts, CF_PC_JACOBI, ws, &L, &fe, &blocks);
+  if (ws_bytes < need) {
+    cf::set_error("cf_mf_t10_plan_apply: workspace %zu < %zu bytes", ws_bytes, need);
+    return CF_ERR_WORKSPACE;
+  }
+  return mf_plan_apply(a, P, counts, x, y, fixed, fe, nullptr, nullptr, cf::as_stream(stream));
+}
+
+extern "C" int cf_pcg_solve_mf_t10_plan(int64_t n_nodes, int64_t n_elem, const int32_t* conn,
+                                        const double* geo, const int32_t* offs,
+                                        const uint32_t* perm, const void* plan,
+                                        const int64_t* counts, double lam, double mu,
+                                        const double* b, double* x, const uint8_t* fixed,
+                                        int32_t precond, double tol, int32_t maxit, void* ws,
+                                        size_t ws_bytes, cf_pcg_stats* stats_host,
+                                        void* stream) {
+  MfArgs a{n_nodes, n_elem, conn, geo, lam, mu, offs, perm, nullptr};
+  int rc = mf_check(a);
+  if (rc) return rc;
+  CF_REQUIRE(plan && counts && b && x && ws && stats_host,
+             "cf_pcg_solve_mf_t10_plan: null argument");
+  CF_REQUIRE(precond == CF_PC_JACOBI || precond == CF_PC_BLOCK3,
+             "cf_pcg_solve_mf_t10_plan: bad precond %d", precond);
+  CF_REQUIRE(tol > 0.0 && maxit > 0, "cf_pcg_solve_mf_t10_plan: tol and maxit must be positive");
+  MfPlan P = mf_plan_layout(n_elem, const_cast<void*>(plan));
+  Layout L{};
+  double *fe, *blocks;
+  const int64_t parts = P.n_chunks + mf_shared_grid(counts[2]);
+  const size_t need = mf_layout(n_nodes, 3 * counts[1], parts, precond, ws, &L, &fe, &blocks);
+  if (ws_bytes < need) {
+    cf::set_error("cf_pcg_solve_mf_t10_plan: workspace %zu < %zu bytes", ws_bytes, need);
+    return CF_ERR_WORKSPACE;
+  }
+  hipStream_t s = cf::as_stream(stream);
+  *stats_host = cf_pcg_stats{};
+  CF_HIP(hipMemsetAsync(L.state, 0, sizeof(PcgState), s));
+  CF_HIP(hipMemsetAsync(L.bad, 0, sizeof(int32_t), s));
+  mf_t10_diag<<<cf::stride_grid(n_nodes, kBlock), kBlock, 0, s>>>(n_nodes, offs, perm, geo, lam,
+                                                                  mu, blocks);
+  CF_LAUNCHED();
+  blocks_to_precond<<<cf::stride_grid(n_nodes, kBlock), kBlock, 0, s>>>(
+      n_nodes, blocks, fixed, precond == CF_PC_BLOCK3, L.dinv, L.bad);
+  CF_LAUNCHED();
+  int32_t bad = 0;
+  CF_HIP(hipMemcpyAsync(&bad, L.bad, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  CF_HIP(hipStreamSynchronize(s));
+  if (bad) {
+    cf::set_error("cf_pcg_solve_mf_t10_plan: %d free diagonal entries/blocks are not positive",
+                  bad);
+    return CF_ERR_BREAKDOWN;
+  }
+  int64_t cnt[3] = {counts[0], counts[1], counts[2]};
+  auto apply = [&](const double* xin, double* yout, PcgState* state, double* partials) {
+    return mf_plan_apply(a, P, cnt, xin, yout, fixed, fe, state, partials, s);
+  };
+  const int64_t n = 3 * n_nodes;
+  const int apply_parts = int(P.n_chunks + (cnt[2] > 0 ? mf_shared_grid(cnt[2]) : 0));
+  if (precond == CF_PC_BLOCK3)
+    return pcg_fp64_core<CF_PC_BLOCK3>(apply, apply_parts, n, n_nodes, b, x, fixed, tol, maxit,
+                                       L, stats_host, s);
+  return pcg_fp64_core<CF_PC_JACOBI>(apply, apply_parts, n, n, b, x, fixed, tol, maxit, L,
+                                     stats_host, s);
 }
 
 extern "C" int cf_invert_permutation(int64_t n, const uint32_t* perm, uint32_t* inv,

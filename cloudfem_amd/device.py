@@ -180,6 +180,28 @@ class DeviceMesh:
             self._mf = (conn, geo, offs, perm, slot)
         return self._mf
 
+    def matfree_plan(self):
+        """(plan buffer, counts (3,) int64 host array) of the chunked
+        matrix-free Tet10 operator (cf_mf_t10_plan_build): Morton-ordered
+        chunks of 256 elements, node sums in LDS."""
+        if getattr(self, "_mfplan", None) is None:
+            lib = N.load()
+            offs, _ = self.incidence()
+            bbox = torch.stack([self.x.min(), self.y.min(), self.z.min(),
+                                self.x.max(), self.y.max(), self.z.max()]).cpu().numpy()
+            nb = ctypes.c_size_t()
+            N.check(lib.cf_mf_t10_plan_bytes(self.n_elem, ctypes.byref(nb)), "cf_mf_t10_plan_bytes")
+            plan = workspace(nb.value, self.dev)
+            counts = np.zeros(3, dtype=np.int64)
+            N.check(lib.cf_mf_t10_plan_build(self.n_nodes, self.n_elem, ptr(self.conn),
+                                             ptr(self.geometry()), ptr(self.x), ptr(self.y),
+                                             ptr(self.z), ptr(offs),
+                                             bbox.astype(np.float64).ctypes.data, ptr(plan),
+                                             nb.value, counts.ctypes.data, stream_handle()),
+                    "cf_mf_t10_plan_build")
+            self._mfplan = (plan, counts)
+        return self._mfplan
+
     # ----------------------------------------------------------- assembly
 
     def matrix_values(self, op, coef=None, lam=0.0, mu=0.0):

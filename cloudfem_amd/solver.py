@@ -115,14 +115,39 @@ def pcg(A, b, x, fixed=None, precision=None, precond=None, tol=None, maxit=None)
     return stats
 
 
-def pcg_matfree_t10(dm, lam, mu, b, x, fixed=None, precond=None, tol=None, maxit=None):
-    """PCG with the matrix-free Tet10 elasticity operator (FP64)."""
+def pcg_matfree_t10(dm, lam, mu, b, x, fixed=None, precond=None, tol=None, maxit=None,
+                    chunked=False):
+    """PCG with the matrix-free Tet10 elasticity operator (FP64).  Default:
+    the two-pass element-force / node-gather operator (cf_pcg_solve_mf_t10),
+    the faster one on MI355X; chunked=True: the LDS-chunked operator
+    (cf_pcg_solve_mf_t10_plan), which moves 2.3x fewer HBM bytes but runs at
+    lower occupancy (88 vs 71 us per product at 384,000 elements)."""
     o = options()
     precond = precond or o["precond"]
     tol = float(tol if tol is not None else o["tol"])
     maxit = int(maxit if maxit is not None else o["maxit"])
     pc = {"jacobi": N.PC_JACOBI, "block3": N.PC_BLOCK3}[precond]
     lib = N.load()
+    if chunked:
+        plan, counts = dm.matfree_plan()
+        offs, perm = dm.incidence()
+        nb = ctypes.c_size_t()
+        N.check(lib.cf_mf_t10_plan_workspace_bytes(dm.n_nodes, dm.n_elem, counts.ctypes.data, pc,
+                                                   ctypes.byref(nb)),
+                "cf_mf_t10_plan_workspace_bytes")
+        ws = workspace(nb.value, b.device)
+        st = N.CfPcgStats()
+        rc = lib.cf_pcg_solve_mf_t10_plan(dm.n_nodes, dm.n_elem, ptr(dm.conn), ptr(dm.geometry()),
+                                          ptr(offs), ptr(perm), ptr(plan), counts.ctypes.data,
+                                          float(lam), float(mu), ptr(b), ptr(x), ptr(fixed), pc,
+                                          tol, maxit, ptr(ws), nb.value, ctypes.byref(st),
+                                          stream_handle())
+        stats = {"iterations": st.iterations, "residual_updates": 0,
+                 "rel_residual": st.rel_residual, "b_norm": st.b_norm, "precision": "fp64",
+                 "precond": precond, "tol": tol, "operator": "matfree"}
+        last_stats.value = stats
+        N.check(rc, "cf_pcg_solve_mf_t10_plan")
+        return stats
     conn, geo, offs, perm, slot = dm.matfree_arrays()
     nb = ctypes.c_size_t()
     N.check(lib.cf_mf_t10_workspace_bytes(dm.n_nodes, dm.n_elem, pc, ctypes.byref(nb)),
@@ -142,9 +167,23 @@ def pcg_matfree_t10(dm, lam, mu, b, x, fixed=None, precond=None, tol=None, maxit
     return stats
 
 
-def matfree_t10_apply(dm, lam, mu, x, fixed=None):
-    """y = K x with the matrix-free Tet10 elasticity operator."""
+def matfree_t10_apply(dm, lam, mu, x, fixed=None, chunked=False):
+    """y = K x with the matrix-free Tet10 elasticity operator (chunked: the
+    LDS-chunked plan, else the two-pass operator)."""
     lib = N.load()
+    if chunked:
+        plan, counts = dm.matfree_plan()
+        nb = ctypes.c_size_t()
+        N.check(lib.cf_mf_t10_plan_workspace_bytes(dm.n_nodes, dm.n_elem, counts.ctypes.data,
+                                                   N.PC_JACOBI, ctypes.byref(nb)),
+                "cf_mf_t10_plan_workspace_bytes")
+        ws = workspace(nb.value, x.device)
+        y = torch.empty_like(x)
+        N.check(lib.cf_mf_t10_plan_apply(dm.n_nodes, dm.n_elem, ptr(plan), counts.ctypes.data,
+                                         float(lam), float(mu), ptr(x), ptr(y), ptr(fixed),
+                                         ptr(ws), nb.value, stream_handle()),
+                "cf_mf_t10_plan_apply")
+        return y
     conn, geo, offs, perm, slot = dm.matfree_arrays()
     nb = ctypes.c_size_t()
     N.check(lib.cf_mf_t10_workspace_bytes(dm.n_nodes, dm.n_elem, N.PC_JACOBI, ctypes.byref(nb)),

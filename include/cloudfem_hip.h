@@ -309,6 +309,39 @@ int cf_pcg_solve_mf_t10(int64_t n_nodes, int64_t n_elem, const int32_t* conn,
                         int32_t precond, double tol, int32_t maxit, void* ws,
                         size_t ws_bytes, cf_pcg_stats* stats_host, void* stream);
 
+/* ---- chunked plan of the matrix-free Tet10 operator.  Elements are put in
+ *      Morton order of their centroids (bbox: host xmin, ymin, zmin, xmax,
+ *      ymax, zmax of the nodes) and cut into chunks of 256; each chunk
+ *      computes its element forces into LDS and sums them per node, so no
+ *      per-element forces go through HBM.  Nodes shared between chunks are
+ *      finished from per-chunk partials in chunk order (deterministic).
+ *      offs = node incidence offsets (cf_incidence_build of the
+ *      connectivity).  plan: caller buffer of cf_mf_t10_plan_bytes;
+ *      counts (host, 3): pairs, shared pairs, shared nodes, written by the
+ *      build and passed back to apply/solve.  Same operator and solve as
+ *      cf_mf_t10_apply / cf_pcg_solve_mf_t10 (which they replace in the
+ *      elastic_tet10.py:214-229 solve); conn/geo/offs/perm of the solve are
+ *      the original (unpermuted) arrays, used for the preconditioner.      */
+int cf_mf_t10_plan_bytes(int64_t n_elem, size_t* bytes);
+int cf_mf_t10_plan_build(int64_t n_nodes, int64_t n_elem, const int32_t* conn,
+                         const double* geo, const double* x, const double* y,
+                         const double* z, const int32_t* offs, const double* bbox,
+                         void* plan, size_t plan_bytes, int64_t* counts, void* stream);
+int cf_mf_t10_plan_workspace_bytes(int64_t n_nodes, int64_t n_elem,
+                                   const int64_t* counts, int32_t precond,
+                                   size_t* bytes);
+int cf_mf_t10_plan_apply(int64_t n_nodes, int64_t n_elem, const void* plan,
+                         const int64_t* counts, double lam, double mu,
+                         const double* x, double* y, const uint8_t* fixed,
+                         void* ws, size_t ws_bytes, void* stream);
+int cf_pcg_solve_mf_t10_plan(int64_t n_nodes, int64_t n_elem, const int32_t* conn,
+                             const double* geo, const int32_t* offs,
+                             const uint32_t* perm, const void* plan,
+                             const int64_t* counts, double lam, double mu,
+                             const double* b, double* x, const uint8_t* fixed,
+                             int32_t precond, double tol, int32_t maxit, void* ws,
+                             size_t ws_bytes, cf_pcg_stats* stats_host, void* stream);
+
 /* ---- inv[perm[j]] = j for a permutation of 0..n-1.                     */
 int cf_invert_permutation(int64_t n, const uint32_t* perm, uint32_t* inv,
                           void* stream);
