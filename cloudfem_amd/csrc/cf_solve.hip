@@ -827,15 +827,14 @@ __constant__ cf::Tet10Tables kT10m = cf::build_tet10_tables();
 // the 5 Keast points: H_q = grad u = (sum_a x_a d_aq^T) Jinv,
 // sigma_q = lam tr(H) I + mu (H + H^T), f_a += V w_q sigma_q Jinv^T d_aq.
 // Exact for affine elements, same integrand and rule as
-// elastic_tet10.py:164-170, ~1.2k FMA per element; out(a, f_a) receives the
-// 10 nodal forces.  The tables stay in constant memory: every lane reads the
+// elastic_tet10.py:164-170, ~1.2k FMA per element; f receives the 10 nodal
+// forces.  The tables stay in constant memory: every lane reads the
 // same entry, served as scalar operands.  (Compile-time literals instead
 // fold the zero gradient entries but cost more registers and moves: 50.7 vs
 // 46.5 us per application at 384,000 elements.)
-template <class Out>
 __device__ __forceinline__ void t10_forces(const int32_t* __restrict__ cn,
                                            const double* __restrict__ g, double lam, double mu,
-                                           const double* __restrict__ x, Out&& out) {
+                                           const double* __restrict__ x, double (&f)[10][3]) {
   double J[3][3];
 #pragma unroll
   for (int r = 0; r < 3; ++r)
@@ -849,7 +848,6 @@ __device__ __forceinline__ void t10_forces(const int32_t* __restrict__ cn,
 #pragma unroll
     for (int c = 0; c < 3; ++c) xa[a][c] = x[3 * nd + c];
   }
-  double f[10][3];
 #pragma unroll
   for (int a = 0; a < 10; ++a)
 #pragma unroll
@@ -886,12 +884,13 @@ __device__ __forceinline__ void t10_forces(const int32_t* __restrict__ cn,
         f[a][c] += kT10m.D[q][a][0] * P[c][0] + kT10m.D[q][a][1] * P[c][1] +
                    kT10m.D[q][a][2] * P[c][2];
   }
-#pragma unroll
-  for (int a = 0; a < 10; ++a) out(a, f[a]);
 }
 
 // fe[e][a][c] (or the node-sorted slot of (e, a)) = forces of element e,
-// summed into the nodes by mf_gather / mf_gather_sorted in fixed order.
+// summed into the nodes by mf_gather / mf_gather_sorted in fixed order.  The
+// same arithmetic as t10_forces, written out: 178 VGPRs (226 when the
+// stores went through a per-node output callback: 60 vs 46.5 us per
+// application at 384,000 elements).
 __global__ void __launch_bounds__(kBlock)
 mf_t10_elem(int64_t m, const int32_t* __restrict__ conn, const double* __restrict__ geo,
             double lam, double mu, const double* __restrict__ x,
@@ -900,14 +899,70 @@ mf_t10_elem(int64_t m, const int32_t* __restrict__ conn, const double* __restric
   if (state && state->halt) return;
   for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < m;
        e += int64_t(gridDim.x) * blockDim.x) {
-    // with slots, each nodal force goes to its position in node-sorted
-    // incidence order, so the node sums read a contiguous range
-    t10_forces(conn + e * 10, geo + e * 10, lam, mu, x, [&](int a, const double (&f)[3]) {
-      double* o = slot ? fe + int64_t(slot[e * 10 + a]) * 3 : fe + e * 30 + 3 * a;
-      o[0] = f[0];
-      o[1] = f[1];
-      o[2] = f[2];
-    });
+    const double* g = geo + e * 10;
+    double J[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) J[r][c] = g[3 * r + c];
+    const double V = g[9];
+    double xa[10][3];
+#pragma unroll
+    for (int a = 0; a < 10; ++a) {
+      const int64_t nd = conn[e * 10 + a];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) xa[a][c] = x[3 * nd + c];
+    }
+    double f[10][3] = {};
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      double U[3][3] = {};
+#pragma unroll
+      for (int a = 0; a < 10; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) U[c][k] += xa[a][c] * kT10m.D[q][a][k];
+      double H[3][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) H[c][i] = U[c][0] * J[0][i] + U[c][1] * J[1][i] + U[c][2] * J[2][i];
+      const double ltr = lam * (H[0][0] + H[1][1] + H[2][2]);
+      const double vw = V * kT10m.w[q];
+      double P[3][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        double sg[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) sg[i] = mu * (H[c][i] + H[i][c]) + (c == i ? ltr : 0.0);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) P[c][k] = vw * (sg[0] * J[k][0] + sg[1] * J[k][1] + sg[2] * J[k][2]);
+      }
+#pragma unroll
+      for (int a = 0; a < 10; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          f[a][c] += kT10m.D[q][a][0] * P[c][0] + kT10m.D[q][a][1] * P[c][1] +
+                     kT10m.D[q][a][2] * P[c][2];
+    }
+    if (slot) {
+      // each nodal force goes to its position in node-sorted incidence order,
+      // so the node sums read a contiguous range
+#pragma unroll
+      for (int a = 0; a < 10; ++a) {
+        double* o = fe + int64_t(slot[e * 10 + a]) * 3;
+        o[0] = f[a][0];
+        o[1] = f[a][1];
+        o[2] = f[a][2];
+      }
+    } else {
+      double* o = fe + e * 30;
+#pragma unroll
+      for (int a = 0; a < 10; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) o[3 * a + c] = f[a][c];
+    }
   }
 }
 
@@ -1137,11 +1192,14 @@ mf_chunk_apply(int64_t m, const int32_t* __restrict__ conn_c, const double* __re
   // forces of the chunk, component-major: fl[(3 a + c) * kChunk + e_local]
   __shared__ double fl[kChunk * 30];
   const int64_t e = int64_t(blockIdx.x) * kChunk + threadIdx.x;
-  if (e < m)
-    t10_forces(conn_c + e * 10, geo_c + e * 10, lam, mu, x, [&](int a, const double (&f)[3]) {
+  if (e < m) {
+    double f[10][3];
+    t10_forces(conn_c + e * 10, geo_c + e * 10, lam, mu, x, f);
 #pragma unroll
-      for (int c = 0; c < 3; ++c) fl[(3 * a + c) * kChunk + threadIdx.x] = f[c];
-    });
+    for (int a = 0; a < 10; ++a)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) fl[(3 * a + c) * kChunk + threadIdx.x] = f[a][c];
+  }
   __syncthreads();
   double dot = 0.0;
   const int32_t p1 = chunk_off[blockIdx.x + 1];
