@@ -118,6 +118,7 @@ def bench_batch_heat(args):
         run_s = float(np.mean([r[1]["run_s"] for r in runs]))
         paths[prec] = {"ms_per_step": 1e3 * float(np.mean([r[0] for r in runs])),
                        "stepping_ms": 1e3 * run_s,
+                       "setup_ms": 1e3 * float(np.mean([r[1]["setup_s"] for r in runs])),
                        "node_steps_per_s_stepping": n_nodes * 50 / run_s,
                        "iterations": st["iterations"], "sweeps": st["sweeps"],
                        "worst_step": st["worst_step"], "graph_replays": st["graph_replays"],

@@ -17,7 +17,7 @@ OUT = os.path.join(PKG, "libcloudfem_hip.so")
 OBJDIR = os.path.join(HERE, "_build")
 
 SOURCES = ["cf_common.cpp", "cf_pattern.hip", "cf_assemble.hip", "cf_solve.hip",
-           "cf_dirichlet.hip", "cf_batch.hip"]
+           "cf_dirichlet.hip", "cf_batch.hip", "cf_mesh.hip"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 ARCH = os.environ.get("CF_OFFLOAD_ARCH", "gfx950")

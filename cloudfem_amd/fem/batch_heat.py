@@ -29,6 +29,7 @@ from .. import _native as N
 from ..autotune import batch_spmv_config
 from ..coef import centroids, eval_scalar
 from ..device import DeviceMesh, ptr, stream_handle, to_dev, workspace
+from ..mesh.device_ops import boundary_faces_device
 from ..mesh.faces import boundary_faces
 from ..mesh.tet_mesh import TetMesh, box_mesh
 from . import _transient
@@ -54,6 +55,8 @@ def _element_values(v, mesh):
             raise ValueError(f"per-element array of shape {a.shape} for "
                              f"{mesh.n_elements} elements")
         return a
+    if not callable(v):
+        return np.full(mesh.n_elements, float(v))
     return eval_scalar(v, centroids(mesh))
 
 
@@ -65,8 +68,32 @@ def _union(meshes):
     return TetMesh(nodes, elems), offs
 
 
-def _faces_and_coef(meshes, offs, specs, pick):
+def _union_faces(um, dm, meshes, offs, names):
+    """{name: (F, 3) union faces} by one device scan per name over the union
+    connectivity (disjoint meshes: the union's exterior faces are the meshes'
+    exterior faces, in mesh order then occurrence order)."""
+    out = {}
+    for name in names:
+        um.boundaries[name] = np.concatenate(
+            [np.asarray(m.boundaries[name], dtype=np.int64) + offs[i]
+             for i, m in enumerate(meshes)])
+        out[name] = boundary_faces_device(um, name, conn=dm.conn)
+    return out
+
+
+def _faces_and_coef(meshes, offs, specs, pick, union=None):
     """Concatenated union faces and per-face values over all meshes."""
+    if union is not None and specs and all(sp is specs[0] for sp in specs):
+        faces, vals = [], []
+        for name, v in (specs[0] or {}).items():
+            fc = union[name]
+            if fc.shape[0]:
+                cen = union["_nodes"][fc].mean(axis=1)
+                faces.append(fc)
+                vals.append(pick(v, cen))
+        if not faces:
+            return None, None
+        return np.concatenate(faces), np.concatenate(vals)
     faces, vals = [], []
     for i, (m, spec) in enumerate(zip(meshes, specs)):
         for name, v in (spec or {}).items():
@@ -117,13 +144,23 @@ def transient_heat_batch(meshes, k=1.0, rho_cp=1.0, f=0.0, neumann=None, robin=N
         dm.add_rhs(N.RHS_P1_SOURCE, to_dev(f_e, torch.float64, dev), F)
     nspec = _per_mesh(neumann, nm, "neumann")
     rspec = _per_mesh(robin, nm, "robin")
-    faces, q = _faces_and_coef(meshes, offs, nspec, lambda v, cen: eval_scalar(v, cen))
+    # same spec for every mesh: one device face scan per name over the union
+    names = set()
+    for spec in (nspec[0], rspec[0]):
+        names |= set(spec or {})
+    union = None
+    if not isinstance(neumann, (list, tuple)) and not isinstance(robin, (list, tuple)) and names:
+        union = _union_faces(um, dm, meshes, offs, sorted(names))
+        union["_nodes"] = um.nodes
+    faces, q = _faces_and_coef(meshes, offs, nspec, lambda v, cen: eval_scalar(v, cen), union)
     if faces is not None:
         dm.add_face_rhs(N.FACE_P1_FLUX, faces, q, F)
-    faces, hv = _faces_and_coef(meshes, offs, rspec, lambda v, cen: eval_scalar(v[0], cen))
+    faces, hv = _faces_and_coef(meshes, offs, rspec, lambda v, cen: eval_scalar(v[0], cen),
+                                union)
     if faces is not None:
         _, ht = _faces_and_coef(meshes, offs, rspec,
-                                lambda v, cen: eval_scalar(v[0], cen) * eval_scalar(v[1], cen))
+                                lambda v, cen: eval_scalar(v[0], cen) * eval_scalar(v[1], cen),
+                                union)
         dm.add_face_rhs(N.FACE_P1_FLUX, faces, ht, F)
         dm.add_subpattern(K, *dm.face_matrix(N.FACE_P1_ROBIN_K, faces, hv))
     A = C.clone()

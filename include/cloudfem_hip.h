@@ -342,6 +342,31 @@ int cf_pcg_solve_mf_t10_plan(int64_t n_nodes, int64_t n_elem, const int32_t* con
                              int32_t precond, double tol, int32_t maxit, void* ws,
                              size_t ws_bytes, cf_pcg_stats* stats_host, void* stream);
 
+/* ---- mesh processing on the GPU.
+ *  cf_tet10_edges: Tet4 -> Tet10 numbering of tet10_mesh.py:77-112 -- the
+ *  midpoint of corner pair (a < b) gets n_corners + rank of (a, b) among the
+ *  unique pairs in lexicographic order (np.unique order).  conn4 has `stride`
+ *  ints per element (4, or 10 to reuse Tet10 corners); writes conn10 (M x 10),
+ *  the unique edges' endpoints edge_a/edge_b (capacity 6 M) and *n_edges
+ *  (device).  cf_edge_midpoints: x/y/z[n_corners + i] = endpoint mean.
+ *  cf_boundary_faces: exterior faces (sorted corner triple in exactly one
+ *  element) whose corners are all in `member` (nullable: all), in first-
+ *  occurrence order with that occurrence's node order (tet_heat.py:219-242,
+ *  tet10.py:329-352, elastic_tet10.py:68-91); width 3 (P1 corner faces) or 6
+ *  (Tet10 faces, corners + midpoints); faces capacity 4 M x width; *count
+ *  (device).  n_nodes < 2^21.  Workspace: cf_mesh_workspace_bytes.       */
+int cf_mesh_workspace_bytes(int64_t n_elem, size_t* bytes);
+int cf_tet10_edges(int64_t n_elem, const int32_t* conn4, int32_t stride,
+                   int64_t n_corners, int32_t* conn10, int32_t* edge_a,
+                   int32_t* edge_b, int64_t* n_edges, void* ws, size_t ws_bytes,
+                   void* stream);
+int cf_edge_midpoints(int64_t n_edges, int64_t n_corners, const int32_t* edge_a,
+                      const int32_t* edge_b, double* x, double* y, double* z,
+                      void* stream);
+int cf_boundary_faces(int64_t n_elem, const int32_t* conn, int32_t k, int64_t n_nodes,
+                      const uint8_t* member, int32_t width, int32_t* faces,
+                      int64_t* count, void* ws, size_t ws_bytes, void* stream);
+
 /* ---- inv[perm[j]] = j for a permutation of 0..n-1.                     */
 int cf_invert_permutation(int64_t n, const uint32_t* perm, uint32_t* inv,
                           void* stream);
