@@ -57,7 +57,7 @@ def parse():
     ap.add_argument("--workload", default="tet10-160x20x20", choices=sorted(WORKLOADS))
     ap.add_argument("--precond64", default="jacobi", choices=["jacobi", "block3"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--paths", default="fp64,matfree,matfree_lds,mixed,mixed16",
+    ap.add_argument("--paths", default="fp64,matfree,matfree_lds,mixed,mixed16,mixed_mf",
                     help="comma-separated solver paths to time (fp64 always runs first)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="before the workload, copy a 256 MiB device buffer 3 times "
@@ -160,7 +160,7 @@ def measured_traffic(workload, path):
         return d.get("matfree_operator_traffic_bytes" if path == "matfree"
                      else "matfree_lds_traffic_bytes")
     kern = {"fp64": "spmv_sell<double, double, double>", "mixed": "cheb_step<float>",
-            "mixed16": "cheb_step<_Float16>"}.get(path)
+            "mixed16": "cheb_step<_Float16>", "mixed_mf": "mf_t10_elem32"}.get(path)
     return d["kernels"].get(kern, {}).get("traffic_bytes")
 
 
@@ -192,7 +192,7 @@ def main():
     from cloudfem_amd.mesh import box_mesh_tet10
     from cloudfem_amd.mesh.faces import boundary_faces10
     from cloudfem_amd.physics.elastic3d import lame
-    from cloudfem_amd.solver import DeviceMatrix, pcg, pcg_matfree_t10
+    from cloudfem_amd.solver import DeviceMatrix, pcg, pcg_matfree_t10, pcg_matfree_t10_mixed
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda", 0)
@@ -225,7 +225,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     paths = set(args.paths.split(","))
-    maxit = {"fp64": 200000, "mixed": 200000, "mixed16": 200000}
+    maxit = {"fp64": 200000, "mixed": 200000, "mixed16": 200000, "mixed_mf": 200000}
     plan_counts = [0, 0, 0]   # pairs, shared pairs, shared nodes of the chunked operator
 
     def step(path, precond):
@@ -238,6 +238,9 @@ def main():
         if path == "matfree":
             stats = pcg_matfree_t10(dm, lam, mu, rhs, x, fixed, precond=precond, tol=TOL,
                                     maxit=maxit["fp64"])
+        elif path == "mixed_mf":
+            stats = pcg_matfree_t10_mixed(dm, lam, mu, rhs, x, fixed, tol=TOL,
+                                          maxit=maxit["mixed"])
         elif path == "matfree_lds":
             stats = pcg_matfree_t10(dm, lam, mu, rhs, x, fixed, precond=precond, tol=TOL,
                                     maxit=maxit["fp64"], chunked=True)
@@ -304,6 +307,24 @@ def main():
                                   "op_launches_sampled": prof[1], "op_bytes": bytes_ml,
                                   "rel_diff_vs_fp64": float(torch.linalg.vector_norm(x_ml - x_64)
                                                             / torch.linalg.vector_norm(x_64))}
+    if "mixed_mf" in paths:
+        try:
+            ms, (x_mm, st, _), prof = timed("mixed_mf", "jacobi")
+            # FP32 matrix-free Chebyshev step: conn 40 + geometry 40 + slot 40 + FP32
+            # force write and read 2 x 120 B per element, offsets; S, d gathered
+            # once, w and z read+written, d' written (28 B per DOF)
+            results["mixed_mf"] = {
+                "ms_per_step": ms, "iterations": st["iterations"],
+                "low_precision_products": st["residual_updates"],
+                "rel_residual": st["rel_residual"],
+                "op": "mf_t10_elem32 + mf_gather_cheb32", "op_avg_ms": prof[2],
+                "op_launches_sampled": prof[3],
+                "op_bytes": 360 * n_elem + 4 * (n_nodes + 1) + 28 * n_dof,
+                "outer_op_avg_ms": prof[0],
+                "rel_diff_vs_fp64": float(torch.linalg.vector_norm(x_mm - x_64)
+                                          / torch.linalg.vector_norm(x_64))}
+        except (N.SolverNotConverged, N.SolverBreakdown) as exc:
+            results["mixed_mf"] = {"converged": False, "error": str(exc)[-300:]}
     for path, vbytes in (("mixed", 36), ("mixed16", 18)):
         if path not in paths:
             continue
@@ -358,7 +379,10 @@ def main():
         "paths": results,
         "speedup_mixed_vs_fp64": {
             k: results["fp64"]["ms_per_step"] / results[k]["ms_per_step"]
-            for k in ("mixed", "mixed16") if "ms_per_step" in results.get(k, {})},
+            for k in ("mixed", "mixed16", "mixed_mf") if "ms_per_step" in results.get(k, {})},
+        "speedup_mixed_mf_vs_matfree_fp64": (
+            results["matfree"]["ms_per_step"] / results["mixed_mf"]["ms_per_step"]
+            if "ms_per_step" in results.get("mixed_mf", {}) and "matfree" in results else None),
         "tip_uz": tip,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,

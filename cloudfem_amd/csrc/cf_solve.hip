@@ -1260,6 +1260,136 @@ mf_shared_gather(int64_t n_sn, const int32_t* __restrict__ shared_node,
   block_partials<1>(v1, partials);
 }
 
+// ---------------------------------------- matrix-free Tet10, FP32 inner operator
+// For the mixed solver's Chebyshev preconditioner: products with the
+// Jacobi-scaled operator S K S in FP32 (geometry rounded once to FP32,
+// forces and sums in FP32).  The element kernel gathers x = S d on the fly;
+// the node kernel scales its force sum by S and applies the Chebyshev
+// update of cheb_step (or, for the power iteration, just stores S K S d).
+
+__global__ void geo_to_f32(int64_t n, const double* __restrict__ g, float* __restrict__ g32) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    g32[i] = float(g[i]);
+}
+
+__global__ void __launch_bounds__(kBlock)
+mf_t10_elem32(int64_t m, const int32_t* __restrict__ conn, const float* __restrict__ geo,
+              float lam, float mu, const float* __restrict__ sc, const float* __restrict__ d,
+              const uint32_t* __restrict__ slot, float* __restrict__ fe,
+              const PcgState* state) {
+  if (state && state->halt) return;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < m;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    const float* g = geo + e * 10;
+    float J[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) J[r][c] = g[3 * r + c];
+    const float V = g[9];
+    float xa[10][3];
+#pragma unroll
+    for (int a = 0; a < 10; ++a) {
+      const int64_t nd = conn[e * 10 + a];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) xa[a][c] = sc[3 * nd + c] * d[3 * nd + c];
+    }
+    float f[10][3] = {};
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      float U[3][3] = {};
+#pragma unroll
+      for (int a = 0; a < 10; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) U[c][k] += xa[a][c] * float(kT10m.D[q][a][k]);
+      float H[3][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) H[c][i] = U[c][0] * J[0][i] + U[c][1] * J[1][i] + U[c][2] * J[2][i];
+      const float ltr = lam * (H[0][0] + H[1][1] + H[2][2]);
+      const float vw = V * float(kT10m.w[q]);
+      float P[3][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        float sg[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) sg[i] = mu * (H[c][i] + H[i][c]) + (c == i ? ltr : 0.0f);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) P[c][k] = vw * (sg[0] * J[k][0] + sg[1] * J[k][1] + sg[2] * J[k][2]);
+      }
+#pragma unroll
+      for (int a = 0; a < 10; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          f[a][c] += float(kT10m.D[q][a][0]) * P[c][0] + float(kT10m.D[q][a][1]) * P[c][1] +
+                     float(kT10m.D[q][a][2]) * P[c][2];
+    }
+#pragma unroll
+    for (int a = 0; a < 10; ++a) {
+      float* o = fe + int64_t(slot[e * 10 + a]) * 3;
+      o[0] = f[a][0];
+      o[1] = f[a][1];
+      o[2] = f[a][2];
+    }
+  }
+}
+
+// node sums of the FP32 forces, o = S sum; mode 0: y = o (power iteration);
+// mode 1: Chebyshev step  w -= o; d' = cd d + cw w; z += d'.
+__global__ void __launch_bounds__(kBlock)
+mf_gather_cheb32(int64_t n_nodes, const int32_t* __restrict__ offs,
+                 const float* __restrict__ fe, const float* __restrict__ sc, int mode,
+                 const float* __restrict__ d, float* __restrict__ dn, float* __restrict__ w,
+                 float* __restrict__ z, float cd, float cw, const PcgState* state) {
+  if (state && state->halt) return;
+  const int64_t i = tile_index(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n_nodes) return;
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+  const float* v = fe + int64_t(offs[i]) * 3;
+  const float* end = fe + int64_t(offs[i + 1]) * 3;
+  for (; v + 12 <= end; v += 12) {
+    float t[12];
+#pragma unroll
+    for (int u = 0; u < 12; ++u) t[u] = v[u];
+#pragma unroll
+    for (int u = 0; u < 12; u += 3) {
+      a0 += t[u];
+      a1 += t[u + 1];
+      a2 += t[u + 2];
+    }
+  }
+  for (; v < end; v += 3) {
+    a0 += v[0];
+    a1 += v[1];
+    a2 += v[2];
+  }
+  const float o[3] = {a0, a1, a2};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int64_t k = 3 * i + c;
+    const float oc = sc[k] * o[c];
+    if (mode == 0) {
+      dn[k] = oc;
+    } else {
+      const float wn = w[k] - oc;
+      w[k] = wn;
+      const float dd = cd * d[k] + cw * wn;
+      dn[k] = dd;
+      z[k] += dd;
+    }
+  }
+}
+
+__global__ void to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    b[i] = float(a[i]);
+}
+
 // inv[perm[j]] = j
 __global__ void invert_permutation(int64_t n, const uint32_t* __restrict__ perm,
                                    uint32_t* __restrict__ inv) {
@@ -2165,40 +2295,23 @@ constexpr int kChebSteps = 8;
 constexpr double kChebRatio = 100.0;
 constexpr int kPowerSteps = 40;
 
-template <class VT>
-int pcg_fcg_cheb(const cf_matrix* A, VT* sval, const double* b, double* x, const uint8_t* fixed,
-                 double tol, int maxit, const Layout& L, cf_pcg_stats* st, hipStream_t s) {
-  const int64_t n = n_dofs(A);
+// The flexible CG + Chebyshev driver, independent of how the products are
+// formed.  outer(xin, y, state): FP64 y = A xin with fixed rows zeroed
+// (state set: p.Ap partials, outer_parts of them); power(v, y): FP32
+// y = S A S v; cheb(cd, cw, d, dn, state): one Chebyshev step (w -= S A S d;
+// dn = cd d + cw w; z += dn on rh / yh).  gersh: Gershgorin bound of S A S,
+// or 0 when none is known (then b = 1.3 x the power estimate).
+template <class Outer, class Power, class Cheb>
+int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, double gersh,
+                  int64_t n, const double* b, double* x, const uint8_t* fixed, double tol,
+                  int maxit, const Layout& L, cf_pcg_stats* st, hipStream_t s) {
   const int G = pcg_grid();
-  const int GS = int(sell_grid(L.n_slices));
   int rc;
   PcgState h{};
-  make_scale<<<G, kBlock, 0, s>>>(n, L.dinv, L.s);
-  CF_LAUNCHED();
-  if ((rc = build_sell(A, L, s)) || (rc = fill_sell<double>(A, L, nullptr, L.sval64, s)) ||
-      (rc = fill_sell<VT>(A, L, L.s, sval, s)))
-    return rc;
-  auto spmv64 = [&](const double* xin, double* y, PcgState* state) {
-    spmv_sell<double, double, double><<<GS, kBlock, 0, s>>>(
-        A->n_rows, L.n_slices, L.rows, L.koff, L.scol, L.sval64, xin, y, fixed, state,
-        L.partials);
-    CF_LAUNCHED();
-    return CF_OK;
-  };
-  // spectrum bound of S A S
-  gershgorin_nb3<<<G, kBlock, 0, s>>>(A->n_rows, A->indptr, A->indices, A->values, L.s,
-                                      L.partials);
-  CF_LAUNCHED();
-  if ((rc = fin(kFinMax, L, s, &L.state->pad, 0.0, G))) return rc;
-  if ((rc = read_state(L.state, &h, s))) return rc;
-  const double gersh = h.pad;
   power_start<<<G, kBlock, 0, s>>>(n, L.s, L.ph);
   CF_LAUNCHED();
   for (int k = 0; k < kPowerSteps; ++k) {
-    spmv_sell<VT, float, float><<<GS, kBlock, 0, s>>>(A->n_rows, L.n_slices, L.rows, L.koff,
-                                                       L.scol, sval, L.ph, L.Aph, nullptr,
-                                                       nullptr, nullptr);
-    CF_LAUNCHED();
+    if ((rc = power(L.ph, L.Aph))) return rc;
     norm2_partials<<<G, kBlock, 0, s>>>(n, L.Aph, L.partials);
     CF_LAUNCHED();
     if ((rc = fin(kFinStore, L, s, &L.state->pad))) return rc;
@@ -2207,7 +2320,7 @@ int pcg_fcg_cheb(const cf_matrix* A, VT* sval, const double* b, double* x, const
   }
   if ((rc = read_state(L.state, &h, s))) return rc;
   const double est = std::sqrt(h.pad);
-  const double bmax = std::min(1.25 * est, gersh);
+  const double bmax = gersh > 0.0 ? std::min(1.25 * est, gersh) : 1.3 * est;
   if (!(bmax > 0.0) || !std::isfinite(bmax)) {
     cf::set_error("cf_pcg_solve: no spectrum bound for the Chebyshev preconditioner "
                   "(power estimate %.3e, Gershgorin %.3e)", est, gersh);
@@ -2236,10 +2349,7 @@ int pcg_fcg_cheb(const cf_matrix* A, VT* sval, const double* b, double* x, const
     for (int k = 1; k < kChebSteps; ++k) {
       int tok = -1;
       if (k == 1) prof_begin(1, sample, s, &tok);
-      cheb_step<VT><<<GS, kBlock, 0, s>>>(A->n_rows, L.n_slices, L.rows, L.koff, L.scol, sval,
-                                          D[(k - 1) & 1], D[k & 1], L.rh, L.yh, cd[k], cw[k],
-                                          L.state);
-      CF_LAUNCHED();
+      if ((r2 = cheb(cd[k], cw[k], D[(k - 1) & 1], D[k & 1]))) return r2;
       if (k == 1) prof_end(tok, s);
     }
     fcg_zdots<<<G, kBlock, 0, s>>>(n, L.r, L.s, L.yh, zc, zo, L.state, L.partials);
@@ -2252,11 +2362,11 @@ int pcg_fcg_cheb(const cf_matrix* A, VT* sval, const double* b, double* x, const
   // reference rhs b_f - A_fF x_F, then r = b - A x
   split_vector<<<G, kBlock, 0, s>>>(n, x, fixed, 1, L.p);
   CF_LAUNCHED();
-  if ((rc = spmv64(L.p, L.Ap, nullptr))) return rc;
+  if ((rc = outer(L.p, L.Ap, nullptr))) return rc;
   residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->pad);
   CF_LAUNCHED();
   if ((rc = fin(kFinStore, L, s, &L.state->pad))) return rc;
-  if ((rc = spmv64(x, L.Ap, nullptr))) return rc;
+  if ((rc = outer(x, L.Ap, nullptr))) return rc;
   residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->rr);
   CF_LAUNCHED();
   if ((rc = fin(kFinStore, L, s, &L.state->rr))) return rc;
@@ -2294,9 +2404,9 @@ int pcg_fcg_cheb(const cf_matrix* A, VT* sval, const double* b, double* x, const
     for (int t = 0; t < batch; ++t, ++enq) {
       int tok;
       prof_begin(0, int(enq), s, &tok);
-      if ((rc = spmv64(L.p, L.Ap, L.state))) return rc;
+      if ((rc = outer(L.p, L.Ap, L.state))) return rc;
       prof_end(tok, s);
-      if ((rc = fin(kFinAlpha, L, s, nullptr, 0.0, GS))) return rc;
+      if ((rc = fin(kFinAlpha, L, s, nullptr, 0.0, outer_parts))) return rc;
       if ((rc = precondition(1, Z[enq & 1], Z[(enq + 1) & 1], int(enq)))) return rc;
     }
     if ((rc = read_state(L.state, &h, s))) return rc;
@@ -2315,7 +2425,7 @@ int pcg_fcg_cheb(const cf_matrix* A, VT* sval, const double* b, double* x, const
   }
   st->iterations = h.iter;
   st->residual_updates = (kChebSteps - 1) * (h.iter + 1);  // low-precision products
-  if ((rc = spmv64(x, L.Ap, nullptr))) return rc;
+  if ((rc = outer(x, L.Ap, nullptr))) return rc;
   residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->pad);
   CF_LAUNCHED();
   if ((rc = fin(kFinStore, L, s, &L.state->pad))) return rc;
@@ -2333,6 +2443,50 @@ int pcg_fcg_cheb(const cf_matrix* A, VT* sval, const double* b, double* x, const
     return CF_ERR_NOT_CONVERGED;
   }
   return CF_OK;
+}
+
+// Assembled NB3 systems: products on the sliced copies (FP64 outer, S A S
+// stored as VT inner).
+template <class VT>
+int pcg_fcg_cheb(const cf_matrix* A, VT* sval, const double* b, double* x, const uint8_t* fixed,
+                 double tol, int maxit, const Layout& L, cf_pcg_stats* st, hipStream_t s) {
+  const int64_t n = n_dofs(A);
+  const int G = pcg_grid();
+  const int GS = int(sell_grid(L.n_slices));
+  int rc;
+  PcgState h{};
+  make_scale<<<G, kBlock, 0, s>>>(n, L.dinv, L.s);
+  CF_LAUNCHED();
+  if ((rc = build_sell(A, L, s)) || (rc = fill_sell<double>(A, L, nullptr, L.sval64, s)) ||
+      (rc = fill_sell<VT>(A, L, L.s, sval, s)))
+    return rc;
+  gershgorin_nb3<<<G, kBlock, 0, s>>>(A->n_rows, A->indptr, A->indices, A->values, L.s,
+                                      L.partials);
+  CF_LAUNCHED();
+  if ((rc = fin(kFinMax, L, s, &L.state->pad, 0.0, G))) return rc;
+  if ((rc = read_state(L.state, &h, s))) return rc;
+  const double gersh = h.pad;
+  auto outer = [&](const double* xin, double* y, PcgState* state) {
+    spmv_sell<double, double, double><<<GS, kBlock, 0, s>>>(
+        A->n_rows, L.n_slices, L.rows, L.koff, L.scol, L.sval64, xin, y, fixed, state,
+        L.partials);
+    CF_LAUNCHED();
+    return CF_OK;
+  };
+  auto power = [&](const float* v, float* y) {
+    spmv_sell<VT, float, float><<<GS, kBlock, 0, s>>>(A->n_rows, L.n_slices, L.rows, L.koff,
+                                                       L.scol, sval, v, y, nullptr, nullptr,
+                                                       nullptr);
+    CF_LAUNCHED();
+    return CF_OK;
+  };
+  auto cheb = [&](float cdk, float cwk, const float* din, float* dout) {
+    cheb_step<VT><<<GS, kBlock, 0, s>>>(A->n_rows, L.n_slices, L.rows, L.koff, L.scol, sval, din,
+                                        dout, L.rh, L.yh, cdk, cwk, L.state);
+    CF_LAUNCHED();
+    return CF_OK;
+  };
+  return fcg_cheb_core(outer, GS, power, cheb, gersh, n, b, x, fixed, tol, maxit, L, st, s);
 }
 
 // Scalar CSR systems (P1 heat, capacity-dominated theta-method steps): FP32
@@ -2962,6 +3116,119 @@ extern "C" int cf_pcg_solve_mf_t10_plan(int64_t n_nodes, int64_t n_elem, const i
                                        L, stats_host, s);
   return pcg_fp64_core<CF_PC_JACOBI>(apply, apply_parts, n, n, b, x, fixed, tol, maxit, L,
                                      stats_host, s);
+}
+
+namespace {
+
+struct MfMixed {
+  double* fe;      // FP64 element forces (outer product)
+  double* blocks;  // diagonal 3x3 blocks
+  float* geo32;    // geometry rounded to FP32
+  float* fe32;     // FP32 element forces (inner products)
+  float* s32;      // Jacobi scaling in FP32
+};
+
+size_t mf_mixed_layout(int64_t n_nodes, int64_t n_elem, void* base, Layout* L, MfMixed* M) {
+  cf::Carve c(base);
+  const int64_t n = 3 * n_nodes;
+  L->r = c.take<double>(n);
+  L->p = c.take<double>(n);
+  L->Ap = c.take<double>(n);
+  L->dinv = c.take<double>(n);
+  L->partials = c.take<double>(4 * size_t(std::max<int64_t>(pcg_grid(), mf_gather_grid(n_nodes))));
+  L->partials_b = c.take<double>(2 * size_t(pcg_vec_grid()));
+  L->state = c.take<PcgState>(1);
+  L->bad = c.take<int32_t>(1);
+  L->s = c.take<double>(n);
+  L->rh = c.take<float>(n);
+  L->ph = c.take<float>(n);
+  L->Aph = c.take<float>(n);
+  L->yh = c.take<float>(n);
+  L->z0 = c.take<double>(n);
+  L->z1 = c.take<double>(n);
+  M->fe = c.take<double>(30 * n_elem);
+  M->blocks = c.take<double>(9 * n_nodes);
+  M->geo32 = c.take<float>(10 * n_elem);
+  M->fe32 = c.take<float>(30 * n_elem);
+  M->s32 = c.take<float>(n);
+  L->total = c.used;
+  return c.used;
+}
+
+}  // namespace
+
+extern "C" int cf_mf_t10_mixed_workspace_bytes(int64_t n_nodes, int64_t n_elem, size_t* bytes) {
+  CF_REQUIRE(n_nodes > 0 && n_elem > 0 && bytes, "cf_mf_t10_mixed_workspace_bytes: bad arguments");
+  Layout L{};
+  MfMixed M{};
+  *bytes = mf_mixed_layout(n_nodes, n_elem, nullptr, &L, &M);
+  return CF_OK;
+}
+
+extern "C" int cf_pcg_solve_mf_t10_mixed(int64_t n_nodes, int64_t n_elem, const int32_t* conn,
+                                         const double* geo, double lam, double mu,
+                                         const int32_t* offs, const uint32_t* perm,
+                                         const uint32_t* slot, const double* b, double* x,
+                                         const uint8_t* fixed, double tol, int32_t maxit,
+                                         void* ws, size_t ws_bytes, cf_pcg_stats* stats_host,
+                                         void* stream) {
+  MfArgs a{n_nodes, n_elem, conn, geo, lam, mu, offs, perm, slot};
+  int rc = mf_check(a);
+  if (rc) return rc;
+  CF_REQUIRE(slot && b && x && ws && stats_host, "cf_pcg_solve_mf_t10_mixed: null argument");
+  CF_REQUIRE(tol > 0.0 && maxit > 0, "cf_pcg_solve_mf_t10_mixed: tol and maxit must be positive");
+  Layout L{};
+  MfMixed M{};
+  const size_t need = mf_mixed_layout(n_nodes, n_elem, ws, &L, &M);
+  if (ws_bytes < need) {
+    cf::set_error("cf_pcg_solve_mf_t10_mixed: workspace %zu < %zu bytes", ws_bytes, need);
+    return CF_ERR_WORKSPACE;
+  }
+  hipStream_t s = cf::as_stream(stream);
+  const int G = pcg_grid();
+  const int64_t n = 3 * n_nodes;
+  *stats_host = cf_pcg_stats{};
+  CF_HIP(hipMemsetAsync(L.state, 0, sizeof(PcgState), s));
+  CF_HIP(hipMemsetAsync(L.bad, 0, sizeof(int32_t), s));
+  mf_t10_diag<<<cf::stride_grid(n_nodes, kBlock), kBlock, 0, s>>>(n_nodes, offs, perm, geo, lam,
+                                                                  mu, M.blocks);
+  CF_LAUNCHED();
+  blocks_to_precond<<<cf::stride_grid(n_nodes, kBlock), kBlock, 0, s>>>(n_nodes, M.blocks, fixed,
+                                                                        0, L.dinv, L.bad);
+  CF_LAUNCHED();
+  int32_t bad = 0;
+  CF_HIP(hipMemcpyAsync(&bad, L.bad, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  CF_HIP(hipStreamSynchronize(s));
+  if (bad) {
+    cf::set_error("cf_pcg_solve_mf_t10_mixed: %d free diagonal entries are not positive", bad);
+    return CF_ERR_BREAKDOWN;
+  }
+  make_scale<<<G, kBlock, 0, s>>>(n, L.dinv, L.s);
+  CF_LAUNCHED();
+  to_f32<<<G, kBlock, 0, s>>>(n, L.s, M.s32);
+  CF_LAUNCHED();
+  geo_to_f32<<<cf::stride_grid(10 * n_elem, kBlock), kBlock, 0, s>>>(10 * n_elem, geo, M.geo32);
+  CF_LAUNCHED();
+  const int ge = cf::stride_grid(n_elem, kBlock);
+  const int gg = int(mf_gather_grid(n_nodes));
+  auto outer = [&](const double* xin, double* y, PcgState* state) {
+    return mf_apply(a, xin, y, fixed, M.fe, state, state ? L.partials : nullptr, s);
+  };
+  auto product32 = [&](const float* din, float* dout, int mode, float cdk, float cwk,
+                       const PcgState* state) {
+    mf_t10_elem32<<<ge, kBlock, 0, s>>>(n_elem, conn, M.geo32, float(lam), float(mu), M.s32, din,
+                                        slot, M.fe32, state);
+    CF_LAUNCHED();
+    mf_gather_cheb32<<<gg, kBlock, 0, s>>>(n_nodes, offs, M.fe32, M.s32, mode, din, dout, L.rh,
+                                           L.yh, cdk, cwk, state);
+    CF_LAUNCHED();
+    return CF_OK;
+  };
+  auto power = [&](const float* v, float* y) { return product32(v, y, 0, 0.0f, 0.0f, nullptr); };
+  auto cheb = [&](float cdk, float cwk, const float* din, float* dout) {
+    return product32(din, dout, 1, cdk, cwk, L.state);
+  };
+  return fcg_cheb_core(outer, gg, power, cheb, 0.0, n, b, x, fixed, tol, maxit, L, stats_host, s);
 }
 
 extern "C" int cf_invert_permutation(int64_t n, const uint32_t* perm, uint32_t* inv,

@@ -85,9 +85,10 @@ def assemble_elasticity_tet10(mesh, E, nu, body=None, traction=None, T=None, alp
 def solve_device(dm, vals, rhs, mesh, bc, lam=None, mu=None, **pcg_kw):
     """Solve on the device; returns (u (3N,) tensor, stats).
 
-    With solver option operator="matfree" (FP64 only) the stiffness is applied
-    matrix-free from the geometry record (needs lam, mu); otherwise the
-    assembled NB3 matrix `vals` is used.
+    With solver option operator="matfree" the stiffness is applied matrix-free
+    from the geometry record (needs lam, mu; precision "mixed" adds the FP32
+    matrix-free Chebyshev preconditioner); otherwise the assembled NB3 matrix
+    `vals` is used.
     """
     n3 = 3 * dm.n_nodes
     if bc:
@@ -97,10 +98,13 @@ def solve_device(dm, vals, rhs, mesh, bc, lam=None, mu=None, **pcg_kw):
     x = to_dev(np.where(mask, fv, 0.0), torch.float64, dm.dev)
     fixed = to_dev(mask.astype(np.uint8), torch.uint8, dm.dev) if mask.any() else None
     o = solver_mod.options()
-    if (o["operator"] == "matfree" and lam is not None and
-            pcg_kw.get("precision", o["precision"]) == "fp64"):
+    prec = pcg_kw.get("precision", o["precision"])
+    if o["operator"] == "matfree" and lam is not None and prec == "fp64":
         kw = {k: v for k, v in pcg_kw.items() if k != "precision"}
         stats = solver_mod.pcg_matfree_t10(dm, lam, mu, rhs, x, fixed, **kw)
+    elif o["operator"] == "matfree" and lam is not None and prec in ("mixed", "mixed16"):
+        kw = {k: v for k, v in pcg_kw.items() if k in ("tol", "maxit")}
+        stats = solver_mod.pcg_matfree_t10_mixed(dm, lam, mu, rhs, x, fixed, **kw)
     else:
         A = DeviceMatrix(N.FMT_NB3, dm.pattern(), vals)
         stats = pcg(A, rhs, x, fixed, **pcg_kw)

@@ -20,7 +20,9 @@ problems:
   maxit      iteration cap (default 200000)
   operator   "assembled" (default: CSR/NB3 SpMV) or "matfree" (Tet10
              elasticity only: element forces recomputed from the geometry
-             record every product, cf_pcg_solve_mf_t10)
+             record every product, cf_pcg_solve_mf_t10; with precision
+             "mixed" the Chebyshev preconditioner's products are matrix-free
+             in FP32, cf_pcg_solve_mf_t10_mixed)
 """
 
 import contextlib
@@ -112,6 +114,31 @@ def pcg(A, b, x, fixed=None, precision=None, precond=None, tol=None, maxit=None)
              "precond": "block3" if pc == N.PC_BLOCK3 else "jacobi", "tol": tol}
     last_stats.value = stats
     N.check(rc, "cf_pcg_solve")
+    return stats
+
+
+def pcg_matfree_t10_mixed(dm, lam, mu, b, x, fixed=None, tol=None, maxit=None):
+    """Mixed precision with the matrix-free Tet10 operator: FP64 flexible CG,
+    FP32 matrix-free Chebyshev preconditioner (cf_pcg_solve_mf_t10_mixed)."""
+    o = options()
+    tol = float(tol if tol is not None else o["tol"])
+    maxit = int(maxit if maxit is not None else o["maxit"])
+    lib = N.load()
+    conn, geo, offs, perm, slot = dm.matfree_arrays()
+    nb = ctypes.c_size_t()
+    N.check(lib.cf_mf_t10_mixed_workspace_bytes(dm.n_nodes, dm.n_elem, ctypes.byref(nb)),
+            "cf_mf_t10_mixed_workspace_bytes")
+    ws = workspace(nb.value, b.device)
+    st = N.CfPcgStats()
+    rc = lib.cf_pcg_solve_mf_t10_mixed(dm.n_nodes, dm.n_elem, ptr(conn), ptr(geo), float(lam),
+                                       float(mu), ptr(offs), ptr(perm), ptr(slot), ptr(b), ptr(x),
+                                       ptr(fixed), tol, maxit, ptr(ws), nb.value,
+                                       ctypes.byref(st), stream_handle())
+    stats = {"iterations": st.iterations, "residual_updates": st.residual_updates,
+             "rel_residual": st.rel_residual, "b_norm": st.b_norm, "precision": "mixed",
+             "precond": "jacobi", "tol": tol, "operator": "matfree"}
+    last_stats.value = stats
+    N.check(rc, "cf_pcg_solve_mf_t10_mixed")
     return stats
 
 

@@ -367,6 +367,20 @@ int cf_boundary_faces(int64_t n_elem, const int32_t* conn, int32_t k, int64_t n_
                       const uint8_t* member, int32_t width, int32_t* faces,
                       int64_t* count, void* ws, size_t ws_bytes, void* stream);
 
+/* ---- mixed precision with the matrix-free operator: FP64 flexible CG on
+ *      the FP64 matrix-free product (as cf_pcg_solve_mf_t10), preconditioned
+ *      by 7 Chebyshev steps on S K S (S = diag(K)^-1/2) whose products are
+ *      matrix-free in FP32 (geometry rounded once to FP32, forces and node
+ *      sums in FP32).  The outer residual, recurrence and solution are FP64,
+ *      so the result meets the same residual bar.  slot is required.        */
+int cf_mf_t10_mixed_workspace_bytes(int64_t n_nodes, int64_t n_elem, size_t* bytes);
+int cf_pcg_solve_mf_t10_mixed(int64_t n_nodes, int64_t n_elem, const int32_t* conn,
+                              const double* geo, double lam, double mu,
+                              const int32_t* offs, const uint32_t* perm,
+                              const uint32_t* slot, const double* b, double* x,
+                              const uint8_t* fixed, double tol, int32_t maxit, void* ws,
+                              size_t ws_bytes, cf_pcg_stats* stats_host, void* stream);
+
 /* ---- inv[perm[j]] = j for a permutation of 0..n-1.                     */
 int cf_invert_permutation(int64_t n, const uint32_t* perm, uint32_t* inv,
                           void* stream);

@@ -107,6 +107,33 @@ def test_mixed_precision_meets_fp64_residual(gpu, precision, dims, tol):
     assert err < 1e-8
 
 
+@pytest.mark.parametrize("dims,tol", [((8, 2, 2), 1e-10), ((20, 3, 3), 1e-9)])
+def test_matfree_mixed_meets_fp64_residual(gpu, dims, tol):
+    """FP64 flexible CG on the matrix-free operator with the FP32 matrix-free
+    Chebyshev preconditioner: FP64 residual bar, error against SuperLU."""
+    from cloudfem_amd import solver
+    from cloudfem_amd.mesh import box_mesh_tet10
+    from cloudfem_amd.physics import elastic_tet10
+    lims = dict(xlim=(0.0, 0.4), ylim=(0.0, 0.05), zlim=(0.0, 0.05))
+    trac = {"xmax": (0.0, 0.0, -1e6)}
+    bc = {"xmin": (0.0, 0.0, 0.0)}
+    m = box_mesh_tet10(*dims, **lims)
+    with solver.solver_options(operator="matfree", precision="mixed", tol=tol):
+        u = np.stack(elastic_tet10.solve_elasticity_tet10(m, 210e9, 0.3, bc=bc, traction=trac),
+                     axis=1).ravel()
+    st = solver.last_stats.value
+    assert st["operator"] == "matfree" and st["precision"] == "mixed"
+    m0 = om.box_mesh_tet10(*dims, **lims)
+    A0, rhs0 = oel.assemble_elasticity_tet10(m0, 210e9, 0.3, traction=trac)
+    u0 = oel.apply_dirichlet_vec3(A0, rhs0, m0, bc)
+    err = rel_err(u, u0)
+    print(f"matfree mixed {dims}: {st['iterations']} FP64 iterations, "
+          f"{st['residual_updates']} FP32 products, rel. residual {st['rel_residual']:.2e}, "
+          f"error vs SuperLU {err:.2e}")
+    assert st["rel_residual"] <= tol and st["residual_updates"] >= 1
+    assert err < 1e-8
+
+
 @pytest.mark.parametrize("precision", ["mixed", "mixed16"])
 def test_p1_mixed_precision_matches_spsolve(gpu, precision):
     """Scalar CSR systems take the FP32 inner CG with FP64 residual
