@@ -46,7 +46,7 @@ WORKLOADS = {
 LIMS = dict(xlim=(0.0, 0.4), ylim=(0.0, 0.05), zlim=(0.0, 0.05))
 E_MOD, NU, TRACTION, TOL = 210e9, 0.3, -1e6, 1e-10
 HBM_PEAK_GBS = 8000.0
-CPU_SAMPLE = (50, 6, 6)   # ~16 s of single-core CPU work
+CPU_SAMPLE = (56, 6, 6)   # ~10 s of single-core CPU work on the GPU host
 
 
 def parse():
@@ -150,18 +150,13 @@ def bench_batch_heat(args):
 def measured_traffic(workload, path):
     """HBM bytes per launch of the path's operator from the committed counter
     summary (two rocprofv3 --pmc passes, FETCH_SIZE and WRITE_SIZE, scaled by
-    a 256 MiB copy), or None when none was taken for this workload."""
+    a 256 MiB copy), or None when none was taken for this workload/path."""
     f = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload}.json")
     if not os.path.exists(f):
         return None
     with open(f) as fh:
         d = json.load(fh)
-    if path in ("matfree", "matfree_lds"):
-        return d.get("matfree_operator_traffic_bytes" if path == "matfree"
-                     else "matfree_lds_traffic_bytes")
-    kern = {"fp64": "spmv_sell<double, double, double>", "mixed": "cheb_step<float>",
-            "mixed16": "cheb_step<_Float16>", "mixed_mf": "mf_t10_elem32"}.get(path)
-    return d["kernels"].get(kern, {}).get("traffic_bytes")
+    return d.get("paths", {}).get(path)
 
 
 def run_cpu_baseline_subprocess():
