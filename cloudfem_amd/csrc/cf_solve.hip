@@ -823,6 +823,87 @@ gershgorin_nb3(int64_t n_nodes, const int32_t* __restrict__ ptr, const int32_t* 
 // assembly kernels' copy; each translation unit owns its constant memory).
 __constant__ cf::Tet10Tables kT10m = cf::build_tet10_tables();
 
+// Keast points in barycentric coordinates (centroid, then L_i = 1/2 with
+// the others 1/6) and weights -4/5, 9/20 (tet10.py:84-100), as compile-time
+// constants for the barycentric form below.
+constexpr double kQL[5][4] = {{0.25, 0.25, 0.25, 0.25},
+                              {0.5, 1.0 / 6, 1.0 / 6, 1.0 / 6},
+                              {1.0 / 6, 0.5, 1.0 / 6, 1.0 / 6},
+                              {1.0 / 6, 1.0 / 6, 0.5, 1.0 / 6},
+                              {1.0 / 6, 1.0 / 6, 1.0 / 6, 0.5}};
+constexpr double kQW[5] = {-0.8, 0.45, 0.45, 0.45, 0.45};
+constexpr int kTE[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+
+// Element forces in barycentric form.  With N_i = L_i (2 L_i - 1) and
+// N_ij = 4 L_i L_j, dN/dL_i is 4 L_i - 1 on corner i and 4 L_j on the
+// midpoints of the edges (i, j), so the barycentric derivatives of u are
+// g_i = (4 L_i - 1) x_i + 4 sum_j L_j x_ij and the reference-coordinate
+// gradient is U[c][k] = g_{k+1}[c] - g_0[c]; the forces are the transpose:
+// f_i += (4 L_i - 1) Q_i, f_ij += 4 (L_j Q_i + L_i Q_j) with
+// Q_{k+1}[c] = P[c][k], Q_0[c] = -sum_k P[c][k].  The same products as the
+// tabulated dN/dxi form (elastic_tet10.py:164-170), in about 1000
+// instead of 1300 multiply-adds per element, the centroid's corner terms
+// vanishing at compile time.
+template <class T>
+__device__ __forceinline__ void t10_forces_bary(const T (&xa)[10][3], const T (&J)[3][3], T V,
+                                                T lam, T mu, T (&f)[10][3]) {
+#pragma unroll
+  for (int a = 0; a < 10; ++a)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f[a][c] = T(0);
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    T g[4][3];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) g[i][c] = T(4.0 * kQL[q][i] - 1.0) * xa[i][c];
+#pragma unroll
+    for (int e = 0; e < 6; ++e) {
+      const int i = kTE[e][0], j = kTE[e][1];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        g[i][c] += T(4.0 * kQL[q][j]) * xa[4 + e][c];
+        g[j][c] += T(4.0 * kQL[q][i]) * xa[4 + e][c];
+      }
+    }
+    T H[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const T u0 = g[1][c] - g[0][c], u1 = g[2][c] - g[0][c], u2 = g[3][c] - g[0][c];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) H[c][i] = u0 * J[0][i] + u1 * J[1][i] + u2 * J[2][i];
+    }
+    const T ltr = lam * (H[0][0] + H[1][1] + H[2][2]);
+    const T vw = V * T(kQW[q]);
+    T Q[4][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      T sg[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) sg[i] = mu * (H[c][i] + H[i][c]) + (c == i ? ltr : T(0));
+      T P[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) P[k] = vw * (sg[0] * J[k][0] + sg[1] * J[k][1] + sg[2] * J[k][2]);
+      Q[1][c] = P[0];
+      Q[2][c] = P[1];
+      Q[3][c] = P[2];
+      Q[0][c] = -(P[0] + P[1] + P[2]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) f[i][c] += T(4.0 * kQL[q][i] - 1.0) * Q[i][c];
+#pragma unroll
+    for (int e = 0; e < 6; ++e) {
+      const int i = kTE[e][0], j = kTE[e][1];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        f[4 + e][c] += T(4.0 * kQL[q][j]) * Q[i][c] + T(4.0 * kQL[q][i]) * Q[j][c];
+    }
+  }
+}
+
 // Element forces f_e = K_e x_e of isotropic Tet10 elasticity, evaluated at
 // the 5 Keast points: H_q = grad u = (sum_a x_a d_aq^T) Jinv,
 // sigma_q = lam tr(H) I + mu (H + H^T), f_a += V w_q sigma_q Jinv^T d_aq.
@@ -848,42 +929,7 @@ __device__ __forceinline__ void t10_forces(const int32_t* __restrict__ cn,
 #pragma unroll
     for (int c = 0; c < 3; ++c) xa[a][c] = x[3 * nd + c];
   }
-#pragma unroll
-  for (int a = 0; a < 10; ++a)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) f[a][c] = 0.0;
-#pragma unroll
-  for (int q = 0; q < 5; ++q) {
-    double U[3][3] = {};
-#pragma unroll
-    for (int a = 0; a < 10; ++a)
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) U[c][k] += xa[a][c] * kT10m.D[q][a][k];
-    double H[3][3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-      for (int i = 0; i < 3; ++i) H[c][i] = U[c][0] * J[0][i] + U[c][1] * J[1][i] + U[c][2] * J[2][i];
-    const double ltr = lam * (H[0][0] + H[1][1] + H[2][2]);
-    const double vw = V * kT10m.w[q];
-    double P[3][3];  // P[c][k] = V w_q sum_i sigma[c][i] Jinv[k][i]
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      double sg[3];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) sg[i] = mu * (H[c][i] + H[i][c]) + (c == i ? ltr : 0.0);
-#pragma unroll
-      for (int k = 0; k < 3; ++k) P[c][k] = vw * (sg[0] * J[k][0] + sg[1] * J[k][1] + sg[2] * J[k][2]);
-    }
-#pragma unroll
-    for (int a = 0; a < 10; ++a)
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-        f[a][c] += kT10m.D[q][a][0] * P[c][0] + kT10m.D[q][a][1] * P[c][1] +
-                   kT10m.D[q][a][2] * P[c][2];
-  }
+  t10_forces_bary<double>(xa, J, V, lam, mu, f);
 }
 
 // fe[e][a][c] (or the node-sorted slot of (e, a)) = forces of element e,
@@ -913,39 +959,8 @@ mf_t10_elem(int64_t m, const int32_t* __restrict__ conn, const double* __restric
 #pragma unroll
       for (int c = 0; c < 3; ++c) xa[a][c] = x[3 * nd + c];
     }
-    double f[10][3] = {};
-#pragma unroll
-    for (int q = 0; q < 5; ++q) {
-      double U[3][3] = {};
-#pragma unroll
-      for (int a = 0; a < 10; ++a)
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-          for (int k = 0; k < 3; ++k) U[c][k] += xa[a][c] * kT10m.D[q][a][k];
-      double H[3][3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int i = 0; i < 3; ++i) H[c][i] = U[c][0] * J[0][i] + U[c][1] * J[1][i] + U[c][2] * J[2][i];
-      const double ltr = lam * (H[0][0] + H[1][1] + H[2][2]);
-      const double vw = V * kT10m.w[q];
-      double P[3][3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        double sg[3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) sg[i] = mu * (H[c][i] + H[i][c]) + (c == i ? ltr : 0.0);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) P[c][k] = vw * (sg[0] * J[k][0] + sg[1] * J[k][1] + sg[2] * J[k][2]);
-      }
-#pragma unroll
-      for (int a = 0; a < 10; ++a)
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-          f[a][c] += kT10m.D[q][a][0] * P[c][0] + kT10m.D[q][a][1] * P[c][1] +
-                     kT10m.D[q][a][2] * P[c][2];
-    }
+    double f[10][3];
+    t10_forces_bary<double>(xa, J, V, lam, mu, f);
     if (slot) {
       // each nodal force goes to its position in node-sorted incidence order,
       // so the node sums read a contiguous range
@@ -1295,39 +1310,8 @@ mf_t10_elem32(int64_t m, const int32_t* __restrict__ conn, const float* __restri
 #pragma unroll
       for (int c = 0; c < 3; ++c) xa[a][c] = sc[3 * nd + c] * d[3 * nd + c];
     }
-    float f[10][3] = {};
-#pragma unroll
-    for (int q = 0; q < 5; ++q) {
-      float U[3][3] = {};
-#pragma unroll
-      for (int a = 0; a < 10; ++a)
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-          for (int k = 0; k < 3; ++k) U[c][k] += xa[a][c] * float(kT10m.D[q][a][k]);
-      float H[3][3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int i = 0; i < 3; ++i) H[c][i] = U[c][0] * J[0][i] + U[c][1] * J[1][i] + U[c][2] * J[2][i];
-      const float ltr = lam * (H[0][0] + H[1][1] + H[2][2]);
-      const float vw = V * float(kT10m.w[q]);
-      float P[3][3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        float sg[3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) sg[i] = mu * (H[c][i] + H[i][c]) + (c == i ? ltr : 0.0f);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) P[c][k] = vw * (sg[0] * J[k][0] + sg[1] * J[k][1] + sg[2] * J[k][2]);
-      }
-#pragma unroll
-      for (int a = 0; a < 10; ++a)
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-          f[a][c] += float(kT10m.D[q][a][0]) * P[c][0] + float(kT10m.D[q][a][1]) * P[c][1] +
-                     float(kT10m.D[q][a][2]) * P[c][2];
-    }
+    float f[10][3];
+    t10_forces_bary<float>(xa, J, V, lam, mu, f);
 #pragma unroll
     for (int a = 0; a < 10; ++a) {
       float* o = fe + int64_t(slot[e * 10 + a]) * 3;
