@@ -137,58 +137,6 @@ __device__ __forceinline__ int64_t tile_index(int64_t b) { return b; }
 template <class T>
 __device__ __forceinline__ T ld(const T* p) { return __builtin_nontemporal_load(p); }
 
-// y = A x for NB3 storage, L lanes per node row.  If `state` is set this is
-// the CG product: y is masked, p.Ap is reduced and alpha = rz / pAp.
-template <class VT, class XT, class AT, int L>
-__global__ void __launch_bounds__(kBlock)
-spmv_nb3(int64_t n_nodes, const int32_t* __restrict__ ptr,
-         const int32_t* __restrict__ idx, const VT* __restrict__ val,
-         const XT* __restrict__ x, XT* __restrict__ y,
-         const uint8_t* __restrict__ fixed, PcgState* state,
-         double* __restrict__ partials) {
-  if (state && state->halt) return;
-  const int lane = threadIdx.x % L;
-  const int group = threadIdx.x / L;
-  constexpr int kGroups = kBlock / L;
-  int64_t r0, r1;
-  block_rows(n_nodes, r0, r1);
-  double dot = 0.0;
-  for (int64_t i = r0 + group; i < r1; i += kGroups) {
-    const int32_t base = ptr[i];
-    const int32_t len = ptr[i + 1] - base;
-    const int32_t l3 = 3 * len, nv = 9 * len;
-    const VT* v = val + int64_t(9) * base;
-    const int32_t* col = idx + base;
-    AT a0 = 0, a1 = 0, a2 = 0;
-    for (int32_t f = lane; f < nv; f += L) {
-      const int c = (f >= l3) + (f >= 2 * l3);
-      const int32_t q = f - c * l3;
-      const int32_t kq = q / 3;
-      const int cp = q - 3 * kq;
-      const AT prod = AT(ld(v + f)) * AT(x[3 * int64_t(col[kq]) + cp]);
-      a0 += c == 0 ? prod : AT(0);
-      a1 += c == 1 ? prod : AT(0);
-      a2 += c == 2 ? prod : AT(0);
-    }
-    a0 = group_sum<L>(a0);
-    a1 = group_sum<L>(a1);
-    a2 = group_sum<L>(a2);
-    if (lane == 0) {
-      AT o[3] = {a0, a1, a2};
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const int64_t d = 3 * i + c;
-        if (fixed && fixed[d]) o[c] = AT(0);
-        y[d] = XT(o[c]);
-        if (state) dot += double(x[d]) * double(XT(o[c]));
-      }
-    }
-  }
-  if (!state) return;
-  double v1[1] = {dot};
-  block_partials<1>(v1, partials);
-}
-
 // NB3 SpMV, one node row per 64-lane wave.  The row's column indices (up to
 // 128 blocks) are loaded once, coalesced, and handed to the lanes that need
 // them with cross-lane permutes, so the x gathers do not wait on a
@@ -405,53 +353,6 @@ __device__ __forceinline__ void precond(const double* __restrict__ dinv,
 }
 
 // ------------------------------------------------------------- FP64 CG
-
-// x += alpha p; r -= alpha Ap; z = M^-1 r; reduce r.r and r.z.
-// Units: DOFs (Jacobi) or nodes (block3, 3 DOFs each).
-template <int PC>
-__global__ void __launch_bounds__(kBlock)
-cg_update(int64_t n_units, double* __restrict__ x, double* __restrict__ r,
-          const double* __restrict__ p, const double* __restrict__ Ap,
-          const double* __restrict__ dinv, PcgState* state,
-          double* __restrict__ partials) {
-  if (state->halt) return;
-  constexpr int W = (PC == CF_PC_BLOCK3) ? 3 : 1;
-  const double alpha = state->alpha;
-  double rr = 0.0, rz = 0.0;
-  for (int64_t u = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; u < n_units;
-       u += int64_t(gridDim.x) * blockDim.x) {
-#pragma unroll
-    for (int c = 0; c < W; ++c) {
-      const int64_t d = W * u + c;
-      x[d] += alpha * p[d];
-      const double rn = r[d] - alpha * Ap[d];
-      r[d] = rn;
-      rr += rn * rn;
-    }
-    double z[3];
-    precond<PC>(dinv, r, u, z);
-#pragma unroll
-    for (int c = 0; c < W; ++c) rz += r[W * u + c] * z[c];
-  }
-  double v[2] = {rr, rz};
-  block_partials<2>(v, partials);
-}
-
-template <int PC>
-__global__ void __launch_bounds__(kBlock)
-cg_direction(int64_t n_units, const double* __restrict__ r, double* __restrict__ p,
-             const double* __restrict__ dinv, const PcgState* state) {
-  if (state->halt) return;
-  constexpr int W = (PC == CF_PC_BLOCK3) ? 3 : 1;
-  const double beta = state->beta;
-  for (int64_t u = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; u < n_units;
-       u += int64_t(gridDim.x) * blockDim.x) {
-    double z[3];
-    precond<PC>(dinv, r, u, z);
-#pragma unroll
-    for (int c = 0; c < W; ++c) p[W * u + c] = z[c] + beta * p[W * u + c];
-  }
-}
 
 // Fused CG scalars.  Each block of the kernel after a reduction folds that
 // reduction's block partials itself (same order in every block, so every
