@@ -658,6 +658,113 @@ fcg_direction(int64_t n, const double* __restrict__ z, double* __restrict__ p,
     p[i] = z[i] + beta * p[i];
 }
 
+// Fused forms of the three kernels above: each block folds the reduction
+// it needs from the previous kernel's block partials (same order in every
+// block) and block 0 records the scalars, so an outer iteration needs no
+// one-block finalize launches.  rzs[slot] holds r.z of the iteration.
+
+// alpha = rzs[slot] / p.Ap (n_pa product partials); x, r update; w, d, z
+// start of the Chebyshev iteration; r.r partials into pb.
+__global__ void __launch_bounds__(kBlock)
+fcg_update_fused(int64_t n, double* __restrict__ x, double* __restrict__ r,
+                 const double* __restrict__ p, const double* __restrict__ Ap,
+                 const double* __restrict__ sc, float* __restrict__ w, float* __restrict__ d,
+                 float* __restrict__ z, float inv_theta, PcgState* state,
+                 const double* __restrict__ pa, int n_pa, double* __restrict__ pb, int slot) {
+  if (state->halt) return;
+  double f[1];
+  fold_partials<1>(pa, n_pa, f);
+  const double pAp = f[0];
+  const bool ok = pAp > 0.0 && isfinite(pAp);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    state->pAp = pAp;
+    if (ok)
+      state->alpha = state->rzs[slot] / pAp;
+    else
+      state->halt = kBreakdown;
+  }
+  if (!ok) return;
+  const double alpha = state->rzs[slot] / pAp;
+  double rr = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    x[i] += alpha * p[i];
+    const double rn = r[i] - alpha * Ap[i];
+    r[i] = rn;
+    rr += rn * rn;
+    const float wv = float(sc[i] * rn);
+    w[i] = wv;
+    const float dv = wv * inv_theta;
+    d[i] = dv;
+    z[i] = dv;
+  }
+  double v1[1] = {rr};
+  block_partials<1>(v1, pb);
+}
+
+// r.r from fcg_update_fused (when `count`: stop test, block 0 records iter,
+// rr, halt); z = S zh with (r.z, r.z_old) partials into pc.
+__global__ void __launch_bounds__(kBlock)
+fcg_zdots_fused(int64_t n, const double* __restrict__ r, const double* __restrict__ sc,
+                const float* __restrict__ zh, double* __restrict__ z,
+                const double* __restrict__ zold, PcgState* state, const double* __restrict__ pb,
+                int n_pb, int count, double* __restrict__ pc) {
+  if (state->halt) return;
+  if (count) {
+    double f[1];
+    fold_partials<1>(pb, n_pb, f);
+    const double rr = f[0];
+    const bool fin = isfinite(rr);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      const int32_t it = state->iter + 1;
+      state->iter = it;
+      state->rr = rr;
+      if (!fin)
+        state->halt = kBreakdown;
+      else if (rr <= state->stop2)
+        state->halt = kConverged;
+      else if (it >= state->maxit)
+        state->halt = kMaxit;
+    }
+    if (!fin || rr <= state->stop2) return;
+  }
+  double rz = 0.0, rzo = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const double zv = sc[i] * double(zh[i]);
+    z[i] = zv;
+    rz += r[i] * zv;
+    rzo += r[i] * zold[i];
+  }
+  double v[2] = {rz, rzo};
+  block_partials<2>(v, pc);
+}
+
+// (r.z, r.z_old) -> Polak-Ribiere beta with rzs[slot]; block 0 records
+// rzs[slot ^ 1]; p = z + beta p.
+__global__ void __launch_bounds__(kBlock)
+fcg_direction_fused(int64_t n, const double* __restrict__ z, double* __restrict__ p,
+                    PcgState* state, const double* __restrict__ pc, int n_pc, int slot) {
+  if (state->halt) return;
+  double v[2];
+  fold_partials<2>(pc, n_pc, v);
+  const bool ok = v[0] > 0.0 && isfinite(v[0]) && isfinite(v[1]);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (!ok) {
+      state->halt = kBreakdown;
+    } else {
+      state->beta = (v[0] - v[1]) / state->rzs[slot];
+      state->rzs[slot ^ 1] = v[0];
+      state->rz = v[0];
+    }
+  }
+  if (!ok) return;
+  const double beta = (v[0] - v[1]) / state->rzs[slot];
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    p[i] = z[i] + beta * p[i];
+}
+
 // Power iteration on S A S for the top of the spectrum: v0 = S 1 spread by a
 // fixed hash, then v = y / ||y|| with y = S A S v.
 __global__ void power_start(int64_t n, const double* __restrict__ sc, float* __restrict__ v) {
@@ -2224,23 +2331,31 @@ int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, do
   const float inv_theta = float(1.0 / theta);
   float* D[2] = {L.ph, L.Aph};
   double* Z[2] = {L.z0, L.z1};
-  // w = rh, zh = yh; z = S p(S A S) S r into zc, r.z against zo
-  auto precondition = [&](int apply_step, double* zc, const double* zo, int sample) {
-    fcg_update<<<G, kBlock, 0, s>>>(n, x, L.r, L.p, L.Ap, L.s, L.rh, D[0], L.yh, inv_theta,
-                                    apply_step, L.state, L.partials);
+  // One outer iteration is: product (p.Ap partials), fcg_update_fused (x, r
+  // and the Chebyshev start), the Chebyshev steps, fcg_zdots_fused (stop test,
+  // z = S zh), fcg_direction_fused -- no finalize launches.  w = rh, zh = yh.
+  const int GV = pcg_vec_grid();
+  auto precondition = [&](int apply_step, double* zc, const double* zo, int sample, int slot) {
+    if (apply_step) {
+      fcg_update_fused<<<GV, kBlock, 0, s>>>(n, x, L.r, L.p, L.Ap, L.s, L.rh, D[0], L.yh,
+                                             inv_theta, L.state, L.partials, outer_parts,
+                                             L.partials_b, slot);
+    } else {
+      fcg_update<<<G, kBlock, 0, s>>>(n, x, L.r, L.p, L.Ap, L.s, L.rh, D[0], L.yh, inv_theta, 0,
+                                      L.state, L.partials);
+    }
     CF_LAUNCHED();
     int r2;
-    if (apply_step && (r2 = fin(kFinFcgCheck, L, s))) return r2;
     for (int k = 1; k < kChebSteps; ++k) {
       int tok = -1;
       if (k == 1) prof_begin(1, sample, s, &tok);
       if ((r2 = cheb(cd[k], cw[k], D[(k - 1) & 1], D[k & 1]))) return r2;
       if (k == 1) prof_end(tok, s);
     }
-    fcg_zdots<<<G, kBlock, 0, s>>>(n, L.r, L.s, L.yh, zc, zo, L.state, L.partials);
+    fcg_zdots_fused<<<GV, kBlock, 0, s>>>(n, L.r, L.s, L.yh, zc, zo, L.state, L.partials_b, GV,
+                                          apply_step, L.partials);
     CF_LAUNCHED();
-    if ((r2 = fin(kFinFcgBeta, L, s))) return r2;
-    fcg_direction<<<G, kBlock, 0, s>>>(n, zc, L.p, L.state);
+    fcg_direction_fused<<<GV, kBlock, 0, s>>>(n, zc, L.p, L.state, L.partials, GV, slot);
     CF_LAUNCHED();
     return CF_OK;
   };
@@ -2275,12 +2390,13 @@ int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, do
   h.maxit = maxit;
   h.iter = 0;
   h.rz = 1.0;
+  h.rzs[0] = h.rzs[1] = 1.0;
   h.halt = h.rr <= h.stop2 ? kConverged : kRunning;
   CF_HIP(hipMemcpyAsync(L.state, &h, sizeof(PcgState), hipMemcpyHostToDevice, s));
-  // first direction: z_old = 0 and p = 0, so p = z and rz = r.z
+  // first direction: z_old = 0 and p = 0, so p = z, and r.z lands in rzs[0]
   CF_HIP(hipMemsetAsync(Z[0], 0, size_t(n) * sizeof(double), s));
   CF_HIP(hipMemsetAsync(L.p, 0, size_t(n) * sizeof(double), s));
-  if (h.halt == kRunning && (rc = precondition(0, Z[1], Z[0], 1))) return rc;
+  if (h.halt == kRunning && (rc = precondition(0, Z[1], Z[0], 1, 1))) return rc;
   int64_t enq = 0;
   int batch = 8;
   double prev_rr = h.rr;
@@ -2291,8 +2407,7 @@ int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, do
       prof_begin(0, int(enq), s, &tok);
       if ((rc = outer(L.p, L.Ap, L.state))) return rc;
       prof_end(tok, s);
-      if ((rc = fin(kFinAlpha, L, s, nullptr, 0.0, outer_parts))) return rc;
-      if ((rc = precondition(1, Z[enq & 1], Z[(enq + 1) & 1], int(enq)))) return rc;
+      if ((rc = precondition(1, Z[enq & 1], Z[(enq + 1) & 1], int(enq), int(enq & 1)))) return rc;
     }
     if ((rc = read_state(L.state, &h, s))) return rc;
     prof_collect();
