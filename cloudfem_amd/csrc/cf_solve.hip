@@ -1535,7 +1535,13 @@ __global__ void blocks_to_precond(int64_t n_nodes, const double* __restrict__ bl
 // at vals[((koff[s] + g) * 9 + q) * 64 + lane], q = 3c + c'.  Every load of
 // the product is a coalesced 64-lane access; the 4 phases of a row are summed
 // with two lane swaps at the end; padding (rows shorter than the slice's
-// longest, about 5 % on Tet10 box meshes) holds zero values and column 0.
+// longest, about 5 % on Tet10 box meshes) holds zero values and column -1.
+// The FP64 product skips the loads of padding lanes: in the last step of a
+// slice of equal-length rows whole phases are padding, so 16 lanes (one
+// 128-byte line of each value plane) are never fetched.  The FP32/FP16
+// copies load them (zeros, gathering x[0]): a padding phase is only half or
+// a quarter of a line there, and the divergent masking cost more than it saved
+// (cheb_step<float> 124 vs 117 us, <_Float16> 226 vs 179 us at 539,601 rows).
 // Four phases give 4x more waves than one-row-per-lane slices, which keeps
 // every SIMD busy on meshes of a few 10^5 nodes.
 constexpr int kSliceRows = 16;
@@ -1595,7 +1601,7 @@ __global__ void sell_fill(int64_t n, int64_t n_slices, const int32_t* __restrict
     for (int32_t g = 0; g < steps; ++g) {
       const int32_t k = g * kPhases + h;
       const int64_t at = int64_t(g0) + g;
-      int32_t col = 0;
+      int32_t col = -1;
       double v[9] = {};
       if (k < len) {
         col = idx[base + k];
@@ -1627,19 +1633,20 @@ __device__ __forceinline__ int64_t sell_rows(int64_t n, int64_t sl,
                                              const VT* __restrict__ sval,
                                              const XT* __restrict__ x, AT (&o)[3]) {
   constexpr int U = kSellUnroll;
+  constexpr bool kMask = sizeof(VT) == 8;
   const int lane = threadIdx.x & 63;
   const int64_t lin = sl * kSliceRows + (lane % kSliceRows);
   const int32_t g0 = koff[sl], g1 = koff[sl + 1];
   AT a0 = 0, a1 = 0, a2 = 0;
   int32_t c[U];
 #pragma unroll
-  for (int u = 0; u < U; ++u) c[u] = g0 + u < g1 ? scol[int64_t(g0 + u) * 64 + lane] : 0;
+  for (int u = 0; u < U; ++u) c[u] = g0 + u < g1 ? scol[int64_t(g0 + u) * 64 + lane] : -1;
   for (int32_t g = g0; g < g1; g += U) {
     AT xv[U][3], v[U][9];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int64_t c3 = 3 * int64_t(c[u]);
-      const bool ok = g + u < g1;
+      const bool ok = g + u < g1 && (!kMask || c[u] >= 0);
+      const int64_t c3 = c[u] >= 0 ? 3 * int64_t(c[u]) : 0;
       xv[u][0] = ok ? AT(x[c3]) : AT(0);
       xv[u][1] = ok ? AT(x[c3 + 1]) : AT(0);
       xv[u][2] = ok ? AT(x[c3 + 2]) : AT(0);
@@ -1647,13 +1654,13 @@ __device__ __forceinline__ int64_t sell_rows(int64_t n, int64_t sl,
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const VT* p = sval + int64_t(g + u) * 9 * 64 + lane;
-      const bool ok = g + u < g1;
+      const bool ok = g + u < g1 && (!kMask || c[u] >= 0);
 #pragma unroll
       for (int q = 0; q < 9; ++q) v[u][q] = ok ? AT(ld(p + q * 64)) : AT(0);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      c[u] = g + U + u < g1 ? scol[int64_t(g + U + u) * 64 + lane] : 0;
+      c[u] = g + U + u < g1 ? scol[int64_t(g + U + u) * 64 + lane] : -1;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       a0 += v[u][0] * xv[u][0] + v[u][1] * xv[u][1] + v[u][2] * xv[u][2];
