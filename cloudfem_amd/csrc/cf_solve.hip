@@ -2284,14 +2284,24 @@ int pcg_fp64(const cf_matrix* A, const double* b, double* x, const uint8_t* fixe
 }
 
 // NB3 systems: FP64 flexible CG, preconditioned by S p(S A S) S with p the
-// Chebyshev iteration of degree kChebSteps on [b / kChebRatio, b] run in FP32
-// on a sliced copy of S A S stored as VT (float or _Float16).  b is the
-// smaller of the Gershgorin bound and 1.25x a 40-step power estimate; p(t) > 0
-// for every t < b + b / kChebRatio, so the preconditioner is SPD even where
+// Chebyshev iteration (steps - 1 products) on [b / ratio, b] run in FP32 on a
+// sliced copy of S A S stored as VT (float or _Float16), or matrix-free.  b is
+// the smaller of the Gershgorin bound and 1.25x a 40-step power estimate;
+// p(t) > 0 for every t < b + b / ratio, so the preconditioner is SPD even where
 // rounding of the stored values shifts the lowest eigenvalues.  Polak-Ribiere
 // beta keeps the outer iteration robust to the FP32 rounding inside p.
-constexpr int kChebSteps = 8;
-constexpr double kChebRatio = 100.0;
+// Degree and ratio, measured on the 160x20x20 Tet10 cantilever (ms per
+// solve, FP32 matrix-free / FP32 sliced / FP16 sliced): (8, 100) 579 / 1218 /
+// 1685; (16, 3000) 502 / 1106 / 1652; (20, 1000) 496 / 1134 / 1709;
+// (24, 3000) 493 / 1095 / 1864; (32, 10000) 500 / 1256 / 1974.  FP16
+// rounding of the stored values makes high degrees lose there.
+struct ChebShape {
+  int steps;
+  double ratio;
+};
+constexpr ChebShape kChebF32{24, 3000.0};
+constexpr ChebShape kChebF16{16, 3000.0};
+constexpr int kChebMaxSteps = 32;
 constexpr int kPowerSteps = 40;
 
 // The flexible CG + Chebyshev driver, independent of how the products are
@@ -2299,10 +2309,11 @@ constexpr int kPowerSteps = 40;
 // (state set: p.Ap partials, outer_parts of them); power(v, y): FP32
 // y = S A S v; cheb(cd, cw, d, dn, state): one Chebyshev step (w -= S A S d;
 // dn = cd d + cw w; z += dn on rh / yh).  gersh: Gershgorin bound of S A S,
-// or 0 when none is known (then b = 1.3 x the power estimate).
+// or 0 when none is known (then b = 1.3 x the power estimate); shape: degree
+// and interval ratio of the polynomial.
 template <class Outer, class Power, class Cheb>
 int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, double gersh,
-                  int64_t n, const double* b, double* x, const uint8_t* fixed, double tol,
+                  ChebShape shape, int64_t n, const double* b, double* x, const uint8_t* fixed, double tol,
                   int maxit, const Layout& L, cf_pcg_stats* st, hipStream_t s) {
   const int G = pcg_grid();
   int rc;
@@ -2325,11 +2336,16 @@ int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, do
                   "(power estimate %.3e, Gershgorin %.3e)", est, gersh);
     return CF_ERR_BREAKDOWN;
   }
-  const double amin = bmax / kChebRatio;
+  if (shape.steps < 2 || shape.steps > kChebMaxSteps || !(shape.ratio > 1.0)) {
+    cf::set_error("cf_pcg_solve: bad Chebyshev shape (%d steps, ratio %g)", shape.steps,
+                  shape.ratio);
+    return CF_ERR_ARG;
+  }
+  const double amin = bmax / shape.ratio;
   const double theta = 0.5 * (bmax + amin), delta = 0.5 * (bmax - amin), sigma = theta / delta;
-  float cd[kChebSteps], cw[kChebSteps];
+  float cd[kChebMaxSteps], cw[kChebMaxSteps];
   double rho0 = 1.0 / sigma;
-  for (int k = 1; k < kChebSteps; ++k) {
+  for (int k = 1; k < shape.steps; ++k) {
     const double rho1 = 1.0 / (2.0 * sigma - rho0);
     cd[k] = float(rho1 * rho0);
     cw[k] = float(2.0 * rho1 / delta);
@@ -2353,7 +2369,7 @@ int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, do
     }
     CF_LAUNCHED();
     int r2;
-    for (int k = 1; k < kChebSteps; ++k) {
+    for (int k = 1; k < shape.steps; ++k) {
       int tok = -1;
       if (k == 1) prof_begin(1, sample, s, &tok);
       if ((r2 = cheb(cd[k], cw[k], D[(k - 1) & 1], D[k & 1]))) return r2;
@@ -2431,7 +2447,7 @@ int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, do
     prev_it = h.iter;
   }
   st->iterations = h.iter;
-  st->residual_updates = (kChebSteps - 1) * (h.iter + 1);  // low-precision products
+  st->residual_updates = (shape.steps - 1) * (h.iter + 1);  // low-precision products
   if ((rc = outer(x, L.Ap, nullptr))) return rc;
   residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->pad);
   CF_LAUNCHED();
@@ -2493,7 +2509,8 @@ int pcg_fcg_cheb(const cf_matrix* A, VT* sval, const double* b, double* x, const
     CF_LAUNCHED();
     return CF_OK;
   };
-  return fcg_cheb_core(outer, GS, power, cheb, gersh, n, b, x, fixed, tol, maxit, L, st, s);
+  return fcg_cheb_core(outer, GS, power, cheb, gersh, sizeof(VT) == 2 ? kChebF16 : kChebF32, n,
+                       b, x, fixed, tol, maxit, L, st, s);
 }
 
 // Scalar CSR systems (P1 heat, capacity-dominated theta-method steps): FP32
@@ -3235,7 +3252,8 @@ extern "C" int cf_pcg_solve_mf_t10_mixed(int64_t n_nodes, int64_t n_elem, const 
   auto cheb = [&](float cdk, float cwk, const float* din, float* dout) {
     return product32(din, dout, 1, cdk, cwk, L.state);
   };
-  return fcg_cheb_core(outer, gg, power, cheb, 0.0, n, b, x, fixed, tol, maxit, L, stats_host, s);
+  return fcg_cheb_core(outer, gg, power, cheb, 0.0, kChebF32, n, b, x, fixed, tol, maxit, L,
+                       stats_host, s);
 }
 
 extern "C" int cf_invert_permutation(int64_t n, const uint32_t* perm, uint32_t* inv,

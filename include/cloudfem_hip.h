@@ -82,7 +82,8 @@ extern "C" {
 #define CF_PREC_FP64 0  /* plain FP64 PCG                                    */
 #define CF_PREC_MIXED 1 /* NB3: FP64 flexible CG preconditioned by an FP32
                            Chebyshev polynomial of the Jacobi-scaled matrix
-                           (FP32 stored values).  CSR: FP32 inner CG on the
+                           (FP32 stored values, 23 products on [b/3000, b];
+                           FP16 stored: 15 products).  CSR: FP32 inner CG on the
                            Jacobi-scaled matrix with FP64 residual
                            replacement + solution accumulation              */
 #define CF_PREC_MIXED16 2 /* as CF_PREC_MIXED, the polynomial's matrix copy
@@ -369,7 +370,7 @@ int cf_boundary_faces(int64_t n_elem, const int32_t* conn, int32_t k, int64_t n_
 
 /* ---- mixed precision with the matrix-free operator: FP64 flexible CG on
  *      the FP64 matrix-free product (as cf_pcg_solve_mf_t10), preconditioned
- *      by 7 Chebyshev steps on S K S (S = diag(K)^-1/2) whose products are
+ *      by 23 Chebyshev steps on S K S (S = diag(K)^-1/2) whose products are
  *      matrix-free in FP32 (geometry rounded once to FP32, forces and node
  *      sums in FP32).  The outer residual, recurrence and solution are FP64,
  *      so the result meets the same residual bar.  slot is required.        */
