@@ -600,7 +600,10 @@ __global__ void mx_accumulate(int64_t n, double* __restrict__ x, float* __restri
 
 // x += alpha p; r -= alpha Ap (skipped when !apply_step); r.r partials; then
 // the first Chebyshev step, which needs no product: w = fp32(S r),
-// d = w / theta, z = d.
+// d = w / theta, z = d.  SQ: the Chebyshev vectors are kept pre-multiplied
+// by S (w = fp32(S^2 r), ...), so a matrix-free product reads its input
+// without scaling it and the final z = S zh is zh itself.
+template <bool SQ>
 __global__ void __launch_bounds__(kBlock)
 fcg_update(int64_t n, double* __restrict__ x, double* __restrict__ r,
            const double* __restrict__ p, const double* __restrict__ Ap,
@@ -619,7 +622,7 @@ fcg_update(int64_t n, double* __restrict__ x, double* __restrict__ r,
       r[i] = rn;
     }
     rr += rn * rn;
-    const float wv = float(sc[i] * rn);
+    const float wv = float(SQ ? sc[i] * sc[i] * rn : sc[i] * rn);
     w[i] = wv;
     const float dv = wv * inv_theta;
     d[i] = dv;
@@ -665,6 +668,7 @@ fcg_direction(int64_t n, const double* __restrict__ z, double* __restrict__ p,
 
 // alpha = rzs[slot] / p.Ap (n_pa product partials); x, r update; w, d, z
 // start of the Chebyshev iteration; r.r partials into pb.
+template <bool SQ>
 __global__ void __launch_bounds__(kBlock)
 fcg_update_fused(int64_t n, double* __restrict__ x, double* __restrict__ r,
                  const double* __restrict__ p, const double* __restrict__ Ap,
@@ -692,7 +696,7 @@ fcg_update_fused(int64_t n, double* __restrict__ x, double* __restrict__ r,
     const double rn = r[i] - alpha * Ap[i];
     r[i] = rn;
     rr += rn * rn;
-    const float wv = float(sc[i] * rn);
+    const float wv = float(SQ ? sc[i] * sc[i] * rn : sc[i] * rn);
     w[i] = wv;
     const float dv = wv * inv_theta;
     d[i] = dv;
@@ -704,6 +708,7 @@ fcg_update_fused(int64_t n, double* __restrict__ x, double* __restrict__ r,
 
 // r.r from fcg_update_fused (when `count`: stop test, block 0 records iter,
 // rr, halt); z = S zh with (r.z, r.z_old) partials into pc.
+template <bool SQ>
 __global__ void __launch_bounds__(kBlock)
 fcg_zdots_fused(int64_t n, const double* __restrict__ r, const double* __restrict__ sc,
                 const float* __restrict__ zh, double* __restrict__ z,
@@ -731,7 +736,7 @@ fcg_zdots_fused(int64_t n, const double* __restrict__ r, const double* __restric
   double rz = 0.0, rzo = 0.0;
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x) {
-    const double zv = sc[i] * double(zh[i]);
+    const double zv = SQ ? double(zh[i]) : sc[i] * double(zh[i]);
     z[i] = zv;
     rz += r[i] * zv;
     rzo += r[i] * zold[i];
@@ -1296,6 +1301,9 @@ __global__ void geo_to_f32(int64_t n, const double* __restrict__ g, float* __res
     g32[i] = float(g[i]);
 }
 
+// PRE: d is already S-scaled (the Chebyshev vectors of the SQ form), so
+// the kernel gathers it as is; otherwise it gathers S d.
+template <bool PRE>
 __global__ void __launch_bounds__(kBlock)
 mf_t10_elem32(int64_t m, const int32_t* __restrict__ conn, const float* __restrict__ geo,
               float lam, float mu, const float* __restrict__ sc, const float* __restrict__ d,
@@ -1316,7 +1324,7 @@ mf_t10_elem32(int64_t m, const int32_t* __restrict__ conn, const float* __restri
     for (int a = 0; a < 10; ++a) {
       const int64_t nd = conn[e * 10 + a];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) xa[a][c] = sc[3 * nd + c] * d[3 * nd + c];
+      for (int c = 0; c < 3; ++c) xa[a][c] = PRE ? d[3 * nd + c] : sc[3 * nd + c] * d[3 * nd + c];
     }
     float f[10][3];
     t10_forces_bary<float>(xa, J, V, lam, mu, f);
@@ -1331,7 +1339,8 @@ mf_t10_elem32(int64_t m, const int32_t* __restrict__ conn, const float* __restri
 }
 
 // node sums of the FP32 forces, o = S sum; mode 0: y = o (power iteration);
-// mode 1: Chebyshev step  w -= o; d' = cd d + cw w; z += d'.
+// mode 1: Chebyshev step  w -= o; d' = cd d + cw w; z += d'.  mode 2: the
+// same step on S-scaled vectors (w, d, z all multiplied by S), o = S^2 sum.
 __global__ void __launch_bounds__(kBlock)
 mf_gather_cheb32(int64_t n_nodes, const int32_t* __restrict__ offs,
                  const float* __restrict__ fe, const float* __restrict__ sc, int mode,
@@ -1363,7 +1372,7 @@ mf_gather_cheb32(int64_t n_nodes, const int32_t* __restrict__ offs,
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const int64_t k = 3 * i + c;
-    const float oc = sc[k] * o[c];
+    const float oc = mode == 2 ? sc[k] * sc[k] * o[c] : sc[k] * o[c];
     if (mode == 0) {
       dn[k] = oc;
     } else {
@@ -2310,8 +2319,9 @@ constexpr int kPowerSteps = 40;
 // y = S A S v; cheb(cd, cw, d, dn, state): one Chebyshev step (w -= S A S d;
 // dn = cd d + cw w; z += dn on rh / yh).  gersh: Gershgorin bound of S A S,
 // or 0 when none is known (then b = 1.3 x the power estimate); shape: degree
-// and interval ratio of the polynomial.
-template <class Outer, class Power, class Cheb>
+// and interval ratio of the polynomial.  SQ: cheb works on S-scaled vectors
+// (see fcg_update).
+template <bool SQ, class Outer, class Power, class Cheb>
 int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, double gersh,
                   ChebShape shape, int64_t n, const double* b, double* x, const uint8_t* fixed, double tol,
                   int maxit, const Layout& L, cf_pcg_stats* st, hipStream_t s) {
@@ -2360,11 +2370,11 @@ int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, do
   const int GV = pcg_vec_grid();
   auto precondition = [&](int apply_step, double* zc, const double* zo, int sample, int slot) {
     if (apply_step) {
-      fcg_update_fused<<<GV, kBlock, 0, s>>>(n, x, L.r, L.p, L.Ap, L.s, L.rh, D[0], L.yh,
+      fcg_update_fused<SQ><<<GV, kBlock, 0, s>>>(n, x, L.r, L.p, L.Ap, L.s, L.rh, D[0], L.yh,
                                              inv_theta, L.state, L.partials, outer_parts,
                                              L.partials_b, slot);
     } else {
-      fcg_update<<<G, kBlock, 0, s>>>(n, x, L.r, L.p, L.Ap, L.s, L.rh, D[0], L.yh, inv_theta, 0,
+      fcg_update<SQ><<<G, kBlock, 0, s>>>(n, x, L.r, L.p, L.Ap, L.s, L.rh, D[0], L.yh, inv_theta, 0,
                                       L.state, L.partials);
     }
     CF_LAUNCHED();
@@ -2375,7 +2385,7 @@ int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, do
       if ((r2 = cheb(cd[k], cw[k], D[(k - 1) & 1], D[k & 1]))) return r2;
       if (k == 1) prof_end(tok, s);
     }
-    fcg_zdots_fused<<<GV, kBlock, 0, s>>>(n, L.r, L.s, L.yh, zc, zo, L.state, L.partials_b, GV,
+    fcg_zdots_fused<SQ><<<GV, kBlock, 0, s>>>(n, L.r, L.s, L.yh, zc, zo, L.state, L.partials_b, GV,
                                           apply_step, L.partials);
     CF_LAUNCHED();
     fcg_direction_fused<<<GV, kBlock, 0, s>>>(n, zc, L.p, L.state, L.partials, GV, slot);
@@ -2509,7 +2519,7 @@ int pcg_fcg_cheb(const cf_matrix* A, VT* sval, const double* b, double* x, const
     CF_LAUNCHED();
     return CF_OK;
   };
-  return fcg_cheb_core(outer, GS, power, cheb, gersh, sizeof(VT) == 2 ? kChebF16 : kChebF32, n,
+  return fcg_cheb_core<false>(outer, GS, power, cheb, gersh, sizeof(VT) == 2 ? kChebF16 : kChebF32, n,
                        b, x, fixed, tol, maxit, L, st, s);
 }
 
@@ -3238,10 +3248,16 @@ extern "C" int cf_pcg_solve_mf_t10_mixed(int64_t n_nodes, int64_t n_elem, const 
   auto outer = [&](const double* xin, double* y, PcgState* state) {
     return mf_apply(a, xin, y, fixed, M.fe, state, state ? L.partials : nullptr, s);
   };
+  // the power iteration multiplies plain vectors (S K S v); the Chebyshev
+  // steps run on S-scaled vectors, so their element kernel gathers no S
   auto product32 = [&](const float* din, float* dout, int mode, float cdk, float cwk,
                        const PcgState* state) {
-    mf_t10_elem32<<<ge, kBlock, 0, s>>>(n_elem, conn, M.geo32, float(lam), float(mu), M.s32, din,
-                                        slot, M.fe32, state);
+    if (mode == 0)
+      mf_t10_elem32<false><<<ge, kBlock, 0, s>>>(n_elem, conn, M.geo32, float(lam), float(mu),
+                                                 M.s32, din, slot, M.fe32, state);
+    else
+      mf_t10_elem32<true><<<ge, kBlock, 0, s>>>(n_elem, conn, M.geo32, float(lam), float(mu),
+                                                M.s32, din, slot, M.fe32, state);
     CF_LAUNCHED();
     mf_gather_cheb32<<<gg, kBlock, 0, s>>>(n_nodes, offs, M.fe32, M.s32, mode, din, dout, L.rh,
                                            L.yh, cdk, cwk, state);
@@ -3250,9 +3266,9 @@ extern "C" int cf_pcg_solve_mf_t10_mixed(int64_t n_nodes, int64_t n_elem, const 
   };
   auto power = [&](const float* v, float* y) { return product32(v, y, 0, 0.0f, 0.0f, nullptr); };
   auto cheb = [&](float cdk, float cwk, const float* din, float* dout) {
-    return product32(din, dout, 1, cdk, cwk, L.state);
+    return product32(din, dout, 2, cdk, cwk, L.state);
   };
-  return fcg_cheb_core(outer, gg, power, cheb, 0.0, kChebF32, n, b, x, fixed, tol, maxit, L,
+  return fcg_cheb_core<true>(outer, gg, power, cheb, 0.0, kChebF32, n, b, x, fixed, tol, maxit, L,
                        stats_host, s);
 }
 
