@@ -6,13 +6,13 @@
 // masked (their rows of A p are zeroed and their entries of r, z, p stay 0),
 // which is the same arithmetic as CG on A[free][:, free].
 //
-// Kernels per CG iteration (all grid-stride over a fixed grid, so the
-// per-block partial sums -- and therefore every dot product -- are
-// bitwise reproducible; the last block to finish folds the partials in a
-// fixed order, no floating-point atomics):
-//   1. spmv      Ap = A p (masked), p.Ap             -> alpha
-//   2. update    x += alpha p, r -= alpha Ap, z = M^-1 r, r.r, r.z -> beta
-//   3. direction p = z + beta p
+// Kernels per CG iteration (fixed grids, so the per-block partial sums --
+// and therefore every dot product -- are bitwise reproducible; each kernel
+// that needs a reduction folds the previous kernel's block partials itself,
+// in the same order in every block; no floating-point atomics):
+//   1. spmv      Ap = A p (masked), p.Ap partials
+//   2. update    alpha; x += alpha p, r -= alpha Ap, z = M^-1 r, r.r, r.z
+//   3. direction convergence test, beta; p = z + beta p
 // A device-side halt flag makes the remaining kernels of a launched batch
 // exit at once after convergence, so the host polls only once per batch.
 //
@@ -131,6 +131,17 @@ __device__ __forceinline__ void block_rows(int64_t n_rows, int64_t& r0, int64_t&
 // Tiled launches: grid = tile_grid(tiles) blocks, block b takes tile b, so
 // every block gets at most one tile (no tail round on a few blocks).
 __device__ __forceinline__ int64_t tile_index(int64_t b) { return b; }
+
+// Workgroups are dispatched round-robin over the kXcd dies, each with its own
+// L2.  xcd_tile(b, nb) maps launch block b of nb to a tile so that each die
+// walks one contiguous range of tiles (a bijection on [0, nb)).  Used where
+// the work per tile is uniform (elements): node neighbourhoods then stay in
+// one die's L2 (FP32 element forces 24.7 vs 26.0 us at 384,000 elements).
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
+  const int64_t q = nb / kXcd, r = nb - q * kXcd;
+  const int64_t d = b % kXcd, i = b / kXcd;
+  return d * q + (d < r ? d : r) + i;
+}
 
 // ------------------------------------------------------------- SpMV
 
@@ -600,9 +611,11 @@ __global__ void mx_accumulate(int64_t n, double* __restrict__ x, float* __restri
 
 // x += alpha p; r -= alpha Ap (skipped when !apply_step); r.r partials; then
 // the first Chebyshev step, which needs no product: w = fp32(S r),
-// d = w / theta, z = d.  SQ: the Chebyshev vectors are kept pre-multiplied
-// by S (w = fp32(S^2 r), ...), so a matrix-free product reads its input
-// without scaling it and the final z = S zh is zh itself.
+// d = w / theta, z = d.  SQ (three-term form, mf_gather_cheb32): the
+// Chebyshev vectors are kept pre-multiplied by S, w = fp32(S^2 r) and the
+// first iterate d = w / theta; z is not written (the steps build it from d),
+// so a matrix-free product reads its input without scaling it and the final
+// z = S zh is zh itself.
 template <bool SQ>
 __global__ void __launch_bounds__(kBlock)
 fcg_update(int64_t n, double* __restrict__ x, double* __restrict__ r,
@@ -626,7 +639,7 @@ fcg_update(int64_t n, double* __restrict__ x, double* __restrict__ r,
     w[i] = wv;
     const float dv = wv * inv_theta;
     d[i] = dv;
-    z[i] = dv;
+    if (!SQ) z[i] = dv;
   }
   double v1[1] = {rr};
   block_partials<1>(v1, partials);
@@ -700,7 +713,7 @@ fcg_update_fused(int64_t n, double* __restrict__ x, double* __restrict__ r,
     w[i] = wv;
     const float dv = wv * inv_theta;
     d[i] = dv;
-    z[i] = dv;
+    if (!SQ) z[i] = dv;
   }
   double v1[1] = {rr};
   block_partials<1>(v1, pb);
@@ -956,7 +969,7 @@ mf_t10_elem(int64_t m, const int32_t* __restrict__ conn, const double* __restric
             const uint32_t* __restrict__ slot, double* __restrict__ fe,
             const PcgState* state) {
   if (state && state->halt) return;
-  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < m;
+  for (int64_t e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; e < m;
        e += int64_t(gridDim.x) * blockDim.x) {
     const double* g = geo + e * 10;
     double J[3][3];
@@ -1310,7 +1323,7 @@ mf_t10_elem32(int64_t m, const int32_t* __restrict__ conn, const float* __restri
               const uint32_t* __restrict__ slot, float* __restrict__ fe,
               const PcgState* state) {
   if (state && state->halt) return;
-  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < m;
+  for (int64_t e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; e < m;
        e += int64_t(gridDim.x) * blockDim.x) {
     const float* g = geo + e * 10;
     float J[3][3];
@@ -1338,14 +1351,20 @@ mf_t10_elem32(int64_t m, const int32_t* __restrict__ conn, const float* __restri
   }
 }
 
-// node sums of the FP32 forces, o = S sum; mode 0: y = o (power iteration);
-// mode 1: Chebyshev step  w -= o; d' = cd d + cw w; z += d'.  mode 2: the
-// same step on S-scaled vectors (w, d, z all multiplied by S), o = S^2 sum.
+// node sums of the FP32 forces.  mode 0 (power iteration): zn = S sum.
+// Modes 1 and 2: one Chebyshev step in three-term form on S-scaled vectors
+// (z = S zhat, w = S^2 r fixed over the steps; the element kernel gathered
+// zc, so sum = K zc and S^2 sum = S (S K S) zhat):
+//   zn = zc + cd (zc - zp) + cw (w - S^2 sum),
+// with zp = 0 in mode 1 (the first step).  Five FP32 vector streams per
+// node (S, w, zc, zp read, zn written) instead of the seven of the
+// residual-recursive form.  zn may alias zp: each entry is read and then
+// written by the same thread.
 __global__ void __launch_bounds__(kBlock)
 mf_gather_cheb32(int64_t n_nodes, const int32_t* __restrict__ offs,
                  const float* __restrict__ fe, const float* __restrict__ sc, int mode,
-                 const float* __restrict__ d, float* __restrict__ dn, float* __restrict__ w,
-                 float* __restrict__ z, float cd, float cw, const PcgState* state) {
+                 const float* __restrict__ zc, const float* zp, float* zn,
+                 const float* __restrict__ w, float cd, float cw, const PcgState* state) {
   if (state && state->halt) return;
   const int64_t i = tile_index(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= n_nodes) return;
@@ -1372,15 +1391,13 @@ mf_gather_cheb32(int64_t n_nodes, const int32_t* __restrict__ offs,
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const int64_t k = 3 * i + c;
-    const float oc = mode == 2 ? sc[k] * sc[k] * o[c] : sc[k] * o[c];
+    const float s = sc[k];
     if (mode == 0) {
-      dn[k] = oc;
+      zn[k] = s * o[c];
     } else {
-      const float wn = w[k] - oc;
-      w[k] = wn;
-      const float dd = cd * d[k] + cw * wn;
-      dn[k] = dd;
-      z[k] += dd;
+      const float zk = zc[k];
+      const float dz = mode == 1 ? zk : zk - zp[k];
+      zn[k] = zk + cd * dz + cw * (w[k] - s * s * o[c]);
     }
   }
 }
@@ -1561,6 +1578,38 @@ constexpr int kSellUnroll = 4;
 // FP64 product at 539,601 node rows), the CG update folds the partials
 constexpr int kSlicesPerWave = 1;
 
+// Slices per die: the product's blocks are dealt round-robin to the kXcd
+// dies, so block b runs on die d = b % kXcd.  xsplit[d] .. xsplit[d + 1] is a
+// contiguous range of slices with about 1/kXcd of the steps, and the blocks
+// of die d walk it in order: the x entries a die gathers then come from one
+// neighbourhood of the mesh and stay in its L2.  (Splitting by slice count
+// instead puts the long corner-node rows on one die.)  FP64 product at
+// 539,601 node rows: 208.0 us against 214.5 us with slices dealt round-robin.
+// Returns the wave's first slice and the end of its die's range.
+__device__ __forceinline__ int64_t sell_first_slice(const int32_t* __restrict__ xsplit,
+                                                    int64_t* end) {
+  const int d = int(blockIdx.x % kXcd);
+  const int64_t i = blockIdx.x / kXcd;
+  *end = xsplit[d + 1];
+  return xsplit[d] + (i * kWaves + threadIdx.x / 64) * kSlicesPerWave;
+}
+
+// xsplit[d] = first slice s with koff[s] >= d W / kXcd (W = total steps);
+// one thread per die boundary.
+__global__ void sell_xcd_split(int64_t n_slices, const int32_t* __restrict__ koff,
+                               int32_t* __restrict__ xsplit) {
+  const int d = threadIdx.x;
+  if (d > kXcd) return;
+  const int64_t W = koff[n_slices];
+  const int64_t target = (W * d + kXcd - 1) / kXcd;
+  int64_t lo = 0, hi = n_slices;  // first s in [0, n_slices] with koff[s] >= target
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) / 2;
+    if (koff[mid] >= target) hi = mid; else lo = mid + 1;
+  }
+  xsplit[d] = int32_t(d == 0 ? 0 : (d == kXcd ? n_slices : lo));
+}
+
 // Sort key: window index above, descending row length below (exact, so the
 // sorted slice lengths never exceed the unsorted ones the workspace is sized by).
 __global__ void sell_keys(int64_t n, const int32_t* __restrict__ ptr, uint64_t* __restrict__ keys,
@@ -1690,17 +1739,20 @@ __device__ __forceinline__ int64_t sell_rows(int64_t n, int64_t sl,
 }
 
 // y = A x on the sliced copy, fixed rows zeroed; with `state`: p.Ap
-// partials.  Grid: sell_grid(n_slices) blocks of 4 waves.
+// partials.  Grid: the product grid of build_sell (blocks of 4 waves, kXcd
+// per-die slice ranges, see sell_first_slice).
 template <class VT, class XT, class AT>
 __global__ void __launch_bounds__(kBlock)
 spmv_sell(int64_t n, int64_t n_slices, const uint32_t* __restrict__ rows,
           const int32_t* __restrict__ koff, const int32_t* __restrict__ scol,
           const VT* __restrict__ sval, const XT* __restrict__ x, XT* __restrict__ y,
-          const uint8_t* __restrict__ fixed, PcgState* state, double* __restrict__ partials) {
+          const uint8_t* __restrict__ fixed, PcgState* state, double* __restrict__ partials,
+          const int32_t* __restrict__ xsplit) {
   if (state && state->halt) return;
-  const int64_t sl0 = (tile_index(blockIdx.x) * (kBlock / 64) + threadIdx.x / 64) * kSlicesPerWave;
+  int64_t end;
+  const int64_t sl0 = sell_first_slice(xsplit, &end);
   double dot = 0.0;
-  for (int64_t sl = sl0; sl < sl0 + kSlicesPerWave && sl < n_slices; ++sl) {
+  for (int64_t sl = sl0; sl < sl0 + kSlicesPerWave && sl < end; ++sl) {
     AT o[3];
     const int64_t r = sell_rows<VT, XT, AT>(n, sl, rows, koff, scol, sval, x, o);
     if (r >= 0) {
@@ -1727,10 +1779,11 @@ cheb_step(int64_t n, int64_t n_slices, const uint32_t* __restrict__ rows,
           const int32_t* __restrict__ koff, const int32_t* __restrict__ scol,
           const VT* __restrict__ sval, const float* __restrict__ d, float* __restrict__ dn,
           float* __restrict__ w, float* __restrict__ z, float cd, float cw,
-          const PcgState* state) {
+          const PcgState* state, const int32_t* __restrict__ xsplit) {
   if (state->halt) return;
-  const int64_t sl0 = (tile_index(blockIdx.x) * (kBlock / 64) + threadIdx.x / 64) * kSlicesPerWave;
-  for (int64_t sl = sl0; sl < sl0 + kSlicesPerWave && sl < n_slices; ++sl) {
+  int64_t end;
+  const int64_t sl0 = sell_first_slice(xsplit, &end);
+  for (int64_t sl = sl0; sl < sl0 + kSlicesPerWave && sl < end; ++sl) {
     float o[3];
     const int64_t r = sell_rows<VT, float, float>(n, sl, rows, koff, scol, sval, d, o);
     if (r < 0) continue;
@@ -1966,6 +2019,7 @@ struct Layout {
   float* vals32;
   // sliced copy of an NB3 matrix (solver-internal product layout)
   int64_t n_slices;
+  int32_t* xsplit;      // kXcd + 1 slice boundaries (sell_xcd_split)
   uint32_t* rows;
   int32_t* koff;
   int32_t* slice_len;
@@ -2037,7 +2091,8 @@ Layout layout(const cf_matrix* A, int precision, int precond, void* base, int64_
   L.Ap = c.take<double>(n);
   L.dinv = c.take<double>(precond == CF_PC_BLOCK3 ? 3 * n : n);
   const int64_t parts =
-      nb3 ? std::max<int64_t>(pcg_grid(), sell_grid((A->n_rows + kSliceRows - 1) / kSliceRows))
+      nb3 ? std::max<int64_t>(pcg_grid(),
+                              kXcd * sell_grid((A->n_rows + kSliceRows - 1) / kSliceRows))
           : pcg_grid();
   L.partials = c.take<double>(4 * size_t(parts));
   L.partials_b = c.take<double>(2 * size_t(pcg_vec_grid()));
@@ -2062,6 +2117,7 @@ Layout layout(const cf_matrix* A, int precision, int precond, void* base, int64_
     L.rows = c.take<uint32_t>(N);
     L.koff = c.take<int32_t>(L.n_slices + 1);
     L.slice_len = c.take<int32_t>(L.n_slices + 1);
+    L.xsplit = c.take<int32_t>(kXcd + 1);
     L.scol = c.take<int32_t>(size_t(64) * lsum);
     L.sval64 = c.take<double>(size_t(9 * 64) * lsum);
     if (precision == CF_PREC_MIXED) L.sval32 = c.take<float>(size_t(9 * 64) * lsum);
@@ -2082,8 +2138,9 @@ Layout layout(const cf_matrix* A, int precision, int precond, void* base, int64_
   return L;
 }
 
-// Slice structure of the sliced copy: window sort, slice steps, offsets.
-int build_sell(const cf_matrix* A, const Layout& L, hipStream_t s) {
+// Slice structure of the sliced copy: window sort, slice steps, offsets, and
+// the per-die slice ranges; *product_grid = blocks of the sliced products.
+int build_sell(const cf_matrix* A, const Layout& L, hipStream_t s, int* product_grid) {
   const int64_t N = A->n_rows;
   const int grid = cf::stride_grid(N, kBlock);
   sell_keys<<<grid, kBlock, 0, s>>>(N, A->indptr, L.keys_in, L.ids_in);
@@ -2099,6 +2156,14 @@ int build_sell(const cf_matrix* A, const Layout& L, hipStream_t s) {
   tb = L.tmp_bytes;
   CF_HIP(rocprim::exclusive_scan(L.tmp, tb, L.slice_len, L.koff, 0, size_t(L.n_slices + 1),
                                  rocprim::plus<int32_t>(), s));
+  sell_xcd_split<<<1, 64, 0, s>>>(L.n_slices, L.koff, L.xsplit);
+  CF_LAUNCHED();
+  int32_t xs[kXcd + 1];
+  CF_HIP(hipMemcpyAsync(xs, L.xsplit, sizeof(xs), hipMemcpyDeviceToHost, s));
+  CF_HIP(hipStreamSynchronize(s));
+  int64_t most = 0;
+  for (int d = 0; d < kXcd; ++d) most = std::max<int64_t>(most, xs[d + 1] - xs[d]);
+  *product_grid = int(kXcd * sell_grid(most));
   return CF_OK;
 }
 
@@ -2275,14 +2340,16 @@ int pcg_fp64(const cf_matrix* A, const double* b, double* x, const uint8_t* fixe
   const int64_t units = PC == CF_PC_BLOCK3 ? A->n_rows : n;
   const int G = pcg_grid();
   int rc;
+  int GS = G;
   if (A->format == CF_FMT_NB3 &&
-      ((rc = build_sell(A, L, s)) || (rc = fill_sell<double>(A, L, nullptr, L.sval64, s))))
+      ((rc = build_sell(A, L, s, &GS)) || (rc = fill_sell<double>(A, L, nullptr, L.sval64, s))))
     return rc;
-  const int GS = A->format == CF_FMT_NB3 ? int(sell_grid(L.n_slices)) : G;
+  const int32_t* xsplit = L.xsplit;
   auto apply = [&](const double* xin, double* y, PcgState* state, double* partials) {
     if (A->format == CF_FMT_NB3) {
       spmv_sell<double, double, double><<<GS, kBlock, 0, s>>>(
-          A->n_rows, L.n_slices, L.rows, L.koff, L.scol, L.sval64, xin, y, fixed, state, partials);
+          A->n_rows, L.n_slices, L.rows, L.koff, L.scol, L.sval64, xin, y, fixed, state, partials,
+          xsplit);
       CF_LAUNCHED();
       return CF_OK;
     }
@@ -2316,11 +2383,12 @@ constexpr int kPowerSteps = 40;
 // The flexible CG + Chebyshev driver, independent of how the products are
 // formed.  outer(xin, y, state): FP64 y = A xin with fixed rows zeroed
 // (state set: p.Ap partials, outer_parts of them); power(v, y): FP32
-// y = S A S v; cheb(cd, cw, d, dn, state): one Chebyshev step (w -= S A S d;
+// y = S A S v; cheb(k, cd, cw, d, dn): Chebyshev step k (w -= S A S d;
 // dn = cd d + cw w; z += dn on rh / yh).  gersh: Gershgorin bound of S A S,
 // or 0 when none is known (then b = 1.3 x the power estimate); shape: degree
 // and interval ratio of the polynomial.  SQ: cheb works on S-scaled vectors
-// (see fcg_update).
+// in three-term form (d = z_k in, dn = z_{k-1} in and z_{k+1} out; see
+// fcg_update and mf_gather_cheb32).
 template <bool SQ, class Outer, class Power, class Cheb>
 int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, double gersh,
                   ChebShape shape, int64_t n, const double* b, double* x, const uint8_t* fixed, double tol,
@@ -2366,7 +2434,8 @@ int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, do
   double* Z[2] = {L.z0, L.z1};
   // One outer iteration is: product (p.Ap partials), fcg_update_fused (x, r
   // and the Chebyshev start), the Chebyshev steps, fcg_zdots_fused (stop test,
-  // z = S zh), fcg_direction_fused -- no finalize launches.  w = rh, zh = yh.
+  // z = S zh), fcg_direction_fused -- no finalize launches.  w = rh, zh = yh
+  // (SQ: zh = the last three-term iterate).
   const int GV = pcg_vec_grid();
   auto precondition = [&](int apply_step, double* zc, const double* zo, int sample, int slot) {
     if (apply_step) {
@@ -2382,10 +2451,12 @@ int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, do
     for (int k = 1; k < shape.steps; ++k) {
       int tok = -1;
       if (k == 1) prof_begin(1, sample, s, &tok);
-      if ((r2 = cheb(cd[k], cw[k], D[(k - 1) & 1], D[k & 1]))) return r2;
+      if ((r2 = cheb(k, cd[k], cw[k], D[(k - 1) & 1], D[k & 1]))) return r2;
       if (k == 1) prof_end(tok, s);
     }
-    fcg_zdots_fused<SQ><<<GV, kBlock, 0, s>>>(n, L.r, L.s, L.yh, zc, zo, L.state, L.partials_b, GV,
+    // SQ: the last three-term step left z in D[(steps - 1) & 1]
+    const float* zh = SQ ? D[(shape.steps - 1) & 1] : L.yh;
+    fcg_zdots_fused<SQ><<<GV, kBlock, 0, s>>>(n, L.r, L.s, zh, zc, zo, L.state, L.partials_b, GV,
                                           apply_step, L.partials);
     CF_LAUNCHED();
     fcg_direction_fused<<<GV, kBlock, 0, s>>>(n, zc, L.p, L.state, L.partials, GV, slot);
@@ -2485,12 +2556,13 @@ int pcg_fcg_cheb(const cf_matrix* A, VT* sval, const double* b, double* x, const
                  double tol, int maxit, const Layout& L, cf_pcg_stats* st, hipStream_t s) {
   const int64_t n = n_dofs(A);
   const int G = pcg_grid();
-  const int GS = int(sell_grid(L.n_slices));
+  int GS = 0;
   int rc;
   PcgState h{};
   make_scale<<<G, kBlock, 0, s>>>(n, L.dinv, L.s);
   CF_LAUNCHED();
-  if ((rc = build_sell(A, L, s)) || (rc = fill_sell<double>(A, L, nullptr, L.sval64, s)) ||
+  const int32_t* xsplit = L.xsplit;
+  if ((rc = build_sell(A, L, s, &GS)) || (rc = fill_sell<double>(A, L, nullptr, L.sval64, s)) ||
       (rc = fill_sell<VT>(A, L, L.s, sval, s)))
     return rc;
   gershgorin_nb3<<<G, kBlock, 0, s>>>(A->n_rows, A->indptr, A->indices, A->values, L.s,
@@ -2502,20 +2574,20 @@ int pcg_fcg_cheb(const cf_matrix* A, VT* sval, const double* b, double* x, const
   auto outer = [&](const double* xin, double* y, PcgState* state) {
     spmv_sell<double, double, double><<<GS, kBlock, 0, s>>>(
         A->n_rows, L.n_slices, L.rows, L.koff, L.scol, L.sval64, xin, y, fixed, state,
-        L.partials);
+        L.partials, xsplit);
     CF_LAUNCHED();
     return CF_OK;
   };
   auto power = [&](const float* v, float* y) {
     spmv_sell<VT, float, float><<<GS, kBlock, 0, s>>>(A->n_rows, L.n_slices, L.rows, L.koff,
                                                        L.scol, sval, v, y, nullptr, nullptr,
-                                                       nullptr);
+                                                       nullptr, xsplit);
     CF_LAUNCHED();
     return CF_OK;
   };
-  auto cheb = [&](float cdk, float cwk, const float* din, float* dout) {
+  auto cheb = [&](int, float cdk, float cwk, const float* din, float* dout) {
     cheb_step<VT><<<GS, kBlock, 0, s>>>(A->n_rows, L.n_slices, L.rows, L.koff, L.scol, sval, din,
-                                        dout, L.rh, L.yh, cdk, cwk, L.state);
+                                        dout, L.rh, L.yh, cdk, cwk, L.state, xsplit);
     CF_LAUNCHED();
     return CF_OK;
   };
@@ -3249,7 +3321,9 @@ extern "C" int cf_pcg_solve_mf_t10_mixed(int64_t n_nodes, int64_t n_elem, const 
     return mf_apply(a, xin, y, fixed, M.fe, state, state ? L.partials : nullptr, s);
   };
   // the power iteration multiplies plain vectors (S K S v); the Chebyshev
-  // steps run on S-scaled vectors, so their element kernel gathers no S
+  // steps run on S-scaled vectors, so their element kernel gathers no S.
+  // Step k reads z_k (din) and z_{k-1} (dout, none at k = 1) and overwrites
+  // dout with z_{k+1}.
   auto product32 = [&](const float* din, float* dout, int mode, float cdk, float cwk,
                        const PcgState* state) {
     if (mode == 0)
@@ -3259,14 +3333,14 @@ extern "C" int cf_pcg_solve_mf_t10_mixed(int64_t n_nodes, int64_t n_elem, const 
       mf_t10_elem32<true><<<ge, kBlock, 0, s>>>(n_elem, conn, M.geo32, float(lam), float(mu),
                                                 M.s32, din, slot, M.fe32, state);
     CF_LAUNCHED();
-    mf_gather_cheb32<<<gg, kBlock, 0, s>>>(n_nodes, offs, M.fe32, M.s32, mode, din, dout, L.rh,
-                                           L.yh, cdk, cwk, state);
+    mf_gather_cheb32<<<gg, kBlock, 0, s>>>(n_nodes, offs, M.fe32, M.s32, mode, din, dout, dout,
+                                           L.rh, cdk, cwk, state);
     CF_LAUNCHED();
     return CF_OK;
   };
   auto power = [&](const float* v, float* y) { return product32(v, y, 0, 0.0f, 0.0f, nullptr); };
-  auto cheb = [&](float cdk, float cwk, const float* din, float* dout) {
-    return product32(din, dout, 2, cdk, cwk, L.state);
+  auto cheb = [&](int k, float cdk, float cwk, const float* din, float* dout) {
+    return product32(din, dout, k == 1 ? 1 : 2, cdk, cwk, L.state);
   };
   return fcg_cheb_core<true>(outer, gg, power, cheb, 0.0, kChebF32, n, b, x, fixed, tol, maxit, L,
                        stats_host, s);

@@ -82,9 +82,11 @@ def test_tet10_elasticity_matches_spsolve(gpu, precision, precond):
 def test_mixed_precision_meets_fp64_residual(gpu, precision, dims, tol):
     """Mixed precision (FP64 flexible CG, FP32 Chebyshev preconditioner on a
     FP32- or FP16-stored copy of the scaled matrix) reaches the FP64 residual
-    bar; its error against SuperLU is stated, not assumed.  The 20x3x3 bar is
-    9x above that mesh's FP64 floor (SuperLU's own residual is 1.1e-10), so
-    the true residual recomputed at exit meets it, not just the recursive one."""
+    bar; its error against SuperLU is stated, not assumed.  The solver stops
+    on the recursive residual; the true residual recomputed at exit must meet
+    the bar or, where the bar is within rounding of the mesh's FP64 floor
+    eps ||A_ff| |x_f|| / ||b_f|| (7.7e-11 at 8x2x2, 1.3e-10 at 20x3x3), three
+    times that floor."""
     from cloudfem_amd import solver
     from cloudfem_amd.mesh import box_mesh_tet10
     from cloudfem_amd.physics import elastic_tet10
@@ -100,10 +102,13 @@ def test_mixed_precision_meets_fp64_residual(gpu, precision, dims, tol):
     A0, rhs0 = oel.assemble_elasticity_tet10(m0, 210e9, 0.3, traction=trac)
     u0 = oel.apply_dirichlet_vec3(A0, rhs0, m0, bc)
     err = rel_err(u, u0)
+    Aff, bf, free = oel.reduced_system_vec3(A0, rhs0, m0, bc)[:3]
+    floor = (np.finfo(np.float64).eps * np.linalg.norm(abs(Aff) @ np.abs(u0[free]))
+             / np.linalg.norm(bf))
     print(f"{precision} {dims}: {st['iterations']} FP64 iterations, {st['residual_updates']} "
-          f"low-precision products, rel. residual {st['rel_residual']:.2e}, "
-          f"error vs SuperLU {err:.2e}")
-    assert st["rel_residual"] <= tol and st["residual_updates"] >= 1
+          f"low-precision products, rel. residual {st['rel_residual']:.2e} (FP64 floor "
+          f"{floor:.2e}), error vs SuperLU {err:.2e}")
+    assert st["rel_residual"] <= max(tol, 3 * floor) and st["residual_updates"] >= 1
     assert err < 1e-8
 
 
