@@ -849,50 +849,60 @@ gershgorin_nb3(int64_t n_nodes, const int32_t* __restrict__ ptr, const int32_t* 
 // assembly kernels' copy; each translation unit owns its constant memory).
 __constant__ cf::Tet10Tables kT10m = cf::build_tet10_tables();
 
-// Keast points in barycentric coordinates (centroid, then L_i = 1/2 with
-// the others 1/6) and weights -4/5, 9/20 (tet10.py:84-100), as compile-time
-// constants for the barycentric form below.
-constexpr double kQL[5][4] = {{0.25, 0.25, 0.25, 0.25},
-                              {0.5, 1.0 / 6, 1.0 / 6, 1.0 / 6},
-                              {1.0 / 6, 0.5, 1.0 / 6, 1.0 / 6},
-                              {1.0 / 6, 1.0 / 6, 0.5, 1.0 / 6},
-                              {1.0 / 6, 1.0 / 6, 1.0 / 6, 0.5}};
-constexpr double kQW[5] = {-0.8, 0.45, 0.45, 0.45, 0.45};
+// Element forces f = K_e x_e of isotropic Tet10 elasticity, K_e = V sum_q w_q
+// B_q^T D B_q, for the affine elements the reference builds (midpoints on
+// straight edges, tet10_mesh.py:77-110).  The integrand is then a polynomial
+// of degree 2, which the symmetric 4-point rule (barycentric (a, b, b, b) and
+// permutations, a = (5 + 3 sqrt 5) / 20, b = (5 - sqrt 5) / 20, weights 1/4)
+// integrates exactly, as does the reference's 5-point Keast rule
+// (tet10.py:84-100, elastic_tet10.py:164-170): the same matrix up to
+// rounding.  With N_i = L_i (2 L_i - 1), N_ij = 4 L_i L_j the barycentric
+// derivatives of u at point q (L_q = a, the others b) are
+//   g_q = (4a - 1) x_q + 4b S_q,   g_i = T_i + 4(a - b) x_iq   (i != q),
+//   S_i = sum of the 3 midpoints on edges at i, T_i = (4b - 1) x_i + 4b S_i,
+// and the reference gradient is U[c][k] = g_{k+1}[c] - g_0[c].  With
+// H = U Jinv, sigma = lam tr(H) I + mu (H + H^T) and P = (V/4) sigma Jinv^T,
+// Q_{k+1} = P[.][k], Q_0 = -sum_k P[.][k], the forces are the transpose:
+//   f_i  = (4b - 1) R_i + 4(a - b) Q_i(q = i),
+//   f_ij = 4b (R_i + R_j) + 4(a - b) (Q_i(q = j) + Q_j(q = i)),
+// R_i = sum_q Q_i(q).  About 580 instead of 960 multiply-adds per element
+// (the 5-point barycentric form), the FP32 element kernel being bound by
+// them (26 us at 384,000 elements with its gathers and stores removed).
+constexpr double kT4a = 0.58541019662496845446;  // (5 + 3 sqrt 5) / 20
+constexpr double kT4b = 0.13819660112501051518;  // (5 - sqrt 5) / 20
 constexpr int kTE[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+// edge of the corner pair (i, j), i != j
+constexpr int kEdgeOf[4][4] = {{-1, 0, 2, 3}, {0, -1, 1, 4}, {2, 1, -1, 5}, {3, 4, 5, -1}};
 
-// Element forces in barycentric form.  With N_i = L_i (2 L_i - 1) and
-// N_ij = 4 L_i L_j, dN/dL_i is 4 L_i - 1 on corner i and 4 L_j on the
-// midpoints of the edges (i, j), so the barycentric derivatives of u are
-// g_i = (4 L_i - 1) x_i + 4 sum_j L_j x_ij and the reference-coordinate
-// gradient is U[c][k] = g_{k+1}[c] - g_0[c]; the forces are the transpose:
-// f_i += (4 L_i - 1) Q_i, f_ij += 4 (L_j Q_i + L_i Q_j) with
-// Q_{k+1}[c] = P[c][k], Q_0[c] = -sum_k P[c][k].  The same products as the
-// tabulated dN/dxi form (elastic_tet10.py:164-170), in about 1000
-// instead of 1300 multiply-adds per element, the centroid's corner terms
-// vanishing at compile time.
 template <class T>
-__device__ __forceinline__ void t10_forces_bary(const T (&xa)[10][3], const T (&J)[3][3], T V,
-                                                T lam, T mu, T (&f)[10][3]) {
+__device__ __forceinline__ void t10_forces_4pt(const T (&xa)[10][3], const T (&J)[3][3], T V,
+                                               T lam, T mu, T (&f)[10][3]) {
+  const T cb = T(4.0 * kT4b - 1.0), c4b = T(4.0 * kT4b), cab = T(4.0 * (kT4a - kT4b));
+  const T lq = T(0.25) * V * lam, mq = T(0.25) * V * mu;
+  T Tg[4][3];
 #pragma unroll
-  for (int a = 0; a < 10; ++a)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int c = 0; c < 3; ++c) f[a][c] = T(0);
+    for (int c = 0; c < 3; ++c) {
+      T sum = T(0);
+      bool first = true;
 #pragma unroll
-  for (int q = 0; q < 5; ++q) {
+      for (int e = 0; e < 6; ++e)
+        if (kTE[e][0] == i || kTE[e][1] == i) {
+          sum = first ? xa[4 + e][c] : sum + xa[4 + e][c];
+          first = false;
+        }
+      Tg[i][c] = cb * xa[i][c] + c4b * sum;
+    }
+  T R[4][3], Ecr[6][3];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
     T g[4][3];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) g[i][c] = T(4.0 * kQL[q][i] - 1.0) * xa[i][c];
-#pragma unroll
-    for (int e = 0; e < 6; ++e) {
-      const int i = kTE[e][0], j = kTE[e][1];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        g[i][c] += T(4.0 * kQL[q][j]) * xa[4 + e][c];
-        g[j][c] += T(4.0 * kQL[q][i]) * xa[4 + e][c];
-      }
-    }
+      for (int c = 0; c < 3; ++c)
+        g[i][c] = Tg[i][c] + cab * (i == q ? xa[i][c] : xa[4 + kEdgeOf[i][q]][c]);
     T H[3][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -900,17 +910,21 @@ __device__ __forceinline__ void t10_forces_bary(const T (&xa)[10][3], const T (&
 #pragma unroll
       for (int i = 0; i < 3; ++i) H[c][i] = u0 * J[0][i] + u1 * J[1][i] + u2 * J[2][i];
     }
-    const T ltr = lam * (H[0][0] + H[1][1] + H[2][2]);
-    const T vw = V * T(kQW[q]);
+    // (V/4) sigma, symmetric
+    const T ltr = lq * (H[0][0] + H[1][1] + H[2][2]);
+    T sg[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      sg[c][c] = T(2) * mq * H[c][c] + ltr;
+#pragma unroll
+      for (int i = c + 1; i < 3; ++i) sg[c][i] = sg[i][c] = mq * (H[c][i] + H[i][c]);
+    }
     T Q[4][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      T sg[3];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) sg[i] = mu * (H[c][i] + H[i][c]) + (c == i ? ltr : T(0));
       T P[3];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) P[k] = vw * (sg[0] * J[k][0] + sg[1] * J[k][1] + sg[2] * J[k][2]);
+      for (int k = 0; k < 3; ++k) P[k] = sg[c][0] * J[k][0] + sg[c][1] * J[k][1] + sg[c][2] * J[k][2];
       Q[1][c] = P[0];
       Q[2][c] = P[1];
       Q[3][c] = P[2];
@@ -919,26 +933,31 @@ __device__ __forceinline__ void t10_forces_bary(const T (&xa)[10][3], const T (&
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) f[i][c] += T(4.0 * kQL[q][i] - 1.0) * Q[i][c];
-#pragma unroll
-    for (int e = 0; e < 6; ++e) {
-      const int i = kTE[e][0], j = kTE[e][1];
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-        f[4 + e][c] += T(4.0 * kQL[q][j]) * Q[i][c] + T(4.0 * kQL[q][i]) * Q[j][c];
-    }
+      for (int c = 0; c < 3; ++c) {
+        R[i][c] = q == 0 ? Q[i][c] : R[i][c] + Q[i][c];
+        if (i == q) {
+          f[i][c] = cab * Q[i][c];
+        } else {
+          // edge (i, q) collects Q_i at q here and Q_q at i when q' = i
+          const int e = kEdgeOf[i][q];
+          const bool first = (i < q ? i : q) == q;  // the smaller corner's point comes first
+          Ecr[e][c] = first ? Q[i][c] : Ecr[e][c] + Q[i][c];
+        }
+      }
   }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f[i][c] = cb * R[i][c] + f[i][c];
+#pragma unroll
+  for (int e = 0; e < 6; ++e)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      f[4 + e][c] = c4b * (R[kTE[e][0]][c] + R[kTE[e][1]][c]) + cab * Ecr[e][c];
 }
 
-// Element forces f_e = K_e x_e of isotropic Tet10 elasticity, evaluated at
-// the 5 Keast points: H_q = grad u = (sum_a x_a d_aq^T) Jinv,
-// sigma_q = lam tr(H) I + mu (H + H^T), f_a += V w_q sigma_q Jinv^T d_aq.
-// Exact for affine elements, same integrand and rule as
-// elastic_tet10.py:164-170, ~1.2k FMA per element; f receives the 10 nodal
-// forces.  The tables stay in constant memory: every lane reads the
-// same entry, served as scalar operands.  (Compile-time literals instead
-// fold the zero gradient entries but cost more registers and moves: 50.7 vs
-// 46.5 us per application at 384,000 elements.)
+// Element forces of one element: gathers its 10 nodal x from the FP64
+// vector and applies t10_forces_4pt; f receives the 10 nodal forces.
 __device__ __forceinline__ void t10_forces(const int32_t* __restrict__ cn,
                                            const double* __restrict__ g, double lam, double mu,
                                            const double* __restrict__ x, double (&f)[10][3]) {
@@ -955,7 +974,7 @@ __device__ __forceinline__ void t10_forces(const int32_t* __restrict__ cn,
 #pragma unroll
     for (int c = 0; c < 3; ++c) xa[a][c] = x[3 * nd + c];
   }
-  t10_forces_bary<double>(xa, J, V, lam, mu, f);
+  t10_forces_4pt<double>(xa, J, V, lam, mu, f);
 }
 
 // fe[e][a][c] (or the node-sorted slot of (e, a)) = forces of element e,
@@ -986,7 +1005,7 @@ mf_t10_elem(int64_t m, const int32_t* __restrict__ conn, const double* __restric
       for (int c = 0; c < 3; ++c) xa[a][c] = x[3 * nd + c];
     }
     double f[10][3];
-    t10_forces_bary<double>(xa, J, V, lam, mu, f);
+    t10_forces_4pt<double>(xa, J, V, lam, mu, f);
     if (slot) {
       // each nodal force goes to its position in node-sorted incidence order,
       // so the node sums read a contiguous range
@@ -1314,44 +1333,189 @@ __global__ void geo_to_f32(int64_t n, const double* __restrict__ g, float* __res
     g32[i] = float(g[i]);
 }
 
+// Run sums of the FP32 inner operator.  Scattered 12-byte stores of the
+// forces into node-sorted slots cost one L2 write request each (3.0 M per
+// product at 384,000 elements, the L2 busy for the whole kernel: 25.6 us).
+// Instead the block of a tile of kBlock consecutive elements stages the
+// tile's forces in LDS sorted by (node, element) -- item (e, a) at its rank
+// k -- so that the items of one node form a contiguous run; it sums each
+// run in element order and writes one partial per (tile, node) pair,
+// coalesced (0.78 M partials instead of 3.84 M forces at 384,000 elements).
+// The node gather then adds its node's partials in tile order (1.4 per node
+// on Tet10 box meshes).  Deterministic: fixed orders throughout.
+//   rank[e * 10 + a]  k of item (e, a) in its tile
+//   toff[t]           first run of tile t (runs numbered tile by tile, by
+//                     node inside a tile); tstart[q] = first k of run q
+//   nq[j]             runs of node n at j in [nro[n], nro[n+1]), tile order
+constexpr int kTileItems = kBlock * 10;
+
+__global__ void iota_u32(int64_t n, uint32_t* __restrict__ v) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    v[i] = uint32_t(i);
+}
+
+// keys[p] = tile of the item in node-sorted slot p (perm[p] = e * 10 + a)
+__global__ void mf_store_keys(int64_t n_items, const uint32_t* __restrict__ perm,
+                              uint32_t* __restrict__ keys, uint32_t* __restrict__ ids) {
+  for (int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; p < n_items;
+       p += int64_t(gridDim.x) * blockDim.x) {
+    keys[p] = perm[p] / uint32_t(kTileItems);
+    ids[p] = uint32_t(p);
+  }
+}
+
+// tslot: the slots sorted by tile (stable, so by slot inside a tile); the
+// rank of an item is its position in its tile's list (every tile but the
+// last holds exactly kTileItems items)
+__global__ void mf_store_rank(int64_t n_items, const uint32_t* __restrict__ perm,
+                              const uint32_t* __restrict__ tslot, uint16_t* __restrict__ rank) {
+  for (int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; k < n_items;
+       k += int64_t(gridDim.x) * blockDim.x)
+    rank[perm[tslot[k]]] = uint16_t(k % kTileItems);
+}
+
+// head[p] = 1 where a run starts: slot p is the first of its node
+// (mf_node_heads) or its item lies in another tile than the item of slot
+// p - 1 (mf_tile_heads); head[n_items] = 0.
+__global__ void mf_tile_heads(int64_t n_items, const uint32_t* __restrict__ perm,
+                              uint32_t* __restrict__ head) {
+  for (int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; p <= n_items;
+       p += int64_t(gridDim.x) * blockDim.x) {
+    uint32_t h = 0;
+    if (p < n_items)
+      h = p == 0 || perm[p] / uint32_t(kTileItems) != perm[p - 1] / uint32_t(kTileItems);
+    head[p] = h;
+  }
+}
+
+__global__ void mf_node_heads(int64_t n_nodes, const int32_t* __restrict__ offs,
+                              uint32_t* __restrict__ head) {
+  for (int64_t n = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; n < n_nodes;
+       n += int64_t(gridDim.x) * blockDim.x)
+    if (offs[n] < offs[n + 1]) head[offs[n]] = 1;
+}
+
+// run r starting at slot p (ridx[p] = r): rk = position of the slot's item in
+// the tile-major layout, rs = p; nro[n] = ridx[offs[n]]
+__global__ void mf_run_fill(int64_t n_items, const uint32_t* __restrict__ perm,
+                            const uint16_t* __restrict__ rank, const uint32_t* __restrict__ head,
+                            const uint32_t* __restrict__ ridx, uint32_t* __restrict__ rk,
+                            uint32_t* __restrict__ rs) {
+  for (int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; p < n_items;
+       p += int64_t(gridDim.x) * blockDim.x) {
+    if (!head[p]) continue;
+    const uint32_t it = perm[p];
+    rk[ridx[p]] = (it / uint32_t(kTileItems)) * uint32_t(kTileItems) + rank[it];
+    rs[ridx[p]] = uint32_t(p);
+  }
+}
+
+__global__ void mf_run_len(int64_t n_items, const uint32_t* __restrict__ ridx,
+                           const uint32_t* __restrict__ rs, uint16_t* __restrict__ rl,
+                           int64_t n_nodes, const int32_t* __restrict__ offs,
+                           int32_t* __restrict__ nro) {
+  const int64_t n_runs = ridx[n_items];
+  for (int64_t r = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; r < n_runs;
+       r += int64_t(gridDim.x) * blockDim.x)
+    rl[r] = uint16_t((r + 1 < n_runs ? int64_t(rs[r + 1]) : n_items) - int64_t(rs[r]));
+  for (int64_t n = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; n <= n_nodes;
+       n += int64_t(gridDim.x) * blockDim.x)
+    nro[n] = int32_t(ridx[offs[n]]);
+}
+
+// tile order of the runs: rq[q] = node-order run id of the q-th smallest
+// start rk; nq[r] = q, tstart[q] = start inside the tile
+__global__ void mf_run_tile_order(int64_t n_runs, const uint32_t* __restrict__ rk_sorted,
+                                  const uint32_t* __restrict__ rq, uint32_t* __restrict__ nq,
+                                  uint16_t* __restrict__ tstart) {
+  for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < n_runs;
+       q += int64_t(gridDim.x) * blockDim.x) {
+    nq[rq[q]] = uint32_t(q);
+    tstart[q] = uint16_t(rk_sorted[q] % uint32_t(kTileItems));
+  }
+}
+
+// toff[t] = first q with rk_sorted[q] >= t * kTileItems (t = 0 .. n_tiles)
+__global__ void mf_tile_run_off(int64_t n_tiles, int64_t n_runs,
+                                const uint32_t* __restrict__ rk_sorted, int32_t* __restrict__ toff) {
+  for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t <= n_tiles;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const uint64_t target = uint64_t(t) * kTileItems;
+    int64_t lo = 0, hi = n_runs;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) / 2;
+      if (uint64_t(rk_sorted[mid]) >= target) hi = mid; else lo = mid + 1;
+    }
+    toff[t] = int32_t(lo);
+  }
+}
+
 // PRE: d is already S-scaled (the Chebyshev vectors of the SQ form), so
-// the kernel gathers it as is; otherwise it gathers S d.
+// the kernel gathers it as is; otherwise it gathers S d.  Writes the run
+// partials described above.
 template <bool PRE>
 __global__ void __launch_bounds__(kBlock)
 mf_t10_elem32(int64_t m, const int32_t* __restrict__ conn, const float* __restrict__ geo,
               float lam, float mu, const float* __restrict__ sc, const float* __restrict__ d,
-              const uint32_t* __restrict__ slot, float* __restrict__ fe,
+              const uint16_t* __restrict__ rank, const int32_t* __restrict__ toff,
+              const uint16_t* __restrict__ tstart, float* __restrict__ part,
               const PcgState* state) {
   if (state && state->halt) return;
-  for (int64_t e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; e < m;
-       e += int64_t(gridDim.x) * blockDim.x) {
-    const float* g = geo + e * 10;
-    float J[3][3];
+  __shared__ float fl[kTileItems * 3];
+  const int64_t n_tiles = (m + kBlock - 1) / kBlock;
+  for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < n_tiles; t += gridDim.x) {
+    const int64_t e = t * kBlock + threadIdx.x;
+    if (e < m) {
+      const float* g = geo + e * 10;
+      float J[3][3];
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
+      for (int r = 0; r < 3; ++r)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) J[r][c] = g[3 * r + c];
-    const float V = g[9];
-    float xa[10][3];
+        for (int c = 0; c < 3; ++c) J[r][c] = g[3 * r + c];
+      const float V = g[9];
+      float xa[10][3];
 #pragma unroll
-    for (int a = 0; a < 10; ++a) {
-      const int64_t nd = conn[e * 10 + a];
+      for (int a = 0; a < 10; ++a) {
+        const int64_t nd = conn[e * 10 + a];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) xa[a][c] = PRE ? d[3 * nd + c] : sc[3 * nd + c] * d[3 * nd + c];
+        for (int c = 0; c < 3; ++c)
+          xa[a][c] = PRE ? d[3 * nd + c] : sc[3 * nd + c] * d[3 * nd + c];
+      }
+      float f[10][3];
+      t10_forces_4pt<float>(xa, J, V, lam, mu, f);
+#pragma unroll
+      for (int a = 0; a < 10; ++a) {
+        const int r = rank[e * 10 + a];
+        fl[3 * r] = f[a][0];
+        fl[3 * r + 1] = f[a][1];
+        fl[3 * r + 2] = f[a][2];
+      }
     }
-    float f[10][3];
-    t10_forces_bary<float>(xa, J, V, lam, mu, f);
-#pragma unroll
-    for (int a = 0; a < 10; ++a) {
-      float* o = fe + int64_t(slot[e * 10 + a]) * 3;
-      o[0] = f[a][0];
-      o[1] = f[a][1];
-      o[2] = f[a][2];
+    __syncthreads();
+    // one partial per run of the tile (runs cover the tile's items)
+    const int64_t k0 = t * kTileItems;
+    const int nt = int((k0 + kTileItems < 10 * m ? k0 + kTileItems : 10 * m) - k0);
+    const int32_t q0 = toff[t], q1 = toff[t + 1];
+    for (int32_t q = q0 + threadIdx.x; q < q1; q += kBlock) {
+      const int kb = tstart[q], ke = q + 1 < q1 ? int(tstart[q + 1]) : nt;
+      float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+      for (int k = kb; k < ke; ++k) {
+        s0 += fl[3 * k];
+        s1 += fl[3 * k + 1];
+        s2 += fl[3 * k + 2];
+      }
+      float* o = part + 3 * int64_t(q);
+      o[0] = s0;
+      o[1] = s1;
+      o[2] = s2;
     }
+    __syncthreads();
   }
 }
 
-// node sums of the FP32 forces.  mode 0 (power iteration): zn = S sum.
+// node sums of the FP32 forces (the node's run partials in tile order).
+// mode 0 (power iteration): zn = S sum.
 // Modes 1 and 2: one Chebyshev step in three-term form on S-scaled vectors
 // (z = S zhat, w = S^2 r fixed over the steps; the element kernel gathered
 // zc, so sum = K zc and S^2 sum = S (S K S) zhat):
@@ -1361,28 +1525,17 @@ mf_t10_elem32(int64_t m, const int32_t* __restrict__ conn, const float* __restri
 // residual-recursive form.  zn may alias zp: each entry is read and then
 // written by the same thread.
 __global__ void __launch_bounds__(kBlock)
-mf_gather_cheb32(int64_t n_nodes, const int32_t* __restrict__ offs,
-                 const float* __restrict__ fe, const float* __restrict__ sc, int mode,
-                 const float* __restrict__ zc, const float* zp, float* zn,
-                 const float* __restrict__ w, float cd, float cw, const PcgState* state) {
+mf_gather_cheb32(int64_t n_nodes, const int32_t* __restrict__ nro,
+                 const uint32_t* __restrict__ nq, const float* __restrict__ part,
+                 const float* __restrict__ sc, int mode, const float* __restrict__ zc,
+                 const float* zp, float* zn, const float* __restrict__ w, float cd, float cw,
+                 const PcgState* state) {
   if (state && state->halt) return;
   const int64_t i = tile_index(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= n_nodes) return;
   float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-  const float* v = fe + int64_t(offs[i]) * 3;
-  const float* end = fe + int64_t(offs[i + 1]) * 3;
-  for (; v + 12 <= end; v += 12) {
-    float t[12];
-#pragma unroll
-    for (int u = 0; u < 12; ++u) t[u] = v[u];
-#pragma unroll
-    for (int u = 0; u < 12; u += 3) {
-      a0 += t[u];
-      a1 += t[u + 1];
-      a2 += t[u + 2];
-    }
-  }
-  for (; v < end; v += 3) {
+  for (int32_t j = nro[i]; j < nro[i + 1]; ++j) {
+    const float* v = part + 3 * int64_t(nq[j]);
     a0 += v[0];
     a1 += v[1];
     a2 += v[2];
@@ -3230,8 +3383,23 @@ struct MfMixed {
   double* fe;      // FP64 element forces (outer product)
   double* blocks;  // diagonal 3x3 blocks
   float* geo32;    // geometry rounded to FP32
-  float* fe32;     // FP32 element forces (inner products)
+  float* fe32;     // FP32 run partials of the inner products
   float* s32;      // Jacobi scaling in FP32
+  // run-sum plan of mf_t10_elem32 / mf_gather_cheb32
+  uint16_t* rank;    // rank of each item in its tile
+  int32_t* toff;     // first run of each tile
+  uint16_t* tstart;  // start of each run inside its tile
+  uint32_t* nq;      // runs of each node, tile order
+  int32_t* nro;      // first entry of each node in nq
+  // plan temporaries
+  uint32_t* keys_in;
+  uint32_t* keys_out;
+  uint32_t* ids_in;
+  uint32_t* tslot;
+  uint32_t* rk;
+  uint16_t* rl;
+  void* tmp;
+  size_t tmp_bytes;
 };
 
 size_t mf_mixed_layout(int64_t n_nodes, int64_t n_elem, void* base, Layout* L, MfMixed* M) {
@@ -3257,8 +3425,76 @@ size_t mf_mixed_layout(int64_t n_nodes, int64_t n_elem, void* base, Layout* L, M
   M->geo32 = c.take<float>(10 * n_elem);
   M->fe32 = c.take<float>(30 * n_elem);
   M->s32 = c.take<float>(n);
+  const int64_t items = 10 * n_elem;
+  const int64_t n_tiles = (n_elem + kBlock - 1) / kBlock;
+  M->rank = c.take<uint16_t>(items);
+  M->toff = c.take<int32_t>(n_tiles + 1);
+  M->tstart = c.take<uint16_t>(items);
+  M->nq = c.take<uint32_t>(items);
+  M->nro = c.take<int32_t>(n_nodes + 1);
+  M->rk = c.take<uint32_t>(items);
+  M->rl = c.take<uint16_t>(items);
+  M->keys_in = c.take<uint32_t>(items + 1);
+  M->keys_out = c.take<uint32_t>(items + 1);
+  M->ids_in = c.take<uint32_t>(items);
+  M->tslot = c.take<uint32_t>(items);
+  size_t sort_bytes = 0, scan_bytes = 0;
+  rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                            (uint32_t*)nullptr, (uint32_t*)nullptr, size_t(items), 0, 32);
+  rocprim::exclusive_scan(nullptr, scan_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, 0u,
+                          size_t(items + 1), rocprim::plus<uint32_t>());
+  M->tmp_bytes = std::max(sort_bytes, scan_bytes);
+  M->tmp = c.take<char>(M->tmp_bytes);
   L->total = c.used;
   return c.used;
+}
+
+// Tile-major layout plan (rank, runs) from the node-sorted incidence list
+// perm and its node offsets offs.
+int mf_store_plan(int64_t n_nodes, int64_t n_elem, const int32_t* offs, const uint32_t* perm,
+                  const MfMixed& M, hipStream_t s) {
+  const int64_t items = 10 * n_elem;
+  const int grid = cf::stride_grid(items + 1, kBlock);
+  mf_store_keys<<<grid, kBlock, 0, s>>>(items, perm, M.keys_in, M.ids_in);
+  CF_LAUNCHED();
+  size_t tb = M.tmp_bytes;
+  CF_HIP(rocprim::radix_sort_pairs(M.tmp, tb, M.keys_in, M.keys_out, M.ids_in, M.tslot,
+                                   size_t(items), 0,
+                                   bits_for((n_elem + kBlock - 1) / kBlock + 1), s));
+  mf_store_rank<<<grid, kBlock, 0, s>>>(items, perm, M.tslot, M.rank);
+  CF_LAUNCHED();
+  uint32_t* head = M.keys_in;
+  uint32_t* ridx = M.keys_out;
+  uint32_t* rs = M.ids_in;
+  mf_tile_heads<<<grid, kBlock, 0, s>>>(items, perm, head);
+  CF_LAUNCHED();
+  mf_node_heads<<<cf::stride_grid(n_nodes, kBlock), kBlock, 0, s>>>(n_nodes, offs, head);
+  CF_LAUNCHED();
+  tb = M.tmp_bytes;
+  CF_HIP(rocprim::exclusive_scan(M.tmp, tb, head, ridx, 0u, size_t(items + 1),
+                                 rocprim::plus<uint32_t>(), s));
+  mf_run_fill<<<grid, kBlock, 0, s>>>(items, perm, M.rank, head, ridx, M.rk, rs);
+  CF_LAUNCHED();
+  mf_run_len<<<grid, kBlock, 0, s>>>(items, ridx, rs, M.rl, n_nodes, offs, M.nro);
+  CF_LAUNCHED();
+  // number the runs tile by tile: sort the run starts rk
+  uint32_t n_runs = 0;
+  CF_HIP(hipMemcpyAsync(&n_runs, ridx + items, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  CF_HIP(hipStreamSynchronize(s));
+  CF_REQUIRE(n_runs > 0 && int64_t(n_runs) <= items, "matrix-free plan: bad run count %u",
+             n_runs);
+  iota_u32<<<grid, kBlock, 0, s>>>(n_runs, M.ids_in);
+  CF_LAUNCHED();
+  tb = M.tmp_bytes;
+  CF_HIP(rocprim::radix_sort_pairs(M.tmp, tb, M.rk, M.keys_out, M.ids_in, M.tslot,
+                                   size_t(n_runs), 0, 32, s));
+  mf_run_tile_order<<<grid, kBlock, 0, s>>>(n_runs, M.keys_out, M.tslot, M.nq, M.tstart);
+  CF_LAUNCHED();
+  const int64_t n_tiles = (n_elem + kBlock - 1) / kBlock;
+  mf_tile_run_off<<<cf::stride_grid(n_tiles + 1, kBlock), kBlock, 0, s>>>(n_tiles, n_runs,
+                                                                          M.keys_out, M.toff);
+  CF_LAUNCHED();
+  return CF_OK;
 }
 
 }  // namespace
@@ -3282,6 +3518,7 @@ extern "C" int cf_pcg_solve_mf_t10_mixed(int64_t n_nodes, int64_t n_elem, const 
   int rc = mf_check(a);
   if (rc) return rc;
   CF_REQUIRE(slot && b && x && ws && stats_host, "cf_pcg_solve_mf_t10_mixed: null argument");
+  CF_REQUIRE(10 * n_elem < (int64_t(1) << 31), "cf_pcg_solve_mf_t10_mixed: too many elements");
   CF_REQUIRE(tol > 0.0 && maxit > 0, "cf_pcg_solve_mf_t10_mixed: tol and maxit must be positive");
   Layout L{};
   MfMixed M{};
@@ -3315,6 +3552,7 @@ extern "C" int cf_pcg_solve_mf_t10_mixed(int64_t n_nodes, int64_t n_elem, const 
   CF_LAUNCHED();
   geo_to_f32<<<cf::stride_grid(10 * n_elem, kBlock), kBlock, 0, s>>>(10 * n_elem, geo, M.geo32);
   CF_LAUNCHED();
+  if ((rc = mf_store_plan(n_nodes, n_elem, offs, perm, M, s))) return rc;
   const int ge = cf::stride_grid(n_elem, kBlock);
   const int gg = int(mf_gather_grid(n_nodes));
   auto outer = [&](const double* xin, double* y, PcgState* state) {
@@ -3328,13 +3566,15 @@ extern "C" int cf_pcg_solve_mf_t10_mixed(int64_t n_nodes, int64_t n_elem, const 
                        const PcgState* state) {
     if (mode == 0)
       mf_t10_elem32<false><<<ge, kBlock, 0, s>>>(n_elem, conn, M.geo32, float(lam), float(mu),
-                                                 M.s32, din, slot, M.fe32, state);
+                                                 M.s32, din, M.rank, M.toff, M.tstart, M.fe32,
+                                                 state);
     else
       mf_t10_elem32<true><<<ge, kBlock, 0, s>>>(n_elem, conn, M.geo32, float(lam), float(mu),
-                                                M.s32, din, slot, M.fe32, state);
+                                                M.s32, din, M.rank, M.toff, M.tstart, M.fe32,
+                                                state);
     CF_LAUNCHED();
-    mf_gather_cheb32<<<gg, kBlock, 0, s>>>(n_nodes, offs, M.fe32, M.s32, mode, din, dout, dout,
-                                           L.rh, cdk, cwk, state);
+    mf_gather_cheb32<<<gg, kBlock, 0, s>>>(n_nodes, M.nro, M.nq, M.fe32, M.s32, mode, din, dout,
+                                           dout, L.rh, cdk, cwk, state);
     CF_LAUNCHED();
     return CF_OK;
   };
