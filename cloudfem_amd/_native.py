@@ -139,9 +139,9 @@ _SIGS = {
     "cf_axpby": ([_I64, _D, _P, _D, _P, _P], _I32),
     "cf_copy_masked": ([_I64, _P, _P, _P, _P], _I32),
     "cf_mf_t10_workspace_bytes": ([_I64, _I64, _I32, _SZ], _I32),
-    "cf_mf_t10_apply": ([_I64, _I64, _P, _P, _D, _D, _P, _P, _P, _P, _P, _P, _P,
+    "cf_mf_t10_apply": ([_I64, _I64, _P, _P, _D, _D, _P, _P, _P, _P, _P, _P,
                          ctypes.c_size_t, _P], _I32),
-    "cf_pcg_solve_mf_t10": ([_I64, _I64, _P, _P, _D, _D, _P, _P, _P, _P, _P, _P, _I32, _D, _I32,
+    "cf_pcg_solve_mf_t10": ([_I64, _I64, _P, _P, _D, _D, _P, _P, _P, _P, _P, _I32, _D, _I32,
                              _P, ctypes.c_size_t, ctypes.POINTER(CfPcgStats), _P], _I32),
     "cf_invert_permutation": ([_I64, _P, _P, _P], _I32),
     "cf_adjoint_dk": ([_P, _I64, _I32, _P, _P, _P, _P, _P], _I32),
@@ -154,7 +154,7 @@ _SIGS = {
     "cf_mf_t10_plan_bytes": ([_I64, _SZ], _I32),
     "cf_mesh_workspace_bytes": ([_I64, _SZ], _I32),
     "cf_mf_t10_mixed_workspace_bytes": ([_I64, _I64, _SZ], _I32),
-    "cf_pcg_solve_mf_t10_mixed": ([_I64, _I64, _P, _P, _D, _D, _P, _P, _P, _P, _P, _P, _D, _I32,
+    "cf_pcg_solve_mf_t10_mixed": ([_I64, _I64, _P, _P, _D, _D, _P, _P, _P, _P, _P, _D, _I32,
                                    _P, ctypes.c_size_t, ctypes.POINTER(CfPcgStats), _P], _I32),
     "cf_tet10_edges": ([_I64, _P, _I32, _I64, _P, _P, _P, _P, _P, ctypes.c_size_t, _P], _I32),
     "cf_edge_midpoints": ([_I64, _I64, _P, _P, _P, _P, _P, _P], _I32),

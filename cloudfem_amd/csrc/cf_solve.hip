@@ -977,87 +977,198 @@ __device__ __forceinline__ void t10_forces(const int32_t* __restrict__ cn,
   t10_forces_4pt<double>(xa, J, V, lam, mu, f);
 }
 
-// fe[e][a][c] (or the node-sorted slot of (e, a)) = forces of element e,
-// summed into the nodes by mf_gather / mf_gather_sorted in fixed order.  The
-// same arithmetic as t10_forces, written out: 178 VGPRs (226 when the
-// stores went through a per-node output callback: 60 vs 46.5 us per
-// application at 384,000 elements).
+// Run sums of the matrix-free operator (FP64 and FP32).  Scattered stores of
+// the element forces into node-sorted slots cost one L2 write request each
+// (FP32: 3.0 M per product at 384,000 elements, the L2 busy for the whole
+// kernel: 25.6 us).
+// Instead the block of a tile of kBlock consecutive elements stages the
+// tile's forces in LDS sorted by (node, element) -- item (e, a) at its rank
+// k -- so that the items of one node form a contiguous run; it sums each
+// run in element order and writes one partial per (tile, node) pair,
+// coalesced (0.78 M partials instead of 3.84 M forces at 384,000 elements).
+// The node gather then adds its node's partials in tile order (1.4 per node
+// on Tet10 box meshes).  Deterministic: fixed orders throughout.
+//   rank[e * 10 + a]  k of item (e, a) in its tile
+//   toff[t]           first run of tile t (runs numbered tile by tile, by
+//                     node inside a tile); tstart[q] = first k of run q
+//   nq[j]             runs of node n at j in [nro[n], nro[n+1]), tile order
+constexpr int kTileItems = kBlock * 10;
+
+__global__ void iota_u32(int64_t n, uint32_t* __restrict__ v) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    v[i] = uint32_t(i);
+}
+
+// keys[p] = tile of the item in node-sorted slot p (perm[p] = e * 10 + a)
+__global__ void mf_store_keys(int64_t n_items, const uint32_t* __restrict__ perm,
+                              uint32_t* __restrict__ keys, uint32_t* __restrict__ ids) {
+  for (int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; p < n_items;
+       p += int64_t(gridDim.x) * blockDim.x) {
+    keys[p] = perm[p] / uint32_t(kTileItems);
+    ids[p] = uint32_t(p);
+  }
+}
+
+// tslot: the slots sorted by tile (stable, so by slot inside a tile); the
+// rank of an item is its position in its tile's list (every tile but the
+// last holds exactly kTileItems items)
+__global__ void mf_store_rank(int64_t n_items, const uint32_t* __restrict__ perm,
+                              const uint32_t* __restrict__ tslot, uint16_t* __restrict__ rank) {
+  for (int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; k < n_items;
+       k += int64_t(gridDim.x) * blockDim.x)
+    rank[perm[tslot[k]]] = uint16_t(k % kTileItems);
+}
+
+// head[p] = 1 where a run starts: slot p is the first of its node
+// (mf_node_heads) or its item lies in another tile than the item of slot
+// p - 1 (mf_tile_heads); head[n_items] = 0.
+__global__ void mf_tile_heads(int64_t n_items, const uint32_t* __restrict__ perm,
+                              uint32_t* __restrict__ head) {
+  for (int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; p <= n_items;
+       p += int64_t(gridDim.x) * blockDim.x) {
+    uint32_t h = 0;
+    if (p < n_items)
+      h = p == 0 || perm[p] / uint32_t(kTileItems) != perm[p - 1] / uint32_t(kTileItems);
+    head[p] = h;
+  }
+}
+
+__global__ void mf_node_heads(int64_t n_nodes, const int32_t* __restrict__ offs,
+                              uint32_t* __restrict__ head) {
+  for (int64_t n = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; n < n_nodes;
+       n += int64_t(gridDim.x) * blockDim.x)
+    if (offs[n] < offs[n + 1]) head[offs[n]] = 1;
+}
+
+// run r starting at slot p (ridx[p] = r): rk = position of the slot's item in
+// the tile-major layout, rs = p; nro[n] = ridx[offs[n]]
+__global__ void mf_run_fill(int64_t n_items, const uint32_t* __restrict__ perm,
+                            const uint16_t* __restrict__ rank, const uint32_t* __restrict__ head,
+                            const uint32_t* __restrict__ ridx, uint32_t* __restrict__ rk,
+                            uint32_t* __restrict__ rs) {
+  for (int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; p < n_items;
+       p += int64_t(gridDim.x) * blockDim.x) {
+    if (!head[p]) continue;
+    const uint32_t it = perm[p];
+    rk[ridx[p]] = (it / uint32_t(kTileItems)) * uint32_t(kTileItems) + rank[it];
+    rs[ridx[p]] = uint32_t(p);
+  }
+}
+
+__global__ void mf_run_len(int64_t n_items, const uint32_t* __restrict__ ridx,
+                           const uint32_t* __restrict__ rs, uint16_t* __restrict__ rl,
+                           int64_t n_nodes, const int32_t* __restrict__ offs,
+                           int32_t* __restrict__ nro) {
+  const int64_t n_runs = ridx[n_items];
+  for (int64_t r = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; r < n_runs;
+       r += int64_t(gridDim.x) * blockDim.x)
+    rl[r] = uint16_t((r + 1 < n_runs ? int64_t(rs[r + 1]) : n_items) - int64_t(rs[r]));
+  for (int64_t n = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; n <= n_nodes;
+       n += int64_t(gridDim.x) * blockDim.x)
+    nro[n] = int32_t(ridx[offs[n]]);
+}
+
+// tile order of the runs: rq[q] = node-order run id of the q-th smallest
+// start rk; nq[r] = q, tstart[q] = start inside the tile
+__global__ void mf_run_tile_order(int64_t n_runs, const uint32_t* __restrict__ rk_sorted,
+                                  const uint32_t* __restrict__ rq, uint32_t* __restrict__ nq,
+                                  uint16_t* __restrict__ tstart) {
+  for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < n_runs;
+       q += int64_t(gridDim.x) * blockDim.x) {
+    nq[rq[q]] = uint32_t(q);
+    tstart[q] = uint16_t(rk_sorted[q] % uint32_t(kTileItems));
+  }
+}
+
+// toff[t] = first q with rk_sorted[q] >= t * kTileItems (t = 0 .. n_tiles)
+__global__ void mf_tile_run_off(int64_t n_tiles, int64_t n_runs,
+                                const uint32_t* __restrict__ rk_sorted, int32_t* __restrict__ toff) {
+  for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t <= n_tiles;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const uint64_t target = uint64_t(t) * kTileItems;
+    int64_t lo = 0, hi = n_runs;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) / 2;
+      if (uint64_t(rk_sorted[mid]) >= target) hi = mid; else lo = mid + 1;
+    }
+    toff[t] = int32_t(lo);
+  }
+}
+
+// FP64 element forces -> run partials part[c * n_runs + q] (component
+// planes), the tile's forces staged in LDS one component at a time (20 KB per
+// block, so the 3 waves per SIMD its registers allow stay resident).
 __global__ void __launch_bounds__(kBlock)
 mf_t10_elem(int64_t m, const int32_t* __restrict__ conn, const double* __restrict__ geo,
             double lam, double mu, const double* __restrict__ x,
-            const uint32_t* __restrict__ slot, double* __restrict__ fe,
+            const uint16_t* __restrict__ rank, const int32_t* __restrict__ toff,
+            const uint16_t* __restrict__ tstart, int64_t n_runs, double* __restrict__ part,
             const PcgState* state) {
   if (state && state->halt) return;
-  for (int64_t e = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; e < m;
-       e += int64_t(gridDim.x) * blockDim.x) {
-    const double* g = geo + e * 10;
-    double J[3][3];
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) J[r][c] = g[3 * r + c];
-    const double V = g[9];
-    double xa[10][3];
-#pragma unroll
-    for (int a = 0; a < 10; ++a) {
-      const int64_t nd = conn[e * 10 + a];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) xa[a][c] = x[3 * nd + c];
-    }
+  __shared__ double fl[kTileItems];
+  const int64_t n_tiles = (m + kBlock - 1) / kBlock;
+  for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < n_tiles; t += gridDim.x) {
+    const int64_t e = t * kBlock + threadIdx.x;
     double f[10][3];
-    t10_forces_4pt<double>(xa, J, V, lam, mu, f);
-    if (slot) {
-      // each nodal force goes to its position in node-sorted incidence order,
-      // so the node sums read a contiguous range
+    int rk[10];
+    if (e < m) {
+      const double* g = geo + e * 10;
+      double J[3][3];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) J[r][c] = g[3 * r + c];
+      const double V = g[9];
+      double xa[10][3];
 #pragma unroll
       for (int a = 0; a < 10; ++a) {
-        double* o = fe + int64_t(slot[e * 10 + a]) * 3;
-        o[0] = f[a][0];
-        o[1] = f[a][1];
-        o[2] = f[a][2];
+        const int64_t nd = conn[e * 10 + a];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) xa[a][c] = x[3 * nd + c];
       }
-    } else {
-      double* o = fe + e * 30;
+      t10_forces_4pt<double>(xa, J, V, lam, mu, f);
 #pragma unroll
-      for (int a = 0; a < 10; ++a)
+      for (int a = 0; a < 10; ++a) rk[a] = rank[e * 10 + a];
+    }
+    const int64_t k0 = t * kTileItems;
+    const int nt = int((k0 + kTileItems < 10 * m ? k0 + kTileItems : 10 * m) - k0);
+    const int32_t q0 = toff[t], q1 = toff[t + 1];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) o[3 * a + c] = f[a][c];
+    for (int c = 0; c < 3; ++c) {
+      if (e < m) {
+#pragma unroll
+        for (int a = 0; a < 10; ++a) fl[rk[a]] = f[a][c];
+      }
+      __syncthreads();
+      for (int32_t q = q0 + threadIdx.x; q < q1; q += kBlock) {
+        const int kb = tstart[q], ke = q + 1 < q1 ? int(tstart[q + 1]) : nt;
+        double sum = 0.0;
+        for (int k = kb; k < ke; ++k) sum += fl[k];
+        part[c * n_runs + q] = sum;
+      }
+      __syncthreads();
     }
   }
 }
 
-// y_i = sum of the node's contiguous force slots fe[offs[i] .. offs[i+1])
-// (written in node-sorted order by mf_t10_elem), fixed rows zeroed; with
+// y_i = sum of node i's run partials in tile order, fixed rows zeroed; with
 // `state`: p.Ap partials.
 __global__ void __launch_bounds__(kBlock)
-mf_gather_sorted(int64_t n_nodes, const int32_t* __restrict__ offs,
-                 const double* __restrict__ fe, const uint8_t* __restrict__ fixed,
-                 const double* __restrict__ x, double* __restrict__ y, PcgState* state,
-                 double* __restrict__ partials) {
+mf_gather_runs(int64_t n_nodes, const int32_t* __restrict__ nro, const uint32_t* __restrict__ nq,
+               int64_t n_runs, const double* __restrict__ part,
+               const uint8_t* __restrict__ fixed, const double* __restrict__ x,
+               double* __restrict__ y, PcgState* state, double* __restrict__ partials) {
   if (state && state->halt) return;
   double dot = 0.0;
   const int64_t i = tile_index(blockIdx.x) * kBlock + threadIdx.x;
   if (i < n_nodes) {
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    const double* v = fe + int64_t(offs[i]) * 3;
-    const double* end = fe + int64_t(offs[i + 1]) * 3;
-    for (; v + 12 <= end; v += 12) {
-      double t[12];
+    double o[3] = {0.0, 0.0, 0.0};
+    for (int32_t j = nro[i]; j < nro[i + 1]; ++j) {
+      const int64_t q = nq[j];
 #pragma unroll
-      for (int u = 0; u < 12; ++u) t[u] = v[u];
-#pragma unroll
-      for (int u = 0; u < 12; u += 3) {
-        a0 += t[u];
-        a1 += t[u + 1];
-        a2 += t[u + 2];
-      }
+      for (int c = 0; c < 3; ++c) o[c] += part[c * n_runs + q];
     }
-    for (; v < end; v += 3) {
-      a0 += v[0];
-      a1 += v[1];
-      a2 += v[2];
-    }
-    double o[3] = {a0, a1, a2};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const int64_t d = 3 * i + c;
@@ -1333,124 +1444,6 @@ __global__ void geo_to_f32(int64_t n, const double* __restrict__ g, float* __res
     g32[i] = float(g[i]);
 }
 
-// Run sums of the FP32 inner operator.  Scattered 12-byte stores of the
-// forces into node-sorted slots cost one L2 write request each (3.0 M per
-// product at 384,000 elements, the L2 busy for the whole kernel: 25.6 us).
-// Instead the block of a tile of kBlock consecutive elements stages the
-// tile's forces in LDS sorted by (node, element) -- item (e, a) at its rank
-// k -- so that the items of one node form a contiguous run; it sums each
-// run in element order and writes one partial per (tile, node) pair,
-// coalesced (0.78 M partials instead of 3.84 M forces at 384,000 elements).
-// The node gather then adds its node's partials in tile order (1.4 per node
-// on Tet10 box meshes).  Deterministic: fixed orders throughout.
-//   rank[e * 10 + a]  k of item (e, a) in its tile
-//   toff[t]           first run of tile t (runs numbered tile by tile, by
-//                     node inside a tile); tstart[q] = first k of run q
-//   nq[j]             runs of node n at j in [nro[n], nro[n+1]), tile order
-constexpr int kTileItems = kBlock * 10;
-
-__global__ void iota_u32(int64_t n, uint32_t* __restrict__ v) {
-  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
-       i += int64_t(gridDim.x) * blockDim.x)
-    v[i] = uint32_t(i);
-}
-
-// keys[p] = tile of the item in node-sorted slot p (perm[p] = e * 10 + a)
-__global__ void mf_store_keys(int64_t n_items, const uint32_t* __restrict__ perm,
-                              uint32_t* __restrict__ keys, uint32_t* __restrict__ ids) {
-  for (int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; p < n_items;
-       p += int64_t(gridDim.x) * blockDim.x) {
-    keys[p] = perm[p] / uint32_t(kTileItems);
-    ids[p] = uint32_t(p);
-  }
-}
-
-// tslot: the slots sorted by tile (stable, so by slot inside a tile); the
-// rank of an item is its position in its tile's list (every tile but the
-// last holds exactly kTileItems items)
-__global__ void mf_store_rank(int64_t n_items, const uint32_t* __restrict__ perm,
-                              const uint32_t* __restrict__ tslot, uint16_t* __restrict__ rank) {
-  for (int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; k < n_items;
-       k += int64_t(gridDim.x) * blockDim.x)
-    rank[perm[tslot[k]]] = uint16_t(k % kTileItems);
-}
-
-// head[p] = 1 where a run starts: slot p is the first of its node
-// (mf_node_heads) or its item lies in another tile than the item of slot
-// p - 1 (mf_tile_heads); head[n_items] = 0.
-__global__ void mf_tile_heads(int64_t n_items, const uint32_t* __restrict__ perm,
-                              uint32_t* __restrict__ head) {
-  for (int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; p <= n_items;
-       p += int64_t(gridDim.x) * blockDim.x) {
-    uint32_t h = 0;
-    if (p < n_items)
-      h = p == 0 || perm[p] / uint32_t(kTileItems) != perm[p - 1] / uint32_t(kTileItems);
-    head[p] = h;
-  }
-}
-
-__global__ void mf_node_heads(int64_t n_nodes, const int32_t* __restrict__ offs,
-                              uint32_t* __restrict__ head) {
-  for (int64_t n = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; n < n_nodes;
-       n += int64_t(gridDim.x) * blockDim.x)
-    if (offs[n] < offs[n + 1]) head[offs[n]] = 1;
-}
-
-// run r starting at slot p (ridx[p] = r): rk = position of the slot's item in
-// the tile-major layout, rs = p; nro[n] = ridx[offs[n]]
-__global__ void mf_run_fill(int64_t n_items, const uint32_t* __restrict__ perm,
-                            const uint16_t* __restrict__ rank, const uint32_t* __restrict__ head,
-                            const uint32_t* __restrict__ ridx, uint32_t* __restrict__ rk,
-                            uint32_t* __restrict__ rs) {
-  for (int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; p < n_items;
-       p += int64_t(gridDim.x) * blockDim.x) {
-    if (!head[p]) continue;
-    const uint32_t it = perm[p];
-    rk[ridx[p]] = (it / uint32_t(kTileItems)) * uint32_t(kTileItems) + rank[it];
-    rs[ridx[p]] = uint32_t(p);
-  }
-}
-
-__global__ void mf_run_len(int64_t n_items, const uint32_t* __restrict__ ridx,
-                           const uint32_t* __restrict__ rs, uint16_t* __restrict__ rl,
-                           int64_t n_nodes, const int32_t* __restrict__ offs,
-                           int32_t* __restrict__ nro) {
-  const int64_t n_runs = ridx[n_items];
-  for (int64_t r = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; r < n_runs;
-       r += int64_t(gridDim.x) * blockDim.x)
-    rl[r] = uint16_t((r + 1 < n_runs ? int64_t(rs[r + 1]) : n_items) - int64_t(rs[r]));
-  for (int64_t n = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; n <= n_nodes;
-       n += int64_t(gridDim.x) * blockDim.x)
-    nro[n] = int32_t(ridx[offs[n]]);
-}
-
-// tile order of the runs: rq[q] = node-order run id of the q-th smallest
-// start rk; nq[r] = q, tstart[q] = start inside the tile
-__global__ void mf_run_tile_order(int64_t n_runs, const uint32_t* __restrict__ rk_sorted,
-                                  const uint32_t* __restrict__ rq, uint32_t* __restrict__ nq,
-                                  uint16_t* __restrict__ tstart) {
-  for (int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; q < n_runs;
-       q += int64_t(gridDim.x) * blockDim.x) {
-    nq[rq[q]] = uint32_t(q);
-    tstart[q] = uint16_t(rk_sorted[q] % uint32_t(kTileItems));
-  }
-}
-
-// toff[t] = first q with rk_sorted[q] >= t * kTileItems (t = 0 .. n_tiles)
-__global__ void mf_tile_run_off(int64_t n_tiles, int64_t n_runs,
-                                const uint32_t* __restrict__ rk_sorted, int32_t* __restrict__ toff) {
-  for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t <= n_tiles;
-       t += int64_t(gridDim.x) * blockDim.x) {
-    const uint64_t target = uint64_t(t) * kTileItems;
-    int64_t lo = 0, hi = n_runs;
-    while (lo < hi) {
-      const int64_t mid = (lo + hi) / 2;
-      if (uint64_t(rk_sorted[mid]) >= target) hi = mid; else lo = mid + 1;
-    }
-    toff[t] = int32_t(lo);
-  }
-}
-
 // PRE: d is already S-scaled (the Chebyshev vectors of the SQ form), so
 // the kernel gathers it as is; otherwise it gathers S d.  Writes the run
 // partials described above.
@@ -1569,60 +1562,6 @@ __global__ void invert_permutation(int64_t n, const uint32_t* __restrict__ perm,
     inv[perm[j]] = uint32_t(j);
 }
 
-// y_i = sum over the node's incidence items (e, a) of fe[e][a] (fixed
-// rows zeroed); with `state`: p.Ap and alpha as in the assembled SpMV.
-__global__ void __launch_bounds__(kBlock)
-mf_gather(int64_t n_nodes, const int32_t* __restrict__ offs, const uint32_t* __restrict__ perm,
-          const double* __restrict__ fe, const uint8_t* __restrict__ fixed,
-          const double* __restrict__ x, double* __restrict__ y, PcgState* state,
-          double* __restrict__ partials) {
-  if (state && state->halt) return;
-  int64_t r0, r1;
-  block_rows(n_nodes, r0, r1);
-  double dot = 0.0;
-  for (int64_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    const int32_t j1 = offs[i + 1];
-    int32_t j = offs[i];
-    // 4 items per step: all index loads, then all force loads, then sums
-    for (; j + 4 <= j1; j += 4) {
-      uint32_t it[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) it[u] = perm[j + u];
-      double f[4][3];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const double* v = fe + int64_t(it[u]) * 3;
-        f[u][0] = v[0];
-        f[u][1] = v[1];
-        f[u][2] = v[2];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        a0 += f[u][0];
-        a1 += f[u][1];
-        a2 += f[u][2];
-      }
-    }
-    for (; j < j1; ++j) {
-      const double* v = fe + int64_t(perm[j]) * 3;
-      a0 += v[0];
-      a1 += v[1];
-      a2 += v[2];
-    }
-    double o[3] = {a0, a1, a2};
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const int64_t d = 3 * i + c;
-      if (fixed && fixed[d]) o[c] = 0.0;
-      y[d] = o[c];
-      if (state) dot += x[d] * o[c];
-    }
-  }
-  if (!state) return;
-  double v1[1] = {dot};
-  block_partials<1>(v1, partials);
-}
 
 // Diagonal 3x3 node blocks of the Tet10 elasticity matrix without
 // assembling it: block_i = sum_(e,a) V (lam M + mu M^T + mu tr(M) I),
@@ -2992,12 +2931,105 @@ struct MfArgs {
   double lam, mu;
   const int32_t* offs;
   const uint32_t* perm;
-  const uint32_t* slot;  // optional: inverse of perm (node-sorted force slots)
 };
 
 int mf_check(const MfArgs& a) {
   CF_REQUIRE(a.n_nodes > 0 && a.n_elem > 0 && a.conn && a.geo && a.offs && a.perm,
              "matrix-free Tet10: bad arguments");
+  return CF_OK;
+}
+
+// Run plan of the matrix-free operator (see mf_t10_elem): built per solve
+// from the node-sorted incidence list, in the solver's workspace.
+struct MfRuns {
+  int64_t n_runs;    // set by mf_runs_build (host)
+  uint16_t* rank;    // rank of each item in its tile
+  int32_t* toff;     // first run of each tile
+  uint16_t* tstart;  // start of each run inside its tile
+  uint32_t* nq;      // runs of each node, tile order
+  int32_t* nro;      // first entry of each node in nq
+  // build temporaries
+  uint32_t* keys_in;
+  uint32_t* keys_out;
+  uint32_t* ids_in;
+  uint32_t* tslot;
+  uint32_t* rk;
+  uint16_t* rl;
+  void* tmp;
+  size_t tmp_bytes;
+};
+
+void mf_runs_layout(int64_t n_nodes, int64_t n_elem, cf::Carve& c, MfRuns* R) {
+  const int64_t items = 10 * n_elem;
+  const int64_t n_tiles = (n_elem + kBlock - 1) / kBlock;
+  R->n_runs = 0;
+  R->rank = c.take<uint16_t>(items);
+  R->toff = c.take<int32_t>(n_tiles + 1);
+  R->tstart = c.take<uint16_t>(items);
+  R->nq = c.take<uint32_t>(items);
+  R->nro = c.take<int32_t>(n_nodes + 1);
+  R->rk = c.take<uint32_t>(items);
+  R->rl = c.take<uint16_t>(items);
+  R->keys_in = c.take<uint32_t>(items + 1);
+  R->keys_out = c.take<uint32_t>(items + 1);
+  R->ids_in = c.take<uint32_t>(items);
+  R->tslot = c.take<uint32_t>(items);
+  size_t sort_bytes = 0, scan_bytes = 0;
+  rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                            (uint32_t*)nullptr, (uint32_t*)nullptr, size_t(items), 0, 32);
+  rocprim::exclusive_scan(nullptr, scan_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, 0u,
+                          size_t(items + 1), rocprim::plus<uint32_t>());
+  R->tmp_bytes = std::max(sort_bytes, scan_bytes);
+  R->tmp = c.take<char>(R->tmp_bytes);
+}
+
+// Run plan from the node-sorted incidence list perm and its node offsets
+// offs (one host synchronisation, for the run count).
+int mf_runs_build(int64_t n_nodes, int64_t n_elem, const int32_t* offs, const uint32_t* perm,
+                  MfRuns* R, hipStream_t s) {
+  const MfRuns& M = *R;
+  const int64_t items = 10 * n_elem;
+  const int grid = cf::stride_grid(items + 1, kBlock);
+  mf_store_keys<<<grid, kBlock, 0, s>>>(items, perm, M.keys_in, M.ids_in);
+  CF_LAUNCHED();
+  size_t tb = M.tmp_bytes;
+  CF_HIP(rocprim::radix_sort_pairs(M.tmp, tb, M.keys_in, M.keys_out, M.ids_in, M.tslot,
+                                   size_t(items), 0,
+                                   bits_for((n_elem + kBlock - 1) / kBlock + 1), s));
+  mf_store_rank<<<grid, kBlock, 0, s>>>(items, perm, M.tslot, M.rank);
+  CF_LAUNCHED();
+  uint32_t* head = M.keys_in;
+  uint32_t* ridx = M.keys_out;
+  uint32_t* rs = M.ids_in;
+  mf_tile_heads<<<grid, kBlock, 0, s>>>(items, perm, head);
+  CF_LAUNCHED();
+  mf_node_heads<<<cf::stride_grid(n_nodes, kBlock), kBlock, 0, s>>>(n_nodes, offs, head);
+  CF_LAUNCHED();
+  tb = M.tmp_bytes;
+  CF_HIP(rocprim::exclusive_scan(M.tmp, tb, head, ridx, 0u, size_t(items + 1),
+                                 rocprim::plus<uint32_t>(), s));
+  mf_run_fill<<<grid, kBlock, 0, s>>>(items, perm, M.rank, head, ridx, M.rk, rs);
+  CF_LAUNCHED();
+  mf_run_len<<<grid, kBlock, 0, s>>>(items, ridx, rs, M.rl, n_nodes, offs, M.nro);
+  CF_LAUNCHED();
+  // number the runs tile by tile: sort the run starts rk
+  uint32_t n_runs = 0;
+  CF_HIP(hipMemcpyAsync(&n_runs, ridx + items, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  CF_HIP(hipStreamSynchronize(s));
+  CF_REQUIRE(n_runs > 0 && int64_t(n_runs) <= items, "matrix-free plan: bad run count %u",
+             n_runs);
+  R->n_runs = n_runs;
+  iota_u32<<<grid, kBlock, 0, s>>>(n_runs, M.ids_in);
+  CF_LAUNCHED();
+  tb = M.tmp_bytes;
+  CF_HIP(rocprim::radix_sort_pairs(M.tmp, tb, M.rk, M.keys_out, M.ids_in, M.tslot,
+                                   size_t(n_runs), 0, 32, s));
+  mf_run_tile_order<<<grid, kBlock, 0, s>>>(n_runs, M.keys_out, M.tslot, M.nq, M.tstart);
+  CF_LAUNCHED();
+  const int64_t n_tiles = (n_elem + kBlock - 1) / kBlock;
+  mf_tile_run_off<<<cf::stride_grid(n_tiles + 1, kBlock), kBlock, 0, s>>>(n_tiles, n_runs,
+                                                                          M.keys_out, M.toff);
+  CF_LAUNCHED();
   return CF_OK;
 }
 
@@ -3008,7 +3040,8 @@ int64_t mf_gather_grid(int64_t n_nodes) { return tile_grid((n_nodes + kBlock - 1
 // shared-node partials (3 per shared pair); parts: most block partials a
 // product writes.
 size_t mf_layout(int64_t n_nodes, int64_t fe_doubles, int64_t parts, int precond, void* base,
-                 Layout* L, double** fe, double** blocks) {
+                 Layout* L, double** fe, double** blocks, MfRuns* R = nullptr,
+                 int64_t n_elem = 0) {
   cf::Carve c(base);
   const int64_t n = 3 * n_nodes;
   L->r = c.take<double>(n);
@@ -3021,28 +3054,26 @@ size_t mf_layout(int64_t n_nodes, int64_t fe_doubles, int64_t parts, int precond
   L->bad = c.take<int32_t>(1);
   *fe = c.take<double>(std::max<int64_t>(fe_doubles, 1));
   *blocks = c.take<double>(9 * n_nodes);
+  if (R) mf_runs_layout(n_nodes, n_elem, c, R);
   L->total = c.used;
   return c.used;
 }
 
+// two-pass operator: fe holds the FP64 run partials (at most 3 per item)
 size_t mf_layout(int64_t n_nodes, int64_t n_elem, int precond, void* base, Layout* L,
-                 double** fe, double** blocks) {
-  return mf_layout(n_nodes, 30 * n_elem, mf_gather_grid(n_nodes), precond, base, L, fe, blocks);
+                 double** fe, double** blocks, MfRuns* R) {
+  return mf_layout(n_nodes, 30 * n_elem, mf_gather_grid(n_nodes), precond, base, L, fe, blocks,
+                   R, n_elem);
 }
 
-int mf_apply(const MfArgs& a, const double* xin, double* y, const uint8_t* fixed, double* fe,
-             PcgState* state, double* partials, hipStream_t s) {
-  const int G = pcg_grid();
+int mf_apply(const MfArgs& a, const MfRuns& R, const double* xin, double* y,
+             const uint8_t* fixed, double* part, PcgState* state, double* partials,
+             hipStream_t s) {
   mf_t10_elem<<<cf::stride_grid(a.n_elem, kBlock), kBlock, 0, s>>>(
-      a.n_elem, a.conn, a.geo, a.lam, a.mu, xin, a.slot, fe, state);
+      a.n_elem, a.conn, a.geo, a.lam, a.mu, xin, R.rank, R.toff, R.tstart, R.n_runs, part, state);
   CF_LAUNCHED();
-  if (a.slot)
-    mf_gather_sorted<<<int(mf_gather_grid(a.n_nodes)), kBlock, 0, s>>>(a.n_nodes, a.offs, fe,
-                                                                      fixed, xin, y, state,
-                                                                      partials);
-  else
-    mf_gather<<<G, kBlock, 0, s>>>(a.n_nodes, a.offs, a.perm, fe, fixed, xin, y, state,
-                                   partials);
+  mf_gather_runs<<<int(mf_gather_grid(a.n_nodes)), kBlock, 0, s>>>(
+      a.n_nodes, R.nro, R.nq, R.n_runs, part, fixed, xin, y, state, partials);
   CF_LAUNCHED();
   return CF_OK;
 }
@@ -3191,53 +3222,58 @@ extern "C" int cf_mf_t10_workspace_bytes(int64_t n_nodes, int64_t n_elem, int32_
   CF_REQUIRE(bytes && n_nodes > 0 && n_elem > 0, "cf_mf_t10_workspace_bytes: bad arguments");
   Layout L{};
   double *fe, *blocks;
-  *bytes = mf_layout(n_nodes, n_elem, precond, nullptr, &L, &fe, &blocks);
+  MfRuns R{};
+  *bytes = mf_layout(n_nodes, n_elem, precond, nullptr, &L, &fe, &blocks, &R);
   return CF_OK;
 }
 
 extern "C" int cf_mf_t10_apply(int64_t n_nodes, int64_t n_elem, const int32_t* conn,
                                const double* geo, double lam, double mu, const int32_t* offs,
-                               const uint32_t* perm, const uint32_t* slot, const double* x,
-                               double* y, const uint8_t* fixed, void* ws, size_t ws_bytes,
-                               void* stream) {
-  MfArgs a{n_nodes, n_elem, conn, geo, lam, mu, offs, perm, slot};
+                               const uint32_t* perm, const double* x, double* y,
+                               const uint8_t* fixed, void* ws, size_t ws_bytes, void* stream) {
+  MfArgs a{n_nodes, n_elem, conn, geo, lam, mu, offs, perm};
   int rc = mf_check(a);
   if (rc) return rc;
   CF_REQUIRE(x && y && ws, "cf_mf_t10_apply: null argument");
+  CF_REQUIRE(10 * n_elem < (int64_t(1) << 31), "cf_mf_t10_apply: too many elements");
   Layout L{};
   double *fe, *blocks;
-  const size_t need = mf_layout(n_nodes, n_elem, CF_PC_JACOBI, ws, &L, &fe, &blocks);
+  MfRuns R{};
+  const size_t need = mf_layout(n_nodes, n_elem, CF_PC_JACOBI, ws, &L, &fe, &blocks, &R);
   if (ws_bytes < need) {
     cf::set_error("cf_mf_t10_apply: workspace %zu < %zu bytes", ws_bytes, need);
     return CF_ERR_WORKSPACE;
   }
-  return mf_apply(a, x, y, fixed, fe, nullptr, nullptr, cf::as_stream(stream));
+  hipStream_t s = cf::as_stream(stream);
+  if ((rc = mf_runs_build(n_nodes, n_elem, offs, perm, &R, s))) return rc;
+  return mf_apply(a, R, x, y, fixed, fe, nullptr, nullptr, s);
 }
 
 extern "C" int cf_pcg_solve_mf_t10(int64_t n_nodes, int64_t n_elem, const int32_t* conn,
                                    const double* geo, double lam, double mu,
                                    const int32_t* offs, const uint32_t* perm,
-                                   const uint32_t* slot, const double* b, double* x,
-                                   const uint8_t* fixed, int32_t precond, double tol,
-                                   int32_t maxit, void* ws, size_t ws_bytes,
-                                   cf_pcg_stats* stats_host, void* stream) {
-  MfArgs a{n_nodes, n_elem, conn, geo, lam, mu, offs, perm, slot};
+                                   const double* b, double* x, const uint8_t* fixed,
+                                   int32_t precond, double tol, int32_t maxit, void* ws,
+                                   size_t ws_bytes, cf_pcg_stats* stats_host, void* stream) {
+  MfArgs a{n_nodes, n_elem, conn, geo, lam, mu, offs, perm};
   int rc = mf_check(a);
   if (rc) return rc;
   CF_REQUIRE(b && x && ws && stats_host, "cf_pcg_solve_mf_t10: null argument");
   CF_REQUIRE(precond == CF_PC_JACOBI || precond == CF_PC_BLOCK3,
              "cf_pcg_solve_mf_t10: bad precond %d", precond);
   CF_REQUIRE(tol > 0.0 && maxit > 0, "cf_pcg_solve_mf_t10: tol and maxit must be positive");
+  CF_REQUIRE(10 * n_elem < (int64_t(1) << 31), "cf_pcg_solve_mf_t10: too many elements");
   Layout L{};
   double *fe, *blocks;
-  const size_t need = mf_layout(n_nodes, n_elem, precond, ws, &L, &fe, &blocks);
+  MfRuns R{};
+  const size_t need = mf_layout(n_nodes, n_elem, precond, ws, &L, &fe, &blocks, &R);
   if (ws_bytes < need) {
     cf::set_error("cf_pcg_solve_mf_t10: workspace %zu < %zu bytes", ws_bytes, need);
     return CF_ERR_WORKSPACE;
   }
   hipStream_t s = cf::as_stream(stream);
-  const int G = pcg_grid();
   *stats_host = cf_pcg_stats{};
+  if ((rc = mf_runs_build(n_nodes, n_elem, offs, perm, &R, s))) return rc;
   CF_HIP(hipMemsetAsync(L.state, 0, sizeof(PcgState), s));
   CF_HIP(hipMemsetAsync(L.bad, 0, sizeof(int32_t), s));
   mf_t10_diag<<<cf::stride_grid(n_nodes, kBlock), kBlock, 0, s>>>(n_nodes, offs, perm, geo, lam,
@@ -3254,10 +3290,10 @@ extern "C" int cf_pcg_solve_mf_t10(int64_t n_nodes, int64_t n_elem, const int32_
     return CF_ERR_BREAKDOWN;
   }
   auto apply = [&](const double* xin, double* y, PcgState* state, double* partials) {
-    return mf_apply(a, xin, y, fixed, fe, state, partials, s);
+    return mf_apply(a, R, xin, y, fixed, fe, state, partials, s);
   };
   const int64_t n = 3 * n_nodes;
-  const int parts = a.slot ? int(mf_gather_grid(n_nodes)) : G;
+  const int parts = int(mf_gather_grid(n_nodes));
   if (precond == CF_PC_BLOCK3)
     return pcg_fp64_core<CF_PC_BLOCK3>(apply, parts, n, n_nodes, b, x, fixed, tol, maxit, L,
                                        stats_host, s);
@@ -3277,7 +3313,7 @@ extern "C" int cf_mf_t10_plan_build(int64_t n_nodes, int64_t n_elem, const int32
                                     const double* z, const int32_t* offs, const double* bbox,
                                     void* plan, size_t plan_bytes, int64_t* counts,
                                     void* stream) {
-  MfArgs a{n_nodes, n_elem, conn, geo, 0.0, 0.0, offs, nullptr, nullptr};
+  MfArgs a{n_nodes, n_elem, conn, geo, 0.0, 0.0, offs, nullptr};
   CF_REQUIRE(n_nodes > 0 && n_elem > 0 && conn && geo && offs && x && y && z && bbox && plan &&
                  counts,
              "cf_mf_t10_plan_build: bad arguments");
@@ -3308,7 +3344,7 @@ extern "C" int cf_mf_t10_plan_apply(int64_t n_nodes, int64_t n_elem, const void*
                                     size_t ws_bytes, void* stream) {
   CF_REQUIRE(n_nodes > 0 && n_elem > 0 && plan && counts && x && y && ws,
              "cf_mf_t10_plan_apply: bad arguments");
-  MfArgs a{n_nodes, n_elem, nullptr, nullptr, lam, mu, nullptr, nullptr, nullptr};
+  MfArgs a{n_nodes, n_elem, nullptr, nullptr, lam, mu, nullptr, nullptr};
   MfPlan P = mf_plan_layout(n_elem, const_cast<void*>(plan));
   Layout L{};
   double *fe, *blocks;
@@ -3329,7 +3365,7 @@ extern "C" int cf_pcg_solve_mf_t10_plan(int64_t n_nodes, int64_t n_elem, const i
                                         int32_t precond, double tol, int32_t maxit, void* ws,
                                         size_t ws_bytes, cf_pcg_stats* stats_host,
                                         void* stream) {
-  MfArgs a{n_nodes, n_elem, conn, geo, lam, mu, offs, perm, nullptr};
+  MfArgs a{n_nodes, n_elem, conn, geo, lam, mu, offs, perm};
   int rc = mf_check(a);
   if (rc) return rc;
   CF_REQUIRE(plan && counts && b && x && ws && stats_host,
@@ -3380,26 +3416,12 @@ extern "C" int cf_pcg_solve_mf_t10_plan(int64_t n_nodes, int64_t n_elem, const i
 namespace {
 
 struct MfMixed {
-  double* fe;      // FP64 element forces (outer product)
+  double* fe;      // FP64 run partials of the outer product (component planes)
   double* blocks;  // diagonal 3x3 blocks
   float* geo32;    // geometry rounded to FP32
-  float* fe32;     // FP32 run partials of the inner products
+  float* fe32;     // FP32 run partials of the inner products (AoS, 3 per run)
   float* s32;      // Jacobi scaling in FP32
-  // run-sum plan of mf_t10_elem32 / mf_gather_cheb32
-  uint16_t* rank;    // rank of each item in its tile
-  int32_t* toff;     // first run of each tile
-  uint16_t* tstart;  // start of each run inside its tile
-  uint32_t* nq;      // runs of each node, tile order
-  int32_t* nro;      // first entry of each node in nq
-  // plan temporaries
-  uint32_t* keys_in;
-  uint32_t* keys_out;
-  uint32_t* ids_in;
-  uint32_t* tslot;
-  uint32_t* rk;
-  uint16_t* rl;
-  void* tmp;
-  size_t tmp_bytes;
+  MfRuns R;          // run plan shared by the FP64 and FP32 products
 };
 
 size_t mf_mixed_layout(int64_t n_nodes, int64_t n_elem, void* base, Layout* L, MfMixed* M) {
@@ -3425,76 +3447,9 @@ size_t mf_mixed_layout(int64_t n_nodes, int64_t n_elem, void* base, Layout* L, M
   M->geo32 = c.take<float>(10 * n_elem);
   M->fe32 = c.take<float>(30 * n_elem);
   M->s32 = c.take<float>(n);
-  const int64_t items = 10 * n_elem;
-  const int64_t n_tiles = (n_elem + kBlock - 1) / kBlock;
-  M->rank = c.take<uint16_t>(items);
-  M->toff = c.take<int32_t>(n_tiles + 1);
-  M->tstart = c.take<uint16_t>(items);
-  M->nq = c.take<uint32_t>(items);
-  M->nro = c.take<int32_t>(n_nodes + 1);
-  M->rk = c.take<uint32_t>(items);
-  M->rl = c.take<uint16_t>(items);
-  M->keys_in = c.take<uint32_t>(items + 1);
-  M->keys_out = c.take<uint32_t>(items + 1);
-  M->ids_in = c.take<uint32_t>(items);
-  M->tslot = c.take<uint32_t>(items);
-  size_t sort_bytes = 0, scan_bytes = 0;
-  rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                            (uint32_t*)nullptr, (uint32_t*)nullptr, size_t(items), 0, 32);
-  rocprim::exclusive_scan(nullptr, scan_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, 0u,
-                          size_t(items + 1), rocprim::plus<uint32_t>());
-  M->tmp_bytes = std::max(sort_bytes, scan_bytes);
-  M->tmp = c.take<char>(M->tmp_bytes);
+  mf_runs_layout(n_nodes, n_elem, c, &M->R);
   L->total = c.used;
   return c.used;
-}
-
-// Tile-major layout plan (rank, runs) from the node-sorted incidence list
-// perm and its node offsets offs.
-int mf_store_plan(int64_t n_nodes, int64_t n_elem, const int32_t* offs, const uint32_t* perm,
-                  const MfMixed& M, hipStream_t s) {
-  const int64_t items = 10 * n_elem;
-  const int grid = cf::stride_grid(items + 1, kBlock);
-  mf_store_keys<<<grid, kBlock, 0, s>>>(items, perm, M.keys_in, M.ids_in);
-  CF_LAUNCHED();
-  size_t tb = M.tmp_bytes;
-  CF_HIP(rocprim::radix_sort_pairs(M.tmp, tb, M.keys_in, M.keys_out, M.ids_in, M.tslot,
-                                   size_t(items), 0,
-                                   bits_for((n_elem + kBlock - 1) / kBlock + 1), s));
-  mf_store_rank<<<grid, kBlock, 0, s>>>(items, perm, M.tslot, M.rank);
-  CF_LAUNCHED();
-  uint32_t* head = M.keys_in;
-  uint32_t* ridx = M.keys_out;
-  uint32_t* rs = M.ids_in;
-  mf_tile_heads<<<grid, kBlock, 0, s>>>(items, perm, head);
-  CF_LAUNCHED();
-  mf_node_heads<<<cf::stride_grid(n_nodes, kBlock), kBlock, 0, s>>>(n_nodes, offs, head);
-  CF_LAUNCHED();
-  tb = M.tmp_bytes;
-  CF_HIP(rocprim::exclusive_scan(M.tmp, tb, head, ridx, 0u, size_t(items + 1),
-                                 rocprim::plus<uint32_t>(), s));
-  mf_run_fill<<<grid, kBlock, 0, s>>>(items, perm, M.rank, head, ridx, M.rk, rs);
-  CF_LAUNCHED();
-  mf_run_len<<<grid, kBlock, 0, s>>>(items, ridx, rs, M.rl, n_nodes, offs, M.nro);
-  CF_LAUNCHED();
-  // number the runs tile by tile: sort the run starts rk
-  uint32_t n_runs = 0;
-  CF_HIP(hipMemcpyAsync(&n_runs, ridx + items, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  CF_HIP(hipStreamSynchronize(s));
-  CF_REQUIRE(n_runs > 0 && int64_t(n_runs) <= items, "matrix-free plan: bad run count %u",
-             n_runs);
-  iota_u32<<<grid, kBlock, 0, s>>>(n_runs, M.ids_in);
-  CF_LAUNCHED();
-  tb = M.tmp_bytes;
-  CF_HIP(rocprim::radix_sort_pairs(M.tmp, tb, M.rk, M.keys_out, M.ids_in, M.tslot,
-                                   size_t(n_runs), 0, 32, s));
-  mf_run_tile_order<<<grid, kBlock, 0, s>>>(n_runs, M.keys_out, M.tslot, M.nq, M.tstart);
-  CF_LAUNCHED();
-  const int64_t n_tiles = (n_elem + kBlock - 1) / kBlock;
-  mf_tile_run_off<<<cf::stride_grid(n_tiles + 1, kBlock), kBlock, 0, s>>>(n_tiles, n_runs,
-                                                                          M.keys_out, M.toff);
-  CF_LAUNCHED();
-  return CF_OK;
 }
 
 }  // namespace
@@ -3510,14 +3465,13 @@ extern "C" int cf_mf_t10_mixed_workspace_bytes(int64_t n_nodes, int64_t n_elem, 
 extern "C" int cf_pcg_solve_mf_t10_mixed(int64_t n_nodes, int64_t n_elem, const int32_t* conn,
                                          const double* geo, double lam, double mu,
                                          const int32_t* offs, const uint32_t* perm,
-                                         const uint32_t* slot, const double* b, double* x,
-                                         const uint8_t* fixed, double tol, int32_t maxit,
-                                         void* ws, size_t ws_bytes, cf_pcg_stats* stats_host,
-                                         void* stream) {
-  MfArgs a{n_nodes, n_elem, conn, geo, lam, mu, offs, perm, slot};
+                                         const double* b, double* x, const uint8_t* fixed,
+                                         double tol, int32_t maxit, void* ws, size_t ws_bytes,
+                                         cf_pcg_stats* stats_host, void* stream) {
+  MfArgs a{n_nodes, n_elem, conn, geo, lam, mu, offs, perm};
   int rc = mf_check(a);
   if (rc) return rc;
-  CF_REQUIRE(slot && b && x && ws && stats_host, "cf_pcg_solve_mf_t10_mixed: null argument");
+  CF_REQUIRE(b && x && ws && stats_host, "cf_pcg_solve_mf_t10_mixed: null argument");
   CF_REQUIRE(10 * n_elem < (int64_t(1) << 31), "cf_pcg_solve_mf_t10_mixed: too many elements");
   CF_REQUIRE(tol > 0.0 && maxit > 0, "cf_pcg_solve_mf_t10_mixed: tol and maxit must be positive");
   Layout L{};
@@ -3552,11 +3506,11 @@ extern "C" int cf_pcg_solve_mf_t10_mixed(int64_t n_nodes, int64_t n_elem, const 
   CF_LAUNCHED();
   geo_to_f32<<<cf::stride_grid(10 * n_elem, kBlock), kBlock, 0, s>>>(10 * n_elem, geo, M.geo32);
   CF_LAUNCHED();
-  if ((rc = mf_store_plan(n_nodes, n_elem, offs, perm, M, s))) return rc;
+  if ((rc = mf_runs_build(n_nodes, n_elem, offs, perm, &M.R, s))) return rc;
   const int ge = cf::stride_grid(n_elem, kBlock);
   const int gg = int(mf_gather_grid(n_nodes));
   auto outer = [&](const double* xin, double* y, PcgState* state) {
-    return mf_apply(a, xin, y, fixed, M.fe, state, state ? L.partials : nullptr, s);
+    return mf_apply(a, M.R, xin, y, fixed, M.fe, state, state ? L.partials : nullptr, s);
   };
   // the power iteration multiplies plain vectors (S K S v); the Chebyshev
   // steps run on S-scaled vectors, so their element kernel gathers no S.
@@ -3566,14 +3520,14 @@ extern "C" int cf_pcg_solve_mf_t10_mixed(int64_t n_nodes, int64_t n_elem, const 
                        const PcgState* state) {
     if (mode == 0)
       mf_t10_elem32<false><<<ge, kBlock, 0, s>>>(n_elem, conn, M.geo32, float(lam), float(mu),
-                                                 M.s32, din, M.rank, M.toff, M.tstart, M.fe32,
+                                                 M.s32, din, M.R.rank, M.R.toff, M.R.tstart, M.fe32,
                                                  state);
     else
       mf_t10_elem32<true><<<ge, kBlock, 0, s>>>(n_elem, conn, M.geo32, float(lam), float(mu),
-                                                M.s32, din, M.rank, M.toff, M.tstart, M.fe32,
+                                                M.s32, din, M.R.rank, M.R.toff, M.R.tstart, M.fe32,
                                                 state);
     CF_LAUNCHED();
-    mf_gather_cheb32<<<gg, kBlock, 0, s>>>(n_nodes, M.nro, M.nq, M.fe32, M.s32, mode, din, dout,
+    mf_gather_cheb32<<<gg, kBlock, 0, s>>>(n_nodes, M.R.nro, M.R.nq, M.fe32, M.s32, mode, din, dout,
                                            dout, L.rh, cdk, cwk, state);
     CF_LAUNCHED();
     return CF_OK;

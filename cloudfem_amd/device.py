@@ -159,11 +159,11 @@ class DeviceMesh:
         return self._incidence
 
     def matfree_arrays(self):
-        """(conn, geo, offs, perm, slot) in node-local element order for the
+        """(conn, geo, offs, perm) in node-local element order for the
         matrix-free operator: elements stably sorted by their first corner
-        (one radix sort), so the elements touching a run of consecutive nodes
-        are themselves near each other and the force gather hits in L2.  The
-        operator is a sum over elements, so the order changes only rounding."""
+        (one radix sort), so a tile of consecutive elements touches few nodes
+        and its per-node partial sums are long.  The operator is a sum over
+        elements, so the order changes only rounding."""
         if getattr(self, "_mf", None) is None:
             _, order = build_incidence(self.conn[:, 0].contiguous(), self.n_nodes, self.dev)
             order = order[:self.n_elem].long()
@@ -174,10 +174,7 @@ class DeviceMesh:
             conn = self.conn.index_select(0, order).contiguous()
             geo = self.geometry().index_select(0, order).contiguous()
             offs, perm = build_incidence(conn.reshape(-1), self.n_nodes, self.dev)
-            slot = torch.empty_like(perm)
-            N.check(N.load().cf_invert_permutation(int(conn.numel()), ptr(perm), ptr(slot),
-                                                   stream_handle()), "cf_invert_permutation")
-            self._mf = (conn, geo, offs, perm, slot)
+            self._mf = (conn, geo, offs, perm)
         return self._mf
 
     def matfree_plan(self):

@@ -124,14 +124,14 @@ def pcg_matfree_t10_mixed(dm, lam, mu, b, x, fixed=None, tol=None, maxit=None):
     tol = float(tol if tol is not None else o["tol"])
     maxit = int(maxit if maxit is not None else o["maxit"])
     lib = N.load()
-    conn, geo, offs, perm, slot = dm.matfree_arrays()
+    conn, geo, offs, perm = dm.matfree_arrays()
     nb = ctypes.c_size_t()
     N.check(lib.cf_mf_t10_mixed_workspace_bytes(dm.n_nodes, dm.n_elem, ctypes.byref(nb)),
             "cf_mf_t10_mixed_workspace_bytes")
     ws = workspace(nb.value, b.device)
     st = N.CfPcgStats()
     rc = lib.cf_pcg_solve_mf_t10_mixed(dm.n_nodes, dm.n_elem, ptr(conn), ptr(geo), float(lam),
-                                       float(mu), ptr(offs), ptr(perm), ptr(slot), ptr(b), ptr(x),
+                                       float(mu), ptr(offs), ptr(perm), ptr(b), ptr(x),
                                        ptr(fixed), tol, maxit, ptr(ws), nb.value,
                                        ctypes.byref(st), stream_handle())
     stats = {"iterations": st.iterations, "residual_updates": st.residual_updates,
@@ -145,10 +145,10 @@ def pcg_matfree_t10_mixed(dm, lam, mu, b, x, fixed=None, tol=None, maxit=None):
 def pcg_matfree_t10(dm, lam, mu, b, x, fixed=None, precond=None, tol=None, maxit=None,
                     chunked=False):
     """PCG with the matrix-free Tet10 elasticity operator (FP64).  Default:
-    the two-pass element-force / node-gather operator (cf_pcg_solve_mf_t10),
-    the faster one on MI355X; chunked=True: the LDS-chunked operator
-    (cf_pcg_solve_mf_t10_plan), which moves 2.3x fewer HBM bytes but runs at
-    lower occupancy (88 vs 71 us per product at 384,000 elements)."""
+    the run-sum operator (cf_pcg_solve_mf_t10: element forces summed per node
+    in LDS per tile of 256 consecutive elements, one partial per (tile, node)
+    gathered per node); chunked=True: the Morton-chunked operator
+    (cf_pcg_solve_mf_t10_plan)."""
     o = options()
     precond = precond or o["precond"]
     tol = float(tol if tol is not None else o["tol"])
@@ -175,15 +175,14 @@ def pcg_matfree_t10(dm, lam, mu, b, x, fixed=None, precond=None, tol=None, maxit
         last_stats.value = stats
         N.check(rc, "cf_pcg_solve_mf_t10_plan")
         return stats
-    conn, geo, offs, perm, slot = dm.matfree_arrays()
+    conn, geo, offs, perm = dm.matfree_arrays()
     nb = ctypes.c_size_t()
     N.check(lib.cf_mf_t10_workspace_bytes(dm.n_nodes, dm.n_elem, pc, ctypes.byref(nb)),
             "cf_mf_t10_workspace_bytes")
     ws = workspace(nb.value, b.device)
     st = N.CfPcgStats()
     rc = lib.cf_pcg_solve_mf_t10(dm.n_nodes, dm.n_elem, ptr(conn), ptr(geo),
-                                 float(lam), float(mu), ptr(offs), ptr(perm), ptr(slot),
-                                 ptr(b), ptr(x),
+                                 float(lam), float(mu), ptr(offs), ptr(perm), ptr(b), ptr(x),
                                  ptr(fixed), pc, tol, maxit, ptr(ws), nb.value,
                                  ctypes.byref(st), stream_handle())
     stats = {"iterations": st.iterations, "residual_updates": 0,
@@ -196,7 +195,7 @@ def pcg_matfree_t10(dm, lam, mu, b, x, fixed=None, precond=None, tol=None, maxit
 
 def matfree_t10_apply(dm, lam, mu, x, fixed=None, chunked=False):
     """y = K x with the matrix-free Tet10 elasticity operator (chunked: the
-    LDS-chunked plan, else the two-pass operator)."""
+    Morton-chunked plan, else the run-sum operator)."""
     lib = N.load()
     if chunked:
         plan, counts = dm.matfree_plan()
@@ -211,15 +210,14 @@ def matfree_t10_apply(dm, lam, mu, x, fixed=None, chunked=False):
                                          ptr(ws), nb.value, stream_handle()),
                 "cf_mf_t10_plan_apply")
         return y
-    conn, geo, offs, perm, slot = dm.matfree_arrays()
+    conn, geo, offs, perm = dm.matfree_arrays()
     nb = ctypes.c_size_t()
     N.check(lib.cf_mf_t10_workspace_bytes(dm.n_nodes, dm.n_elem, N.PC_JACOBI, ctypes.byref(nb)),
             "cf_mf_t10_workspace_bytes")
     ws = workspace(nb.value, x.device)
     y = torch.empty_like(x)
     N.check(lib.cf_mf_t10_apply(dm.n_nodes, dm.n_elem, ptr(conn), ptr(geo),
-                                float(lam), float(mu), ptr(offs), ptr(perm), ptr(slot),
-                                ptr(x), ptr(y),
+                                float(lam), float(mu), ptr(offs), ptr(perm), ptr(x), ptr(y),
                                 ptr(fixed), ptr(ws), nb.value, stream_handle()),
             "cf_mf_t10_apply")
     return y

@@ -284,29 +284,30 @@ int cf_pcg_solve(const cf_matrix* A, const double* b, double* x,
                  cf_pcg_stats* stats_host, void* stream);
 
 /* ---- matrix-free Tet10 elasticity operator (no assembled matrix): element
- *      forces K_e x_e from the geometry record at the 5 Keast points, summed
- *      into the nodes over the node incidence lists (offs/perm =
- *      cf_incidence_build of the Tet10 connectivity) in fixed order.  Same
- *      operator as CF_OP_T10_ELAST (isotropic lam, mu; affine elements), so
- *      it replaces the stiffness in the solve of elastic_tet10.py:214-229
- *      without the 12 B/value matrix traffic.  slot (nullable) = the inverse
- *      of perm (cf_invert_permutation): with it every element writes its
- *      nodal forces straight into node-sorted order and the node sums read
- *      contiguous memory.  Workspace: 30 doubles per element + 9 per node +
- *      the PCG vectors.                                                       */
+ *      forces K_e x_e from the geometry record (exact 4-point rule for the
+ *      degree-2 integrand of affine elements; the reference's 5-point rule
+ *      gives the same matrix up to rounding), summed into the nodes in fixed
+ *      order.  offs/perm = cf_incidence_build of the Tet10 connectivity
+ *      (conn, in the element order of the caller's choice; consecutive
+ *      elements sharing nodes make the node sums cheaper).  Each call first
+ *      builds a run plan in the workspace: per tile of 256 elements, the
+ *      forces are summed per node in LDS and one partial per (tile, node) is
+ *      written, so no per-element force goes through HBM.  Same operator as
+ *      CF_OP_T10_ELAST (isotropic lam, mu; affine elements), so it replaces
+ *      the stiffness in the solve of elastic_tet10.py:214-229 without the
+ *      12 B/value matrix traffic.  Workspace: about 52 doubles per element
+ *      + 9 per node + the PCG vectors.                                       */
 int cf_mf_t10_workspace_bytes(int64_t n_nodes, int64_t n_elem, int32_t precond,
                               size_t* bytes);
 int cf_mf_t10_apply(int64_t n_nodes, int64_t n_elem, const int32_t* conn,
                     const double* geo, double lam, double mu,
                     const int32_t* offs, const uint32_t* perm,
-                    const uint32_t* slot, const double* x, double* y,
-                    const uint8_t* fixed, void* ws, size_t ws_bytes,
-                    void* stream);
+                    const double* x, double* y, const uint8_t* fixed,
+                    void* ws, size_t ws_bytes, void* stream);
 int cf_pcg_solve_mf_t10(int64_t n_nodes, int64_t n_elem, const int32_t* conn,
                         const double* geo, double lam, double mu,
                         const int32_t* offs, const uint32_t* perm,
-                        const uint32_t* slot, const double* b, double* x,
-                        const uint8_t* fixed,
+                        const double* b, double* x, const uint8_t* fixed,
                         int32_t precond, double tol, int32_t maxit, void* ws,
                         size_t ws_bytes, cf_pcg_stats* stats_host, void* stream);
 
@@ -372,15 +373,15 @@ int cf_boundary_faces(int64_t n_elem, const int32_t* conn, int32_t k, int64_t n_
  *      the FP64 matrix-free product (as cf_pcg_solve_mf_t10), preconditioned
  *      by 23 Chebyshev steps on S K S (S = diag(K)^-1/2) whose products are
  *      matrix-free in FP32 (geometry rounded once to FP32, forces and node
- *      sums in FP32).  The outer residual, recurrence and solution are FP64,
- *      so the result meets the same residual bar.  slot is required.        */
+ *      sums in FP32, same run plan).  The outer residual, recurrence and
+ *      solution are FP64, so the result meets the same residual bar.        */
 int cf_mf_t10_mixed_workspace_bytes(int64_t n_nodes, int64_t n_elem, size_t* bytes);
 int cf_pcg_solve_mf_t10_mixed(int64_t n_nodes, int64_t n_elem, const int32_t* conn,
                               const double* geo, double lam, double mu,
                               const int32_t* offs, const uint32_t* perm,
-                              const uint32_t* slot, const double* b, double* x,
-                              const uint8_t* fixed, double tol, int32_t maxit, void* ws,
-                              size_t ws_bytes, cf_pcg_stats* stats_host, void* stream);
+                              const double* b, double* x, const uint8_t* fixed,
+                              double tol, int32_t maxit, void* ws, size_t ws_bytes,
+                              cf_pcg_stats* stats_host, void* stream);
 
 /* ---- inv[perm[j]] = j for a permutation of 0..n-1.                     */
 int cf_invert_permutation(int64_t n, const uint32_t* perm, uint32_t* inv,
