@@ -10,12 +10,14 @@ timed region and resident in HBM when it starts.
 One step = the whole drop-in path on device-resident inputs: element
 geometry, node-pair CSR pattern (radix sort), stiffness values, traction
 load, Dirichlet-masked PCG to ||b_f - A_ff x_f|| / ||b_f|| <= 1e-10 (FP64
-residual).  Four solver paths are timed on the same mesh, each after the
+residual).  Six solver paths are timed on the same mesh, each after the
 same full CSR assembly: plain FP64 PCG on the assembled NB3 matrix (`fp64`),
 FP64 PCG with the matrix-free Tet10 operator (`matfree`, Jacobi from the
-same diagonal), and mixed precision (`mixed`, `mixed16`: FP64 flexible CG
-on the assembled matrix, preconditioned by an FP32 Chebyshev polynomial of
-the Jacobi-scaled matrix whose copy is stored in FP32 / FP16).  `value` is
+same diagonal; `matfree_lds`: the Morton-chunked variant of it), and mixed
+precision (`mixed`, `mixed16`: FP64 flexible CG on the assembled matrix,
+preconditioned by an FP32 Chebyshev polynomial of the Jacobi-scaled matrix
+whose copy is stored in FP32 / FP16; `mixed_mf`: the same with FP64 and
+FP32 matrix-free products).  `value` is
 the fastest path that reaches the FP64 residual bar; `config.solver` names
 it.  DOF counts all nodal DOF (3N).
 
@@ -267,6 +269,11 @@ def main():
         return 1e3 * float(np.mean(times)), last, (a64.value, c64.value, a32.value, c32.value)
 
     n_elem = mesh.n_elements
+    # runs of the matrix-free operators: distinct (tile of 256 elements, node)
+    # pairs with the elements in first-corner order (DeviceMesh.matfree_arrays)
+    els = mesh.elements[np.argsort(mesh.elements[:, 0], kind="stable")].astype(np.int64)
+    n_runs = int(np.unique((np.arange(n_elem, dtype=np.int64)[:, None] // 256) * n_nodes
+                           + els).size)
     results = {}
     ms, (x_64, st, nnz_b), prof = timed("fp64", args.precond64)
     bytes_spmv64 = 76 * nnz_b + 4 * (n_nodes + 1) + 16 * n_dof
@@ -277,13 +284,16 @@ def main():
     maxit["mixed"] = maxit["mixed16"] = 4 * st["iterations"]   # bound the mixed attempts
     if "matfree" in paths:
         ms, (x_mf, st, _), prof = timed("matfree", args.precond64)
-        # two-pass operator: conn 40 + geometry 80 + slot 40 + force write and
-        # read 2 x 240 B per element, offsets, x and y
+        # run-sum operator: per element conn 40 + geometry 80 + ranks 20 B; per
+        # run its start 2 B, an FP64 partial written and read 2 x 24 B, its
+        # node-list entry 4 B; node offsets; per DOF x gathered 8, x read for
+        # p.Ap 8, y written 8, fixed flag 1 B
         results["matfree"] = {"ms_per_step": ms, "iterations": st["iterations"],
                               "rel_residual": st["rel_residual"], "precond": st["precond"],
-                              "op": "mf_t10_elem + mf_gather_sorted", "op_avg_ms": prof[0],
-                              "op_launches_sampled": prof[1],
-                              "op_bytes": 640 * n_elem + 4 * (n_nodes + 1) + 16 * n_dof,
+                              "op": "mf_t10_elem + mf_gather_runs", "op_avg_ms": prof[0],
+                              "op_launches_sampled": prof[1], "runs": n_runs,
+                              "op_bytes": 140 * n_elem + 54 * n_runs + 4 * (n_nodes + 1)
+                              + 25 * n_dof,
                               "rel_diff_vs_fp64": float(torch.linalg.vector_norm(x_mf - x_64)
                                                         / torch.linalg.vector_norm(x_64))}
     if "matfree_lds" in paths:
@@ -305,16 +315,17 @@ def main():
     if "mixed_mf" in paths:
         try:
             ms, (x_mm, st, _), prof = timed("mixed_mf", "jacobi")
-            # FP32 matrix-free Chebyshev step: conn 40 + geometry 40 + slot 40 + FP32
-            # force write and read 2 x 120 B per element, offsets; S, d gathered
-            # once, w and z read+written, d' written (28 B per DOF)
+            # FP32 matrix-free Chebyshev step: per element conn 40 + FP32 geometry
+            # 40 + ranks 20 B; per run its start 2 B, an FP32 partial written
+            # and read 2 x 12 B, its node-list entry 4 B; node offsets; per DOF
+            # z_k gathered 4, S, w, z_k, z_k-1 read 16, z_k+1 written 4 B
             results["mixed_mf"] = {
                 "ms_per_step": ms, "iterations": st["iterations"],
                 "low_precision_products": st["residual_updates"],
                 "rel_residual": st["rel_residual"],
                 "op": "mf_t10_elem32 + mf_gather_cheb32", "op_avg_ms": prof[2],
-                "op_launches_sampled": prof[3],
-                "op_bytes": 360 * n_elem + 4 * (n_nodes + 1) + 28 * n_dof,
+                "op_launches_sampled": prof[3], "runs": n_runs,
+                "op_bytes": 100 * n_elem + 30 * n_runs + 4 * (n_nodes + 1) + 24 * n_dof,
                 "outer_op_avg_ms": prof[0],
                 "rel_diff_vs_fp64": float(torch.linalg.vector_norm(x_mm - x_64)
                                           / torch.linalg.vector_norm(x_64))}
