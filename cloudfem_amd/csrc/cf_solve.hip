@@ -2322,6 +2322,25 @@ namespace {
 // launches y = A xin with fixed rows zeroed; when `state` is set it also
 // writes `apply_parts` block partials of p.Ap (the CG product).  `dinv` holds the
 // preconditioner (Jacobi 1/A_ii, or inverse 3x3 node blocks).
+// Iterations to enqueue before the next host check.  The remaining count is
+// predicted from the faster of two convergence rates (since the start, and
+// over the last batch when the residual fell there), and 80 % of it is
+// enqueued: iterations launched after convergence exit at once but still
+// cost a launch each, and CG residuals are not monotone, so the prediction
+// must not overshoot.  Without a usable rate the batch doubles.
+int next_batch(const PcgState& h, double rr_start, double prev_rr, int prev_it, int batch,
+               int lo, int hi) {
+  if (h.halt != kRunning) return batch;
+  double rate = 0.0;  // log residual decrease per iteration (< 0 when converging)
+  if (h.iter > 0 && h.rr < rr_start) rate = std::log(h.rr / rr_start) / h.iter;
+  const int done_it = h.iter - prev_it;
+  if (done_it > 0 && h.rr < prev_rr) rate = std::min(rate, std::log(h.rr / prev_rr) / done_it);
+  if (!(rate < 0.0)) return batch * 2 > hi ? hi : batch * 2;
+  const double need = std::log(h.stop2 / h.rr) / rate;
+  const int nb = int(0.8 * need) + 2;
+  return nb < lo ? lo : (nb > hi ? hi : nb);
+}
+
 template <int PC, class Apply>
 int pcg_fp64_core(Apply&& apply, int apply_parts, int64_t n, int64_t units, const double* b,
                   double* x, const uint8_t* fixed, double tol, int maxit, const Layout& L,
@@ -2369,6 +2388,7 @@ int pcg_fp64_core(Apply&& apply, int apply_parts, int64_t n, int64_t units, cons
   CF_HIP(hipMemcpyAsync(L.state, &h, sizeof(PcgState), hipMemcpyHostToDevice, s));
   int batch = 8;
   double prev_rr = h.rr;
+  const double rr_start = h.rr;
   int prev_it = 0;
   const int GV = pcg_vec_grid();
   int64_t enq = 0;
@@ -2391,16 +2411,7 @@ int pcg_fp64_core(Apply&& apply, int apply_parts, int64_t n, int64_t units, cons
     if ((rc = read_state(L.state, &h, s))) return rc;
     prof_collect();
     // size the next batch from the observed convergence rate
-    const int done_it = h.iter - prev_it;
-    if (h.halt == kRunning && done_it > 0 && h.rr < prev_rr) {
-      const double rate = std::log(h.rr / prev_rr) / done_it;  // < 0
-      const double need = std::log(h.stop2 / h.rr) / rate;
-      int nb = int(need * 0.9) + 4;
-      nb = nb < 8 ? 8 : (nb > 512 ? 512 : nb);
-      batch = nb;
-    } else {
-      batch = batch * 2 > 512 ? 512 : batch * 2;
-    }
+    batch = next_batch(h, rr_start, prev_rr, prev_it, batch, 8, 512);
     prev_rr = h.rr;
     prev_it = h.iter;
   }
@@ -2596,6 +2607,7 @@ int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, do
   int64_t enq = 0;
   int batch = 8;
   double prev_rr = h.rr;
+  const double rr_start = h.rr;
   int prev_it = 0;
   while (h.halt == kRunning) {
     for (int t = 0; t < batch; ++t, ++enq) {
@@ -2607,15 +2619,7 @@ int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, do
     }
     if ((rc = read_state(L.state, &h, s))) return rc;
     prof_collect();
-    const int done_it = h.iter - prev_it;
-    if (h.halt == kRunning && done_it > 0 && h.rr < prev_rr) {
-      const double rate = std::log(h.rr / prev_rr) / done_it;
-      const double need = std::log(h.stop2 / h.rr) / rate;
-      int nb = int(need * 0.9) + 2;
-      batch = nb < 4 ? 4 : (nb > 128 ? 128 : nb);
-    } else {
-      batch = batch * 2 > 128 ? 128 : batch * 2;
-    }
+    batch = next_batch(h, rr_start, prev_rr, prev_it, batch, 4, 128);
     prev_rr = h.rr;
     prev_it = h.iter;
   }
