@@ -59,7 +59,7 @@ def parse():
     ap.add_argument("--workload", default="tet10-160x20x20", choices=sorted(WORKLOADS))
     ap.add_argument("--precond64", default="jacobi", choices=["jacobi", "block3"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--paths", default="fp64,matfree,matfree_lds,mixed,mixed16,mixed_mf",
+    ap.add_argument("--paths", default="fp64,matfree,matfree_lds,mixed,mixed16,mixed_mf,mixed_mf2l",
                     help="comma-separated solver paths to time (fp64 always runs first)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="before the workload, copy a 256 MiB device buffer 3 times "
@@ -189,7 +189,8 @@ def main():
     from cloudfem_amd.mesh import box_mesh_tet10
     from cloudfem_amd.mesh.faces import boundary_faces10
     from cloudfem_amd.physics.elastic3d import lame
-    from cloudfem_amd.solver import DeviceMatrix, pcg, pcg_matfree_t10, pcg_matfree_t10_mixed
+    from cloudfem_amd.solver import (DeviceMatrix, pcg, pcg_matfree_t10, pcg_matfree_t10_mixed,
+                                     pcg_matfree_t10_twolevel)
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda", 0)
@@ -238,6 +239,9 @@ def main():
         elif path == "mixed_mf":
             stats = pcg_matfree_t10_mixed(dm, lam, mu, rhs, x, fixed, tol=TOL,
                                           maxit=maxit["mixed"])
+        elif path == "mixed_mf2l":
+            stats = pcg_matfree_t10_twolevel(dm, lam, mu, rhs, x, fixed, tol=TOL,
+                                             maxit=maxit["mixed"])
         elif path == "matfree_lds":
             stats = pcg_matfree_t10(dm, lam, mu, rhs, x, fixed, precond=precond, tol=TOL,
                                     maxit=maxit["fp64"], chunked=True)
@@ -331,6 +335,22 @@ def main():
                                           / torch.linalg.vector_norm(x_64))}
         except (N.SolverNotConverged, N.SolverBreakdown) as exc:
             results["mixed_mf"] = {"converged": False, "error": str(exc)[-300:]}
+    if "mixed_mf2l" in paths:
+        try:
+            ms, (x_m2, st, _), prof = timed("mixed_mf2l", "jacobi")
+            # same FP32 Chebyshev step as mixed_mf (bytes as there)
+            results["mixed_mf2l"] = {
+                "ms_per_step": ms, "iterations": st["iterations"],
+                "low_precision_products": st["residual_updates"],
+                "rel_residual": st["rel_residual"],
+                "op": "mf_t10_elem32 + mf_gather_cheb32", "op_avg_ms": prof[2],
+                "op_launches_sampled": prof[3], "runs": n_runs,
+                "op_bytes": 100 * n_elem + 30 * n_runs + 4 * (n_nodes + 1) + 24 * n_dof,
+                "outer_op_avg_ms": prof[0],
+                "rel_diff_vs_fp64": float(torch.linalg.vector_norm(x_m2 - x_64)
+                                          / torch.linalg.vector_norm(x_64))}
+        except (N.SolverNotConverged, N.SolverBreakdown, N.CFError) as exc:
+            results["mixed_mf2l"] = {"converged": False, "error": str(exc)[-300:]}
     for path, vbytes in (("mixed", 36), ("mixed16", 18)):
         if path not in paths:
             continue
@@ -385,7 +405,8 @@ def main():
         "paths": results,
         "speedup_mixed_vs_fp64": {
             k: results["fp64"]["ms_per_step"] / results[k]["ms_per_step"]
-            for k in ("mixed", "mixed16", "mixed_mf") if "ms_per_step" in results.get(k, {})},
+            for k in ("mixed", "mixed16", "mixed_mf", "mixed_mf2l")
+            if "ms_per_step" in results.get(k, {})},
         "speedup_mixed_mf_vs_matfree_fp64": (
             results["matfree"]["ms_per_step"] / results["mixed_mf"]["ms_per_step"]
             if "ms_per_step" in results.get("mixed_mf", {}) and "matfree" in results else None),

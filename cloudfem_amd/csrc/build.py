@@ -20,6 +20,7 @@ SOURCES = ["cf_common.cpp", "cf_pattern.hip", "cf_assemble.hip", "cf_solve.hip",
            "cf_dirichlet.hip", "cf_batch.hip", "cf_mesh.hip"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+ROCM_LIB = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
 ARCH = os.environ.get("CF_OFFLOAD_ARCH", "gfx950")
 
 COMMON = [
@@ -46,7 +47,8 @@ def build(verbose=False):
     with cf.ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs]
+    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs,
+           "-L", ROCM_LIB, f"-Wl,-rpath,{ROCM_LIB}", "-lrocsolver", "-lrocblas"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stderr}")

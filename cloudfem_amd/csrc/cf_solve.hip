@@ -27,8 +27,11 @@
 #include <cstring>
 #include <vector>
 
+#include <rocblas/rocblas.h>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
+#include <rocsolver/rocsolver.h>
+#include <type_traits>
 
 #include "cf_common.hpp"
 #include "cf_tables.hpp"
@@ -783,6 +786,56 @@ fcg_direction_fused(int64_t n, const double* __restrict__ z, double* __restrict_
     p[i] = z[i] + beta * p[i];
 }
 
+// Two-level form of the outer update: alpha from the product partials (when
+// APPLY), x += alpha p, r -= alpha Ap, r.r partials into pb, and an FP32 copy
+// of r for the preconditioner.
+template <bool APPLY>
+__global__ void __launch_bounds__(kBlock)
+fcg_update2l(int64_t n, double* __restrict__ x, double* __restrict__ r,
+             const double* __restrict__ p, const double* __restrict__ Ap,
+             float* __restrict__ r32, PcgState* state, const double* __restrict__ pa, int n_pa,
+             double* __restrict__ pb, int slot) {
+  if (state->halt) return;
+  double alpha = 0.0;
+  if (APPLY) {
+    double f[1];
+    fold_partials<1>(pa, n_pa, f);
+    const double pAp = f[0];
+    const bool ok = pAp > 0.0 && isfinite(pAp);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      state->pAp = pAp;
+      if (ok)
+        state->alpha = state->rzs[slot] / pAp;
+      else
+        state->halt = kBreakdown;
+    }
+    if (!ok) return;
+    alpha = state->rzs[slot] / pAp;
+  }
+  double rr = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    double rn = r[i];
+    if (APPLY) {
+      x[i] += alpha * p[i];
+      rn -= alpha * Ap[i];
+      r[i] = rn;
+    }
+    rr += rn * rn;
+    r32[i] = float(rn);
+  }
+  double v1[1] = {rr};
+  block_partials<1>(v1, pb);
+}
+
+__global__ void vec_add32(int64_t n, float* __restrict__ u, const float* __restrict__ v,
+                          const PcgState* state) {
+  if (state && state->halt) return;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    u[i] += v[i];
+}
+
 // Power iteration on S A S for the top of the spectrum: v0 = S 1 spread by a
 // fixed hash, then v = y / ||y|| with y = S A S v.
 __global__ void power_start(int64_t n, const double* __restrict__ sc, float* __restrict__ v) {
@@ -1508,7 +1561,10 @@ mf_t10_elem32(int64_t m, const int32_t* __restrict__ conn, const float* __restri
 }
 
 // node sums of the FP32 forces (the node's run partials in tile order).
-// mode 0 (power iteration): zn = S sum.
+// mode 0 (power iteration): zn = S sum.  Modes 3 and 4 serve the two-level
+// preconditioner (product K zc of its coarse iterate): mode 3 starts the
+// Chebyshev iteration on the residual rin - K zc, w = S^2 (rin - sum),
+// zn = cw w (cw = 1 / theta); mode 4 returns the residual zn = rin - sum.
 // Modes 1 and 2: one Chebyshev step in three-term form on S-scaled vectors
 // (z = S zhat, w = S^2 r fixed over the steps; the element kernel gathered
 // zc, so sum = K zc and S^2 sum = S (S K S) zhat):
@@ -1521,8 +1577,8 @@ __global__ void __launch_bounds__(kBlock)
 mf_gather_cheb32(int64_t n_nodes, const int32_t* __restrict__ nro,
                  const uint32_t* __restrict__ nq, const float* __restrict__ part,
                  const float* __restrict__ sc, int mode, const float* __restrict__ zc,
-                 const float* zp, float* zn, const float* __restrict__ w, float cd, float cw,
-                 const PcgState* state) {
+                 const float* zp, float* zn, float* __restrict__ w, float cd, float cw,
+                 const float* __restrict__ rin, const PcgState* state) {
   if (state && state->halt) return;
   const int64_t i = tile_index(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= n_nodes) return;
@@ -1540,6 +1596,12 @@ mf_gather_cheb32(int64_t n_nodes, const int32_t* __restrict__ nro,
     const float s = sc[k];
     if (mode == 0) {
       zn[k] = s * o[c];
+    } else if (mode == 3) {
+      const float wk = s * s * (rin[k] - o[c]);
+      w[k] = wk;
+      zn[k] = cw * wk;
+    } else if (mode == 4) {
+      zn[k] = rin[k] - o[c];
     } else {
       const float zk = zc[k];
       const float dz = mode == 1 ? zk : zk - zp[k];
@@ -2492,10 +2554,12 @@ constexpr int kPowerSteps = 40;
 // and interval ratio of the polynomial.  SQ: cheb works on S-scaled vectors
 // in three-term form (d = z_k in, dn = z_{k-1} in and z_{k+1} out; see
 // fcg_update and mf_gather_cheb32).
-template <bool SQ, class Outer, class Power, class Cheb>
+template <bool SQ, class Outer, class Power, class Cheb, class Pre2L = std::nullptr_t>
 int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, double gersh,
                   ChebShape shape, int64_t n, const double* b, double* x, const uint8_t* fixed, double tol,
-                  int maxit, const Layout& L, cf_pcg_stats* st, hipStream_t s) {
+                  int maxit, const Layout& L, cf_pcg_stats* st, hipStream_t s,
+                  Pre2L&& pre2l = nullptr, const float* zh2l = nullptr, int extra_products = 0) {
+  constexpr bool kTwoLevel = !std::is_same<std::decay_t<Pre2L>, std::nullptr_t>::value;
   const int G = pcg_grid();
   int rc;
   PcgState h{};
@@ -2541,6 +2605,17 @@ int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, do
   // (SQ: zh = the last three-term iterate).
   const int GV = pcg_vec_grid();
   auto precondition = [&](int apply_step, double* zc, const double* zo, int sample, int slot) {
+    if constexpr (kTwoLevel) {
+      // two-level: the hook updates x, r and forms the preconditioned residual in zh2l
+      int r2 = pre2l(apply_step, slot, sample, cd, cw, inv_theta, shape.steps);
+      if (r2) return r2;
+      fcg_zdots_fused<SQ><<<GV, kBlock, 0, s>>>(n, L.r, L.s, zh2l, zc, zo, L.state, L.partials_b,
+                                                GV, apply_step, L.partials);
+      CF_LAUNCHED();
+      fcg_direction_fused<<<GV, kBlock, 0, s>>>(n, zc, L.p, L.state, L.partials, GV, slot);
+      CF_LAUNCHED();
+      return CF_OK;
+    }
     if (apply_step) {
       fcg_update_fused<SQ><<<GV, kBlock, 0, s>>>(n, x, L.r, L.p, L.Ap, L.s, L.rh, D[0], L.yh,
                                              inv_theta, L.state, L.partials, outer_parts,
@@ -2624,7 +2699,7 @@ int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, do
     prev_it = h.iter;
   }
   st->iterations = h.iter;
-  st->residual_updates = (shape.steps - 1) * (h.iter + 1);  // low-precision products
+  st->residual_updates = (shape.steps - 1 + extra_products) * (h.iter + 1);  // low-precision products
   if ((rc = outer(x, L.Ap, nullptr))) return rc;
   residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->pad);
   CF_LAUNCHED();
@@ -3428,8 +3503,7 @@ struct MfMixed {
   MfRuns R;          // run plan shared by the FP64 and FP32 products
 };
 
-size_t mf_mixed_layout(int64_t n_nodes, int64_t n_elem, void* base, Layout* L, MfMixed* M) {
-  cf::Carve c(base);
+void mf_mixed_layout_base(int64_t n_nodes, int64_t n_elem, cf::Carve& c, Layout* L, MfMixed* M) {
   const int64_t n = 3 * n_nodes;
   L->r = c.take<double>(n);
   L->p = c.take<double>(n);
@@ -3452,37 +3526,613 @@ size_t mf_mixed_layout(int64_t n_nodes, int64_t n_elem, void* base, Layout* L, M
   M->fe32 = c.take<float>(30 * n_elem);
   M->s32 = c.take<float>(n);
   mf_runs_layout(n_nodes, n_elem, c, &M->R);
-  L->total = c.used;
-  return c.used;
 }
 
 }  // namespace
 
-extern "C" int cf_mf_t10_mixed_workspace_bytes(int64_t n_nodes, int64_t n_elem, size_t* bytes) {
-  CF_REQUIRE(n_nodes > 0 && n_elem > 0 && bytes, "cf_mf_t10_mixed_workspace_bytes: bad arguments");
-  Layout L{};
-  MfMixed M{};
-  *bytes = mf_mixed_layout(n_nodes, n_elem, nullptr, &L, &M);
+namespace {
+
+// ------------------------------------------------------------- two-level preconditioner
+// Coarse space of the mixed matrix-free solver: the six rigid-body modes of
+// node aggregates.  Aggregate of a node = the group of kAggTiles element
+// tiles (kAggTiles * kBlock elements in the caller's element order) that
+// holds the first element touching it.  Basis rows of node i in aggregate g:
+// translations e_k and rotations (e_k x (x_i - c_g)) / L_g (c_g centroid, L_g
+// RMS node distance from it), zero on fixed DOFs.  Coarse matrix
+// A_c = P^T K_ff P by probing: aggregates get a distance-2 colouring, the
+// modes of all aggregates of one colour are applied together with the FP64
+// matrix-free operator, and because no two aggregates of a colour share a
+// neighbour each coarse block is read off exactly (no atomics).  A_c is
+// factored and inverted with rocSOLVER; the preconditioner is
+//   z1 = C r,  z2 = z1 + Cheb(r - K z1),  z = z2 + C (r - K z2),
+// C = P A_c^-1 P^T, Cheb the FP32 Chebyshev polynomial of S K S.
+constexpr int kAggTiles = 2;
+constexpr int kMaxColors = 64;
+constexpr int kPairsPerElem = 45;  // distinct aggregate pairs of 10 nodes
+
+__device__ __forceinline__ void rigid_row(int m, double dx, double dy, double dz, double invl,
+                                          double (&o)[3]) {
+  o[0] = o[1] = o[2] = 0.0;
+  if (m < 3) {
+    o[m] = 1.0;
+  } else if (m == 3) {  // e_x x d
+    o[1] = -dz * invl;
+    o[2] = dy * invl;
+  } else if (m == 4) {  // e_y x d
+    o[0] = dz * invl;
+    o[2] = -dx * invl;
+  } else {              // e_z x d
+    o[0] = -dy * invl;
+    o[1] = dx * invl;
+  }
+}
+
+// group of the first element touching node i (-1: none); used[g] = 1
+__global__ void agg_first(int64_t n_nodes, const int32_t* __restrict__ offs,
+                          const uint32_t* __restrict__ perm, int32_t* __restrict__ graw,
+                          uint32_t* __restrict__ used) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n_nodes;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    int32_t g = -1;
+    if (offs[i] < offs[i + 1]) {
+      g = int32_t(perm[offs[i]] / uint32_t(kTileItems * kAggTiles));
+      used[g] = 1u;
+    }
+    graw[i] = g;
+  }
+}
+
+// compact aggregate ids; sort keys (aggregate, or n_agg for untouched nodes)
+__global__ void agg_compact(int64_t n_nodes, const int32_t* __restrict__ graw,
+                            const uint32_t* __restrict__ gid, int64_t n_groups,
+                            int32_t* __restrict__ agg, uint32_t* __restrict__ keys,
+                            uint32_t* __restrict__ ids) {
+  const uint32_t n_agg = gid[n_groups];
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n_nodes;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int32_t g = graw[i] >= 0 ? int32_t(gid[graw[i]]) : -1;
+    agg[i] = g;
+    keys[i] = g >= 0 ? uint32_t(g) : n_agg;
+    ids[i] = uint32_t(i);
+  }
+}
+
+// agg_off[g] = first position of aggregate g in the sorted keys (g = 0..n_agg)
+__global__ void agg_offsets(int64_t n_nodes, int64_t n_agg, const uint32_t* __restrict__ keys,
+                            int32_t* __restrict__ agg_off) {
+  for (int64_t g = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; g <= n_agg;
+       g += int64_t(gridDim.x) * blockDim.x) {
+    int64_t lo = 0, hi = n_nodes;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) / 2;
+      if (int64_t(keys[mid]) >= g) hi = mid; else lo = mid + 1;
+    }
+    agg_off[g] = int32_t(lo);
+  }
+}
+
+// centroid and 1 / RMS radius of each aggregate, over its nodes in order
+__global__ void agg_frame(int64_t n_agg, const int32_t* __restrict__ agg_off,
+                          const uint32_t* __restrict__ agg_nodes, const double* __restrict__ x,
+                          const double* __restrict__ y, const double* __restrict__ z,
+                          double* __restrict__ frame) {
+  for (int64_t g = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; g < n_agg;
+       g += int64_t(gridDim.x) * blockDim.x) {
+    const int32_t b = agg_off[g], e = agg_off[g + 1];
+    double c[3] = {0.0, 0.0, 0.0};
+    for (int32_t k = b; k < e; ++k) {
+      const uint32_t i = agg_nodes[k];
+      c[0] += x[i];
+      c[1] += y[i];
+      c[2] += z[i];
+    }
+    const double inv_n = e > b ? 1.0 / double(e - b) : 0.0;
+    c[0] *= inv_n;
+    c[1] *= inv_n;
+    c[2] *= inv_n;
+    double r2 = 0.0;
+    for (int32_t k = b; k < e; ++k) {
+      const uint32_t i = agg_nodes[k];
+      const double dx = x[i] - c[0], dy = y[i] - c[1], dz = z[i] - c[2];
+      r2 += dx * dx + dy * dy + dz * dz;
+    }
+    r2 *= inv_n;
+    frame[4 * g] = c[0];
+    frame[4 * g + 1] = c[1];
+    frame[4 * g + 2] = c[2];
+    frame[4 * g + 3] = r2 > 0.0 ? 1.0 / sqrt(r2) : 1.0;
+  }
+}
+
+// distinct aggregates of element e's nodes, ascending; returns their count
+__device__ __forceinline__ int elem_aggs(const int32_t* __restrict__ conn, int64_t e,
+                                         const int32_t* __restrict__ agg, int32_t (&g)[10]) {
+  int n = 0;
+  for (int a = 0; a < 10; ++a) {
+    const int32_t v = agg[conn[e * 10 + a]];
+    if (v < 0) continue;
+    int p = n;
+    bool dup = false;
+    for (int q = 0; q < n; ++q) dup |= g[q] == v;
+    if (dup) continue;
+    while (p > 0 && g[p - 1] > v) {
+      g[p] = g[p - 1];
+      --p;
+    }
+    g[p] = v;
+    ++n;
+  }
+  return n;
+}
+
+// number of aggregate pairs (g < h) of each element
+__global__ void agg_pair_count(int64_t m, const int32_t* __restrict__ conn,
+                               const int32_t* __restrict__ agg, uint32_t* __restrict__ cnt) {
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e <= m;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    if (e == m) {
+      cnt[e] = 0;
+      continue;
+    }
+    int32_t g[10];
+    const int n = elem_aggs(conn, e, agg, g);
+    cnt[e] = uint32_t(n * (n - 1) / 2);
+  }
+}
+
+__global__ void agg_pair_fill(int64_t m, const int32_t* __restrict__ conn,
+                              const int32_t* __restrict__ agg, const uint32_t* __restrict__ off,
+                              int64_t n_agg, uint64_t* __restrict__ keys) {
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < m;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    int32_t g[10];
+    const int n = elem_aggs(conn, e, agg, g);
+    uint32_t o = off[e];
+    for (int a = 0; a < n; ++a)
+      for (int b = a + 1; b < n; ++b) keys[o++] = uint64_t(g[a]) * uint64_t(n_agg) + uint64_t(g[b]);
+  }
+}
+
+// probe vectors of colour c: v_m = mode m of every aggregate of colour c
+__global__ void probe_build(int64_t n_nodes, const int32_t* __restrict__ agg,
+                            const int32_t* __restrict__ color, int c,
+                            const double* __restrict__ frame, const double* __restrict__ x,
+                            const double* __restrict__ y, const double* __restrict__ z,
+                            const uint8_t* __restrict__ fixed, double* __restrict__ v) {
+  const int64_t n = 3 * n_nodes;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n_nodes;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int32_t g = agg[i];
+    const bool on = g >= 0 && color[g] == c;
+    const double* f = frame + 4 * (g >= 0 ? g : 0);
+    for (int mm = 0; mm < 6; ++mm) {
+      double o[3] = {0.0, 0.0, 0.0};
+      if (on) rigid_row(mm, x[i] - f[0], y[i] - f[1], z[i] - f[2], f[3], o);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int64_t d = 3 * i + k;
+        v[mm * n + d] = (fixed && fixed[d]) ? 0.0 : o[k];
+      }
+    }
+  }
+}
+
+// A_c[(h, mp), (g, mm)] = sum over the nodes i of h of mode mp(i) . y_mm(i), g =
+// the colour-c neighbour of h (if any); one workgroup per aggregate h, fixed
+// node split and reduction tree.  Column-major, leading dimension 6 n_agg.
+__global__ void __launch_bounds__(kBlock)
+probe_extract(int64_t n_agg, int n_col, int c, const int32_t* __restrict__ nbc,
+              const int32_t* __restrict__ agg_off, const uint32_t* __restrict__ agg_nodes,
+              const double* __restrict__ frame, const double* __restrict__ x,
+              const double* __restrict__ y, const double* __restrict__ z, int64_t n,
+              const double* __restrict__ yv, double* __restrict__ A) {
+  const int64_t nc = 6 * n_agg;
+  for (int64_t h = blockIdx.x; h < n_agg; h += gridDim.x) {
+    const int32_t g = nbc[h * n_col + c];
+    if (g < 0) continue;  // uniform over the block
+    const double* f = frame + 4 * h;
+    double acc[36];
+#pragma unroll
+    for (int q = 0; q < 36; ++q) acc[q] = 0.0;
+    for (int32_t k = agg_off[h] + threadIdx.x; k < agg_off[h + 1]; k += blockDim.x) {
+      const uint32_t i = agg_nodes[k];
+      const double dx = x[i] - f[0], dy = y[i] - f[1], dz = z[i] - f[2];
+      double yy[6][3];
+#pragma unroll
+      for (int mm = 0; mm < 6; ++mm)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) yy[mm][q] = yv[mm * n + 3 * int64_t(i) + q];
+#pragma unroll
+      for (int mp = 0; mp < 6; ++mp) {
+        double o[3];
+        rigid_row(mp, dx, dy, dz, f[3], o);
+#pragma unroll
+        for (int mm = 0; mm < 6; ++mm)
+          acc[6 * mp + mm] += o[0] * yy[mm][0] + o[1] * yy[mm][1] + o[2] * yy[mm][2];
+      }
+    }
+    block_sum<36>(acc);
+    if (threadIdx.x == 0)
+#pragma unroll
+      for (int q = 0; q < 36; ++q) A[(6 * h + q / 6) + (6 * int64_t(g) + q % 6) * nc] = acc[q];
+    __syncthreads();
+  }
+}
+
+// A = (A + A^T) / 2; a vanishing diagonal (aggregate without free DOFs, or a
+// degenerate mode) becomes an identity row and column.  One thread per column
+// j, rows i <= j; maxd = largest diagonal entry.
+__global__ void coarse_symmetrize(int64_t nc, double* __restrict__ A) {
+  for (int64_t j = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; j < nc;
+       j += int64_t(gridDim.x) * blockDim.x) {
+    for (int64_t i = 0; i < j; ++i) {
+      const double v = 0.5 * (A[i + j * nc] + A[j + i * nc]);
+      A[i + j * nc] = v;
+      A[j + i * nc] = v;
+    }
+  }
+}
+
+__global__ void coarse_regularize(int64_t nc, double* __restrict__ A, double tiny) {
+  for (int64_t j = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; j < nc;
+       j += int64_t(gridDim.x) * blockDim.x) {
+    if (A[j + j * nc] > tiny) continue;
+    for (int64_t i = 0; i < nc; ++i) {
+      A[i + j * nc] = 0.0;
+      A[j + i * nc] = 0.0;
+    }
+    A[j + j * nc] = 1.0;
+  }
+}
+
+// lower triangle of the inverse (rocSOLVER potri) copied to the upper one
+__global__ void coarse_fill_upper(int64_t nc, double* __restrict__ A) {
+  for (int64_t j = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; j < nc;
+       j += int64_t(gridDim.x) * blockDim.x)
+    for (int64_t i = 0; i < j; ++i) A[i + j * nc] = A[j + i * nc];
+}
+
+// rc[(g, m)] = sum over the nodes of g of mode m . r (r FP64 or FP32); one
+// workgroup per aggregate, fixed node split and reduction tree.
+template <class RT>
+__global__ void __launch_bounds__(kBlock)
+coarse_restrict(int64_t n_agg, const int32_t* __restrict__ agg_off,
+                const uint32_t* __restrict__ agg_nodes, const double* __restrict__ frame,
+                const double* __restrict__ x, const double* __restrict__ y,
+                const double* __restrict__ z, const RT* __restrict__ r, double* __restrict__ rc,
+                const PcgState* state) {
+  if (state && state->halt) return;
+  for (int64_t g = blockIdx.x; g < n_agg; g += gridDim.x) {
+    const double* f = frame + 4 * g;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int32_t k = agg_off[g] + threadIdx.x; k < agg_off[g + 1]; k += blockDim.x) {
+      const uint32_t i = agg_nodes[k];
+      const double dx = x[i] - f[0], dy = y[i] - f[1], dz = z[i] - f[2];
+      const double r0 = double(r[3 * int64_t(i)]), r1 = double(r[3 * int64_t(i) + 1]),
+                   r2 = double(r[3 * int64_t(i) + 2]);
+#pragma unroll
+      for (int mm = 0; mm < 6; ++mm) {
+        double o[3];
+        rigid_row(mm, dx, dy, dz, f[3], o);
+        acc[mm] += o[0] * r0 + o[1] * r1 + o[2] * r2;
+      }
+    }
+    block_sum<6>(acc);
+    if (threadIdx.x == 0)
+#pragma unroll
+      for (int q = 0; q < 6; ++q) rc[6 * g + q] = acc[q];
+    __syncthreads();
+  }
+}
+
+// zc = Ainv rc, Ainv symmetric (column i read as row i): one wave per entry
+__global__ void __launch_bounds__(kBlock)
+coarse_gemv(int64_t nc, const double* __restrict__ Ainv, const double* __restrict__ rc,
+            double* __restrict__ zc, const PcgState* state) {
+  if (state && state->halt) return;
+  const int lane = threadIdx.x & 63;
+  for (int64_t i = (blockIdx.x * int64_t(blockDim.x) + threadIdx.x) / 64; i < nc;
+       i += int64_t(gridDim.x) * blockDim.x / 64) {
+    const double* col = Ainv + i * nc;
+    double acc = 0.0;
+    for (int64_t j = lane; j < nc; j += 64) acc += col[j] * rc[j];
+    acc = wave_sum(acc);
+    if (lane == 0) zc[i] = acc;
+  }
+}
+
+// u (FP32) = [ADD ? u : 0] + P zc (zero on fixed DOFs)
+template <bool ADD>
+__global__ void coarse_prolong(int64_t n_nodes, const int32_t* __restrict__ agg,
+                               const double* __restrict__ frame, const double* __restrict__ x,
+                               const double* __restrict__ y, const double* __restrict__ z,
+                               const uint8_t* __restrict__ fixed, const double* __restrict__ zc,
+                               float* __restrict__ u, const PcgState* state) {
+  if (state && state->halt) return;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n_nodes;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int32_t g = agg[i];
+    double s[3] = {0.0, 0.0, 0.0};
+    if (g >= 0) {
+      const double* f = frame + 4 * g;
+      const double dx = x[i] - f[0], dy = y[i] - f[1], dz = z[i] - f[2];
+      for (int mm = 0; mm < 6; ++mm) {
+        double o[3];
+        rigid_row(mm, dx, dy, dz, f[3], o);
+        const double c = zc[6 * int64_t(g) + mm];
+        s[0] += o[0] * c;
+        s[1] += o[1] * c;
+        s[2] += o[2] * c;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int64_t d = 3 * i + k;
+      const float v = (fixed && fixed[d]) ? 0.0f : float(s[k]);
+      u[d] = ADD ? u[d] + v : v;
+    }
+  }
+}
+
+}  // namespace
+
+namespace {
+
+struct Coarse {
+  int64_t n_groups = 0, n_agg = 0, nc = 0;  // host
+  int n_col = 0;
+  int32_t* graw;
+  uint32_t* used;
+  uint32_t* gid;
+  int32_t* agg;        // node -> aggregate (-1: no element)
+  int32_t* agg_off;    // aggregate -> positions in agg_nodes
+  uint32_t* keys_in;
+  uint32_t* keys_out;
+  uint32_t* ids_in;
+  uint32_t* agg_nodes; // nodes sorted by aggregate
+  double* frame;       // centroid, 1 / RMS radius
+  uint32_t* pcnt;
+  uint32_t* poff;
+  uint64_t* pkeys_in;
+  uint64_t* pkeys_out;
+  int32_t* color;
+  int32_t* nbc;        // n_agg x n_col: neighbour (or self) of each colour, -1
+  double* A;           // coarse matrix, then its inverse (column-major nc x nc)
+  double* rc;
+  double* zc;
+  double* probe;       // 6 probe vectors, 6 products
+  float* u32;          // preconditioned residual being built
+  float* r32;          // FP32 copy of the outer residual
+  float* r2;           // residual after the smoothing step
+  int32_t* info;
+  void* tmp;
+  size_t tmp_bytes = 0;
+};
+
+void coarse_layout(int64_t n_nodes, int64_t n_elem, cf::Carve& c, Coarse* C) {
+  const int64_t n = 3 * n_nodes, n_tiles = (n_elem + kBlock - 1) / kBlock;
+  const int64_t G = (n_tiles + kAggTiles - 1) / kAggTiles, ncmax = 6 * G;
+  const int64_t maxp = kPairsPerElem * n_elem;
+  C->n_groups = G;
+  C->graw = c.take<int32_t>(n_nodes);
+  C->used = c.take<uint32_t>(G + 1);
+  C->gid = c.take<uint32_t>(G + 1);
+  C->agg = c.take<int32_t>(n_nodes);
+  C->agg_off = c.take<int32_t>(G + 1);
+  C->keys_in = c.take<uint32_t>(n_nodes);
+  C->keys_out = c.take<uint32_t>(n_nodes);
+  C->ids_in = c.take<uint32_t>(n_nodes);
+  C->agg_nodes = c.take<uint32_t>(n_nodes);
+  C->frame = c.take<double>(4 * G);
+  C->pcnt = c.take<uint32_t>(n_elem + 1);
+  C->poff = c.take<uint32_t>(n_elem + 1);
+  C->pkeys_in = c.take<uint64_t>(maxp);
+  C->pkeys_out = c.take<uint64_t>(maxp);
+  C->color = c.take<int32_t>(G);
+  C->nbc = c.take<int32_t>(G * kMaxColors);
+  C->A = c.take<double>(ncmax * ncmax);
+  C->rc = c.take<double>(ncmax);
+  C->zc = c.take<double>(ncmax);
+  C->probe = c.take<double>(12 * n);
+  C->u32 = c.take<float>(n);
+  C->r32 = c.take<float>(n);
+  C->r2 = c.take<float>(n);
+  C->info = c.take<int32_t>(1);
+  size_t b1 = 0, b2 = 0, b3 = 0, b4 = 0;
+  rocprim::radix_sort_pairs(nullptr, b1, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                            (uint32_t*)nullptr, (uint32_t*)nullptr, size_t(n_nodes), 0, 32);
+  rocprim::exclusive_scan(nullptr, b2, (uint32_t*)nullptr, (uint32_t*)nullptr, 0u,
+                          size_t(G + 1), rocprim::plus<uint32_t>());
+  rocprim::exclusive_scan(nullptr, b3, (uint32_t*)nullptr, (uint32_t*)nullptr, 0u,
+                          size_t(n_elem + 1), rocprim::plus<uint32_t>());
+  rocprim::radix_sort_keys(nullptr, b4, (uint64_t*)nullptr, (uint64_t*)nullptr, size_t(maxp), 0,
+                           64);
+  C->tmp_bytes = std::max(std::max(b1, b2), std::max(b3, b4));
+  C->tmp = c.take<char>(C->tmp_bytes);
+}
+
+// workspace of the mixed matrix-free solver; C (two-level only) may be null
+size_t mf_mixed_layout(int64_t n_nodes, int64_t n_elem, void* base, Layout* L, MfMixed* M,
+                       Coarse* C) {
+  cf::Carve c(base);
+  mf_mixed_layout_base(n_nodes, n_elem, c, L, M);
+  if (C) coarse_layout(n_nodes, n_elem, c, C);
+  L->total = c.used;
+  return c.used;
+}
+
+// rocBLAS handle of the current device for the rocSOLVER factorisation (one
+// per device for the life of the process; not thread-safe).
+rocblas_handle solver_handle() {
+  static rocblas_handle handles[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  if (!handles[dev] && rocblas_create_handle(&handles[dev]) != rocblas_status_success)
+    handles[dev] = nullptr;
+  return handles[dev];
+}
+
+// Aggregates, colouring, coarse matrix by probing, and its inverse.
+int coarse_setup(const MfArgs& a, const MfRuns& R, const double* xs, const double* ys,
+                 const double* zs, const uint8_t* fixed, double* part, Coarse* C,
+                 hipStream_t s) {
+  const int64_t n_nodes = a.n_nodes, n_elem = a.n_elem, n = 3 * n_nodes, G = C->n_groups;
+  const int gn = cf::stride_grid(n_nodes, kBlock), ge = cf::stride_grid(n_elem + 1, kBlock);
+  size_t tb;
+  // aggregates: first-touching element group, compacted ids
+  CF_HIP(hipMemsetAsync(C->used, 0, sizeof(uint32_t) * (G + 1), s));
+  agg_first<<<gn, kBlock, 0, s>>>(n_nodes, a.offs, a.perm, C->graw, C->used);
+  CF_LAUNCHED();
+  tb = C->tmp_bytes;
+  CF_HIP(rocprim::exclusive_scan(C->tmp, tb, C->used, C->gid, 0u, size_t(G + 1),
+                                 rocprim::plus<uint32_t>(), s));
+  agg_compact<<<gn, kBlock, 0, s>>>(n_nodes, C->graw, C->gid, G, C->agg, C->keys_in, C->ids_in);
+  CF_LAUNCHED();
+  uint32_t n_agg = 0;
+  CF_HIP(hipMemcpyAsync(&n_agg, C->gid + G, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  CF_HIP(hipStreamSynchronize(s));
+  CF_REQUIRE(n_agg > 0 && int64_t(n_agg) <= G, "two-level: bad aggregate count %u", n_agg);
+  C->n_agg = n_agg;
+  C->nc = 6 * int64_t(n_agg);
+  tb = C->tmp_bytes;
+  CF_HIP(rocprim::radix_sort_pairs(C->tmp, tb, C->keys_in, C->keys_out, C->ids_in, C->agg_nodes,
+                                   size_t(n_nodes), 0, bits_for(int64_t(n_agg) + 1), s));
+  agg_offsets<<<cf::stride_grid(n_agg + 1, kBlock), kBlock, 0, s>>>(n_nodes, n_agg, C->keys_out,
+                                                                    C->agg_off);
+  CF_LAUNCHED();
+  agg_frame<<<cf::stride_grid(n_agg, 64), 64, 0, s>>>(n_agg, C->agg_off, C->agg_nodes, xs, ys, zs,
+                                                      C->frame);
+  CF_LAUNCHED();
+  // aggregate adjacency through elements
+  agg_pair_count<<<ge, kBlock, 0, s>>>(n_elem, a.conn, C->agg, C->pcnt);
+  CF_LAUNCHED();
+  tb = C->tmp_bytes;
+  CF_HIP(rocprim::exclusive_scan(C->tmp, tb, C->pcnt, C->poff, 0u, size_t(n_elem + 1),
+                                 rocprim::plus<uint32_t>(), s));
+  uint32_t n_pairs = 0;
+  CF_HIP(hipMemcpyAsync(&n_pairs, C->poff + n_elem, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  CF_HIP(hipStreamSynchronize(s));
+  CF_REQUIRE(int64_t(n_pairs) <= kPairsPerElem * n_elem, "two-level: %u aggregate pairs", n_pairs);
+  std::vector<uint64_t> pk(n_pairs);
+  if (n_pairs) {
+    agg_pair_fill<<<ge, kBlock, 0, s>>>(n_elem, a.conn, C->agg, C->poff, n_agg, C->pkeys_in);
+    CF_LAUNCHED();
+    tb = C->tmp_bytes;
+    CF_HIP(rocprim::radix_sort_keys(C->tmp, tb, C->pkeys_in, C->pkeys_out, size_t(n_pairs), 0,
+                                    bits_for(int64_t(n_agg) * n_agg + 1), s));
+    CF_HIP(hipMemcpyAsync(pk.data(), C->pkeys_out, sizeof(uint64_t) * n_pairs,
+                          hipMemcpyDeviceToHost, s));
+    CF_HIP(hipStreamSynchronize(s));
+  }
+  // distance-2 greedy colouring on the host (n_agg is small)
+  std::vector<std::vector<int32_t>> adj(n_agg);
+  for (uint32_t g = 0; g < n_agg; ++g) adj[g].push_back(int32_t(g));
+  uint64_t last = ~uint64_t(0);
+  for (uint64_t key : pk) {
+    if (key == last) continue;
+    last = key;
+    const int32_t g = int32_t(key / n_agg), h = int32_t(key % n_agg);
+    adj[g].push_back(h);
+    adj[h].push_back(g);
+  }
+  std::vector<int32_t> color(n_agg, -1);
+  std::vector<uint32_t> stamp(kMaxColors, 0);
+  int n_col = 0;
+  for (uint32_t g = 0; g < n_agg; ++g) {
+    for (int32_t h : adj[g])
+      for (int32_t k : adj[h])
+        if (color[k] >= 0) stamp[color[k]] = g + 1;
+    int c = 0;
+    while (c < kMaxColors && stamp[c] == g + 1) ++c;
+    CF_REQUIRE(c < kMaxColors, "two-level: aggregate graph needs more than %d colours",
+               kMaxColors);
+    color[g] = c;
+    n_col = std::max(n_col, c + 1);
+  }
+  std::vector<int32_t> nbc(size_t(n_agg) * n_col, -1);
+  for (uint32_t h = 0; h < n_agg; ++h)
+    for (int32_t g : adj[h]) nbc[size_t(h) * n_col + color[g]] = g;
+  C->n_col = n_col;
+  CF_HIP(hipMemcpyAsync(C->color, color.data(), sizeof(int32_t) * n_agg, hipMemcpyHostToDevice,
+                        s));
+  CF_HIP(hipMemcpyAsync(C->nbc, nbc.data(), sizeof(int32_t) * nbc.size(), hipMemcpyHostToDevice,
+                        s));
+  // coarse matrix: 6 products per colour
+  const int64_t nc = C->nc;
+  CF_HIP(hipMemsetAsync(C->A, 0, sizeof(double) * nc * nc, s));
+  double* v = C->probe;
+  double* yv = C->probe + 6 * n;
+  int rc;
+  for (int c = 0; c < n_col; ++c) {
+    probe_build<<<gn, kBlock, 0, s>>>(n_nodes, C->agg, C->color, c, C->frame, xs, ys, zs, fixed, v);
+    CF_LAUNCHED();
+    for (int mm = 0; mm < 6; ++mm)
+      if ((rc = mf_apply(a, R, v + mm * n, yv + mm * n, fixed, part, nullptr, nullptr, s)))
+        return rc;
+    probe_extract<<<int(std::min<int64_t>(n_agg, 65535)), kBlock, 0, s>>>(
+        n_agg, n_col, c, C->nbc, C->agg_off, C->agg_nodes, C->frame, xs, ys, zs, n, yv, C->A);
+    CF_LAUNCHED();
+  }
+  const int gc = cf::stride_grid(nc, 64);
+  coarse_symmetrize<<<gc, 64, 0, s>>>(nc, C->A);
+  CF_LAUNCHED();
+  std::vector<double> diag(nc);
+  CF_HIP(hipMemcpy2DAsync(diag.data(), sizeof(double), C->A, sizeof(double) * (nc + 1),
+                          sizeof(double), nc, hipMemcpyDeviceToHost, s));
+  CF_HIP(hipStreamSynchronize(s));
+  double dmax = 0.0;
+  for (double d : diag) dmax = std::max(dmax, d);
+  CF_REQUIRE(dmax > 0.0 && std::isfinite(dmax), "two-level: bad coarse matrix (max diag %g)",
+             dmax);
+  coarse_regularize<<<gc, 64, 0, s>>>(nc, C->A, 1e-12 * dmax);
+  CF_LAUNCHED();
+  // factor and invert (rocSOLVER)
+  rocblas_handle hnd = solver_handle();
+  CF_REQUIRE(hnd != nullptr, "two-level: no rocBLAS handle");
+  CF_REQUIRE(rocblas_set_stream(hnd, s) == rocblas_status_success, "two-level: rocblas_set_stream");
+  CF_REQUIRE(rocsolver_dpotrf(hnd, rocblas_fill_lower, rocblas_int(nc), C->A, rocblas_int(nc),
+                              C->info) == rocblas_status_success,
+             "two-level: rocsolver_dpotrf failed");
+  int32_t info = 0;
+  CF_HIP(hipMemcpyAsync(&info, C->info, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  CF_HIP(hipStreamSynchronize(s));
+  CF_REQUIRE(info == 0, "two-level: coarse matrix not positive definite (potrf info %d)", info);
+  CF_REQUIRE(rocsolver_dpotri(hnd, rocblas_fill_lower, rocblas_int(nc), C->A, rocblas_int(nc),
+                              C->info) == rocblas_status_success,
+             "two-level: rocsolver_dpotri failed");
+  CF_HIP(hipMemcpyAsync(&info, C->info, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  CF_HIP(hipStreamSynchronize(s));
+  CF_REQUIRE(info == 0, "two-level: coarse inverse failed (potri info %d)", info);
+  coarse_fill_upper<<<gc, 64, 0, s>>>(nc, C->A);
+  CF_LAUNCHED();
   return CF_OK;
 }
 
-extern "C" int cf_pcg_solve_mf_t10_mixed(int64_t n_nodes, int64_t n_elem, const int32_t* conn,
-                                         const double* geo, double lam, double mu,
-                                         const int32_t* offs, const uint32_t* perm,
-                                         const double* b, double* x, const uint8_t* fixed,
-                                         double tol, int32_t maxit, void* ws, size_t ws_bytes,
-                                         cf_pcg_stats* stats_host, void* stream) {
+}  // namespace
+
+namespace {
+
+// Mixed precision with the matrix-free operator (Chebyshev-only or, with
+// xyz, two-level).
+int mf_mixed_solve(const char* who, int64_t n_nodes, int64_t n_elem, const int32_t* conn,
+                   const double* geo, double lam, double mu, const int32_t* offs,
+                   const uint32_t* perm, const double* xs, const double* ys, const double* zs,
+                   bool two_level, const double* b, double* x, const uint8_t* fixed, double tol,
+                   int32_t maxit, void* ws, size_t ws_bytes, cf_pcg_stats* stats_host,
+                   void* stream) {
   MfArgs a{n_nodes, n_elem, conn, geo, lam, mu, offs, perm};
   int rc = mf_check(a);
   if (rc) return rc;
-  CF_REQUIRE(b && x && ws && stats_host, "cf_pcg_solve_mf_t10_mixed: null argument");
-  CF_REQUIRE(10 * n_elem < (int64_t(1) << 31), "cf_pcg_solve_mf_t10_mixed: too many elements");
-  CF_REQUIRE(tol > 0.0 && maxit > 0, "cf_pcg_solve_mf_t10_mixed: tol and maxit must be positive");
+  CF_REQUIRE(b && x && ws && stats_host, "%s: null argument", who);
+  CF_REQUIRE(!two_level || (xs && ys && zs), "%s: node coordinates required", who);
+  CF_REQUIRE(10 * n_elem < (int64_t(1) << 31), "%s: too many elements", who);
+  CF_REQUIRE(tol > 0.0 && maxit > 0, "%s: tol and maxit must be positive", who);
   Layout L{};
   MfMixed M{};
-  const size_t need = mf_mixed_layout(n_nodes, n_elem, ws, &L, &M);
+  Coarse C{};
+  const size_t need = mf_mixed_layout(n_nodes, n_elem, ws, &L, &M, two_level ? &C : nullptr);
   if (ws_bytes < need) {
-    cf::set_error("cf_pcg_solve_mf_t10_mixed: workspace %zu < %zu bytes", ws_bytes, need);
+    cf::set_error("%s: workspace %zu < %zu bytes", who, ws_bytes, need);
     return CF_ERR_WORKSPACE;
   }
   hipStream_t s = cf::as_stream(stream);
@@ -3501,7 +4151,7 @@ extern "C" int cf_pcg_solve_mf_t10_mixed(int64_t n_nodes, int64_t n_elem, const 
   CF_HIP(hipMemcpyAsync(&bad, L.bad, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   CF_HIP(hipStreamSynchronize(s));
   if (bad) {
-    cf::set_error("cf_pcg_solve_mf_t10_mixed: %d free diagonal entries are not positive", bad);
+    cf::set_error("%s: %d free diagonal entries are not positive", who, bad);
     return CF_ERR_BREAKDOWN;
   }
   make_scale<<<G, kBlock, 0, s>>>(n, L.dinv, L.s);
@@ -3511,6 +4161,7 @@ extern "C" int cf_pcg_solve_mf_t10_mixed(int64_t n_nodes, int64_t n_elem, const 
   geo_to_f32<<<cf::stride_grid(10 * n_elem, kBlock), kBlock, 0, s>>>(10 * n_elem, geo, M.geo32);
   CF_LAUNCHED();
   if ((rc = mf_runs_build(n_nodes, n_elem, offs, perm, &M.R, s))) return rc;
+  if (two_level && (rc = coarse_setup(a, M.R, xs, ys, zs, fixed, M.fe, &C, s))) return rc;
   const int ge = cf::stride_grid(n_elem, kBlock);
   const int gg = int(mf_gather_grid(n_nodes));
   auto outer = [&](const double* xin, double* y, PcgState* state) {
@@ -3519,7 +4170,7 @@ extern "C" int cf_pcg_solve_mf_t10_mixed(int64_t n_nodes, int64_t n_elem, const 
   // the power iteration multiplies plain vectors (S K S v); the Chebyshev
   // steps run on S-scaled vectors, so their element kernel gathers no S.
   // Step k reads z_k (din) and z_{k-1} (dout, none at k = 1) and overwrites
-  // dout with z_{k+1}.
+  // dout with z_{k+1}.  Modes 3 and 4: products of the two-level form.
   auto product32 = [&](const float* din, float* dout, int mode, float cdk, float cwk,
                        const PcgState* state) {
     if (mode == 0)
@@ -3532,7 +4183,8 @@ extern "C" int cf_pcg_solve_mf_t10_mixed(int64_t n_nodes, int64_t n_elem, const 
                                                 state);
     CF_LAUNCHED();
     mf_gather_cheb32<<<gg, kBlock, 0, s>>>(n_nodes, M.R.nro, M.R.nq, M.fe32, M.s32, mode, din, dout,
-                                           dout, L.rh, cdk, cwk, state);
+                                           dout, L.rh, cdk, cwk, two_level ? C.r32 : nullptr,
+                                           state);
     CF_LAUNCHED();
     return CF_OK;
   };
@@ -3540,8 +4192,99 @@ extern "C" int cf_pcg_solve_mf_t10_mixed(int64_t n_nodes, int64_t n_elem, const 
   auto cheb = [&](int k, float cdk, float cwk, const float* din, float* dout) {
     return product32(din, dout, k == 1 ? 1 : 2, cdk, cwk, L.state);
   };
-  return fcg_cheb_core<true>(outer, gg, power, cheb, 0.0, kChebF32, n, b, x, fixed, tol, maxit, L,
-                       stats_host, s);
+  if (!two_level)
+    return fcg_cheb_core<true>(outer, gg, power, cheb, 0.0, kChebF32, n, b, x, fixed, tol, maxit,
+                               L, stats_host, s);
+  // two-level: z1 = C r; z2 = z1 + Cheb(r - K z1); z = z2 + C (r - K z2)
+  const int GV = pcg_vec_grid();
+  const int ga = int(std::min<int64_t>(C.n_agg, 65535));
+  const int gc = int(std::min<int64_t>((C.nc + kWaves - 1) / kWaves, 65535));
+  auto coarse_apply = [&](auto* rin, bool add) {
+    coarse_restrict<<<ga, kBlock, 0, s>>>(C.n_agg, C.agg_off, C.agg_nodes, C.frame, xs, ys, zs,
+                                          rin, C.rc, L.state);
+    CF_LAUNCHED();
+    coarse_gemv<<<gc, kBlock, 0, s>>>(C.nc, C.A, C.rc, C.zc, L.state);
+    CF_LAUNCHED();
+    if (add)
+      coarse_prolong<true><<<G, kBlock, 0, s>>>(n_nodes, C.agg, C.frame, xs, ys, zs, fixed, C.zc,
+                                                C.u32, L.state);
+    else
+      coarse_prolong<false><<<G, kBlock, 0, s>>>(n_nodes, C.agg, C.frame, xs, ys, zs, fixed, C.zc,
+                                                 C.u32, L.state);
+    CF_LAUNCHED();
+    return CF_OK;
+  };
+  float* D[2] = {L.ph, L.Aph};
+  auto pre2l = [&](int apply_step, int slot, int sample, const float* cd, const float* cw,
+                   float inv_theta, int steps) {
+    if (apply_step)
+      fcg_update2l<true><<<GV, kBlock, 0, s>>>(n, x, L.r, L.p, L.Ap, C.r32, L.state, L.partials,
+                                               gg, L.partials_b, slot);
+    else
+      fcg_update2l<false><<<GV, kBlock, 0, s>>>(n, x, L.r, L.p, L.Ap, C.r32, L.state, nullptr, 0,
+                                                L.partials_b, slot);
+    CF_LAUNCHED();
+    int r2;
+    if ((r2 = coarse_apply(L.r, false))) return r2;                       // u = C r
+    if ((r2 = product32(C.u32, D[0], 3, 0.0f, inv_theta, L.state))) return r2;  // Cheb start
+    for (int k = 1; k < steps; ++k) {
+      int tok = -1;
+      if (k == 1) prof_begin(1, sample, s, &tok);
+      if ((r2 = cheb(k, cd[k], cw[k], D[(k - 1) & 1], D[k & 1]))) return r2;
+      if (k == 1) prof_end(tok, s);
+    }
+    vec_add32<<<G, kBlock, 0, s>>>(n, C.u32, D[(steps - 1) & 1], L.state);  // u = z1 + Cheb
+    CF_LAUNCHED();
+    if ((r2 = product32(C.u32, C.r2, 4, 0.0f, 0.0f, L.state))) return r2;   // r2 = r - K u
+    return coarse_apply(C.r2, true);                                         // u += C r2
+  };
+  return fcg_cheb_core<true>(outer, gg, power, cheb, 0.0, kChebF32, n, b, x, fixed, tol, maxit,
+                             L, stats_host, s, pre2l, C.u32, 2);
+}
+
+}  // namespace
+
+extern "C" int cf_mf_t10_mixed_workspace_bytes(int64_t n_nodes, int64_t n_elem, size_t* bytes) {
+  CF_REQUIRE(n_nodes > 0 && n_elem > 0 && bytes, "cf_mf_t10_mixed_workspace_bytes: bad arguments");
+  Layout L{};
+  MfMixed M{};
+  *bytes = mf_mixed_layout(n_nodes, n_elem, nullptr, &L, &M, nullptr);
+  return CF_OK;
+}
+
+extern "C" int cf_pcg_solve_mf_t10_mixed(int64_t n_nodes, int64_t n_elem, const int32_t* conn,
+                                         const double* geo, double lam, double mu,
+                                         const int32_t* offs, const uint32_t* perm,
+                                         const double* b, double* x, const uint8_t* fixed,
+                                         double tol, int32_t maxit, void* ws, size_t ws_bytes,
+                                         cf_pcg_stats* stats_host, void* stream) {
+  return mf_mixed_solve("cf_pcg_solve_mf_t10_mixed", n_nodes, n_elem, conn, geo, lam, mu, offs,
+                        perm, nullptr, nullptr, nullptr, false, b, x, fixed, tol, maxit, ws,
+                        ws_bytes, stats_host, stream);
+}
+
+extern "C" int cf_mf_t10_twolevel_workspace_bytes(int64_t n_nodes, int64_t n_elem,
+                                                  size_t* bytes) {
+  CF_REQUIRE(n_nodes > 0 && n_elem > 0 && bytes,
+             "cf_mf_t10_twolevel_workspace_bytes: bad arguments");
+  Layout L{};
+  MfMixed M{};
+  Coarse C{};
+  *bytes = mf_mixed_layout(n_nodes, n_elem, nullptr, &L, &M, &C);
+  return CF_OK;
+}
+
+extern "C" int cf_pcg_solve_mf_t10_twolevel(int64_t n_nodes, int64_t n_elem, const int32_t* conn,
+                                            const double* geo, double lam, double mu,
+                                            const int32_t* offs, const uint32_t* perm,
+                                            const double* x_nodes, const double* y_nodes,
+                                            const double* z_nodes, const double* b, double* x,
+                                            const uint8_t* fixed, double tol, int32_t maxit,
+                                            void* ws, size_t ws_bytes, cf_pcg_stats* stats_host,
+                                            void* stream) {
+  return mf_mixed_solve("cf_pcg_solve_mf_t10_twolevel", n_nodes, n_elem, conn, geo, lam, mu, offs,
+                        perm, x_nodes, y_nodes, z_nodes, true, b, x, fixed, tol, maxit, ws,
+                        ws_bytes, stats_host, stream);
 }
 
 extern "C" int cf_invert_permutation(int64_t n, const uint32_t* perm, uint32_t* inv,

@@ -87,7 +87,8 @@ def solve_device(dm, vals, rhs, mesh, bc, lam=None, mu=None, **pcg_kw):
 
     With solver option operator="matfree" the stiffness is applied matrix-free
     from the geometry record (needs lam, mu; precision "mixed" adds the FP32
-    matrix-free Chebyshev preconditioner); otherwise the assembled NB3 matrix
+    matrix-free Chebyshev preconditioner, and precond "twolevel" wraps it in
+    the rigid-body coarse correction); otherwise the assembled NB3 matrix
     `vals` is used.
     """
     n3 = 3 * dm.n_nodes
@@ -104,7 +105,10 @@ def solve_device(dm, vals, rhs, mesh, bc, lam=None, mu=None, **pcg_kw):
         stats = solver_mod.pcg_matfree_t10(dm, lam, mu, rhs, x, fixed, **kw)
     elif o["operator"] == "matfree" and lam is not None and prec in ("mixed", "mixed16"):
         kw = {k: v for k, v in pcg_kw.items() if k in ("tol", "maxit")}
-        stats = solver_mod.pcg_matfree_t10_mixed(dm, lam, mu, rhs, x, fixed, **kw)
+        if pcg_kw.get("precond", o["precond"]) == "twolevel":
+            stats = solver_mod.pcg_matfree_t10_twolevel(dm, lam, mu, rhs, x, fixed, **kw)
+        else:
+            stats = solver_mod.pcg_matfree_t10_mixed(dm, lam, mu, rhs, x, fixed, **kw)
     else:
         A = DeviceMatrix(N.FMT_NB3, dm.pattern(), vals)
         stats = pcg(A, rhs, x, fixed, **pcg_kw)

@@ -15,7 +15,10 @@ problems:
              test is the FP64 residual in every case, so the error against
              FP64 is bounded by the same tolerance (cf_pcg_solve)
   precond    "jacobi" (default) or "block3" (3x3 node blocks, vector
-             problems, FP64 only)
+             problems, FP64 only); "twolevel" with operator "matfree" and
+             precision "mixed": the FP32 Chebyshev polynomial wrapped in a
+             coarse correction on the rigid-body modes of node aggregates
+             (cf_pcg_solve_mf_t10_twolevel)
   tol        relative residual ||b_f - A_ff x_f|| / ||b_f|| (default 1e-12)
   maxit      iteration cap (default 200000)
   operator   "assembled" (default: CSR/NB3 SpMV) or "matfree" (Tet10
@@ -62,6 +65,14 @@ def solver_options(**kw):
 last_stats = threading.local()
 
 
+def _pc_code(precond):
+    codes = {"jacobi": N.PC_JACOBI, "block3": N.PC_BLOCK3}
+    if precond not in codes:
+        raise ValueError(f"precond {precond!r} is not available here (jacobi, block3; twolevel "
+                         "needs operator='matfree' and precision='mixed')")
+    return codes[precond]
+
+
 class DeviceMatrix:
     """A square matrix in HBM: scalar CSR or NB3 (3x3 node blocks)."""
 
@@ -96,7 +107,7 @@ def pcg(A, b, x, fixed=None, precision=None, precond=None, tol=None, maxit=None)
     tol = float(tol if tol is not None else o["tol"])
     maxit = int(maxit if maxit is not None else o["maxit"])
     prec = {"fp64": N.PREC_FP64, "mixed": N.PREC_MIXED, "mixed16": N.PREC_MIXED16}[precision]
-    pc = {"jacobi": N.PC_JACOBI, "block3": N.PC_BLOCK3}[precond]
+    pc = _pc_code(precond)
     if pc == N.PC_BLOCK3 and (A.fmt != N.FMT_NB3 or prec != N.PREC_FP64):
         pc = N.PC_JACOBI
     lib = N.load()
@@ -142,6 +153,33 @@ def pcg_matfree_t10_mixed(dm, lam, mu, b, x, fixed=None, tol=None, maxit=None):
     return stats
 
 
+def pcg_matfree_t10_twolevel(dm, lam, mu, b, x, fixed=None, tol=None, maxit=None):
+    """Two-level mixed precision with the matrix-free Tet10 operator: FP64
+    flexible CG, preconditioned by a coarse correction on the rigid-body modes
+    of node aggregates around the FP32 matrix-free Chebyshev polynomial
+    (cf_pcg_solve_mf_t10_twolevel)."""
+    o = options()
+    tol = float(tol if tol is not None else o["tol"])
+    maxit = int(maxit if maxit is not None else o["maxit"])
+    lib = N.load()
+    conn, geo, offs, perm = dm.matfree_arrays()
+    nb = ctypes.c_size_t()
+    N.check(lib.cf_mf_t10_twolevel_workspace_bytes(dm.n_nodes, dm.n_elem, ctypes.byref(nb)),
+            "cf_mf_t10_twolevel_workspace_bytes")
+    ws = workspace(nb.value, b.device)
+    st = N.CfPcgStats()
+    rc = lib.cf_pcg_solve_mf_t10_twolevel(dm.n_nodes, dm.n_elem, ptr(conn), ptr(geo), float(lam),
+                                          float(mu), ptr(offs), ptr(perm), ptr(dm.x), ptr(dm.y),
+                                          ptr(dm.z), ptr(b), ptr(x), ptr(fixed), tol, maxit,
+                                          ptr(ws), nb.value, ctypes.byref(st), stream_handle())
+    stats = {"iterations": st.iterations, "residual_updates": st.residual_updates,
+             "rel_residual": st.rel_residual, "b_norm": st.b_norm, "precision": "mixed",
+             "precond": "two-level", "tol": tol, "operator": "matfree"}
+    last_stats.value = stats
+    N.check(rc, "cf_pcg_solve_mf_t10_twolevel")
+    return stats
+
+
 def pcg_matfree_t10(dm, lam, mu, b, x, fixed=None, precond=None, tol=None, maxit=None,
                     chunked=False):
     """PCG with the matrix-free Tet10 elasticity operator (FP64).  Default:
@@ -153,7 +191,7 @@ def pcg_matfree_t10(dm, lam, mu, b, x, fixed=None, precond=None, tol=None, maxit
     precond = precond or o["precond"]
     tol = float(tol if tol is not None else o["tol"])
     maxit = int(maxit if maxit is not None else o["maxit"])
-    pc = {"jacobi": N.PC_JACOBI, "block3": N.PC_BLOCK3}[precond]
+    pc = _pc_code(precond)
     lib = N.load()
     if chunked:
         plan, counts = dm.matfree_plan()

@@ -383,6 +383,28 @@ int cf_pcg_solve_mf_t10_mixed(int64_t n_nodes, int64_t n_elem, const int32_t* co
                               double tol, int32_t maxit, void* ws, size_t ws_bytes,
                               cf_pcg_stats* stats_host, void* stream);
 
+/* ---- two-level mixed precision with the matrix-free operator (no
+ *      reference counterpart; replaces the spsolve of elastic_tet10.py:214-229
+ *      like cf_pcg_solve_mf_t10).  FP64 flexible CG on the FP64 matrix-free
+ *      product, preconditioned by z1 = C r, z2 = z1 + Cheb(r - K z1),
+ *      z = z2 + C (r - K z2): Cheb the FP32 matrix-free Chebyshev polynomial
+ *      of cf_pcg_solve_mf_t10_mixed, C = P A_c^-1 P^T a coarse correction on
+ *      the six rigid-body modes of node aggregates (the group of 512
+ *      consecutive elements that first touches a node), A_c = P^T K_ff P
+ *      assembled by probing with distance-2 coloured aggregates (no atomics)
+ *      and inverted with rocSOLVER (dpotrf + dpotri; rocBLAS allocates its own
+ *      small workspace and handle, once per device).  x_nodes/y_nodes/z_nodes:
+ *      node coordinates (device, FP64).  Same FP64 residual bar as the other
+ *      solvers.                                                              */
+int cf_mf_t10_twolevel_workspace_bytes(int64_t n_nodes, int64_t n_elem, size_t* bytes);
+int cf_pcg_solve_mf_t10_twolevel(int64_t n_nodes, int64_t n_elem, const int32_t* conn,
+                                 const double* geo, double lam, double mu,
+                                 const int32_t* offs, const uint32_t* perm,
+                                 const double* x_nodes, const double* y_nodes,
+                                 const double* z_nodes, const double* b, double* x,
+                                 const uint8_t* fixed, double tol, int32_t maxit, void* ws,
+                                 size_t ws_bytes, cf_pcg_stats* stats_host, void* stream);
+
 /* ---- inv[perm[j]] = j for a permutation of 0..n-1.                     */
 int cf_invert_permutation(int64_t n, const uint32_t* perm, uint32_t* inv,
                           void* stream);

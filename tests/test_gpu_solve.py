@@ -142,6 +142,59 @@ def test_matfree_mixed_meets_fp64_residual(gpu, dims, tol):
     assert err < 1e-8
 
 
+@pytest.mark.parametrize("dims", [(8, 2, 2), (20, 3, 3), (40, 5, 5)])
+def test_matfree_twolevel_meets_fp64_residual(gpu, dims):
+    """Two-level mixed solver (rigid-body coarse correction around the FP32
+    matrix-free Chebyshev polynomial): FP64 residual bar (3x the FP64 floor
+    where the bar is within rounding of it, as above) and the solution
+    against SuperLU.  8x2x2 has one aggregate, 40x5x5 twelve."""
+    from cloudfem_amd import solver
+    from cloudfem_amd.mesh import box_mesh_tet10
+    from cloudfem_amd.physics import elastic_tet10
+    tol = 1e-10
+    lims = dict(xlim=(0.0, 0.4), ylim=(0.0, 0.05), zlim=(0.0, 0.05))
+    trac = {"xmax": (0.0, 0.0, -1e6)}
+    bc = {"xmin": (0.0, 0.0, 0.0)}
+    m = box_mesh_tet10(*dims, **lims)
+    with solver.solver_options(operator="matfree", precision="mixed", precond="twolevel",
+                               tol=tol):
+        u = np.stack(elastic_tet10.solve_elasticity_tet10(m, 210e9, 0.3, bc=bc, traction=trac),
+                     axis=1).ravel()
+    st = solver.last_stats.value
+    assert st["operator"] == "matfree" and st["precond"] == "two-level"
+    m0 = om.box_mesh_tet10(*dims, **lims)
+    A0, rhs0 = oel.assemble_elasticity_tet10(m0, 210e9, 0.3, traction=trac)
+    u0 = oel.apply_dirichlet_vec3(A0, rhs0, m0, bc)
+    Aff, bf, free = oel.reduced_system_vec3(A0, rhs0, m0, bc)[:3]
+    floor = (np.finfo(np.float64).eps * np.linalg.norm(abs(Aff) @ np.abs(u0[free]))
+             / np.linalg.norm(bf))
+    err = rel_err(u, u0)
+    print(f"two-level {dims}: {st['iterations']} FP64 iterations, {st['residual_updates']} "
+          f"FP32 products, rel. residual {st['rel_residual']:.2e} (floor {floor:.2e}), "
+          f"error vs SuperLU {err:.2e}")
+    assert st["rel_residual"] <= max(2 * tol, 3 * floor)
+    assert err < 1e-8
+
+
+def test_twolevel_fewer_iterations_than_chebyshev(gpu):
+    """The coarse correction must pay: on the 40x5x5 cantilever the two-level
+    solver needs fewer outer iterations than the Chebyshev-only solver."""
+    from cloudfem_amd import solver
+    from cloudfem_amd.mesh import box_mesh_tet10
+    from cloudfem_amd.physics import elastic_tet10
+    lims = dict(xlim=(0.0, 0.4), ylim=(0.0, 0.05), zlim=(0.0, 0.05))
+    trac = {"xmax": (0.0, 0.0, -1e6)}
+    bc = {"xmin": (0.0, 0.0, 0.0)}
+    m = box_mesh_tet10(40, 5, 5, **lims)
+    its = {}
+    for pc in ("jacobi", "twolevel"):
+        with solver.solver_options(operator="matfree", precision="mixed", precond=pc, tol=1e-10):
+            elastic_tet10.solve_elasticity_tet10(m, 210e9, 0.3, bc=bc, traction=trac)
+        its[pc] = solver.last_stats.value["iterations"]
+    print("outer iterations", its)
+    assert its["twolevel"] < its["jacobi"]
+
+
 @pytest.mark.parametrize("precision", ["mixed", "mixed16"])
 def test_p1_mixed_precision_matches_spsolve(gpu, precision):
     """Scalar CSR systems take the FP32 inner CG with FP64 residual
