@@ -2541,6 +2541,8 @@ struct ChebShape {
   double ratio;
 };
 constexpr ChebShape kChebF32{24, 3000.0};
+// inside the two-level preconditioner the coarse space takes the lowest modes
+constexpr ChebShape kCheb2L{24, 3000.0};
 constexpr ChebShape kChebF16{16, 3000.0};
 constexpr int kChebMaxSteps = 32;
 constexpr int kPowerSteps = 40;
@@ -3546,7 +3548,7 @@ namespace {
 // factored and inverted with rocSOLVER; the preconditioner is
 //   z1 = C r,  z2 = z1 + Cheb(r - K z1),  z = z2 + C (r - K z2),
 // C = P A_c^-1 P^T, Cheb the FP32 Chebyshev polynomial of S K S.
-constexpr int kAggTiles = 2;
+constexpr int kAggTiles = 6;  // 1536 elements: 2, 3, 4, 6, 8, 12, 16, 24 tiles measured
 constexpr int kMaxColors = 64;
 constexpr int kPairsPerElem = 45;  // distinct aggregate pairs of 10 nodes
 
@@ -3611,36 +3613,42 @@ __global__ void agg_offsets(int64_t n_nodes, int64_t n_agg, const uint32_t* __re
   }
 }
 
-// centroid and 1 / RMS radius of each aggregate, over its nodes in order
-__global__ void agg_frame(int64_t n_agg, const int32_t* __restrict__ agg_off,
-                          const uint32_t* __restrict__ agg_nodes, const double* __restrict__ x,
-                          const double* __restrict__ y, const double* __restrict__ z,
-                          double* __restrict__ frame) {
-  for (int64_t g = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; g < n_agg;
-       g += int64_t(gridDim.x) * blockDim.x) {
+// centroid and 1 / RMS radius of each aggregate; one workgroup per
+// aggregate, fixed node split and reduction tree
+__global__ void __launch_bounds__(kBlock)
+agg_frame(int64_t n_agg, const int32_t* __restrict__ agg_off,
+          const uint32_t* __restrict__ agg_nodes, const double* __restrict__ x,
+          const double* __restrict__ y, const double* __restrict__ z,
+          double* __restrict__ frame) {
+  for (int64_t g = blockIdx.x; g < n_agg; g += gridDim.x) {
     const int32_t b = agg_off[g], e = agg_off[g + 1];
     double c[3] = {0.0, 0.0, 0.0};
-    for (int32_t k = b; k < e; ++k) {
+    for (int32_t k = b + threadIdx.x; k < e; k += blockDim.x) {
       const uint32_t i = agg_nodes[k];
       c[0] += x[i];
       c[1] += y[i];
       c[2] += z[i];
     }
+    block_sum<3>(c);
     const double inv_n = e > b ? 1.0 / double(e - b) : 0.0;
     c[0] *= inv_n;
     c[1] *= inv_n;
     c[2] *= inv_n;
-    double r2 = 0.0;
-    for (int32_t k = b; k < e; ++k) {
+    double r2[1] = {0.0};
+    for (int32_t k = b + threadIdx.x; k < e; k += blockDim.x) {
       const uint32_t i = agg_nodes[k];
       const double dx = x[i] - c[0], dy = y[i] - c[1], dz = z[i] - c[2];
-      r2 += dx * dx + dy * dy + dz * dz;
+      r2[0] += dx * dx + dy * dy + dz * dz;
     }
-    r2 *= inv_n;
-    frame[4 * g] = c[0];
-    frame[4 * g + 1] = c[1];
-    frame[4 * g + 2] = c[2];
-    frame[4 * g + 3] = r2 > 0.0 ? 1.0 / sqrt(r2) : 1.0;
+    block_sum<1>(r2);
+    if (threadIdx.x == 0) {
+      const double rr = r2[0] * inv_n;
+      frame[4 * g] = c[0];
+      frame[4 * g + 1] = c[1];
+      frame[4 * g + 2] = c[2];
+      frame[4 * g + 3] = rr > 0.0 ? 1.0 / sqrt(rr) : 1.0;
+    }
+    __syncthreads();
   }
 }
 
@@ -3759,37 +3767,59 @@ probe_extract(int64_t n_agg, int n_col, int c, const int32_t* __restrict__ nbc,
   }
 }
 
-// A = (A + A^T) / 2; a vanishing diagonal (aggregate without free DOFs, or a
-// degenerate mode) becomes an identity row and column.  One thread per column
-// j, rows i <= j; maxd = largest diagonal entry.
-__global__ void coarse_symmetrize(int64_t nc, double* __restrict__ A) {
-  for (int64_t j = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; j < nc;
-       j += int64_t(gridDim.x) * blockDim.x) {
-    for (int64_t i = 0; i < j; ++i) {
-      const double v = 0.5 * (A[i + j * nc] + A[j + i * nc]);
-      A[i + j * nc] = v;
-      A[j + i * nc] = v;
-    }
+// Largest diagonal entry of A (one block; deterministic tree).
+__global__ void __launch_bounds__(kBlock)
+coarse_diag_max(int64_t nc, const double* __restrict__ A, double* __restrict__ out) {
+  __shared__ double sh[kBlock];
+  double m = 0.0;
+  for (int64_t j = threadIdx.x; j < nc; j += blockDim.x) m = fmax(m, A[j + j * nc]);
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
   }
+  if (threadIdx.x == 0) out[0] = sh[0];
 }
 
-__global__ void coarse_regularize(int64_t nc, double* __restrict__ A, double tiny) {
+// A vanishing diagonal (an aggregate without free DOFs, or a degenerate mode)
+// becomes an identity row and column of the lower triangle rocSOLVER reads.
+// The probed blocks A[h, g] and A[g, h] agree to rounding, and potrf uses the
+// lower one, so no symmetrisation is needed.
+__global__ void coarse_regularize(int64_t nc, double* __restrict__ A,
+                                  const double* __restrict__ dmax) {
+  const double tiny = 1e-12 * dmax[0];
   for (int64_t j = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; j < nc;
        j += int64_t(gridDim.x) * blockDim.x) {
     if (A[j + j * nc] > tiny) continue;
-    for (int64_t i = 0; i < nc; ++i) {
-      A[i + j * nc] = 0.0;
-      A[j + i * nc] = 0.0;
-    }
+    for (int64_t i = j + 1; i < nc; ++i) A[i + j * nc] = 0.0;  // column j below the diagonal
+    for (int64_t k = 0; k < j; ++k) A[j + k * nc] = 0.0;       // row j left of it
     A[j + j * nc] = 1.0;
   }
 }
 
-// lower triangle of the inverse (rocSOLVER potri) copied to the upper one
-__global__ void coarse_fill_upper(int64_t nc, double* __restrict__ A) {
-  for (int64_t j = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; j < nc;
-       j += int64_t(gridDim.x) * blockDim.x)
-    for (int64_t i = 0; i < j; ++i) A[i + j * nc] = A[j + i * nc];
+// upper triangle := lower triangle (after potri), 32 x 32 tiles through LDS
+__global__ void __launch_bounds__(kBlock)
+coarse_fill_upper(int64_t nc, double* __restrict__ A) {
+  __shared__ double t[32][33];
+  const int64_t nt = (nc + 31) / 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int64_t b = blockIdx.x; b < nt * nt; b += gridDim.x) {
+    const int64_t bi = b / nt, bj = b - bi * nt;  // source tile: rows bi, columns bj
+    if (bj > bi) continue;                        // lower tiles only (uniform per block)
+    // t[c][rr] = A[bi*32 + rr, bj*32 + c], loaded down the columns (coalesced)
+    for (int r = ty; r < 32; r += 8) {
+      const int64_t i = bi * 32 + tx, j = bj * 32 + r;
+      t[r][tx] = (i < nc && j < nc) ? A[i + j * nc] : 0.0;
+    }
+    __syncthreads();
+    // A[j, i] = A[i, j] for j < i, i = bi*32 + r, j = bj*32 + tx (coalesced)
+    for (int r = ty; r < 32; r += 8) {
+      const int64_t i = bi * 32 + r, j = bj * 32 + tx;
+      if (i < nc && j < nc && j < i) A[j + i * nc] = t[tx][r];
+    }
+    __syncthreads();
+  }
 }
 
 // rc[(g, m)] = sum over the nodes of g of mode m . r (r FP64 or FP32); one
@@ -4000,8 +4030,9 @@ int coarse_setup(const MfArgs& a, const MfRuns& R, const double* xs, const doubl
   agg_offsets<<<cf::stride_grid(n_agg + 1, kBlock), kBlock, 0, s>>>(n_nodes, n_agg, C->keys_out,
                                                                     C->agg_off);
   CF_LAUNCHED();
-  agg_frame<<<cf::stride_grid(n_agg, 64), 64, 0, s>>>(n_agg, C->agg_off, C->agg_nodes, xs, ys, zs,
-                                                      C->frame);
+  agg_frame<<<int(std::min<int64_t>(n_agg, 65535)), kBlock, 0, s>>>(n_agg, C->agg_off,
+                                                                    C->agg_nodes, xs, ys, zs,
+                                                                    C->frame);
   CF_LAUNCHED();
   // aggregate adjacency through elements
   agg_pair_count<<<ge, kBlock, 0, s>>>(n_elem, a.conn, C->agg, C->pcnt);
@@ -4074,17 +4105,9 @@ int coarse_setup(const MfArgs& a, const MfRuns& R, const double* xs, const doubl
     CF_LAUNCHED();
   }
   const int gc = cf::stride_grid(nc, 64);
-  coarse_symmetrize<<<gc, 64, 0, s>>>(nc, C->A);
+  coarse_diag_max<<<1, kBlock, 0, s>>>(nc, C->A, C->zc);
   CF_LAUNCHED();
-  std::vector<double> diag(nc);
-  CF_HIP(hipMemcpy2DAsync(diag.data(), sizeof(double), C->A, sizeof(double) * (nc + 1),
-                          sizeof(double), nc, hipMemcpyDeviceToHost, s));
-  CF_HIP(hipStreamSynchronize(s));
-  double dmax = 0.0;
-  for (double d : diag) dmax = std::max(dmax, d);
-  CF_REQUIRE(dmax > 0.0 && std::isfinite(dmax), "two-level: bad coarse matrix (max diag %g)",
-             dmax);
-  coarse_regularize<<<gc, 64, 0, s>>>(nc, C->A, 1e-12 * dmax);
+  coarse_regularize<<<gc, 64, 0, s>>>(nc, C->A, C->zc);
   CF_LAUNCHED();
   // factor and invert (rocSOLVER)
   rocblas_handle hnd = solver_handle();
@@ -4103,7 +4126,8 @@ int coarse_setup(const MfArgs& a, const MfRuns& R, const double* xs, const doubl
   CF_HIP(hipMemcpyAsync(&info, C->info, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   CF_HIP(hipStreamSynchronize(s));
   CF_REQUIRE(info == 0, "two-level: coarse inverse failed (potri info %d)", info);
-  coarse_fill_upper<<<gc, 64, 0, s>>>(nc, C->A);
+  const int64_t nt = (nc + 31) / 32;
+  coarse_fill_upper<<<int(std::min<int64_t>(nt * nt, 65535)), kBlock, 0, s>>>(nc, C->A);
   CF_LAUNCHED();
   return CF_OK;
 }
@@ -4238,7 +4262,7 @@ int mf_mixed_solve(const char* who, int64_t n_nodes, int64_t n_elem, const int32
     if ((r2 = product32(C.u32, C.r2, 4, 0.0f, 0.0f, L.state))) return r2;   // r2 = r - K u
     return coarse_apply(C.r2, true);                                         // u += C r2
   };
-  return fcg_cheb_core<true>(outer, gg, power, cheb, 0.0, kChebF32, n, b, x, fixed, tol, maxit,
+  return fcg_cheb_core<true>(outer, gg, power, cheb, 0.0, kCheb2L, n, b, x, fixed, tol, maxit,
                              L, stats_host, s, pre2l, C.u32, 2);
 }
 
