@@ -176,6 +176,34 @@ def test_matfree_twolevel_meets_fp64_residual(gpu, dims):
     assert err < 1e-8
 
 
+def test_twolevel_jittered_mesh_body_load(gpu):
+    """Two-level solver on an irregular Tet10 mesh (interior and boundary
+    nodes of a 36x6x6 Kuhn box jittered by up to 0.2 h, 7,776 elements, six
+    aggregates) under a body load, clamped at x = 0, against SuperLU."""
+    from cloudfem_amd import solver
+    from cloudfem_amd.physics import elastic_tet10
+    m4 = om.box_mesh(36, 6, 6, xlim=(0.0, 1.0), ylim=(0.0, 0.2), zlim=(0.0, 0.2))
+    rng = np.random.default_rng(7)
+    h = np.array([1.0 / 36, 0.2 / 6, 0.2 / 6])
+    nodes = m4.nodes + rng.uniform(-0.2, 0.2, size=m4.nodes.shape) * h
+    nodes[m4.boundaries["xmin"], 0] = 0.0   # keep the clamped face planar
+    m = om.tet4_to_tet10(om.OracleTetMesh(nodes, m4.elements, m4.boundaries))
+    body = (0.0, 0.0, -7850.0 * 9.81)
+    bc = {"xmin": (0.0, 0.0, 0.0)}
+    with solver.solver_options(operator="matfree", precision="mixed", precond="twolevel",
+                               tol=1e-10):
+        u = np.stack(elastic_tet10.solve_elasticity_tet10(m, 210e9, 0.3, bc=bc, body=body),
+                     axis=1).ravel()
+    st = solver.last_stats.value
+    A0, rhs0 = oel.assemble_elasticity_tet10(m, 210e9, 0.3, body=body)
+    u0 = oel.apply_dirichlet_vec3(A0, rhs0, m, bc)
+    err = rel_err(u, u0)
+    print(f"two-level jittered: {st['iterations']} FP64 iterations, rel. residual "
+          f"{st['rel_residual']:.2e}, error vs SuperLU {err:.2e}")
+    assert st["rel_residual"] <= 1e-9
+    assert err < 1e-8
+
+
 def test_twolevel_fewer_iterations_than_chebyshev(gpu):
     """The coarse correction must pay: on the 40x5x5 cantilever the two-level
     solver needs fewer outer iterations than the Chebyshev-only solver."""
