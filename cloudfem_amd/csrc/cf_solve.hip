@@ -24,6 +24,7 @@
 // matrix ("reliable updates", Clark et al., Comput. Phys. Commun. 181 (2010)
 // 1517), so the stopping test is applied to the true FP64 residual.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -82,10 +83,10 @@ __device__ __forceinline__ float group_sum(float v) {
   return v;
 }
 
-// Block sum of NV values; result valid in every thread.
-template <int NV>
+// Block sum of NV values over W waves; result valid in every thread.
+template <int NV, int W = kWaves>
 __device__ void block_sum(double (&v)[NV]) {
-  __shared__ double sh[NV][kWaves];
+  __shared__ double sh[NV][W];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
@@ -98,7 +99,7 @@ __device__ void block_sum(double (&v)[NV]) {
   for (int k = 0; k < NV; ++k) {
     double s = 0.0;
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) s += sh[k][w];
+    for (int w = 0; w < W; ++w) s += sh[k][w];
     v[k] = s;
   }
 }
@@ -1214,20 +1215,42 @@ mf_gather_runs(int64_t n_nodes, const int32_t* __restrict__ nro, const uint32_t*
                double* __restrict__ y, PcgState* state, double* __restrict__ partials) {
   if (state && state->halt) return;
   double dot = 0.0;
-  const int64_t i = tile_index(blockIdx.x) * kBlock + threadIdx.x;
-  if (i < n_nodes) {
-    double o[3] = {0.0, 0.0, 0.0};
-    for (int32_t j = nro[i]; j < nro[i + 1]; ++j) {
-      const int64_t q = nq[j];
+  // two nodes per thread, their partial chains interleaved (as in
+  // mf_gather_cheb32)
+  const int64_t i0 = tile_index(blockIdx.x) * (2 * kBlock) + threadIdx.x;
+  double acc[2][3] = {};
+  int32_t jb[2] = {0, 0}, je[2] = {0, 0};
 #pragma unroll
-      for (int c = 0; c < 3; ++c) o[c] += part[c * n_runs + q];
+  for (int h = 0; h < 2; ++h) {
+    const int64_t i = i0 + h * kBlock;
+    if (i >= n_nodes) continue;
+    jb[h] = nro[i];
+    je[h] = nro[i + 1];
+  }
+  for (int32_t j = 0;; ++j) {
+    const bool m0 = jb[0] + j < je[0], m1 = jb[1] + j < je[1];
+    if (!m0 && !m1) break;
+    if (m0) {
+      const int64_t q = nq[jb[0] + j];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) acc[0][c] += part[c * n_runs + q];
     }
+    if (m1) {
+      const int64_t q = nq[jb[1] + j];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) acc[1][c] += part[c * n_runs + q];
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int64_t i = i0 + h * kBlock;
+    if (i >= n_nodes) continue;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const int64_t d = 3 * i + c;
-      if (fixed && fixed[d]) o[c] = 0.0;
-      y[d] = o[c];
-      if (state) dot += x[d] * o[c];
+      const double o = fixed && fixed[d] ? 0.0 : acc[h][c];
+      y[d] = o;
+      if (state) dot += x[d] * o;
     }
   }
   if (!state) return;
@@ -1573,6 +1596,11 @@ mf_t10_elem32(int64_t m, const int32_t* __restrict__ conn, const float* __restri
 // node (S, w, zc, zp read, zn written) instead of the seven of the
 // residual-recursive form.  zn may alias zp: each entry is read and then
 // written by the same thread.
+// Each thread takes two nodes (i and i + kBlock of its block's 2 kBlock), so
+// one launch fits in a single round of resident waves (one node per thread
+// needs 1.03 rounds at 539,601 nodes: 14.8 vs 13.2 us), and it loads the
+// node vectors before the dependent nro -> nq -> partial chain, whose loads
+// it interleaves over the two nodes.
 __global__ void __launch_bounds__(kBlock)
 mf_gather_cheb32(int64_t n_nodes, const int32_t* __restrict__ nro,
                  const uint32_t* __restrict__ nq, const float* __restrict__ part,
@@ -1580,32 +1608,64 @@ mf_gather_cheb32(int64_t n_nodes, const int32_t* __restrict__ nro,
                  const float* zp, float* zn, float* __restrict__ w, float cd, float cw,
                  const float* __restrict__ rin, const PcgState* state) {
   if (state && state->halt) return;
-  const int64_t i = tile_index(blockIdx.x) * kBlock + threadIdx.x;
-  if (i >= n_nodes) return;
-  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-  for (int32_t j = nro[i]; j < nro[i + 1]; ++j) {
-    const float* v = part + 3 * int64_t(nq[j]);
-    a0 += v[0];
-    a1 += v[1];
-    a2 += v[2];
-  }
-  const float o[3] = {a0, a1, a2};
+  const int64_t i0 = tile_index(blockIdx.x) * (2 * kBlock) + threadIdx.x;
+  // per node: S, then zc and w (modes 1, 2) or rin (modes 3, 4), zp (mode 2)
+  float acc[2][3] = {}, sv[2][3] = {}, zv[2][3] = {}, wv[2][3] = {}, pv[2][3] = {};
+  int32_t jb[2] = {0, 0}, je[2] = {0, 0};
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const int64_t k = 3 * i + c;
-    const float s = sc[k];
-    if (mode == 0) {
-      zn[k] = s * o[c];
-    } else if (mode == 3) {
-      const float wk = s * s * (rin[k] - o[c]);
-      w[k] = wk;
-      zn[k] = cw * wk;
-    } else if (mode == 4) {
-      zn[k] = rin[k] - o[c];
-    } else {
-      const float zk = zc[k];
-      const float dz = mode == 1 ? zk : zk - zp[k];
-      zn[k] = zk + cd * dz + cw * (w[k] - s * s * o[c]);
+  for (int h = 0; h < 2; ++h) {
+    const int64_t i = i0 + h * kBlock;
+    if (i >= n_nodes) continue;
+    jb[h] = nro[i];
+    je[h] = nro[i + 1];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t k = 3 * i + c;
+      if (mode != 4) sv[h][c] = sc[k];
+      if (mode == 1 || mode == 2) {
+        zv[h][c] = zc[k];
+        wv[h][c] = w[k];
+      } else if (mode >= 3) {
+        wv[h][c] = rin[k];
+      }
+      if (mode == 2) pv[h][c] = zp[k];
+    }
+  }
+  for (int32_t j = 0;; ++j) {
+    const bool m0 = jb[0] + j < je[0], m1 = jb[1] + j < je[1];
+    if (!m0 && !m1) break;
+    if (m0) {
+      const float* v = part + 3 * int64_t(nq[jb[0] + j]);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) acc[0][c] += v[c];
+    }
+    if (m1) {
+      const float* v = part + 3 * int64_t(nq[jb[1] + j]);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) acc[1][c] += v[c];
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int64_t i = i0 + h * kBlock;
+    if (i >= n_nodes) continue;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t k = 3 * i + c;
+      const float s = sv[h][c], o = acc[h][c];
+      if (mode == 0) {
+        zn[k] = s * o;
+      } else if (mode == 3) {
+        const float wk = s * s * (wv[h][c] - o);
+        w[k] = wk;
+        zn[k] = cw * wk;
+      } else if (mode == 4) {
+        zn[k] = wv[h][c] - o;
+      } else {
+        const float zk = zv[h][c];
+        const float dz = mode == 1 ? zk : zk - pv[h][c];
+        zn[k] = zk + cd * dz + cw * (wv[h][c] - s * s * o);
+      }
     }
   }
 }
@@ -3114,8 +3174,11 @@ int mf_runs_build(int64_t n_nodes, int64_t n_elem, const int32_t* offs, const ui
   return CF_OK;
 }
 
-// one node per thread, kBlock nodes per block
-int64_t mf_gather_grid(int64_t n_nodes) { return tile_grid((n_nodes + kBlock - 1) / kBlock); }
+// blocks of the run-sum node kernels (mf_gather_runs, mf_gather_cheb32):
+// two nodes per thread, 2 kBlock nodes per block
+int64_t mf_gather_grid(int64_t n_nodes) {
+  return tile_grid((n_nodes + 2 * kBlock - 1) / (2 * kBlock));
+}
 
 // fe_doubles: element forces (30 per element) or, for the chunked plan,
 // shared-node partials (3 per shared pair); parts: most block partials a
@@ -3823,9 +3886,13 @@ coarse_fill_upper(int64_t nc, double* __restrict__ A) {
 }
 
 // rc[(g, m)] = sum over the nodes of g of mode m . r (r FP64 or FP32); one
-// workgroup per aggregate, fixed node split and reduction tree.
+// workgroup of kRestrictBlock threads per aggregate (a few hundred
+// aggregates: wide groups keep each thread's chain of gathers short), fixed
+// node split and reduction tree.
+constexpr int kRestrictBlock = 1024;
+
 template <class RT>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kRestrictBlock)
 coarse_restrict(int64_t n_agg, const int32_t* __restrict__ agg_off,
                 const uint32_t* __restrict__ agg_nodes, const double* __restrict__ frame,
                 const double* __restrict__ x, const double* __restrict__ y,
@@ -3847,7 +3914,7 @@ coarse_restrict(int64_t n_agg, const int32_t* __restrict__ agg_off,
         acc[mm] += o[0] * r0 + o[1] * r1 + o[2] * r2;
       }
     }
-    block_sum<6>(acc);
+    block_sum<6, kRestrictBlock / 64>(acc);
     if (threadIdx.x == 0)
 #pragma unroll
       for (int q = 0; q < 6; ++q) rc[6 * g + q] = acc[q];
@@ -3904,12 +3971,248 @@ __global__ void coarse_prolong(int64_t n_nodes, const int32_t* __restrict__ agg,
   }
 }
 
+// ------------------------------------------ banded coarse factorisation
+// Aggregates are numbered in the solver's element order, so two aggregates
+// that share an element lie a few ids apart and the coarse matrix is block
+// banded (half-bandwidth 6 * 2 + 5 = 17 at 250 aggregates for a 160x20x20
+// beam).  band_cholesky factors it in one workgroup, band_inverse forms the
+// dense inverse column by column; together they replace rocSOLVER's dense
+// potrf + potri (6.4 ms at 1,500 unknowns, mostly panel kernels and launch
+// gaps) whenever the half-bandwidth is at most kBandW - 2.
+constexpr int kBandW = 64;         // LDS window of the factorisation, ring of the inverse
+constexpr int kBandChunk = 32;     // band rows fetched per chunk of factorisation steps
+constexpr int kBandThreads = 256;  // threads of band_cholesky
+constexpr int kBandNV = kBandChunk * kBandW / kBandThreads;
+
+// v <- rows r0 .. r0 + kBandChunk - 1 of the band, v[q] = A(r, r - bw + t) for
+// the (r, t) of slot threadIdx.x + q kBandThreads (zero outside the band)
+__device__ __forceinline__ void band_fetch(int nc, int bw, const double* __restrict__ A,
+                                           int64_t r0, double (&v)[kBandNV]) {
+#pragma unroll
+  for (int q = 0; q < kBandNV; ++q) {
+    const int idx = int(threadIdx.x) + q * kBandThreads;
+    const int64_t r = r0 + idx / kBandW;
+    const int t = idx % kBandW;
+    v[q] = (t <= bw && r < nc) ? A[r + (r - bw + t) * int64_t(nc)] : 0.0;
+  }
+}
+
+// Right-looking Cholesky A = L L^T of the SPD matrix A (column-major nc x nc,
+// lower triangle read, zero beyond half-bandwidth bw <= kBandW - 2), one
+// column per step: column k is scaled by its pivot, then the trailing
+// bw x bw triangle is updated.  The active rows live in an LDS window
+// (entry (i, j) at win[i % kBandW][j % kBandW]); row k + 1 + bw enters at
+// step k from a staging buffer that is filled one chunk of kBandChunk steps
+// ahead, so the HBM latency of the band rows stays hidden.  Writes the
+// factor by columns (Lc[k kBandW + t] = L(k + t, k)) and by rows
+// (Lr[i kBandW + t] = L(i, i - t)), dinv[k] = 1 / L(k, k), and info = 1 +
+// the first non-positive pivot (0 if none).
+__global__ void __launch_bounds__(kBandThreads)
+band_cholesky(int nc, int bw, const double* __restrict__ A, double* __restrict__ Lc,
+              double* __restrict__ Lr, double* __restrict__ dinv, int32_t* __restrict__ info) {
+  constexpr int W = kBandW;
+  __shared__ double win[W][W];
+  __shared__ double stg[kBandChunk][W];
+  // the trailing triangle's entries (a, b), b <= a < bw, as a | b << 8
+  __shared__ uint16_t pab[(W - 2) * (W - 1) / 2];
+  const int th = threadIdx.x;
+  const int np = bw * (bw + 1) / 2;
+  for (int p = th; p < np; p += kBandThreads) {
+    int a = int((sqrtf(8.0f * float(p) + 1.0f) - 1.0f) * 0.5f);
+    while ((a + 1) * (a + 2) / 2 <= p) ++a;
+    while (a * (a + 1) / 2 > p) --a;
+    pab[p] = uint16_t(a | ((p - a * (a + 1) / 2) << 8));
+  }
+  for (int idx = th; idx < W * W; idx += kBandThreads) {
+    const int i = idx / W, j = idx % W;
+    if (i <= bw && i < nc && j <= i) win[i][j] = A[i + int64_t(j) * nc];
+  }
+  double v[kBandNV];
+  band_fetch(nc, bw, A, 1 + bw, v);
+#pragma unroll
+  for (int q = 0; q < kBandNV; ++q) {
+    const int idx = th + q * kBandThreads;
+    stg[idx / W][idx % W] = v[q];
+  }
+  __syncthreads();
+  band_fetch(nc, bw, A, 1 + bw + kBandChunk, v);
+  int32_t bad = 0;
+  for (int k0 = 0; k0 < nc; k0 += kBandChunk) {
+    for (int s = 0; s < kBandChunk && k0 + s < nc; ++s) {
+      const int k = k0 + s, ks = k % W;
+      double d = win[ks][ks];
+      if (!(d > 0.0)) {
+        if (bad == 0) bad = k + 1;
+        d = 1.0;
+      }
+      const double l = sqrt(d), rl = 1.0 / l;
+      if (th <= bw) {
+        const int i = k + th;
+        if (i < nc) {
+          const double lik = th == 0 ? l : win[i % W][ks] * rl;
+          if (th > 0) win[i % W][ks] = lik;
+          Lc[int64_t(k) * W + th] = lik;
+          Lr[int64_t(i) * W + th] = lik;
+        } else {
+          Lc[int64_t(k) * W + th] = 0.0;
+        }
+        if (th == 0) dinv[k] = rl;
+      }
+      __syncthreads();
+      for (int p = th; p < np; p += kBandThreads) {
+        const int a = pab[p] & 255, b = pab[p] >> 8;  // entry (k + 1 + a, k + 1 + b)
+        if (k + 1 + a < nc) {
+          const int i = (k + 1 + a) % W, j = (k + 1 + b) % W;
+          win[i][j] -= win[i][ks] * win[j][ks];
+        }
+      }
+      const int r = k + 1 + bw;
+      if (r < nc && th <= bw) win[r % W][(r - bw + th) % W] = stg[s][th];
+      __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < kBandNV; ++q) {
+      const int idx = th + q * kBandThreads;
+      stg[idx / W][idx % W] = v[q];
+    }
+    __syncthreads();
+    band_fetch(nc, bw, A, 1 + bw + int64_t(k0) + 2 * kBandChunk, v);
+  }
+  if (th == 0) *info = bad;
+}
+
+// Dense inverse from the band factor, one column per lane: lane j solves
+// L y = e_j forward over rows j0 .. nc - 1 (j0 = first column of its wave,
+// y = 0 above row j) and L^T x = y backward over rows nc - 1 .. 0, and
+// stores Ainv[i nc + j] = x_i (y_i first, read back by the backward pass
+// kBandPf rows ahead).  The lane's last kBandW values stay in an LDS ring
+// (lane-major, no bank conflicts); the factor entries are wave-uniform
+// loads.  Fixed summation order: bitwise reproducible.
+constexpr int kBandPf = 4;
+
+__global__ void __launch_bounds__(64)
+band_inverse(int nc, int bw, const double* __restrict__ Lc, const double* __restrict__ Lr,
+             const double* __restrict__ dinv, double* __restrict__ Ainv) {
+  constexpr int W = kBandW;
+  __shared__ double ring[W][64];
+  const int lane = threadIdx.x;
+  const int j0 = blockIdx.x * 64, j = j0 + lane;
+  const bool live = j < nc;
+  for (int i = j0; i < nc; ++i) {
+    const double* lr = Lr + int64_t(i) * W;
+    const int tmax = bw < i - j0 ? bw : i - j0;
+    double s0 = i == j ? 1.0 : 0.0, s1 = 0.0;
+    int t = 1;
+    for (; t + 1 <= tmax; t += 2) {
+      s0 -= lr[t] * ring[(i - t) % W][lane];
+      s1 -= lr[t + 1] * ring[(i - t - 1) % W][lane];
+    }
+    if (t <= tmax) s0 -= lr[t] * ring[(i - t) % W][lane];
+    const double y = (s0 + s1) * dinv[i];
+    ring[i % W][lane] = y;
+    if (live) Ainv[int64_t(i) * nc + j] = y;
+  }
+  // y_i for the next kBandPf rows of the backward pass (0 above row j0)
+  double pf[kBandPf];
+#pragma unroll
+  for (int u = 0; u < kBandPf; ++u) {
+    const int i = nc - 1 - u;
+    pf[u] = (live && i >= j0) ? Ainv[int64_t(i) * nc + j] : 0.0;
+  }
+  for (int ib = nc - 1; ib >= 0; ib -= kBandPf) {
+#pragma unroll
+    for (int u = 0; u < kBandPf; ++u) {
+      const int i = ib - u;
+      if (i < 0) break;
+      double s0 = pf[u], s1 = 0.0;
+      const int ip = i - kBandPf;
+      pf[u] = (live && ip >= j0) ? Ainv[int64_t(ip) * nc + j] : 0.0;
+      const double* lc = Lc + int64_t(i) * W;
+      const int tmax = bw < nc - 1 - i ? bw : nc - 1 - i;
+      int t = 1;
+      for (; t + 1 <= tmax; t += 2) {
+        s0 -= lc[t] * ring[(i + t) % W][lane];
+        s1 -= lc[t + 1] * ring[(i + t + 1) % W][lane];
+      }
+      if (t <= tmax) s0 -= lc[t] * ring[(i + t) % W][lane];
+      const double x = (s0 + s1) * dinv[i];
+      ring[i % W][lane] = x;
+      if (live) Ainv[int64_t(i) * nc + j] = x;
+    }
+  }
+}
+
+// band_inverse for half-bandwidths below BWM, with the lane's last BWM - 1
+// values in registers instead of LDS: row r sits in slot (r - j0) mod BWM
+// during the forward pass and (nc - 1 - r) mod BWM during the backward pass,
+// and the step loop is unrolled BWM times, so every slot index is a
+// constant.  The factor arrays are cleared before the factorisation, so the
+// entries beyond the band (and past the last row) are zero and every step
+// runs the same BWM - 1 multiply-adds with no bounds.
+template <int BWM>
+__global__ void __launch_bounds__(64)
+band_inverse_reg(int nc, const double* __restrict__ Lc, const double* __restrict__ Lr,
+                 const double* __restrict__ dinv, double* __restrict__ Ainv) {
+  static_assert(BWM % kBandPf == 0 && BWM <= kBandW, "slot rotation");
+  const int lane = threadIdx.x;
+  const int j0 = blockIdx.x * 64, j = j0 + lane;
+  const bool live = j < nc;
+  double h[BWM];
+#pragma unroll
+  for (int t = 0; t < BWM; ++t) h[t] = 0.0;
+  for (int ib = j0; ib < nc; ib += BWM) {
+#pragma unroll
+    for (int u = 0; u < BWM; ++u) {
+      const int i = ib + u;
+      if (i >= nc) break;
+      const double* lr = Lr + int64_t(i) * kBandW;
+      double s0 = i == j ? 1.0 : 0.0, s1 = 0.0;
+#pragma unroll
+      for (int t = 1; t < BWM; ++t) {
+        const double v = lr[t] * h[(u - t + BWM) % BWM];
+        if (t & 1) s0 -= v; else s1 -= v;
+      }
+      const double y = (s0 + s1) * dinv[i];
+      h[u] = y;
+      if (live) Ainv[int64_t(i) * nc + j] = y;
+    }
+  }
+  double pf[kBandPf];
+#pragma unroll
+  for (int u = 0; u < kBandPf; ++u) {
+    const int i = nc - 1 - u;
+    pf[u] = (live && i >= j0) ? Ainv[int64_t(i) * nc + j] : 0.0;
+  }
+  for (int ib = 0; ib < nc; ib += BWM) {
+#pragma unroll
+    for (int u = 0; u < BWM; ++u) {
+      const int i = nc - 1 - (ib + u);
+      if (i < 0) break;
+      const int q = u % kBandPf;
+      double s0 = pf[q], s1 = 0.0;
+      const int ip = i - kBandPf;
+      pf[q] = (live && ip >= j0) ? Ainv[int64_t(ip) * nc + j] : 0.0;
+      const double* lc = Lc + int64_t(i) * kBandW;
+#pragma unroll
+      for (int t = 1; t < BWM; ++t) {
+        const double v = lc[t] * h[(u - t + BWM) % BWM];
+        if (t & 1) s0 -= v; else s1 -= v;
+      }
+      const double x = (s0 + s1) * dinv[i];
+      h[u] = x;
+      if (live) Ainv[int64_t(i) * nc + j] = x;
+    }
+  }
+}
+constexpr int kBandReg = 24;  // band_inverse_reg<kBandReg> serves half-bandwidths below it
+
 }  // namespace
 
 namespace {
 
 struct Coarse {
   int64_t n_groups = 0, n_agg = 0, nc = 0;  // host
+  int64_t bw = 0;                            // half-bandwidth of the coarse matrix
   int n_col = 0;
   int32_t* graw;
   uint32_t* used;
@@ -3967,7 +4270,7 @@ void coarse_layout(int64_t n_nodes, int64_t n_elem, cf::Carve& c, Coarse* C) {
   C->u32 = c.take<float>(n);
   C->r32 = c.take<float>(n);
   C->r2 = c.take<float>(n);
-  C->info = c.take<int32_t>(1);
+  C->info = c.take<int32_t>(2);
   size_t b1 = 0, b2 = 0, b3 = 0, b4 = 0;
   rocprim::radix_sort_pairs(nullptr, b1, (uint32_t*)nullptr, (uint32_t*)nullptr,
                             (uint32_t*)nullptr, (uint32_t*)nullptr, size_t(n_nodes), 0, 32);
@@ -4059,12 +4362,14 @@ int coarse_setup(const MfArgs& a, const MfRuns& R, const double* xs, const doubl
   std::vector<std::vector<int32_t>> adj(n_agg);
   for (uint32_t g = 0; g < n_agg; ++g) adj[g].push_back(int32_t(g));
   uint64_t last = ~uint64_t(0);
+  int64_t agg_band = 0;  // largest id distance of two coupled aggregates
   for (uint64_t key : pk) {
     if (key == last) continue;
     last = key;
     const int32_t g = int32_t(key / n_agg), h = int32_t(key % n_agg);
     adj[g].push_back(h);
     adj[h].push_back(g);
+    agg_band = std::max<int64_t>(agg_band, g > h ? g - h : h - g);
   }
   std::vector<int32_t> color(n_agg, -1);
   std::vector<uint32_t> stamp(kMaxColors, 0);
@@ -4109,28 +4414,97 @@ int coarse_setup(const MfArgs& a, const MfRuns& R, const double* xs, const doubl
   CF_LAUNCHED();
   coarse_regularize<<<gc, 64, 0, s>>>(nc, C->A, C->zc);
   CF_LAUNCHED();
-  // factor and invert (rocSOLVER)
-  rocblas_handle hnd = solver_handle();
-  CF_REQUIRE(hnd != nullptr, "two-level: no rocBLAS handle");
-  CF_REQUIRE(rocblas_set_stream(hnd, s) == rocblas_status_success, "two-level: rocblas_set_stream");
-  CF_REQUIRE(rocsolver_dpotrf(hnd, rocblas_fill_lower, rocblas_int(nc), C->A, rocblas_int(nc),
-                              C->info) == rocblas_status_success,
-             "two-level: rocsolver_dpotrf failed");
-  int32_t info = 0;
-  CF_HIP(hipMemcpyAsync(&info, C->info, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  CF_HIP(hipStreamSynchronize(s));
-  CF_REQUIRE(info == 0, "two-level: coarse matrix not positive definite (potrf info %d)", info);
-  CF_REQUIRE(rocsolver_dpotri(hnd, rocblas_fill_lower, rocblas_int(nc), C->A, rocblas_int(nc),
-                              C->info) == rocblas_status_success,
-             "two-level: rocsolver_dpotri failed");
-  CF_HIP(hipMemcpyAsync(&info, C->info, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  CF_HIP(hipStreamSynchronize(s));
-  CF_REQUIRE(info == 0, "two-level: coarse inverse failed (potri info %d)", info);
+  C->bw = 6 * agg_band + 5;  // scalar half-bandwidth
+  return CF_OK;
+}
+
+// Factor and invert the coarse matrix on stream s, with no host
+// synchronisation (coarse_factor_check reads the pivot flags later).
+// Narrow bands use the banded kernels, wide ones rocSOLVER's dense
+// potrf/potri.  The environment variable CF_COARSE_FACTOR=dense (rocSOLVER)
+// or =ring (LDS-ring inverse) pins a path, so that each one can be checked
+// on narrow-band problems.
+int coarse_factor(Coarse* C, int64_t n, hipStream_t s) {
+  const int64_t nc = C->nc, bw = C->bw;
+  const char* pin = std::getenv("CF_COARSE_FACTOR");
+  const bool pin_dense = pin && std::strcmp(pin, "dense") == 0;
+  const bool pin_ring = pin && std::strcmp(pin, "ring") == 0;
+  CF_HIP(hipMemsetAsync(C->info, 0, 2 * sizeof(int32_t), s));
+  if (!pin_dense && bw <= kBandW - 2 && 2 * nc * kBandW + nc <= 12 * n) {
+    // banded factor and inverse; the probe vectors' space holds the factor
+    double* Lc = C->probe;
+    double* Lr = Lc + nc * kBandW;
+    double* dinv = Lr + nc * kBandW;
+    CF_HIP(hipMemsetAsync(Lc, 0, sizeof(double) * 2 * nc * kBandW, s));
+    band_cholesky<<<1, kBandThreads, 0, s>>>(int(nc), int(bw), C->A, Lc, Lr, dinv, C->info);
+    CF_LAUNCHED();
+    const int gi = int((nc + 63) / 64);
+    if (bw < kBandReg && !pin_ring)
+      band_inverse_reg<kBandReg><<<gi, 64, 0, s>>>(int(nc), Lc, Lr, dinv, C->A);
+    else
+      band_inverse<<<gi, 64, 0, s>>>(int(nc), int(bw), Lc, Lr, dinv, C->A);
+    CF_LAUNCHED();
+  } else {
+    rocblas_handle hnd = solver_handle();
+    CF_REQUIRE(hnd != nullptr, "two-level: no rocBLAS handle");
+    CF_REQUIRE(rocblas_set_stream(hnd, s) == rocblas_status_success,
+               "two-level: rocblas_set_stream");
+    CF_REQUIRE(rocsolver_dpotrf(hnd, rocblas_fill_lower, rocblas_int(nc), C->A, rocblas_int(nc),
+                                C->info) == rocblas_status_success,
+               "two-level: rocsolver_dpotrf failed");
+    CF_REQUIRE(rocsolver_dpotri(hnd, rocblas_fill_lower, rocblas_int(nc), C->A, rocblas_int(nc),
+                                C->info + 1) == rocblas_status_success,
+               "two-level: rocsolver_dpotri failed");
+  }
+  // exact symmetry: the lower triangle (column-major) mirrored onto the upper
   const int64_t nt = (nc + 31) / 32;
   coarse_fill_upper<<<int(std::min<int64_t>(nt * nt, 65535)), kBlock, 0, s>>>(nc, C->A);
   CF_LAUNCHED();
   return CF_OK;
 }
+
+// Pivot flags of coarse_factor (waits for stream s).
+int coarse_factor_check(const Coarse& C, hipStream_t s) {
+  int32_t info[2] = {0, 0};
+  CF_HIP(hipMemcpyAsync(info, C.info, sizeof(info), hipMemcpyDeviceToHost, s));
+  CF_HIP(hipStreamSynchronize(s));
+  CF_REQUIRE(info[0] == 0, "two-level: coarse matrix not positive definite (pivot %d)", info[0]);
+  CF_REQUIRE(info[1] == 0, "two-level: coarse inverse failed (potri info %d)", info[1]);
+  return CF_OK;
+}
+
+// Side stream (default priority, non-blocking) and two events per device
+// for the coarse factorisation, which runs beside the power iteration;
+// created on first use and kept until the process ends.  Solves that
+// overlap on one device share them.
+struct Side {
+  hipStream_t s = nullptr;
+  hipEvent_t ready = nullptr, done = nullptr;
+};
+
+const Side* side_stream() {
+  static Side side[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  Side& S = side[dev];
+  if (!S.s && hipStreamCreateWithFlags(&S.s, hipStreamNonBlocking) != hipSuccess) S.s = nullptr;
+  if (S.s && !S.ready && hipEventCreateWithFlags(&S.ready, hipEventDisableTiming) != hipSuccess)
+    S.ready = nullptr;
+  if (S.ready && !S.done && hipEventCreateWithFlags(&S.done, hipEventDisableTiming) != hipSuccess)
+    S.done = nullptr;
+  return S.done ? &S : nullptr;
+}
+
+// Makes stream s wait for the side stream's work when it goes out of scope,
+// on every return path, so that the caller's stream-ordered frees of the
+// workspace come after it.
+struct SideJoin {
+  hipStream_t s = nullptr;
+  hipEvent_t done = nullptr;
+  ~SideJoin() {
+    if (done) (void)hipStreamWaitEvent(s, done, 0);
+  }
+};
 
 }  // namespace
 
@@ -4186,6 +4560,25 @@ int mf_mixed_solve(const char* who, int64_t n_nodes, int64_t n_elem, const int32
   CF_LAUNCHED();
   if ((rc = mf_runs_build(n_nodes, n_elem, offs, perm, &M.R, s))) return rc;
   if (two_level && (rc = coarse_setup(a, M.R, xs, ys, zs, fixed, M.fe, &C, s))) return rc;
+  // the coarse factorisation runs on a side stream beside the power
+  // iteration; the first two-level application waits for it
+  const Side* side = two_level ? side_stream() : nullptr;
+  const hipStream_t sf = side ? side->s : s;
+  SideJoin join;
+  if (two_level) {
+    if (side) {
+      CF_HIP(hipEventRecord(side->ready, s));
+      CF_HIP(hipStreamWaitEvent(sf, side->ready, 0));
+    }
+    rc = coarse_factor(&C, n, sf);
+    if (side) {
+      CF_HIP(hipEventRecord(side->done, sf));
+      join.s = s;
+      join.done = side->done;
+    }
+    if (rc) return rc;
+  }
+  bool factor_joined = false;
   const int ge = cf::stride_grid(n_elem, kBlock);
   const int gg = int(mf_gather_grid(n_nodes));
   auto outer = [&](const double* xin, double* y, PcgState* state) {
@@ -4206,9 +4599,9 @@ int mf_mixed_solve(const char* who, int64_t n_nodes, int64_t n_elem, const int32
                                                 M.s32, din, M.R.rank, M.R.toff, M.R.tstart, M.fe32,
                                                 state);
     CF_LAUNCHED();
-    mf_gather_cheb32<<<gg, kBlock, 0, s>>>(n_nodes, M.R.nro, M.R.nq, M.fe32, M.s32, mode, din, dout,
-                                           dout, L.rh, cdk, cwk, two_level ? C.r32 : nullptr,
-                                           state);
+    mf_gather_cheb32<<<gg, kBlock, 0, s>>>(n_nodes, M.R.nro, M.R.nq, M.fe32, M.s32, mode, din,
+                                           dout, dout, L.rh, cdk, cwk,
+                                           two_level ? C.r32 : nullptr, state);
     CF_LAUNCHED();
     return CF_OK;
   };
@@ -4224,8 +4617,8 @@ int mf_mixed_solve(const char* who, int64_t n_nodes, int64_t n_elem, const int32
   const int ga = int(std::min<int64_t>(C.n_agg, 65535));
   const int gc = int(std::min<int64_t>((C.nc + kWaves - 1) / kWaves, 65535));
   auto coarse_apply = [&](auto* rin, bool add) {
-    coarse_restrict<<<ga, kBlock, 0, s>>>(C.n_agg, C.agg_off, C.agg_nodes, C.frame, xs, ys, zs,
-                                          rin, C.rc, L.state);
+    coarse_restrict<<<ga, kRestrictBlock, 0, s>>>(C.n_agg, C.agg_off, C.agg_nodes, C.frame, xs,
+                                                  ys, zs, rin, C.rc, L.state);
     CF_LAUNCHED();
     coarse_gemv<<<gc, kBlock, 0, s>>>(C.nc, C.A, C.rc, C.zc, L.state);
     CF_LAUNCHED();
@@ -4241,6 +4634,12 @@ int mf_mixed_solve(const char* who, int64_t n_nodes, int64_t n_elem, const int32
   float* D[2] = {L.ph, L.Aph};
   auto pre2l = [&](int apply_step, int slot, int sample, const float* cd, const float* cw,
                    float inv_theta, int steps) {
+    if (!factor_joined) {
+      factor_joined = true;
+      int r0;
+      if ((r0 = coarse_factor_check(C, sf))) return r0;
+      if (side) CF_HIP(hipStreamWaitEvent(s, side->done, 0));
+    }
     if (apply_step)
       fcg_update2l<true><<<GV, kBlock, 0, s>>>(n, x, L.r, L.p, L.Ap, C.r32, L.state, L.partials,
                                                gg, L.partials_b, slot);

@@ -147,7 +147,8 @@ def test_matfree_twolevel_meets_fp64_residual(gpu, dims):
     """Two-level mixed solver (rigid-body coarse correction around the FP32
     matrix-free Chebyshev polynomial): FP64 residual bar (3x the FP64 floor
     where the bar is within rounding of it, as above) and the solution
-    against SuperLU.  8x2x2 has one aggregate, 40x5x5 twelve."""
+    against SuperLU.  8x2x2 has one aggregate, 40x5x5 four (groups of
+    1,536 elements)."""
     from cloudfem_amd import solver
     from cloudfem_amd.mesh import box_mesh_tet10
     from cloudfem_amd.physics import elastic_tet10
@@ -202,6 +203,40 @@ def test_twolevel_jittered_mesh_body_load(gpu):
           f"{st['rel_residual']:.2e}, error vs SuperLU {err:.2e}")
     assert st["rel_residual"] <= 1e-9
     assert err < 1e-8
+
+
+@pytest.mark.parametrize("dims", [(40, 5, 5), (80, 6, 6)])
+def test_twolevel_coarse_factor_paths(gpu, monkeypatch, dims):
+    """The three factorisations of the coarse matrix (banded Cholesky with
+    the register or the LDS-ring inverse, rocSOLVER's dense potrf/potri)
+    give the same solve: each meets the FP64 residual bar, and the solutions
+    agree with SuperLU.  80x6x6 has 23,040 elements, 15 aggregates."""
+    from cloudfem_amd import solver
+    from cloudfem_amd.mesh import box_mesh_tet10
+    from cloudfem_amd.physics import elastic_tet10
+    lims = dict(xlim=(0.0, 0.4), ylim=(0.0, 0.05), zlim=(0.0, 0.05))
+    trac = {"xmax": (0.0, 0.0, -1e6)}
+    bc = {"xmin": (0.0, 0.0, 0.0)}
+    m = box_mesh_tet10(*dims, **lims)
+    m0 = om.box_mesh_tet10(*dims, **lims)
+    A0, rhs0 = oel.assemble_elasticity_tet10(m0, 210e9, 0.3, traction=trac)
+    u0 = oel.apply_dirichlet_vec3(A0, rhs0, m0, bc)
+    its = {}
+    for path in ("auto", "ring", "dense"):
+        monkeypatch.setenv("CF_COARSE_FACTOR", path)
+        with solver.solver_options(operator="matfree", precision="mixed", precond="twolevel",
+                                   tol=1e-10):
+            u = np.stack(elastic_tet10.solve_elasticity_tet10(m, 210e9, 0.3, bc=bc,
+                                                              traction=trac),
+                         axis=1).ravel()
+        st = solver.last_stats.value
+        its[path] = st["iterations"]
+        err = rel_err(u, u0)
+        print(f"{dims} coarse factor {path}: {st['iterations']} iterations, rel. residual "
+              f"{st['rel_residual']:.2e}, error vs SuperLU {err:.2e}")
+        assert st["rel_residual"] <= 1e-9
+        assert err < 1e-8
+    assert max(its.values()) - min(its.values()) <= 2
 
 
 def test_twolevel_fewer_iterations_than_chebyshev(gpu):
