@@ -1594,8 +1594,10 @@ mf_t10_elem32(int64_t m, const int32_t* __restrict__ conn, const float* __restri
 //   zn = zc + cd (zc - zp) + cw (w - S^2 sum),
 // with zp = 0 in mode 1 (the first step).  Five FP32 vector streams per
 // node (S, w, zc, zp read, zn written) instead of the seven of the
-// residual-recursive form.  zn may alias zp: each entry is read and then
-// written by the same thread.
+// residual-recursive form.  Mode 5 is mode 2 for the last step of the
+// two-level form, which adds its iterate to zn (zn += ..., the coarse
+// iterate) instead of storing it.  zn may alias zp: each entry is read and
+// then written by the same thread.
 // Each thread takes two nodes (i and i + kBlock of its block's 2 kBlock), so
 // one launch fits in a single round of resident waves (one node per thread
 // needs 1.03 rounds at 539,601 nodes: 14.8 vs 13.2 us), and it loads the
@@ -1609,8 +1611,10 @@ mf_gather_cheb32(int64_t n_nodes, const int32_t* __restrict__ nro,
                  const float* __restrict__ rin, const PcgState* state) {
   if (state && state->halt) return;
   const int64_t i0 = tile_index(blockIdx.x) * (2 * kBlock) + threadIdx.x;
-  // per node: S, then zc and w (modes 1, 2) or rin (modes 3, 4), zp (mode 2)
-  float acc[2][3] = {}, sv[2][3] = {}, zv[2][3] = {}, wv[2][3] = {}, pv[2][3] = {};
+  // per node: S, then zc and w (modes 1, 2, 5) or rin (modes 3, 4), zp
+  // (modes 2, 5), the sum zn is added to (mode 5)
+  float acc[2][3] = {}, sv[2][3] = {}, zv[2][3] = {}, wv[2][3] = {}, pv[2][3] = {},
+        uv[2][3] = {};
   int32_t jb[2] = {0, 0}, je[2] = {0, 0};
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
@@ -1622,13 +1626,14 @@ mf_gather_cheb32(int64_t n_nodes, const int32_t* __restrict__ nro,
     for (int c = 0; c < 3; ++c) {
       const int64_t k = 3 * i + c;
       if (mode != 4) sv[h][c] = sc[k];
-      if (mode == 1 || mode == 2) {
+      if (mode == 3 || mode == 4) {
+        wv[h][c] = rin[k];
+      } else if (mode != 0) {
         zv[h][c] = zc[k];
         wv[h][c] = w[k];
-      } else if (mode >= 3) {
-        wv[h][c] = rin[k];
       }
-      if (mode == 2) pv[h][c] = zp[k];
+      if (mode == 2 || mode == 5) pv[h][c] = zp[k];
+      if (mode == 5) uv[h][c] = zn[k];
     }
   }
   for (int32_t j = 0;; ++j) {
@@ -1664,7 +1669,8 @@ mf_gather_cheb32(int64_t n_nodes, const int32_t* __restrict__ nro,
       } else {
         const float zk = zv[h][c];
         const float dz = mode == 1 ? zk : zk - pv[h][c];
-        zn[k] = zk + cd * dz + cw * (wv[h][c] - s * s * o);
+        const float znew = zk + cd * dz + cw * (wv[h][c] - s * s * o);
+        zn[k] = mode == 5 ? uv[h][c] + znew : znew;
       }
     }
   }
@@ -3922,19 +3928,37 @@ coarse_restrict(int64_t n_agg, const int32_t* __restrict__ agg_off,
   }
 }
 
-// zc = Ainv rc, Ainv symmetric (column i read as row i): one wave per entry
+// zc = Ainv rc with the FP32 copy of the symmetric inverse (column i read as
+// row i; half the bytes of the FP64 one, products summed in FP64): one wave
+// per entry, two entries per lane and load (nc = 6 x aggregates is even),
+// four loads in flight
 __global__ void __launch_bounds__(kBlock)
-coarse_gemv(int64_t nc, const double* __restrict__ Ainv, const double* __restrict__ rc,
+coarse_gemv(int64_t nc, const float* __restrict__ Ainv, const double* __restrict__ rc,
             double* __restrict__ zc, const PcgState* state) {
   if (state && state->halt) return;
   const int lane = threadIdx.x & 63;
+  const int64_t half = nc / 2;
   for (int64_t i = (blockIdx.x * int64_t(blockDim.x) + threadIdx.x) / 64; i < nc;
        i += int64_t(gridDim.x) * blockDim.x / 64) {
-    const double* col = Ainv + i * nc;
-    double acc = 0.0;
-    for (int64_t j = lane; j < nc; j += 64) acc += col[j] * rc[j];
-    acc = wave_sum(acc);
-    if (lane == 0) zc[i] = acc;
+    const float2* col = reinterpret_cast<const float2*>(Ainv + i * nc);
+    const double2* r2 = reinterpret_cast<const double2*>(rc);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    int64_t j = lane;
+    for (; j + 192 < half; j += 256) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float2 a = col[j + 64 * u];
+        const double2 r = r2[j + 64 * u];
+        acc[u] += double(a.x) * r.x + double(a.y) * r.y;
+      }
+    }
+    for (; j < half; j += 64) {
+      const float2 a = col[j];
+      const double2 r = r2[j];
+      acc[0] += double(a.x) * r.x + double(a.y) * r.y;
+    }
+    const double sum = wave_sum((acc[0] + acc[1]) + (acc[2] + acc[3]));
+    if (lane == 0) zc[i] = sum;
   }
 }
 
@@ -4231,6 +4255,7 @@ struct Coarse {
   int32_t* color;
   int32_t* nbc;        // n_agg x n_col: neighbour (or self) of each colour, -1
   double* A;           // coarse matrix, then its inverse (column-major nc x nc)
+  float* A32;          // FP32 copy of the symmetric inverse
   double* rc;
   double* zc;
   double* probe;       // 6 probe vectors, 6 products
@@ -4264,6 +4289,7 @@ void coarse_layout(int64_t n_nodes, int64_t n_elem, cf::Carve& c, Coarse* C) {
   C->color = c.take<int32_t>(G);
   C->nbc = c.take<int32_t>(G * kMaxColors);
   C->A = c.take<double>(ncmax * ncmax);
+  C->A32 = c.take<float>(ncmax * ncmax);
   C->rc = c.take<double>(ncmax);
   C->zc = c.take<double>(ncmax);
   C->probe = c.take<double>(12 * n);
@@ -4460,6 +4486,8 @@ int coarse_factor(Coarse* C, int64_t n, hipStream_t s) {
   const int64_t nt = (nc + 31) / 32;
   coarse_fill_upper<<<int(std::min<int64_t>(nt * nt, 65535)), kBlock, 0, s>>>(nc, C->A);
   CF_LAUNCHED();
+  to_f32<<<cf::stride_grid(nc * nc, kBlock), kBlock, 0, s>>>(nc * nc, C->A, C->A32);
+  CF_LAUNCHED();
   return CF_OK;
 }
 
@@ -4588,8 +4616,9 @@ int mf_mixed_solve(const char* who, int64_t n_nodes, int64_t n_elem, const int32
   // steps run on S-scaled vectors, so their element kernel gathers no S.
   // Step k reads z_k (din) and z_{k-1} (dout, none at k = 1) and overwrites
   // dout with z_{k+1}.  Modes 3 and 4: products of the two-level form.
+  // zout: where mode 5 adds its step (else the step overwrites dout)
   auto product32 = [&](const float* din, float* dout, int mode, float cdk, float cwk,
-                       const PcgState* state) {
+                       const PcgState* state, float* zout = nullptr) {
     if (mode == 0)
       mf_t10_elem32<false><<<ge, kBlock, 0, s>>>(n_elem, conn, M.geo32, float(lam), float(mu),
                                                  M.s32, din, M.R.rank, M.R.toff, M.R.tstart, M.fe32,
@@ -4600,7 +4629,7 @@ int mf_mixed_solve(const char* who, int64_t n_nodes, int64_t n_elem, const int32
                                                 state);
     CF_LAUNCHED();
     mf_gather_cheb32<<<gg, kBlock, 0, s>>>(n_nodes, M.R.nro, M.R.nq, M.fe32, M.s32, mode, din,
-                                           dout, dout, L.rh, cdk, cwk,
+                                           dout, zout ? zout : dout, L.rh, cdk, cwk,
                                            two_level ? C.r32 : nullptr, state);
     CF_LAUNCHED();
     return CF_OK;
@@ -4620,7 +4649,7 @@ int mf_mixed_solve(const char* who, int64_t n_nodes, int64_t n_elem, const int32
     coarse_restrict<<<ga, kRestrictBlock, 0, s>>>(C.n_agg, C.agg_off, C.agg_nodes, C.frame, xs,
                                                   ys, zs, rin, C.rc, L.state);
     CF_LAUNCHED();
-    coarse_gemv<<<gc, kBlock, 0, s>>>(C.nc, C.A, C.rc, C.zc, L.state);
+    coarse_gemv<<<gc, kBlock, 0, s>>>(C.nc, C.A32, C.rc, C.zc, L.state);
     CF_LAUNCHED();
     if (add)
       coarse_prolong<true><<<G, kBlock, 0, s>>>(n_nodes, C.agg, C.frame, xs, ys, zs, fixed, C.zc,
@@ -4653,11 +4682,17 @@ int mf_mixed_solve(const char* who, int64_t n_nodes, int64_t n_elem, const int32
     for (int k = 1; k < steps; ++k) {
       int tok = -1;
       if (k == 1) prof_begin(1, sample, s, &tok);
-      if ((r2 = cheb(k, cd[k], cw[k], D[(k - 1) & 1], D[k & 1]))) return r2;
+      if (k > 1 && k == steps - 1)  // the last step adds its iterate to u = z1 (mode 5)
+        r2 = product32(D[(k - 1) & 1], D[k & 1], 5, cd[k], cw[k], L.state, C.u32);
+      else
+        r2 = cheb(k, cd[k], cw[k], D[(k - 1) & 1], D[k & 1]);
+      if (r2) return r2;
       if (k == 1) prof_end(tok, s);
     }
-    vec_add32<<<G, kBlock, 0, s>>>(n, C.u32, D[(steps - 1) & 1], L.state);  // u = z1 + Cheb
-    CF_LAUNCHED();
+    if (steps == 2) {
+      vec_add32<<<G, kBlock, 0, s>>>(n, C.u32, D[1], L.state);  // u = z1 + Cheb
+      CF_LAUNCHED();
+    }
     if ((r2 = product32(C.u32, C.r2, 4, 0.0f, 0.0f, L.state))) return r2;   // r2 = r - K u
     return coarse_apply(C.r2, true);                                         // u += C r2
   };
