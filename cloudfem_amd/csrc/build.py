@@ -48,7 +48,7 @@ def build(verbose=False):
         objs = list(ex.map(_compile, SOURCES))
     tmp = OUT + ".tmp"
     cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs,
-           "-L", ROCM_LIB, f"-Wl,-rpath,{ROCM_LIB}", "-lrocsolver", "-lrocblas"]
+           "-L", ROCM_LIB, f"-Wl,-rpath,{ROCM_LIB}", "-lrocsolver", "-lrocblas", "-pthread"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stderr}")

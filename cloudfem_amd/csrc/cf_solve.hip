@@ -26,6 +26,8 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include <rocblas/rocblas.h>
@@ -2613,6 +2615,26 @@ constexpr ChebShape kChebF16{16, 3000.0};
 constexpr int kChebMaxSteps = 32;
 constexpr int kPowerSteps = 40;
 
+// Enqueues the power iteration for the largest eigenvalue of S A S on
+// stream s (power(v, y): y = S A S v on that stream); the square of the
+// estimate ends in state->pad, read with read_state.
+template <class Power>
+int power_launch(Power&& power, int64_t n, const Layout& L, hipStream_t s) {
+  const int G = pcg_grid();
+  int rc;
+  power_start<<<G, kBlock, 0, s>>>(n, L.s, L.ph);
+  CF_LAUNCHED();
+  for (int k = 0; k < kPowerSteps; ++k) {
+    if ((rc = power(L.ph, L.Aph))) return rc;
+    norm2_partials<<<G, kBlock, 0, s>>>(n, L.Aph, L.partials);
+    CF_LAUNCHED();
+    if ((rc = fin(kFinStore, L, s, &L.state->pad))) return rc;
+    power_scale<<<G, kBlock, 0, s>>>(n, L.Aph, L.ph, L.state);
+    CF_LAUNCHED();
+  }
+  return CF_OK;
+}
+
 // The flexible CG + Chebyshev driver, independent of how the products are
 // formed.  outer(xin, y, state): FP64 y = A xin with fixed rows zeroed
 // (state set: p.Ap partials, outer_parts of them); power(v, y): FP32
@@ -2626,23 +2648,19 @@ template <bool SQ, class Outer, class Power, class Cheb, class Pre2L = std::null
 int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, double gersh,
                   ChebShape shape, int64_t n, const double* b, double* x, const uint8_t* fixed, double tol,
                   int maxit, const Layout& L, cf_pcg_stats* st, hipStream_t s,
-                  Pre2L&& pre2l = nullptr, const float* zh2l = nullptr, int extra_products = 0) {
+                  Pre2L&& pre2l = nullptr, const float* zh2l = nullptr, int extra_products = 0,
+                  double power_est = 0.0) {
   constexpr bool kTwoLevel = !std::is_same<std::decay_t<Pre2L>, std::nullptr_t>::value;
   const int G = pcg_grid();
   int rc;
   PcgState h{};
-  power_start<<<G, kBlock, 0, s>>>(n, L.s, L.ph);
-  CF_LAUNCHED();
-  for (int k = 0; k < kPowerSteps; ++k) {
-    if ((rc = power(L.ph, L.Aph))) return rc;
-    norm2_partials<<<G, kBlock, 0, s>>>(n, L.Aph, L.partials);
-    CF_LAUNCHED();
-    if ((rc = fin(kFinStore, L, s, &L.state->pad))) return rc;
-    power_scale<<<G, kBlock, 0, s>>>(n, L.Aph, L.ph, L.state);
-    CF_LAUNCHED();
+  // power_est > 0: the caller ran power_launch already (state->pad = its square)
+  double est = power_est;
+  if (!(est > 0.0)) {
+    if ((rc = power_launch(power, n, L, s))) return rc;
+    if ((rc = read_state(L.state, &h, s))) return rc;
+    est = std::sqrt(h.pad);
   }
-  if ((rc = read_state(L.state, &h, s))) return rc;
-  const double est = std::sqrt(h.pad);
   const double bmax = gersh > 0.0 ? std::min(1.25 * est, gersh) : 1.3 * est;
   if (!(bmax > 0.0) || !std::isfinite(bmax)) {
     cf::set_error("cf_pcg_solve: no spectrum bound for the Chebyshev preconditioner "
@@ -4534,6 +4552,38 @@ struct SideJoin {
   }
 };
 
+// Runs fn() (launches only) on a second host thread with the caller's
+// device current, so that its launch calls overlap the caller's host work;
+// joined by join() or when the object goes out of scope.  join() returns
+// fn's status and moves its error text to the calling thread.
+class HostTask {
+ public:
+  template <class F>
+  explicit HostTask(F&& fn) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    t_ = std::thread([this, dev, fn]() mutable {
+      rc_ = hipSetDevice(dev) == hipSuccess ? fn() : CF_ERR_HIP;
+      if (rc_) msg_ = cf_last_error();
+    });
+  }
+  HostTask(const HostTask&) = delete;
+  HostTask& operator=(const HostTask&) = delete;
+  ~HostTask() { join(); }
+  int join() {
+    if (t_.joinable()) {
+      t_.join();
+      if (rc_) cf::set_error("%s", msg_.c_str());
+    }
+    return rc_;
+  }
+
+ private:
+  int rc_ = CF_OK;
+  std::string msg_;
+  std::thread t_;
+};
+
 }  // namespace
 
 namespace {
@@ -4587,26 +4637,6 @@ int mf_mixed_solve(const char* who, int64_t n_nodes, int64_t n_elem, const int32
   geo_to_f32<<<cf::stride_grid(10 * n_elem, kBlock), kBlock, 0, s>>>(10 * n_elem, geo, M.geo32);
   CF_LAUNCHED();
   if ((rc = mf_runs_build(n_nodes, n_elem, offs, perm, &M.R, s))) return rc;
-  if (two_level && (rc = coarse_setup(a, M.R, xs, ys, zs, fixed, M.fe, &C, s))) return rc;
-  // the coarse factorisation runs on a side stream beside the power
-  // iteration; the first two-level application waits for it
-  const Side* side = two_level ? side_stream() : nullptr;
-  const hipStream_t sf = side ? side->s : s;
-  SideJoin join;
-  if (two_level) {
-    if (side) {
-      CF_HIP(hipEventRecord(side->ready, s));
-      CF_HIP(hipStreamWaitEvent(sf, side->ready, 0));
-    }
-    rc = coarse_factor(&C, n, sf);
-    if (side) {
-      CF_HIP(hipEventRecord(side->done, sf));
-      join.s = s;
-      join.done = side->done;
-    }
-    if (rc) return rc;
-  }
-  bool factor_joined = false;
   const int ge = cf::stride_grid(n_elem, kBlock);
   const int gg = int(mf_gather_grid(n_nodes));
   auto outer = [&](const double* xin, double* y, PcgState* state) {
@@ -4616,21 +4646,23 @@ int mf_mixed_solve(const char* who, int64_t n_nodes, int64_t n_elem, const int32
   // steps run on S-scaled vectors, so their element kernel gathers no S.
   // Step k reads z_k (din) and z_{k-1} (dout, none at k = 1) and overwrites
   // dout with z_{k+1}.  Modes 3 and 4: products of the two-level form.
-  // zout: where mode 5 adds its step (else the step overwrites dout)
+  // zout: where mode 5 adds its step (else the step overwrites dout); ps:
+  // the stream (default s)
   auto product32 = [&](const float* din, float* dout, int mode, float cdk, float cwk,
-                       const PcgState* state, float* zout = nullptr) {
+                       const PcgState* state, float* zout = nullptr, hipStream_t ps = nullptr) {
+    if (!ps) ps = s;
     if (mode == 0)
-      mf_t10_elem32<false><<<ge, kBlock, 0, s>>>(n_elem, conn, M.geo32, float(lam), float(mu),
-                                                 M.s32, din, M.R.rank, M.R.toff, M.R.tstart, M.fe32,
-                                                 state);
+      mf_t10_elem32<false><<<ge, kBlock, 0, ps>>>(n_elem, conn, M.geo32, float(lam), float(mu),
+                                                  M.s32, din, M.R.rank, M.R.toff, M.R.tstart,
+                                                  M.fe32, state);
     else
-      mf_t10_elem32<true><<<ge, kBlock, 0, s>>>(n_elem, conn, M.geo32, float(lam), float(mu),
-                                                M.s32, din, M.R.rank, M.R.toff, M.R.tstart, M.fe32,
-                                                state);
+      mf_t10_elem32<true><<<ge, kBlock, 0, ps>>>(n_elem, conn, M.geo32, float(lam), float(mu),
+                                                 M.s32, din, M.R.rank, M.R.toff, M.R.tstart,
+                                                 M.fe32, state);
     CF_LAUNCHED();
-    mf_gather_cheb32<<<gg, kBlock, 0, s>>>(n_nodes, M.R.nro, M.R.nq, M.fe32, M.s32, mode, din,
-                                           dout, zout ? zout : dout, L.rh, cdk, cwk,
-                                           two_level ? C.r32 : nullptr, state);
+    mf_gather_cheb32<<<gg, kBlock, 0, ps>>>(n_nodes, M.R.nro, M.R.nq, M.fe32, M.s32, mode, din,
+                                            dout, zout ? zout : dout, L.rh, cdk, cwk,
+                                            two_level ? C.r32 : nullptr, state);
     CF_LAUNCHED();
     return CF_OK;
   };
@@ -4638,6 +4670,41 @@ int mf_mixed_solve(const char* who, int64_t n_nodes, int64_t n_elem, const int32
   auto cheb = [&](int k, float cdk, float cwk, const float* din, float* dout) {
     return product32(din, dout, k == 1 ? 1 : 2, cdk, cwk, L.state);
   };
+  // Two-level: the power iteration runs on a side stream while this stream
+  // builds, factors and inverts the coarse matrix (whose host colouring and
+  // one-workgroup factorisation leave most of the GPU idle).
+  double power_est = 0.0;
+  SideJoin join;
+  if (two_level) {
+    const Side* side = side_stream();
+    const hipStream_t sp = side ? side->s : s;
+    if (side) {
+      CF_HIP(hipEventRecord(side->ready, s));
+      CF_HIP(hipStreamWaitEvent(sp, side->ready, 0));
+      join.s = s;
+      join.done = side->done;
+    }
+    auto power_side = [&](const float* v, float* y) {
+      return product32(v, y, 0, 0.0f, 0.0f, nullptr, nullptr, sp);
+    };
+    // its ~200 launches go out from a second host thread
+    HostTask launches([&]() {
+      int r = power_launch(power_side, n, L, sp);
+      if (side && hipEventRecord(side->done, sp) != hipSuccess && r == CF_OK) {
+        cf::set_error("two-level: hipEventRecord on the side stream failed");
+        r = CF_ERR_HIP;
+      }
+      return r;
+    });
+    if ((rc = coarse_setup(a, M.R, xs, ys, zs, fixed, M.fe, &C, s))) return rc;
+    if ((rc = coarse_factor(&C, n, s))) return rc;
+    if ((rc = coarse_factor_check(C, s))) return rc;
+    if ((rc = launches.join())) return rc;
+    PcgState h{};
+    if ((rc = read_state(L.state, &h, sp))) return rc;
+    power_est = std::sqrt(h.pad);
+    if (side) CF_HIP(hipStreamWaitEvent(s, side->done, 0));
+  }
   if (!two_level)
     return fcg_cheb_core<true>(outer, gg, power, cheb, 0.0, kChebF32, n, b, x, fixed, tol, maxit,
                                L, stats_host, s);
@@ -4663,12 +4730,6 @@ int mf_mixed_solve(const char* who, int64_t n_nodes, int64_t n_elem, const int32
   float* D[2] = {L.ph, L.Aph};
   auto pre2l = [&](int apply_step, int slot, int sample, const float* cd, const float* cw,
                    float inv_theta, int steps) {
-    if (!factor_joined) {
-      factor_joined = true;
-      int r0;
-      if ((r0 = coarse_factor_check(C, sf))) return r0;
-      if (side) CF_HIP(hipStreamWaitEvent(s, side->done, 0));
-    }
     if (apply_step)
       fcg_update2l<true><<<GV, kBlock, 0, s>>>(n, x, L.r, L.p, L.Ap, C.r32, L.state, L.partials,
                                                gg, L.partials_b, slot);
@@ -4697,7 +4758,7 @@ int mf_mixed_solve(const char* who, int64_t n_nodes, int64_t n_elem, const int32
     return coarse_apply(C.r2, true);                                         // u += C r2
   };
   return fcg_cheb_core<true>(outer, gg, power, cheb, 0.0, kCheb2L, n, b, x, fixed, tol, maxit,
-                             L, stats_host, s, pre2l, C.u32, 2);
+                             L, stats_host, s, pre2l, C.u32, 2, power_est);
 }
 
 }  // namespace
