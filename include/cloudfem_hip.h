@@ -392,10 +392,16 @@ int cf_pcg_solve_mf_t10_mixed(int64_t n_nodes, int64_t n_elem, const int32_t* co
  *      the six rigid-body modes of node aggregates (the group of 1536
  *      consecutive elements that first touches a node), A_c = P^T K_ff P
  *      assembled by probing with distance-2 coloured aggregates (no atomics)
- *      and inverted with rocSOLVER (dpotrf + dpotri; rocBLAS allocates its own
- *      small workspace and handle, once per device).  x_nodes/y_nodes/z_nodes:
- *      node coordinates (device, FP64).  Same FP64 residual bar as the other
- *      solvers.                                                              */
+ *      and inverted by a banded Cholesky factorisation and a column-parallel
+ *      inverse when its half-bandwidth is at most 62 (the aggregates follow
+ *      the element order), else by rocSOLVER (dpotrf + dpotri; rocBLAS
+ *      allocates its own small workspace and handle, once per device).  The
+ *      power iteration for the Chebyshev bound runs on a side stream (created
+ *      once per device) beside the coarse set-up, launched from a second host
+ *      thread; on return the caller's stream is ordered after both.  Solves
+ *      that overlap on one device share that stream: run them one at a time.
+ *      x_nodes/y_nodes/z_nodes: node coordinates (device, FP64).  Same FP64
+ *      residual bar as the other solvers.                                    */
 int cf_mf_t10_twolevel_workspace_bytes(int64_t n_nodes, int64_t n_elem, size_t* bytes);
 int cf_pcg_solve_mf_t10_twolevel(int64_t n_nodes, int64_t n_elem, const int32_t* conn,
                                  const double* geo, double lam, double mu,
