@@ -4555,17 +4555,27 @@ struct SideJoin {
 // Runs fn() (launches only) on a second host thread with the caller's
 // device current, so that its launch calls overlap the caller's host work;
 // joined by join() or when the object goes out of scope.  join() returns
-// fn's status and moves its error text to the calling thread.
+// fn's status and moves its error text to the calling thread.  If no thread
+// can be started, fn runs inline (no exception crosses the C ABI).
 class HostTask {
  public:
   template <class F>
   explicit HostTask(F&& fn) {
     int dev = 0;
     (void)hipGetDevice(&dev);
-    t_ = std::thread([this, dev, fn]() mutable {
-      rc_ = hipSetDevice(dev) == hipSuccess ? fn() : CF_ERR_HIP;
-      if (rc_) msg_ = cf_last_error();
-    });
+    try {
+      t_ = std::thread([this, dev, fn]() mutable {
+        if (hipSetDevice(dev) == hipSuccess) {
+          rc_ = fn();
+        } else {
+          cf::set_error("hipSetDevice(%d) failed on the launch thread", dev);
+          rc_ = CF_ERR_HIP;
+        }
+        if (rc_) msg_ = cf_last_error();
+      });
+    } catch (...) {
+      rc_ = fn();
+    }
   }
   HostTask(const HostTask&) = delete;
   HostTask& operator=(const HostTask&) = delete;
