@@ -3700,6 +3700,87 @@ __global__ void agg_offsets(int64_t n_nodes, int64_t n_agg, const uint32_t* __re
   }
 }
 
+// Fallback aggregates (a node numbering that scatters the element-order
+// groups): the nodes that belong to an element, sorted by their coordinate
+// along the longest side of the node bounding box (stable: ties keep the
+// node order) and cut into equal runs -- slabs across that axis whatever
+// the numbering, so aggregates that share an element lie a few ids apart.
+
+// per-block minimum and maximum of x, y, z:
+// part[6 b .. 6 b + 5] = min x, min y, min z, max x, max y, max z
+__global__ void __launch_bounds__(kBlock)
+node_bbox(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
+          const double* __restrict__ z, double* __restrict__ part) {
+  __shared__ double sh[6][kBlock];
+  double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const double c[3] = {x[i], y[i], z[i]};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      v[k] = fmin(v[k], c[k]);
+      v[3 + k] = fmax(v[3 + k], c[k]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) sh[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (int(threadIdx.x) < w)
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        sh[k][threadIdx.x] = k < 3 ? fmin(sh[k][threadIdx.x], sh[k][threadIdx.x + w])
+                                   : fmax(sh[k][threadIdx.x], sh[k][threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) part[6 * int64_t(blockIdx.x) + k] = sh[k][0];
+}
+
+// sort keys: the coordinate as an order-preserving uint32 (UINT32_MAX for a
+// node in no element, which sorts last), node ids
+__global__ void slab_keys(int64_t n_nodes, const int32_t* __restrict__ offs,
+                          const double* __restrict__ coord, uint32_t* __restrict__ keys,
+                          uint32_t* __restrict__ ids) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n_nodes;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    uint32_t k = 0xffffffffu;
+    if (offs[i] < offs[i + 1]) {
+      const uint32_t b = __float_as_uint(float(coord[i]));
+      k = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+      if (k == 0xffffffffu) k = 0xfffffffeu;
+    }
+    keys[i] = k;
+    ids[i] = uint32_t(i);
+  }
+}
+
+// count[0] = number of sorted keys below UINT32_MAX (the nodes in elements)
+__global__ void slab_count(int64_t n_nodes, const uint32_t* __restrict__ keys,
+                           uint32_t* __restrict__ count) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  int64_t lo = 0, hi = n_nodes;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) / 2;
+    if (keys[mid] == 0xffffffffu) hi = mid; else lo = mid + 1;
+  }
+  count[0] = uint32_t(lo);
+}
+
+// agg[node] = position / npa for the sorted nodes in elements (agg is -1
+// beforehand); agg_off[g] = g npa capped at n_in (g = 0..n_agg)
+__global__ void slab_assign(int64_t n_in, int64_t npa, int64_t n_agg,
+                            const uint32_t* __restrict__ sorted_ids, int32_t* __restrict__ agg,
+                            int32_t* __restrict__ agg_off) {
+  for (int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; p < n_in;
+       p += int64_t(gridDim.x) * blockDim.x)
+    agg[sorted_ids[p]] = int32_t(p / npa);
+  for (int64_t g = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; g <= n_agg;
+       g += int64_t(gridDim.x) * blockDim.x)
+    agg_off[g] = int32_t(std::min<int64_t>(g * npa, n_in));
+}
+
 // centroid and 1 / RMS radius of each aggregate; one workgroup per
 // aggregate, fixed node split and reduction tree
 __global__ void __launch_bounds__(kBlock)
@@ -4356,78 +4437,131 @@ int coarse_setup(const MfArgs& a, const MfRuns& R, const double* xs, const doubl
   const int64_t n_nodes = a.n_nodes, n_elem = a.n_elem, n = 3 * n_nodes, G = C->n_groups;
   const int gn = cf::stride_grid(n_nodes, kBlock), ge = cf::stride_grid(n_elem + 1, kBlock);
   size_t tb;
-  // aggregates: first-touching element group, compacted ids
-  CF_HIP(hipMemsetAsync(C->used, 0, sizeof(uint32_t) * (G + 1), s));
-  agg_first<<<gn, kBlock, 0, s>>>(n_nodes, a.offs, a.perm, C->graw, C->used);
-  CF_LAUNCHED();
-  tb = C->tmp_bytes;
-  CF_HIP(rocprim::exclusive_scan(C->tmp, tb, C->used, C->gid, 0u, size_t(G + 1),
-                                 rocprim::plus<uint32_t>(), s));
-  agg_compact<<<gn, kBlock, 0, s>>>(n_nodes, C->graw, C->gid, G, C->agg, C->keys_in, C->ids_in);
-  CF_LAUNCHED();
+  // Aggregates from the element order; when their graph needs more than
+  // kMaxColors colours (a node numbering that scatters the groups), slabs
+  // across the longest side of the bounding box instead.
   uint32_t n_agg = 0;
-  CF_HIP(hipMemcpyAsync(&n_agg, C->gid + G, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  CF_HIP(hipStreamSynchronize(s));
-  CF_REQUIRE(n_agg > 0 && int64_t(n_agg) <= G, "two-level: bad aggregate count %u", n_agg);
-  C->n_agg = n_agg;
-  C->nc = 6 * int64_t(n_agg);
-  tb = C->tmp_bytes;
-  CF_HIP(rocprim::radix_sort_pairs(C->tmp, tb, C->keys_in, C->keys_out, C->ids_in, C->agg_nodes,
-                                   size_t(n_nodes), 0, bits_for(int64_t(n_agg) + 1), s));
-  agg_offsets<<<cf::stride_grid(n_agg + 1, kBlock), kBlock, 0, s>>>(n_nodes, n_agg, C->keys_out,
-                                                                    C->agg_off);
-  CF_LAUNCHED();
-  agg_frame<<<int(std::min<int64_t>(n_agg, 65535)), kBlock, 0, s>>>(n_agg, C->agg_off,
-                                                                    C->agg_nodes, xs, ys, zs,
-                                                                    C->frame);
-  CF_LAUNCHED();
-  // aggregate adjacency through elements
-  agg_pair_count<<<ge, kBlock, 0, s>>>(n_elem, a.conn, C->agg, C->pcnt);
-  CF_LAUNCHED();
-  tb = C->tmp_bytes;
-  CF_HIP(rocprim::exclusive_scan(C->tmp, tb, C->pcnt, C->poff, 0u, size_t(n_elem + 1),
-                                 rocprim::plus<uint32_t>(), s));
-  uint32_t n_pairs = 0;
-  CF_HIP(hipMemcpyAsync(&n_pairs, C->poff + n_elem, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  CF_HIP(hipStreamSynchronize(s));
-  CF_REQUIRE(int64_t(n_pairs) <= kPairsPerElem * n_elem, "two-level: %u aggregate pairs", n_pairs);
-  std::vector<uint64_t> pk(n_pairs);
-  if (n_pairs) {
-    agg_pair_fill<<<ge, kBlock, 0, s>>>(n_elem, a.conn, C->agg, C->poff, n_agg, C->pkeys_in);
+  int64_t agg_band = 0;  // largest id distance of two coupled aggregates
+  int n_col = 0;
+  std::vector<int32_t> color;
+  std::vector<std::vector<int32_t>> adj;
+  for (int attempt = 0;; ++attempt) {
+    if (attempt == 0) {
+      // first-touching element group, compacted ids
+      CF_HIP(hipMemsetAsync(C->used, 0, sizeof(uint32_t) * (G + 1), s));
+      agg_first<<<gn, kBlock, 0, s>>>(n_nodes, a.offs, a.perm, C->graw, C->used);
+      CF_LAUNCHED();
+      tb = C->tmp_bytes;
+      CF_HIP(rocprim::exclusive_scan(C->tmp, tb, C->used, C->gid, 0u, size_t(G + 1),
+                                     rocprim::plus<uint32_t>(), s));
+      agg_compact<<<gn, kBlock, 0, s>>>(n_nodes, C->graw, C->gid, G, C->agg, C->keys_in,
+                                        C->ids_in);
+      CF_LAUNCHED();
+      CF_HIP(hipMemcpyAsync(&n_agg, C->gid + G, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+      CF_HIP(hipStreamSynchronize(s));
+      CF_REQUIRE(n_agg > 0 && int64_t(n_agg) <= G, "two-level: bad aggregate count %u", n_agg);
+      tb = C->tmp_bytes;
+      CF_HIP(rocprim::radix_sort_pairs(C->tmp, tb, C->keys_in, C->keys_out, C->ids_in,
+                                       C->agg_nodes, size_t(n_nodes), 0,
+                                       bits_for(int64_t(n_agg) + 1), s));
+      agg_offsets<<<cf::stride_grid(n_agg + 1, kBlock), kBlock, 0, s>>>(n_nodes, n_agg,
+                                                                        C->keys_out, C->agg_off);
+      CF_LAUNCHED();
+    } else {
+      // slabs: nodes sorted along the longest side of the bounding box
+      std::vector<double> box(6 * size_t(gn));
+      node_bbox<<<gn, kBlock, 0, s>>>(n_nodes, xs, ys, zs, C->probe);
+      CF_LAUNCHED();
+      CF_HIP(hipMemcpyAsync(box.data(), C->probe, sizeof(double) * box.size(),
+                            hipMemcpyDeviceToHost, s));
+      CF_HIP(hipStreamSynchronize(s));
+      double ext[3];
+      for (int k = 0; k < 3; ++k) {
+        double lo = INFINITY, hi = -INFINITY;
+        for (int blk = 0; blk < gn; ++blk) {
+          lo = std::min(lo, box[6 * size_t(blk) + k]);
+          hi = std::max(hi, box[6 * size_t(blk) + 3 + k]);
+        }
+        ext[k] = hi - lo;
+      }
+      const int axis = ext[0] >= ext[1] && ext[0] >= ext[2] ? 0 : (ext[1] >= ext[2] ? 1 : 2);
+      slab_keys<<<gn, kBlock, 0, s>>>(n_nodes, a.offs, axis == 0 ? xs : axis == 1 ? ys : zs,
+                                      C->keys_in, C->ids_in);
+      CF_LAUNCHED();
+      tb = C->tmp_bytes;
+      CF_HIP(rocprim::radix_sort_pairs(C->tmp, tb, C->keys_in, C->keys_out, C->ids_in,
+                                       C->agg_nodes, size_t(n_nodes), 0, 32, s));
+      slab_count<<<1, 64, 0, s>>>(n_nodes, C->keys_out, C->used);
+      CF_LAUNCHED();
+      uint32_t n_in = 0;
+      CF_HIP(hipMemcpyAsync(&n_in, C->used, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+      CF_HIP(hipStreamSynchronize(s));
+      CF_REQUIRE(n_in > 0, "two-level: no node belongs to an element");
+      const int64_t npa = (int64_t(n_in) + G - 1) / G;  // nodes per aggregate
+      n_agg = uint32_t((int64_t(n_in) + npa - 1) / npa);
+      CF_HIP(hipMemsetAsync(C->agg, 0xff, sizeof(int32_t) * n_nodes, s));
+      slab_assign<<<gn, kBlock, 0, s>>>(n_in, npa, n_agg, C->agg_nodes, C->agg, C->agg_off);
+      CF_LAUNCHED();
+    }
+    C->n_agg = n_agg;
+    C->nc = 6 * int64_t(n_agg);
+    agg_frame<<<int(std::min<int64_t>(n_agg, 65535)), kBlock, 0, s>>>(n_agg, C->agg_off,
+                                                                      C->agg_nodes, xs, ys, zs,
+                                                                      C->frame);
+    CF_LAUNCHED();
+    // aggregate adjacency through elements
+    agg_pair_count<<<ge, kBlock, 0, s>>>(n_elem, a.conn, C->agg, C->pcnt);
     CF_LAUNCHED();
     tb = C->tmp_bytes;
-    CF_HIP(rocprim::radix_sort_keys(C->tmp, tb, C->pkeys_in, C->pkeys_out, size_t(n_pairs), 0,
-                                    bits_for(int64_t(n_agg) * n_agg + 1), s));
-    CF_HIP(hipMemcpyAsync(pk.data(), C->pkeys_out, sizeof(uint64_t) * n_pairs,
-                          hipMemcpyDeviceToHost, s));
+    CF_HIP(rocprim::exclusive_scan(C->tmp, tb, C->pcnt, C->poff, 0u, size_t(n_elem + 1),
+                                   rocprim::plus<uint32_t>(), s));
+    uint32_t n_pairs = 0;
+    CF_HIP(hipMemcpyAsync(&n_pairs, C->poff + n_elem, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                          s));
     CF_HIP(hipStreamSynchronize(s));
-  }
-  // distance-2 greedy colouring on the host (n_agg is small)
-  std::vector<std::vector<int32_t>> adj(n_agg);
-  for (uint32_t g = 0; g < n_agg; ++g) adj[g].push_back(int32_t(g));
-  uint64_t last = ~uint64_t(0);
-  int64_t agg_band = 0;  // largest id distance of two coupled aggregates
-  for (uint64_t key : pk) {
-    if (key == last) continue;
-    last = key;
-    const int32_t g = int32_t(key / n_agg), h = int32_t(key % n_agg);
-    adj[g].push_back(h);
-    adj[h].push_back(g);
-    agg_band = std::max<int64_t>(agg_band, g > h ? g - h : h - g);
-  }
-  std::vector<int32_t> color(n_agg, -1);
-  std::vector<uint32_t> stamp(kMaxColors, 0);
-  int n_col = 0;
-  for (uint32_t g = 0; g < n_agg; ++g) {
-    for (int32_t h : adj[g])
-      for (int32_t k : adj[h])
-        if (color[k] >= 0) stamp[color[k]] = g + 1;
-    int c = 0;
-    while (c < kMaxColors && stamp[c] == g + 1) ++c;
-    CF_REQUIRE(c < kMaxColors, "two-level: aggregate graph needs more than %d colours",
+    CF_REQUIRE(int64_t(n_pairs) <= kPairsPerElem * n_elem, "two-level: %u aggregate pairs",
+               n_pairs);
+    std::vector<uint64_t> pk(n_pairs);
+    if (n_pairs) {
+      agg_pair_fill<<<ge, kBlock, 0, s>>>(n_elem, a.conn, C->agg, C->poff, n_agg, C->pkeys_in);
+      CF_LAUNCHED();
+      tb = C->tmp_bytes;
+      CF_HIP(rocprim::radix_sort_keys(C->tmp, tb, C->pkeys_in, C->pkeys_out, size_t(n_pairs), 0,
+                                      bits_for(int64_t(n_agg) * n_agg + 1), s));
+      CF_HIP(hipMemcpyAsync(pk.data(), C->pkeys_out, sizeof(uint64_t) * n_pairs,
+                            hipMemcpyDeviceToHost, s));
+      CF_HIP(hipStreamSynchronize(s));
+    }
+    // distance-2 greedy colouring on the host (n_agg is small)
+    adj.assign(n_agg, {});
+    for (uint32_t g = 0; g < n_agg; ++g) adj[g].push_back(int32_t(g));
+    uint64_t last = ~uint64_t(0);
+    agg_band = 0;
+    for (uint64_t key : pk) {
+      if (key == last) continue;
+      last = key;
+      const int32_t g = int32_t(key / n_agg), h = int32_t(key % n_agg);
+      adj[g].push_back(h);
+      adj[h].push_back(g);
+      agg_band = std::max<int64_t>(agg_band, g > h ? g - h : h - g);
+    }
+    color.assign(n_agg, -1);
+    std::vector<uint32_t> stamp(kMaxColors, 0);
+    n_col = 0;
+    bool fits = true;
+    for (uint32_t g = 0; g < n_agg && fits; ++g) {
+      for (int32_t h : adj[g])
+        for (int32_t k : adj[h])
+          if (color[k] >= 0) stamp[color[k]] = g + 1;
+      int c = 0;
+      while (c < kMaxColors && stamp[c] == g + 1) ++c;
+      fits = c < kMaxColors;
+      color[g] = c;
+      n_col = std::max(n_col, c + 1);
+    }
+    if (fits) break;
+    CF_REQUIRE(attempt == 0, "two-level: aggregate graph needs more than %d colours",
                kMaxColors);
-    color[g] = c;
-    n_col = std::max(n_col, c + 1);
   }
   std::vector<int32_t> nbc(size_t(n_agg) * n_col, -1);
   for (uint32_t h = 0; h < n_agg; ++h)

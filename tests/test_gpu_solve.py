@@ -239,6 +239,41 @@ def test_twolevel_coarse_factor_paths(gpu, monkeypatch, dims):
     assert max(its.values()) - min(its.values()) <= 2
 
 
+def test_twolevel_scattered_node_numbering(gpu):
+    """A 160x12x12 cantilever (138,240 elements, 90 aggregates) with its node
+    ids randomly permuted: the element-order aggregates then all touch each
+    other and need more than 64 colours, so the solver falls back to slabs
+    along the longest axis.  The solution, mapped back, must agree with the
+    solve on the original numbering, and both meet the FP64 residual bar."""
+    from cloudfem_amd import solver
+    from cloudfem_amd.mesh import box_mesh_tet10
+    from cloudfem_amd.mesh.tet10_mesh import Tet10Mesh
+    from cloudfem_amd.physics import elastic_tet10
+    lims = dict(xlim=(0.0, 0.4), ylim=(0.0, 0.05), zlim=(0.0, 0.05))
+    trac = {"xmax": (0.0, 0.0, -1e6)}
+    bc = {"xmin": (0.0, 0.0, 0.0)}
+    m = box_mesh_tet10(160, 12, 12, **lims)
+    perm = np.random.default_rng(3).permutation(m.n_nodes)   # old id -> new id
+    nodes = np.empty_like(m.nodes)
+    nodes[perm] = m.nodes
+    mp = Tet10Mesh(nodes, perm[m.elements], {k: perm[v] for k, v in m.boundaries.items()},
+                   n_corners=m.n_corners)
+    sol = {}
+    for name, mesh in (("original", m), ("permuted", mp)):
+        with solver.solver_options(operator="matfree", precision="mixed", precond="twolevel",
+                                   tol=1e-10):
+            u = np.stack(elastic_tet10.solve_elasticity_tet10(mesh, 210e9, 0.3, bc=bc,
+                                                              traction=trac), axis=1)
+        st = solver.last_stats.value
+        print(f"{name}: {st['iterations']} outer iterations, rel. residual "
+              f"{st['rel_residual']:.2e}")
+        assert st["rel_residual"] <= 1e-9
+        sol[name] = u if name == "original" else u[perm]
+    err = rel_err(sol["permuted"].ravel(), sol["original"].ravel())
+    print(f"permuted vs original numbering: {err:.2e}")
+    assert err < 1e-8
+
+
 def test_twolevel_fewer_iterations_than_chebyshev(gpu):
     """The coarse correction must pay: on the 40x5x5 cantilever the two-level
     solver needs fewer outer iterations than the Chebyshev-only solver."""
