@@ -33,6 +33,7 @@
 #include <rocblas/rocblas.h>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_select.hpp>
 #include <rocsolver/rocsolver.h>
 #include <type_traits>
 
@@ -4405,7 +4406,10 @@ void coarse_layout(int64_t n_nodes, int64_t n_elem, cf::Carve& c, Coarse* C) {
                           size_t(n_elem + 1), rocprim::plus<uint32_t>());
   rocprim::radix_sort_keys(nullptr, b4, (uint64_t*)nullptr, (uint64_t*)nullptr, size_t(maxp), 0,
                            64);
-  C->tmp_bytes = std::max(std::max(b1, b2), std::max(b3, b4));
+  size_t b5 = 0;
+  rocprim::unique(nullptr, b5, (uint64_t*)nullptr, (uint64_t*)nullptr, (uint32_t*)nullptr,
+                  size_t(maxp), rocprim::equal_to<uint64_t>());
+  C->tmp_bytes = std::max(std::max(std::max(b1, b2), std::max(b3, b4)), b5);
   C->tmp = c.take<char>(C->tmp_bytes);
 }
 
@@ -4521,14 +4525,24 @@ int coarse_setup(const MfArgs& a, const MfRuns& R, const double* xs, const doubl
     CF_HIP(hipStreamSynchronize(s));
     CF_REQUIRE(int64_t(n_pairs) <= kPairsPerElem * n_elem, "two-level: %u aggregate pairs",
                n_pairs);
-    std::vector<uint64_t> pk(n_pairs);
+    std::vector<uint64_t> pk;
     if (n_pairs) {
       agg_pair_fill<<<ge, kBlock, 0, s>>>(n_elem, a.conn, C->agg, C->poff, n_agg, C->pkeys_in);
       CF_LAUNCHED();
       tb = C->tmp_bytes;
       CF_HIP(rocprim::radix_sort_keys(C->tmp, tb, C->pkeys_in, C->pkeys_out, size_t(n_pairs), 0,
                                       bits_for(int64_t(n_agg) * n_agg + 1), s));
-      CF_HIP(hipMemcpyAsync(pk.data(), C->pkeys_out, sizeof(uint64_t) * n_pairs,
+      // distinct pairs only (about 3 per aggregate instead of one per element
+      // on a slab boundary) cross to the host
+      tb = C->tmp_bytes;
+      CF_HIP(rocprim::unique(C->tmp, tb, C->pkeys_out, C->pkeys_in, C->used, size_t(n_pairs),
+                             rocprim::equal_to<uint64_t>(), s));
+      uint32_t n_unique = 0;
+      CF_HIP(hipMemcpyAsync(&n_unique, C->used, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+      CF_HIP(hipStreamSynchronize(s));
+      CF_REQUIRE(n_unique > 0 && n_unique <= n_pairs, "two-level: %u distinct pairs", n_unique);
+      pk.resize(n_unique);
+      CF_HIP(hipMemcpyAsync(pk.data(), C->pkeys_in, sizeof(uint64_t) * n_unique,
                             hipMemcpyDeviceToHost, s));
       CF_HIP(hipStreamSynchronize(s));
     }
