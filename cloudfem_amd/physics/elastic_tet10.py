@@ -40,13 +40,14 @@ def _qpoints(mesh):
 
 
 def _assemble_device(mesh, E, nu, body=None, traction=None, T=None, alpha=0.0, T_ref=0.0,
-                     dm=None):
-    """(DeviceMesh, NB3 stiffness values, rhs (3N,)) on the GPU."""
+                     dm=None, values=True):
+    """(DeviceMesh, NB3 stiffness values, rhs (3N,)) on the GPU; with values=False the
+    stiffness values are not assembled (None), for the matrix-free solvers."""
     dm = dm or DeviceMesh(mesh)
     if dm.k != 10:
         raise ValueError("assemble_elasticity_tet10 needs a 10-node mesh")
     lam, mu = lame(E, nu)
-    vals = dm.matrix_values(N.OP_T10_ELAST, lam=lam, mu=mu)
+    vals = dm.matrix_values(N.OP_T10_ELAST, lam=lam, mu=mu) if values else None
     rhs = torch.zeros(3 * dm.n_nodes, dtype=torch.float64, device=dm.dev)
     if body is not None:
         xq = _qpoints(mesh).reshape(-1, 3)
@@ -110,6 +111,8 @@ def solve_device(dm, vals, rhs, mesh, bc, lam=None, mu=None, **pcg_kw):
         else:
             stats = solver_mod.pcg_matfree_t10_mixed(dm, lam, mu, rhs, x, fixed, **kw)
     else:
+        if vals is None:
+            vals = dm.matrix_values(N.OP_T10_ELAST, lam=lam, mu=mu)
         A = DeviceMatrix(N.FMT_NB3, dm.pattern(), vals)
         stats = pcg(A, rhs, x, fixed, **pcg_kw)
     return x, stats
@@ -117,8 +120,11 @@ def solve_device(dm, vals, rhs, mesh, bc, lam=None, mu=None, **pcg_kw):
 
 def solve_elasticity_tet10(mesh, E, nu, bc=None, body=None, traction=None, T=None, alpha=0.0,
                            T_ref=0.0):
-    """(ux, uy, uz) over all Tet10 nodes (elastic_tet10.py:214-229)."""
-    dm, vals, rhs = _assemble_device(mesh, E, nu, body, traction, T, alpha, T_ref)
+    """(ux, uy, uz) over all Tet10 nodes (elastic_tet10.py:214-229).  With the
+    matrix-free operator selected the stiffness matrix is never formed."""
+    matfree = solver_mod.options()["operator"] == "matfree"
+    dm, vals, rhs = _assemble_device(mesh, E, nu, body, traction, T, alpha, T_ref,
+                                     values=not matfree)
     lam, mu = lame(E, nu)
     x, _ = solve_device(dm, vals, rhs, mesh, bc, lam=lam, mu=mu)
     u = x.cpu().numpy()
