@@ -31,7 +31,7 @@ def test_p1_stiffness_and_rhs(gpu, dims):
     assert_vec_close(rhs, rhs0)
 
 
-def test_p1_cfg1_pattern_size(gpu):
+def test_p1_box20_pattern_size(gpu):
     from cloudfem_amd.fem import tet_poisson
     from cloudfem_amd.mesh import box_mesh
     A, _ = tet_poisson.assemble(box_mesh(20, 20, 20), k=1.0, f=SRC)
@@ -76,7 +76,7 @@ def test_tet10_elasticity(gpu, dims):
     assert_vec_close(rhs, rhs0)
 
 
-def test_tet10_elasticity_cfg2_structure(gpu):
+def test_tet10_elasticity_80x10x10_structure(gpu):
     """80x10x10 Tet10 box: the structure counts of the full benchmark mesh."""
     from cloudfem_amd.mesh import box_mesh_tet10
     from cloudfem_amd.physics import elastic_tet10

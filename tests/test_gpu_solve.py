@@ -35,7 +35,7 @@ def test_p1_manufactured_rates(gpu):
     assert all(1.7 < r < 2.3 for r in rates), rates
 
 
-def test_p1_cfg1_solution_matches_spsolve(gpu):
+def test_p1_box20_poisson_matches_spsolve(gpu):
     """20^3 Kuhn box, f = 3 pi^2 sin sin sin, T = 0 on all faces."""
     from cloudfem_amd.fem.tet_poisson import solve_tet_poisson
     from cloudfem_amd.mesh import box_mesh
