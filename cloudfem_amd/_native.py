@@ -94,7 +94,8 @@ class CfMatrix(ctypes.Structure):
 class CfPcgStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32),
                 ("residual_updates", ctypes.c_int32),
-                ("rel_residual", ctypes.c_double), ("b_norm", ctypes.c_double)]
+                ("rel_residual", ctypes.c_double), ("b_norm", ctypes.c_double),
+                ("restarts", ctypes.c_int32), ("floor_limited", ctypes.c_int32)]
 
 
 class CfBatchStats(ctypes.Structure):

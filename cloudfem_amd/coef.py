@@ -1,14 +1,39 @@
 """Host evaluation of the reference's float-or-callable coefficients.
 
 The reference calls user callables f(x, y, z) once per centroid, quadrature
-point or node (tet_poisson.py:48-56, tet10.py:152-153,
-elastic_tet10.py:176-186).  Python callables cannot run on the GPU, so they
-are evaluated here into arrays that are then uploaded.  A callable is first
-tried once on whole coordinate arrays; if that raises or returns the wrong
-shape it is called point by point, exactly as the reference does.
+point or node, with numpy float64 scalars (tet_poisson.py:48-56,
+tet10.py:152-153, elastic_tet10.py:176-186).  Python callables cannot run on
+the GPU, so they are evaluated here into arrays that are then uploaded.
+
+A callable is first tried once on whole coordinate arrays, which is only a
+shortcut: a vectorised result is kept only if it has one value per point and
+agrees with per-point calls at sampled points (first, last and a spread of
+others).  A callable that returns a single value for array inputs is called
+point by point, since it may reduce over its inputs (``np.linalg.norm([x, y,
+z])`` does).  Any other outcome (an exception, a wrong shape, a mismatch at a
+sampled point) falls back to one call per point, which is exactly what the
+reference does.
 """
 
 import numpy as np
+
+# Sampled points compared with per-point calls before a vectorised result is
+# trusted; relative tolerance of that comparison (array math may round the
+# last bit differently from scalar math).
+_N_CHECK = 16
+_CHECK_RTOL = 1e-13
+
+
+def _per_point(v, pts):
+    return np.array([float(v(x, y, z)) for x, y, z in pts], dtype=float)
+
+
+def _sample_ids(n):
+    if n <= _N_CHECK:
+        return np.arange(n)
+    rng = np.random.default_rng(n)
+    mid = rng.choice(np.arange(1, n - 1), size=_N_CHECK - 2, replace=False)
+    return np.unique(np.concatenate([[0, n - 1], mid]))
 
 
 def eval_scalar(v, pts):
@@ -21,13 +46,16 @@ def eval_scalar(v, pts):
         return np.zeros(0)
     try:
         out = np.asarray(v(pts[:, 0], pts[:, 1], pts[:, 2]), dtype=float)
-        if out.shape == ():
-            return np.full(n, float(out))
-        if out.shape == (n,):
-            return out
     except Exception:  # noqa: BLE001 - any failure means "not vectorisable"
-        pass
-    return np.array([float(v(x, y, z)) for x, y, z in pts], dtype=float)
+        out = None
+    if out is None or out.shape != (n,):
+        return _per_point(v, pts)
+    ids = _sample_ids(n)
+    ref = _per_point(v, pts[ids])
+    tol = _CHECK_RTOL * np.maximum(np.abs(ref), np.abs(out[ids])) + 1e-300
+    if not np.all(np.abs(out[ids] - ref) <= tol):
+        return _per_point(v, pts)
+    return out
 
 
 def eval_components(vals, pts):

@@ -2472,6 +2472,37 @@ int next_batch(const PcgState& h, double rr_start, double prev_rr, int prev_it, 
   return nb < lo ? lo : (nb > hi ? hi : nb);
 }
 
+// Residual replacement at the end of a CG solve.  CG stops on its
+// recursive residual, which drifts from the true one b - A x by rounding.
+// When the true residual misses the bar, CG restarts from it (the true
+// residual replaces the recursive one) for a segment that ends when the
+// recursive residual has fallen 10x below the restart value (or to the bar).
+// If a restart does not halve the true residual, the bar lies below what
+// FP64 can resolve for this system (about eps ||A_ff| |x_f|| / ||b_f||):
+// the solve stops with floor_limited set and the true residual reported.
+struct TrueResidual {
+  static constexpr int kMaxRestarts = 8;
+  double bar2;            // (tol ||b_f||)^2
+  double last2 = 0.0;     // last true r.r
+  double prev2 = HUGE_VAL;// true r.r at the previous restart
+  int restarts = 0;
+  int floor_limited = 0;
+  explicit TrueResidual(double bar) : bar2(bar) {}
+  // true r.r of the current iterate -> whether to restart from it
+  bool restart(double rr2) {
+    last2 = rr2;
+    if (!(rr2 > bar2)) return false;  // met (or NaN: reported, no restart)
+    if (restarts >= kMaxRestarts || rr2 > 0.25 * prev2) {
+      floor_limited = 1;
+      return false;
+    }
+    prev2 = rr2;
+    ++restarts;
+    return true;
+  }
+  double segment_stop2() const { return std::max(bar2, 0.01 * last2); }
+};
+
 template <int PC, class Apply>
 int pcg_fp64_core(Apply&& apply, int apply_parts, int64_t n, int64_t units, const double* b,
                   double* x, const uint8_t* fixed, double tol, int maxit, const Layout& L,
@@ -2499,6 +2530,8 @@ int pcg_fp64_core(Apply&& apply, int apply_parts, int64_t n, int64_t units, cons
   st->b_norm = std::sqrt(bb);
   st->iterations = 0;
   st->residual_updates = 0;
+  st->restarts = 0;
+  st->floor_limited = 0;
   if (!std::isfinite(bb) || !std::isfinite(h.rr)) {
     cf::set_error("cf_pcg_solve: non-finite right-hand side");
     return CF_ERR_BREAKDOWN;
@@ -2519,42 +2552,61 @@ int pcg_fp64_core(Apply&& apply, int apply_parts, int64_t n, int64_t units, cons
   CF_HIP(hipMemcpyAsync(L.state, &h, sizeof(PcgState), hipMemcpyHostToDevice, s));
   int batch = 8;
   double prev_rr = h.rr;
-  const double rr_start = h.rr;
+  double rr_start = h.rr;
   int prev_it = 0;
   const int GV = pcg_vec_grid();
   int64_t enq = 0;
+  TrueResidual tr(tol * tol * bb);
   // three launches per iteration: product (+ p.Ap partials), update (folds
   // p.Ap itself), direction (folds r.r, r.z itself)
-  while (h.halt == kRunning) {
-    for (int t = 0; t < batch; ++t, ++enq) {
-      int tok;
-      prof_begin(0, h.iter + t, s, &tok);
-      if ((rc = apply(L.p, L.Ap, L.state, L.partials))) return rc;
-      prof_end(tok, s);
-      const int slot = int(enq & 1);
-      cg_update_fused<PC><<<GV, kBlock, 0, s>>>(units, x, L.r, L.p, L.Ap, L.dinv, L.state,
-                                                L.partials, apply_parts, L.partials_b, slot);
-      CF_LAUNCHED();
-      cg_direction_fused<PC><<<GV, kBlock, 0, s>>>(units, L.r, L.p, L.dinv, L.state,
-                                                   L.partials_b, GV, slot);
-      CF_LAUNCHED();
+  for (;;) {
+    while (h.halt == kRunning) {
+      for (int t = 0; t < batch; ++t, ++enq) {
+        int tok;
+        prof_begin(0, h.iter + t, s, &tok);
+        if ((rc = apply(L.p, L.Ap, L.state, L.partials))) return rc;
+        prof_end(tok, s);
+        const int slot = int(enq & 1);
+        cg_update_fused<PC><<<GV, kBlock, 0, s>>>(units, x, L.r, L.p, L.Ap, L.dinv, L.state,
+                                                  L.partials, apply_parts, L.partials_b, slot);
+        CF_LAUNCHED();
+        cg_direction_fused<PC><<<GV, kBlock, 0, s>>>(units, L.r, L.p, L.dinv, L.state,
+                                                     L.partials_b, GV, slot);
+        CF_LAUNCHED();
+      }
+      if ((rc = read_state(L.state, &h, s))) return rc;
+      prof_collect();
+      // size the next batch from the observed convergence rate
+      batch = next_batch(h, rr_start, prev_rr, prev_it, batch, 8, 512);
+      prev_rr = h.rr;
+      prev_it = h.iter;
     }
+    // true FP64 residual of the free system (r = b - A x, its r.r into pad)
+    if ((rc = apply(x, L.Ap, nullptr, nullptr))) return rc;
+    residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->pad);
+    CF_LAUNCHED();
+    if ((rc = fin(kFinStore, L, s, &L.state->pad))) return rc;
+    PcgState ts{};
+    if ((rc = read_state(L.state, &ts, s))) return rc;
+    tr.last2 = ts.pad;
+    if (h.halt != kConverged || !tr.restart(ts.pad)) break;
+    // residual replacement: restart CG from the true residual now in r
+    cg_start<PC><<<G, kBlock, 0, s>>>(units, L.r, L.p, L.dinv, L.state, L.partials);
+    CF_LAUNCHED();
+    if ((rc = fin(kFinStart, L, s))) return rc;
     if ((rc = read_state(L.state, &h, s))) return rc;
-    prof_collect();
-    // size the next batch from the observed convergence rate
-    batch = next_batch(h, rr_start, prev_rr, prev_it, batch, 8, 512);
-    prev_rr = h.rr;
+    h.stop2 = tr.segment_stop2();
+    h.halt = h.rr <= h.stop2 ? kConverged : (h.iter >= maxit ? kMaxit : kRunning);
+    CF_HIP(hipMemcpyAsync(L.state, &h, sizeof(PcgState), hipMemcpyHostToDevice, s));
+    enq = 0;
+    batch = 8;
+    prev_rr = rr_start = h.rr;
     prev_it = h.iter;
   }
   st->iterations = h.iter;
-  // true FP64 residual of the free system
-  if ((rc = apply(x, L.Ap, nullptr, nullptr))) return rc;
-  residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->pad);
-  CF_LAUNCHED();
-  if ((rc = fin(kFinStore, L, s, &L.state->pad))) return rc;
-  PcgState fin{};
-  if ((rc = read_state(L.state, &fin, s))) return rc;
-  st->rel_residual = std::sqrt(fin.pad / bb);
+  st->rel_residual = std::sqrt(tr.last2 / bb);
+  st->restarts = tr.restarts;
+  st->floor_limited = tr.floor_limited;
   if (h.halt == kBreakdown) {
     cf::set_error("cf_pcg_solve: breakdown after %d iterations (NaN/Inf or p.Ap <= 0)", h.iter);
     return CF_ERR_BREAKDOWN;
@@ -2744,6 +2796,8 @@ int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, do
   st->b_norm = std::sqrt(bb);
   st->iterations = 0;
   st->residual_updates = 0;
+  st->restarts = 0;
+  st->floor_limited = 0;
   if (!std::isfinite(bb) || !std::isfinite(h.rr)) {
     cf::set_error("cf_pcg_solve: non-finite right-hand side");
     return CF_ERR_BREAKDOWN;
@@ -2765,35 +2819,67 @@ int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, do
   // first direction: z_old = 0 and p = 0, so p = z, and r.z lands in rzs[0]
   CF_HIP(hipMemsetAsync(Z[0], 0, size_t(n) * sizeof(double), s));
   CF_HIP(hipMemsetAsync(L.p, 0, size_t(n) * sizeof(double), s));
-  if (h.halt == kRunning && (rc = precondition(0, Z[1], Z[0], 1, 1))) return rc;
+  int opened = 0;  // preconditioner applications that opened a CG segment
+  if (h.halt == kRunning) {
+    if ((rc = precondition(0, Z[1], Z[0], 1, 1))) return rc;
+    ++opened;
+  }
   int64_t enq = 0;
   int batch = 8;
   double prev_rr = h.rr;
-  const double rr_start = h.rr;
+  double rr_start = h.rr;
   int prev_it = 0;
-  while (h.halt == kRunning) {
-    for (int t = 0; t < batch; ++t, ++enq) {
-      int tok;
-      prof_begin(0, int(enq), s, &tok);
-      if ((rc = outer(L.p, L.Ap, L.state))) return rc;
-      prof_end(tok, s);
-      if ((rc = precondition(1, Z[enq & 1], Z[(enq + 1) & 1], int(enq), int(enq & 1)))) return rc;
+  TrueResidual tr(tol * tol * bb);
+  for (;;) {
+    while (h.halt == kRunning) {
+      for (int t = 0; t < batch; ++t, ++enq) {
+        int tok;
+        prof_begin(0, int(enq), s, &tok);
+        if ((rc = outer(L.p, L.Ap, L.state))) return rc;
+        prof_end(tok, s);
+        if ((rc = precondition(1, Z[enq & 1], Z[(enq + 1) & 1], int(enq), int(enq & 1))))
+          return rc;
+      }
+      if ((rc = read_state(L.state, &h, s))) return rc;
+      prof_collect();
+      batch = next_batch(h, rr_start, prev_rr, prev_it, batch, 4, 128);
+      prev_rr = h.rr;
+      prev_it = h.iter;
     }
-    if ((rc = read_state(L.state, &h, s))) return rc;
-    prof_collect();
-    batch = next_batch(h, rr_start, prev_rr, prev_it, batch, 4, 128);
-    prev_rr = h.rr;
+    // true FP64 residual (r = b - A x, its r.r into pad)
+    if ((rc = outer(x, L.Ap, nullptr))) return rc;
+    residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->pad);
+    CF_LAUNCHED();
+    if ((rc = fin(kFinStore, L, s, &L.state->pad))) return rc;
+    PcgState fin_state{};
+    if ((rc = read_state(L.state, &fin_state, s))) return rc;
+    tr.last2 = fin_state.pad;
+    if (h.halt != kConverged || !tr.restart(fin_state.pad)) break;
+    // residual replacement: a fresh flexible CG from the true residual in r
+    h.stop2 = tr.segment_stop2();
+    h.rr = fin_state.pad;
+    h.rz = 1.0;
+    h.rzs[0] = h.rzs[1] = 1.0;
+    h.halt = h.rr <= h.stop2 ? kConverged : (h.iter >= maxit ? kMaxit : kRunning);
+    CF_HIP(hipMemcpyAsync(L.state, &h, sizeof(PcgState), hipMemcpyHostToDevice, s));
+    CF_HIP(hipMemsetAsync(Z[0], 0, size_t(n) * sizeof(double), s));
+    CF_HIP(hipMemsetAsync(L.p, 0, size_t(n) * sizeof(double), s));
+    if (h.halt == kRunning) {
+      if ((rc = precondition(0, Z[1], Z[0], 1, 1))) return rc;
+      ++opened;
+    }
+    enq = 0;
+    batch = 8;
+    prev_rr = rr_start = h.rr;
     prev_it = h.iter;
   }
   st->iterations = h.iter;
-  st->residual_updates = (shape.steps - 1 + extra_products) * (h.iter + 1);  // low-precision products
-  if ((rc = outer(x, L.Ap, nullptr))) return rc;
-  residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->pad);
-  CF_LAUNCHED();
-  if ((rc = fin(kFinStore, L, s, &L.state->pad))) return rc;
-  PcgState fin_state{};
-  if ((rc = read_state(L.state, &fin_state, s))) return rc;
-  st->rel_residual = std::sqrt(fin_state.pad / bb);
+  // low-precision products: every outer iteration and every segment start
+  // applies the polynomial once (steps - 1 products, plus the two-level extras)
+  st->residual_updates = (shape.steps - 1 + extra_products) * (h.iter + opened);
+  st->rel_residual = std::sqrt(tr.last2 / bb);
+  st->restarts = tr.restarts;
+  st->floor_limited = tr.floor_limited;
   if (h.halt == kBreakdown) {
     cf::set_error("cf_pcg_solve: flexible CG breakdown after %d iterations (NaN/Inf, "
                   "p.Ap <= 0 or r.z <= 0)", h.iter);
@@ -2895,6 +2981,8 @@ int pcg_mixed(const cf_matrix* A, const double* b, double* x, const uint8_t* fix
   st->b_norm = std::sqrt(bb);
   st->iterations = 0;
   st->residual_updates = 0;
+  st->restarts = 0;
+  st->floor_limited = 0;
   if (!std::isfinite(bb) || !std::isfinite(h.rr)) {
     cf::set_error("cf_pcg_solve: non-finite right-hand side");
     return CF_ERR_BREAKDOWN;

@@ -65,6 +65,15 @@ def solver_options(**kw):
 last_stats = threading.local()
 
 
+def _base_stats(st):
+    """Fields every solver reports.  floor_limited: the true FP64 residual
+    stopped falling above tol (tol below the attainable FP64 floor of the
+    system); rel_residual is then that true residual."""
+    return {"iterations": st.iterations, "residual_updates": st.residual_updates,
+            "rel_residual": st.rel_residual, "b_norm": st.b_norm,
+            "restarts": st.restarts, "floor_limited": bool(st.floor_limited)}
+
+
 def _pc_code(precond):
     codes = {"jacobi": N.PC_JACOBI, "block3": N.PC_BLOCK3}
     if precond not in codes:
@@ -119,8 +128,7 @@ def pcg(A, b, x, fixed=None, precision=None, precond=None, tol=None, maxit=None)
     st = N.CfPcgStats()
     rc = lib.cf_pcg_solve(ctypes.byref(As), ptr(b), ptr(x), ptr(fixed), prec, pc, tol, maxit,
                           ptr(ws), nb.value, ctypes.byref(st), stream_handle())
-    stats = {"iterations": st.iterations, "residual_updates": st.residual_updates,
-             "rel_residual": st.rel_residual, "b_norm": st.b_norm,
+    stats = {**_base_stats(st),
              "precision": precision,
              "precond": "block3" if pc == N.PC_BLOCK3 else "jacobi", "tol": tol}
     last_stats.value = stats
@@ -145,8 +153,7 @@ def pcg_matfree_t10_mixed(dm, lam, mu, b, x, fixed=None, tol=None, maxit=None):
                                        float(mu), ptr(offs), ptr(perm), ptr(b), ptr(x),
                                        ptr(fixed), tol, maxit, ptr(ws), nb.value,
                                        ctypes.byref(st), stream_handle())
-    stats = {"iterations": st.iterations, "residual_updates": st.residual_updates,
-             "rel_residual": st.rel_residual, "b_norm": st.b_norm, "precision": "mixed",
+    stats = {**_base_stats(st), "precision": "mixed",
              "precond": "jacobi", "tol": tol, "operator": "matfree"}
     last_stats.value = stats
     N.check(rc, "cf_pcg_solve_mf_t10_mixed")
@@ -172,8 +179,7 @@ def pcg_matfree_t10_twolevel(dm, lam, mu, b, x, fixed=None, tol=None, maxit=None
                                           float(mu), ptr(offs), ptr(perm), ptr(dm.x), ptr(dm.y),
                                           ptr(dm.z), ptr(b), ptr(x), ptr(fixed), tol, maxit,
                                           ptr(ws), nb.value, ctypes.byref(st), stream_handle())
-    stats = {"iterations": st.iterations, "residual_updates": st.residual_updates,
-             "rel_residual": st.rel_residual, "b_norm": st.b_norm, "precision": "mixed",
+    stats = {**_base_stats(st), "precision": "mixed",
              "precond": "two-level", "tol": tol, "operator": "matfree"}
     last_stats.value = stats
     N.check(rc, "cf_pcg_solve_mf_t10_twolevel")
@@ -207,8 +213,7 @@ def pcg_matfree_t10(dm, lam, mu, b, x, fixed=None, precond=None, tol=None, maxit
                                           float(lam), float(mu), ptr(b), ptr(x), ptr(fixed), pc,
                                           tol, maxit, ptr(ws), nb.value, ctypes.byref(st),
                                           stream_handle())
-        stats = {"iterations": st.iterations, "residual_updates": 0,
-                 "rel_residual": st.rel_residual, "b_norm": st.b_norm, "precision": "fp64",
+        stats = {**_base_stats(st), "residual_updates": 0, "precision": "fp64",
                  "precond": precond, "tol": tol, "operator": "matfree"}
         last_stats.value = stats
         N.check(rc, "cf_pcg_solve_mf_t10_plan")
@@ -223,8 +228,7 @@ def pcg_matfree_t10(dm, lam, mu, b, x, fixed=None, precond=None, tol=None, maxit
                                  float(lam), float(mu), ptr(offs), ptr(perm), ptr(b), ptr(x),
                                  ptr(fixed), pc, tol, maxit, ptr(ws), nb.value,
                                  ctypes.byref(st), stream_handle())
-    stats = {"iterations": st.iterations, "residual_updates": 0,
-             "rel_residual": st.rel_residual, "b_norm": st.b_norm, "precision": "fp64",
+    stats = {**_base_stats(st), "residual_updates": 0, "precision": "fp64",
              "precond": precond, "tol": tol, "operator": "matfree"}
     last_stats.value = stats
     N.check(rc, "cf_pcg_solve_mf_t10")

@@ -101,10 +101,18 @@ typedef struct cf_matrix {
 } cf_matrix;
 
 typedef struct cf_pcg_stats {
-  int32_t iterations;        /* inner iterations performed                   */
-  int32_t residual_updates;  /* FP64 residual replacements (mixed only)      */
-  double rel_residual;       /* ||b_f - A_ff x_f|| / ||b_f||, FP64, at exit  */
+  int32_t iterations;        /* CG iterations (flexible CG: outer FP64 ones)  */
+  int32_t residual_updates;  /* scalar CSR mixed: FP64 residual replacements;
+                                Chebyshev-preconditioned solvers: products in
+                                low precision                               */
+  double rel_residual;       /* true ||b_f - A_ff x_f|| / ||b_f||, FP64, at exit */
   double b_norm;             /* ||b_f||                                      */
+  int32_t restarts;          /* FP64 and flexible CG: restarts from the true
+                                residual after the recursive one met tol     */
+  int32_t floor_limited;     /* 1: a restart no longer halved the true
+                                residual, i.e. tol lies below what FP64 resolves
+                                for this system (~eps ||A_ff| |x_f|| / ||b_f||);
+                                the call returns CF_OK with rel_residual > tol */
 } cf_pcg_stats;
 
 /* ---- library ------------------------------------------------------------ */

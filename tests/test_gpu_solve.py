@@ -7,7 +7,8 @@ import pytest
 from oracle import elastic as oel
 from oracle import meshes as om
 from oracle import p1 as op1
-from tests._helpers import FACES, assert_csr_parity, assert_vec_close, rel_err
+from tests._helpers import (FACES, assert_csr_parity, assert_residual_bar, assert_vec_close,
+                            fp64_floor, rel_err)
 
 pytestmark = pytest.mark.gpu
 
@@ -78,15 +79,16 @@ def test_tet10_elasticity_matches_spsolve(gpu, precision, precond):
 
 
 @pytest.mark.parametrize("precision", ["mixed", "mixed16"])
-@pytest.mark.parametrize("dims,tol", [((8, 2, 2), 1e-10), ((20, 3, 3), 1e-9)])
-def test_mixed_precision_meets_fp64_residual(gpu, precision, dims, tol):
+@pytest.mark.parametrize("dims", [(8, 2, 2), (20, 3, 3)])
+def test_mixed_precision_meets_fp64_residual(gpu, precision, dims):
     """Mixed precision (FP64 flexible CG, FP32 Chebyshev preconditioner on a
     FP32- or FP16-stored copy of the scaled matrix) reaches the FP64 residual
-    bar; its error against SuperLU is stated, not assumed.  The solver stops
-    on the recursive residual; the true residual recomputed at exit must meet
-    the bar or, where the bar is within rounding of the mesh's FP64 floor
-    eps ||A_ff| |x_f|| / ||b_f|| (7.7e-11 at 8x2x2, 1.3e-10 at 20x3x3), three
-    times that floor."""
+    bar and agrees with SuperLU to 1e-10.  tol 1e-12 lies below the FP64 floor
+    eps ||A_ff| |x_f|| / ||b_f|| of these cantilevers (7.7e-11 at 8x2x2,
+    1.3e-10 at 20x3x3): the solver restarts from the true residual until it
+    no longer falls, and must then report floor_limited with the true
+    residual within 3x the floor."""
+    tol = 1e-12
     from cloudfem_amd import solver
     from cloudfem_amd.mesh import box_mesh_tet10
     from cloudfem_amd.physics import elastic_tet10
@@ -103,19 +105,21 @@ def test_mixed_precision_meets_fp64_residual(gpu, precision, dims, tol):
     u0 = oel.apply_dirichlet_vec3(A0, rhs0, m0, bc)
     err = rel_err(u, u0)
     Aff, bf, free = oel.reduced_system_vec3(A0, rhs0, m0, bc)[:3]
-    floor = (np.finfo(np.float64).eps * np.linalg.norm(abs(Aff) @ np.abs(u0[free]))
-             / np.linalg.norm(bf))
+    floor = fp64_floor(Aff, u0[free], bf)
     print(f"{precision} {dims}: {st['iterations']} FP64 iterations, {st['residual_updates']} "
-          f"low-precision products, rel. residual {st['rel_residual']:.2e} (FP64 floor "
-          f"{floor:.2e}), error vs SuperLU {err:.2e}")
-    assert st["rel_residual"] <= max(tol, 3 * floor) and st["residual_updates"] >= 1
-    assert err < 1e-8
+          f"low-precision products, {st['restarts']} restarts, rel. residual "
+          f"{st['rel_residual']:.2e} (FP64 floor {floor:.2e}), error vs SuperLU {err:.2e}")
+    assert_residual_bar(st, tol, floor)
+    assert st["residual_updates"] >= 1
+    assert err < 1e-10
 
 
-@pytest.mark.parametrize("dims,tol", [((8, 2, 2), 1e-10), ((20, 3, 3), 1e-9)])
-def test_matfree_mixed_meets_fp64_residual(gpu, dims, tol):
+@pytest.mark.parametrize("dims", [(8, 2, 2), (20, 3, 3)])
+def test_matfree_mixed_meets_fp64_residual(gpu, dims):
     """FP64 flexible CG on the matrix-free operator with the FP32 matrix-free
-    Chebyshev preconditioner: FP64 residual bar, error against SuperLU."""
+    Chebyshev preconditioner: FP64 residual bar (tol 1e-12, or floor_limited
+    within 3x the FP64 floor), error against SuperLU below 1e-10."""
+    tol = 1e-12
     from cloudfem_amd import solver
     from cloudfem_amd.mesh import box_mesh_tet10
     from cloudfem_amd.physics import elastic_tet10
@@ -128,31 +132,31 @@ def test_matfree_mixed_meets_fp64_residual(gpu, dims, tol):
                      axis=1).ravel()
     st = solver.last_stats.value
     assert st["operator"] == "matfree" and st["precision"] == "mixed"
-    # the stop test is the recursive FP64 residual; the true residual recomputed
-    # at exit may exceed it by the FP64 rounding of the recurrence (8x2x2 at
-    # 1e-10 sits at that mesh's floor)
     m0 = om.box_mesh_tet10(*dims, **lims)
     A0, rhs0 = oel.assemble_elasticity_tet10(m0, 210e9, 0.3, traction=trac)
     u0 = oel.apply_dirichlet_vec3(A0, rhs0, m0, bc)
     err = rel_err(u, u0)
+    Aff, bf, free = oel.reduced_system_vec3(A0, rhs0, m0, bc)[:3]
+    floor = fp64_floor(Aff, u0[free], bf)
     print(f"matfree mixed {dims}: {st['iterations']} FP64 iterations, "
-          f"{st['residual_updates']} FP32 products, rel. residual {st['rel_residual']:.2e}, "
-          f"error vs SuperLU {err:.2e}")
-    assert st["rel_residual"] <= 2 * tol and st["residual_updates"] >= 1
-    assert err < 1e-8
+          f"{st['residual_updates']} FP32 products, {st['restarts']} restarts, rel. residual "
+          f"{st['rel_residual']:.2e} (floor {floor:.2e}), error vs SuperLU {err:.2e}")
+    assert_residual_bar(st, tol, floor)
+    assert st["residual_updates"] >= 1
+    assert err < 1e-10
 
 
 @pytest.mark.parametrize("dims", [(8, 2, 2), (20, 3, 3), (40, 5, 5)])
 def test_matfree_twolevel_meets_fp64_residual(gpu, dims):
     """Two-level mixed solver (rigid-body coarse correction around the FP32
-    matrix-free Chebyshev polynomial): FP64 residual bar (3x the FP64 floor
-    where the bar is within rounding of it, as above) and the solution
-    against SuperLU.  8x2x2 has one aggregate, 40x5x5 four (groups of
-    1,536 elements)."""
+    matrix-free Chebyshev polynomial): FP64 residual bar (tol 1e-12, or
+    floor_limited within 3x the FP64 floor) and the solution against SuperLU
+    below 1e-10.  8x2x2 has one aggregate, 40x5x5 four (groups of 1,536
+    elements)."""
     from cloudfem_amd import solver
     from cloudfem_amd.mesh import box_mesh_tet10
     from cloudfem_amd.physics import elastic_tet10
-    tol = 1e-10
+    tol = 1e-12
     lims = dict(xlim=(0.0, 0.4), ylim=(0.0, 0.05), zlim=(0.0, 0.05))
     trac = {"xmax": (0.0, 0.0, -1e6)}
     bc = {"xmin": (0.0, 0.0, 0.0)}
@@ -167,14 +171,13 @@ def test_matfree_twolevel_meets_fp64_residual(gpu, dims):
     A0, rhs0 = oel.assemble_elasticity_tet10(m0, 210e9, 0.3, traction=trac)
     u0 = oel.apply_dirichlet_vec3(A0, rhs0, m0, bc)
     Aff, bf, free = oel.reduced_system_vec3(A0, rhs0, m0, bc)[:3]
-    floor = (np.finfo(np.float64).eps * np.linalg.norm(abs(Aff) @ np.abs(u0[free]))
-             / np.linalg.norm(bf))
+    floor = fp64_floor(Aff, u0[free], bf)
     err = rel_err(u, u0)
     print(f"two-level {dims}: {st['iterations']} FP64 iterations, {st['residual_updates']} "
-          f"FP32 products, rel. residual {st['rel_residual']:.2e} (floor {floor:.2e}), "
-          f"error vs SuperLU {err:.2e}")
-    assert st["rel_residual"] <= max(2 * tol, 3 * floor)
-    assert err < 1e-8
+          f"FP32 products, {st['restarts']} restarts, rel. residual {st['rel_residual']:.2e} "
+          f"(floor {floor:.2e}), error vs SuperLU {err:.2e}")
+    assert_residual_bar(st, tol, floor)
+    assert err < 1e-10
 
 
 def test_twolevel_jittered_mesh_body_load(gpu):
@@ -192,17 +195,19 @@ def test_twolevel_jittered_mesh_body_load(gpu):
     body = (0.0, 0.0, -7850.0 * 9.81)
     bc = {"xmin": (0.0, 0.0, 0.0)}
     with solver.solver_options(operator="matfree", precision="mixed", precond="twolevel",
-                               tol=1e-10):
+                               tol=1e-12):
         u = np.stack(elastic_tet10.solve_elasticity_tet10(m, 210e9, 0.3, bc=bc, body=body),
                      axis=1).ravel()
     st = solver.last_stats.value
     A0, rhs0 = oel.assemble_elasticity_tet10(m, 210e9, 0.3, body=body)
     u0 = oel.apply_dirichlet_vec3(A0, rhs0, m, bc)
     err = rel_err(u, u0)
+    Aff, bf, free = oel.reduced_system_vec3(A0, rhs0, m, bc)[:3]
+    floor = fp64_floor(Aff, u0[free], bf)
     print(f"two-level jittered: {st['iterations']} FP64 iterations, rel. residual "
-          f"{st['rel_residual']:.2e}, error vs SuperLU {err:.2e}")
-    assert st["rel_residual"] <= 1e-9
-    assert err < 1e-8
+          f"{st['rel_residual']:.2e} (floor {floor:.2e}), error vs SuperLU {err:.2e}")
+    assert_residual_bar(st, 1e-12, floor)
+    assert err < 1e-10
 
 
 @pytest.mark.parametrize("dims", [(40, 5, 5), (80, 6, 6)])
@@ -221,11 +226,13 @@ def test_twolevel_coarse_factor_paths(gpu, monkeypatch, dims):
     m0 = om.box_mesh_tet10(*dims, **lims)
     A0, rhs0 = oel.assemble_elasticity_tet10(m0, 210e9, 0.3, traction=trac)
     u0 = oel.apply_dirichlet_vec3(A0, rhs0, m0, bc)
+    Aff, bf, free = oel.reduced_system_vec3(A0, rhs0, m0, bc)[:3]
+    floor = fp64_floor(Aff, u0[free], bf)
     its = {}
     for path in ("auto", "ring", "dense"):
         monkeypatch.setenv("CF_COARSE_FACTOR", path)
         with solver.solver_options(operator="matfree", precision="mixed", precond="twolevel",
-                                   tol=1e-10):
+                                   tol=1e-12):
             u = np.stack(elastic_tet10.solve_elasticity_tet10(m, 210e9, 0.3, bc=bc,
                                                               traction=trac),
                          axis=1).ravel()
@@ -233,10 +240,10 @@ def test_twolevel_coarse_factor_paths(gpu, monkeypatch, dims):
         its[path] = st["iterations"]
         err = rel_err(u, u0)
         print(f"{dims} coarse factor {path}: {st['iterations']} iterations, rel. residual "
-              f"{st['rel_residual']:.2e}, error vs SuperLU {err:.2e}")
-        assert st["rel_residual"] <= 1e-9
-        assert err < 1e-8
-    assert max(its.values()) - min(its.values()) <= 2
+              f"{st['rel_residual']:.2e} (floor {floor:.2e}), error vs SuperLU {err:.2e}")
+        assert_residual_bar(st, 1e-12, floor)
+        assert err < 1e-10
+    assert max(its.values()) - min(its.values()) <= 0.1 * min(its.values()) + 2
 
 
 def test_twolevel_scattered_node_numbering(gpu):
@@ -261,17 +268,18 @@ def test_twolevel_scattered_node_numbering(gpu):
     sol = {}
     for name, mesh in (("original", m), ("permuted", mp)):
         with solver.solver_options(operator="matfree", precision="mixed", precond="twolevel",
-                                   tol=1e-10):
+                                   tol=1e-12):
             u = np.stack(elastic_tet10.solve_elasticity_tet10(mesh, 210e9, 0.3, bc=bc,
                                                               traction=trac), axis=1)
         st = solver.last_stats.value
-        print(f"{name}: {st['iterations']} outer iterations, rel. residual "
-              f"{st['rel_residual']:.2e}")
+        print(f"{name}: {st['iterations']} outer iterations, {st['restarts']} restarts, "
+              f"rel. residual {st['rel_residual']:.2e}, floor_limited {st['floor_limited']}")
+        assert st["rel_residual"] <= 1e-12 or st["floor_limited"]
         assert st["rel_residual"] <= 1e-9
         sol[name] = u if name == "original" else u[perm]
     err = rel_err(sol["permuted"].ravel(), sol["original"].ravel())
     print(f"permuted vs original numbering: {err:.2e}")
-    assert err < 1e-8
+    assert err < 1e-10
 
 
 def test_twolevel_fewer_iterations_than_chebyshev(gpu):
@@ -302,12 +310,12 @@ def test_p1_mixed_precision_matches_spsolve(gpu, precision):
     from cloudfem_amd.mesh import box_mesh
     src = lambda x, y, z: 3 * np.pi ** 2 * np.sin(np.pi * x) * np.sin(np.pi * y) * np.sin(np.pi * z)  # noqa: E731
     bc = {f: 0.0 for f in FACES}
-    with solver.solver_options(precision=precision, tol=1e-10):
+    with solver.solver_options(precision=precision, tol=1e-12):
         u = solve_tet_poisson(box_mesh(14, 14, 14), 1.0, src, bc)
     st = solver.last_stats.value
     u0 = op1.solve_tet_poisson(om.box_mesh(14, 14, 14), 1.0, src, bc)
-    assert st["rel_residual"] <= 1e-10
-    assert rel_err(u, u0) < 1e-8
+    assert st["rel_residual"] <= 1e-12
+    assert rel_err(u, u0) < 1e-10
 
 
 def test_tet10_cantilever_vs_beam_theory(gpu):
@@ -416,8 +424,7 @@ def test_chunked_matfree_solve_matches_two_pass(gpu, dims):
     for chunked in (False, True):
         x = torch.zeros_like(b)
         st = pcg_matfree_t10(dm, lam, mu, b, x, fixed, tol=1e-12, chunked=chunked)
-        # true residual; the stop test is 1e-12 on the recursive one, and the
-        # FP64 floor of these cantilevers is ~1e-10 (SuperLU's own residual)
-        assert st["rel_residual"] < 1e-9
+        # true residual: tol, or the FP64 floor of these cantilevers (~1e-10)
+        assert st["rel_residual"] <= 1e-12 or (st["floor_limited"] and st["rel_residual"] < 1e-9)
         xs.append(x)
-    assert float((xs[0] - xs[1]).abs().max() / xs[0].abs().max()) < 1e-9
+    assert float((xs[0] - xs[1]).abs().max() / xs[0].abs().max()) < 1e-10
