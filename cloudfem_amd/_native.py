@@ -70,6 +70,7 @@ EXPORTED = (
     "cf_mesh_workspace_bytes", "cf_tet10_edges", "cf_edge_midpoints", "cf_boundary_faces",
     "cf_mf_t10_mixed_workspace_bytes", "cf_pcg_solve_mf_t10_mixed",
     "cf_mf_t10_twolevel_workspace_bytes", "cf_pcg_solve_mf_t10_twolevel",
+    "cf_t10_twolevel_workspace_bytes", "cf_pcg_solve_t10_twolevel", "cf_spmv_abs",
 )
 
 
@@ -162,6 +163,12 @@ _SIGS = {
     "cf_pcg_solve_mf_t10_twolevel": ([_I64, _I64, _P, _P, _D, _D, _P, _P, _P, _P, _P, _P, _P, _P,
                                       _D, _I32, _P, ctypes.c_size_t, ctypes.POINTER(CfPcgStats),
                                       _P], _I32),
+    "cf_t10_twolevel_workspace_bytes": ([ctypes.POINTER(CfMatrix), _I64, _I64, _I32, _SZ],
+                                        _I32),
+    "cf_spmv_abs": ([ctypes.POINTER(CfMatrix), _P, _P, _P, _P], _I32),
+    "cf_pcg_solve_t10_twolevel": ([ctypes.POINTER(CfMatrix), _I64, _I64, _P, _P, _D, _D, _P, _P, _P, _P, _P, _P, _P,
+                                   _P, _I32, _D, _I32, _P, ctypes.c_size_t,
+                                   ctypes.POINTER(CfPcgStats), _P], _I32),
     "cf_tet10_edges": ([_I64, _P, _I32, _I64, _P, _P, _P, _P, _P, ctypes.c_size_t, _P], _I32),
     "cf_edge_midpoints": ([_I64, _I64, _P, _P, _P, _P, _P, _P], _I32),
     "cf_boundary_faces": ([_I64, _P, _I32, _I64, _P, _I32, _P, _P, _P, ctypes.c_size_t, _P], _I32),

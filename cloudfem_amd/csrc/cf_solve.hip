@@ -623,12 +623,12 @@ __global__ void mx_accumulate(int64_t n, double* __restrict__ x, float* __restri
 // first iterate d = w / theta; z is not written (the steps build it from d),
 // so a matrix-free product reads its input without scaling it and the final
 // z = S zh is zh itself.
-template <bool SQ>
+template <bool SQ, class T>
 __global__ void __launch_bounds__(kBlock)
 fcg_update(int64_t n, double* __restrict__ x, double* __restrict__ r,
            const double* __restrict__ p, const double* __restrict__ Ap,
-           const double* __restrict__ sc, float* __restrict__ w, float* __restrict__ d,
-           float* __restrict__ z, float inv_theta, int apply_step, PcgState* state,
+           const double* __restrict__ sc, T* __restrict__ w, T* __restrict__ d,
+           T* __restrict__ z, T inv_theta, int apply_step, PcgState* state,
            double* __restrict__ partials) {
   if (apply_step && state->halt) return;
   const double alpha = apply_step ? state->alpha : 0.0;
@@ -642,9 +642,9 @@ fcg_update(int64_t n, double* __restrict__ x, double* __restrict__ r,
       r[i] = rn;
     }
     rr += rn * rn;
-    const float wv = float(SQ ? sc[i] * sc[i] * rn : sc[i] * rn);
+    const T wv = T(SQ ? sc[i] * sc[i] * rn : sc[i] * rn);
     w[i] = wv;
-    const float dv = wv * inv_theta;
+    const T dv = wv * inv_theta;
     d[i] = dv;
     if (!SQ) z[i] = dv;
   }
@@ -688,12 +688,12 @@ fcg_direction(int64_t n, const double* __restrict__ z, double* __restrict__ p,
 
 // alpha = rzs[slot] / p.Ap (n_pa product partials); x, r update; w, d, z
 // start of the Chebyshev iteration; r.r partials into pb.
-template <bool SQ>
+template <bool SQ, class T>
 __global__ void __launch_bounds__(kBlock)
 fcg_update_fused(int64_t n, double* __restrict__ x, double* __restrict__ r,
                  const double* __restrict__ p, const double* __restrict__ Ap,
-                 const double* __restrict__ sc, float* __restrict__ w, float* __restrict__ d,
-                 float* __restrict__ z, float inv_theta, PcgState* state,
+                 const double* __restrict__ sc, T* __restrict__ w, T* __restrict__ d,
+                 T* __restrict__ z, T inv_theta, PcgState* state,
                  const double* __restrict__ pa, int n_pa, double* __restrict__ pb, int slot) {
   if (state->halt) return;
   double f[1];
@@ -716,9 +716,9 @@ fcg_update_fused(int64_t n, double* __restrict__ x, double* __restrict__ r,
     const double rn = r[i] - alpha * Ap[i];
     r[i] = rn;
     rr += rn * rn;
-    const float wv = float(SQ ? sc[i] * sc[i] * rn : sc[i] * rn);
+    const T wv = T(SQ ? sc[i] * sc[i] * rn : sc[i] * rn);
     w[i] = wv;
-    const float dv = wv * inv_theta;
+    const T dv = wv * inv_theta;
     d[i] = dv;
     if (!SQ) z[i] = dv;
   }
@@ -728,10 +728,10 @@ fcg_update_fused(int64_t n, double* __restrict__ x, double* __restrict__ r,
 
 // r.r from fcg_update_fused (when `count`: stop test, block 0 records iter,
 // rr, halt); z = S zh with (r.z, r.z_old) partials into pc.
-template <bool SQ>
+template <bool SQ, class T>
 __global__ void __launch_bounds__(kBlock)
 fcg_zdots_fused(int64_t n, const double* __restrict__ r, const double* __restrict__ sc,
-                const float* __restrict__ zh, double* __restrict__ z,
+                const T* __restrict__ zh, double* __restrict__ z,
                 const double* __restrict__ zold, PcgState* state, const double* __restrict__ pb,
                 int n_pb, int count, double* __restrict__ pc) {
   if (state->halt) return;
@@ -793,11 +793,11 @@ fcg_direction_fused(int64_t n, const double* __restrict__ z, double* __restrict_
 // Two-level form of the outer update: alpha from the product partials (when
 // APPLY), x += alpha p, r -= alpha Ap, r.r partials into pb, and an FP32 copy
 // of r for the preconditioner.
-template <bool APPLY>
+template <bool APPLY, class T>
 __global__ void __launch_bounds__(kBlock)
 fcg_update2l(int64_t n, double* __restrict__ x, double* __restrict__ r,
              const double* __restrict__ p, const double* __restrict__ Ap,
-             float* __restrict__ r32, PcgState* state, const double* __restrict__ pa, int n_pa,
+             T* __restrict__ r32, PcgState* state, const double* __restrict__ pa, int n_pa,
              double* __restrict__ pb, int slot) {
   if (state->halt) return;
   double alpha = 0.0;
@@ -826,14 +826,15 @@ fcg_update2l(int64_t n, double* __restrict__ x, double* __restrict__ r,
       r[i] = rn;
     }
     rr += rn * rn;
-    r32[i] = float(rn);
+    r32[i] = T(rn);
   }
   double v1[1] = {rr};
   block_partials<1>(v1, pb);
 }
 
-__global__ void vec_add32(int64_t n, float* __restrict__ u, const float* __restrict__ v,
-                          const PcgState* state) {
+template <class T>
+__global__ void vec_add(int64_t n, T* __restrict__ u, const T* __restrict__ v,
+                        const PcgState* state) {
   if (state && state->halt) return;
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x)
@@ -842,16 +843,18 @@ __global__ void vec_add32(int64_t n, float* __restrict__ u, const float* __restr
 
 // Power iteration on S A S for the top of the spectrum: v0 = S 1 spread by a
 // fixed hash, then v = y / ||y|| with y = S A S v.
-__global__ void power_start(int64_t n, const double* __restrict__ sc, float* __restrict__ v) {
+template <class T>
+__global__ void power_start(int64_t n, const double* __restrict__ sc, T* __restrict__ v) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x) {
     const uint32_t hsh = uint32_t(i) * 2654435761u;
-    v[i] = sc[i] > 0.0 ? 1.0f + float(hsh >> 22) * (1.0f / 1024.0f) : 0.0f;
+    v[i] = sc[i] > 0.0 ? T(1.0f + float(hsh >> 22) * (1.0f / 1024.0f)) : T(0);
   }
 }
 
+template <class T>
 __global__ void __launch_bounds__(kBlock)
-norm2_partials(int64_t n, const float* __restrict__ y, double* __restrict__ partials) {
+norm2_partials(int64_t n, const T* __restrict__ y, double* __restrict__ partials) {
   double a = 0.0;
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x)
@@ -860,9 +863,10 @@ norm2_partials(int64_t n, const float* __restrict__ y, double* __restrict__ part
   block_partials<1>(v1, partials);
 }
 
-__global__ void power_scale(int64_t n, const float* __restrict__ y, float* __restrict__ v,
+template <class T>
+__global__ void power_scale(int64_t n, const T* __restrict__ y, T* __restrict__ v,
                             const PcgState* state) {
-  const float inv = state->pad > 0.0 ? float(1.0 / sqrt(state->pad)) : 0.0f;
+  const T inv = state->pad > 0.0 ? T(1.0 / sqrt(state->pad)) : T(0);
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x)
     v[i] = y[i] * inv;
@@ -1517,36 +1521,31 @@ mf_shared_gather(int64_t n_sn, const int32_t* __restrict__ shared_node,
 // the node kernel scales its force sum by S and applies the Chebyshev
 // update of cheb_step (or, for the power iteration, just stores S K S d).
 
-__global__ void geo_to_f32(int64_t n, const double* __restrict__ g, float* __restrict__ g32) {
-  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
-       i += int64_t(gridDim.x) * blockDim.x)
-    g32[i] = float(g[i]);
-}
 
 // PRE: d is already S-scaled (the Chebyshev vectors of the SQ form), so
 // the kernel gathers it as is; otherwise it gathers S d.  Writes the run
 // partials described above.
-template <bool PRE>
+template <class T, bool PRE>
 __global__ void __launch_bounds__(kBlock)
-mf_t10_elem32(int64_t m, const int32_t* __restrict__ conn, const float* __restrict__ geo,
-              float lam, float mu, const float* __restrict__ sc, const float* __restrict__ d,
-              const uint16_t* __restrict__ rank, const int32_t* __restrict__ toff,
-              const uint16_t* __restrict__ tstart, float* __restrict__ part,
+mf_t10_elem_lp(int64_t m, const int32_t* __restrict__ conn, const T* __restrict__ geo,
+               T lam, T mu, const T* __restrict__ sc, const T* __restrict__ d,
+               const uint16_t* __restrict__ rank, const int32_t* __restrict__ toff,
+               const uint16_t* __restrict__ tstart, T* __restrict__ part,
               const PcgState* state) {
   if (state && state->halt) return;
-  __shared__ float fl[kTileItems * 3];
+  __shared__ T fl[kTileItems * 3];
   const int64_t n_tiles = (m + kBlock - 1) / kBlock;
   for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < n_tiles; t += gridDim.x) {
     const int64_t e = t * kBlock + threadIdx.x;
     if (e < m) {
-      const float* g = geo + e * 10;
-      float J[3][3];
+      const T* g = geo + e * 10;
+      T J[3][3];
 #pragma unroll
       for (int r = 0; r < 3; ++r)
 #pragma unroll
         for (int c = 0; c < 3; ++c) J[r][c] = g[3 * r + c];
-      const float V = g[9];
-      float xa[10][3];
+      const T V = g[9];
+      T xa[10][3];
 #pragma unroll
       for (int a = 0; a < 10; ++a) {
         const int64_t nd = conn[e * 10 + a];
@@ -1554,8 +1553,8 @@ mf_t10_elem32(int64_t m, const int32_t* __restrict__ conn, const float* __restri
         for (int c = 0; c < 3; ++c)
           xa[a][c] = PRE ? d[3 * nd + c] : sc[3 * nd + c] * d[3 * nd + c];
       }
-      float f[10][3];
-      t10_forces_4pt<float>(xa, J, V, lam, mu, f);
+      T f[10][3];
+      t10_forces_4pt<T>(xa, J, V, lam, mu, f);
 #pragma unroll
       for (int a = 0; a < 10; ++a) {
         const int r = rank[e * 10 + a];
@@ -1571,13 +1570,13 @@ mf_t10_elem32(int64_t m, const int32_t* __restrict__ conn, const float* __restri
     const int32_t q0 = toff[t], q1 = toff[t + 1];
     for (int32_t q = q0 + threadIdx.x; q < q1; q += kBlock) {
       const int kb = tstart[q], ke = q + 1 < q1 ? int(tstart[q + 1]) : nt;
-      float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+      T s0 = T(0), s1 = T(0), s2 = T(0);
       for (int k = kb; k < ke; ++k) {
         s0 += fl[3 * k];
         s1 += fl[3 * k + 1];
         s2 += fl[3 * k + 2];
       }
-      float* o = part + 3 * int64_t(q);
+      T* o = part + 3 * int64_t(q);
       o[0] = s0;
       o[1] = s1;
       o[2] = s2;
@@ -1606,18 +1605,19 @@ mf_t10_elem32(int64_t m, const int32_t* __restrict__ conn, const float* __restri
 // needs 1.03 rounds at 539,601 nodes: 14.8 vs 13.2 us), and it loads the
 // node vectors before the dependent nro -> nq -> partial chain, whose loads
 // it interleaves over the two nodes.
+template <class T>
 __global__ void __launch_bounds__(kBlock)
-mf_gather_cheb32(int64_t n_nodes, const int32_t* __restrict__ nro,
-                 const uint32_t* __restrict__ nq, const float* __restrict__ part,
-                 const float* __restrict__ sc, int mode, const float* __restrict__ zc,
-                 const float* zp, float* zn, float* __restrict__ w, float cd, float cw,
-                 const float* __restrict__ rin, const PcgState* state) {
+mf_gather_cheb(int64_t n_nodes, const int32_t* __restrict__ nro,
+               const uint32_t* __restrict__ nq, const T* __restrict__ part,
+               const T* __restrict__ sc, int mode, const T* __restrict__ zc,
+               const T* zp, T* zn, T* __restrict__ w, T cd, T cw,
+               const T* __restrict__ rin, const PcgState* state) {
   if (state && state->halt) return;
   const int64_t i0 = tile_index(blockIdx.x) * (2 * kBlock) + threadIdx.x;
   // per node: S, then zc and w (modes 1, 2, 5) or rin (modes 3, 4), zp
   // (modes 2, 5), the sum zn is added to (mode 5)
-  float acc[2][3] = {}, sv[2][3] = {}, zv[2][3] = {}, wv[2][3] = {}, pv[2][3] = {},
-        uv[2][3] = {};
+  T acc[2][3] = {}, sv[2][3] = {}, zv[2][3] = {}, wv[2][3] = {}, pv[2][3] = {},
+    uv[2][3] = {};
   int32_t jb[2] = {0, 0}, je[2] = {0, 0};
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
@@ -1643,12 +1643,12 @@ mf_gather_cheb32(int64_t n_nodes, const int32_t* __restrict__ nro,
     const bool m0 = jb[0] + j < je[0], m1 = jb[1] + j < je[1];
     if (!m0 && !m1) break;
     if (m0) {
-      const float* v = part + 3 * int64_t(nq[jb[0] + j]);
+      const T* v = part + 3 * int64_t(nq[jb[0] + j]);
 #pragma unroll
       for (int c = 0; c < 3; ++c) acc[0][c] += v[c];
     }
     if (m1) {
-      const float* v = part + 3 * int64_t(nq[jb[1] + j]);
+      const T* v = part + 3 * int64_t(nq[jb[1] + j]);
 #pragma unroll
       for (int c = 0; c < 3; ++c) acc[1][c] += v[c];
     }
@@ -1660,29 +1660,30 @@ mf_gather_cheb32(int64_t n_nodes, const int32_t* __restrict__ nro,
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const int64_t k = 3 * i + c;
-      const float s = sv[h][c], o = acc[h][c];
+      const T s = sv[h][c], o = acc[h][c];
       if (mode == 0) {
         zn[k] = s * o;
       } else if (mode == 3) {
-        const float wk = s * s * (wv[h][c] - o);
+        const T wk = s * s * (wv[h][c] - o);
         w[k] = wk;
         zn[k] = cw * wk;
       } else if (mode == 4) {
         zn[k] = wv[h][c] - o;
       } else {
-        const float zk = zv[h][c];
-        const float dz = mode == 1 ? zk : zk - pv[h][c];
-        const float znew = zk + cd * dz + cw * (wv[h][c] - s * s * o);
+        const T zk = zv[h][c];
+        const T dz = mode == 1 ? zk : zk - pv[h][c];
+        const T znew = zk + cd * dz + cw * (wv[h][c] - s * s * o);
         zn[k] = mode == 5 ? uv[h][c] + znew : znew;
       }
     }
   }
 }
 
-__global__ void to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+template <class T>
+__global__ void to_type(int64_t n, const double* __restrict__ a, T* __restrict__ b) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x)
-    b[i] = float(a[i]);
+    b[i] = T(a[i]);
 }
 
 // inv[perm[j]] = j
@@ -2303,6 +2304,37 @@ int sell_bound(const cf_matrix* A, int64_t* lsum) {
   return CF_OK;
 }
 
+// Carves the sliced copy of an NB3 matrix (slice structure, FP64 values and,
+// for the mixed precisions, the low-precision values) and its sort space.
+void sell_layout(const cf_matrix* A, int precision, int64_t lsum, cf::Carve& c, Layout& L) {
+  const int64_t N = A->n_rows;
+  L.n_slices = (N + kSliceRows - 1) / kSliceRows;
+  L.rows = c.take<uint32_t>(N);
+  L.koff = c.take<int32_t>(L.n_slices + 1);
+  L.slice_len = c.take<int32_t>(L.n_slices + 1);
+  L.xsplit = c.take<int32_t>(kXcd + 1);
+  L.scol = c.take<int32_t>(size_t(64) * lsum);
+  L.sval64 = c.take<double>(size_t(9 * 64) * lsum);
+  if (precision == CF_PREC_MIXED) L.sval32 = c.take<float>(size_t(9 * 64) * lsum);
+  if (precision == CF_PREC_MIXED16) L.sval16 = c.take<_Float16>(size_t(9 * 64) * lsum);
+  L.keys_in = c.take<uint64_t>(N);
+  L.keys_out = c.take<uint64_t>(N);
+  L.ids_in = c.take<uint32_t>(N);
+  size_t sort_bytes = 0, scan_bytes = 0;
+  const int end_bit = bits_for((N / kSigma + 1)) + 32;
+  rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint64_t*)nullptr, (uint64_t*)nullptr,
+                            (uint32_t*)nullptr, (uint32_t*)nullptr, size_t(N), 0, end_bit);
+  rocprim::exclusive_scan(nullptr, scan_bytes, (int32_t*)nullptr, (int32_t*)nullptr, 0,
+                          size_t(L.n_slices + 1), rocprim::plus<int32_t>());
+  L.tmp_bytes = std::max(sort_bytes, scan_bytes);
+  L.tmp = c.take<char>(L.tmp_bytes);
+}
+
+// partial sums written by one sliced product (its grid, at most)
+int64_t sell_parts(int64_t n_rows) {
+  return kXcd * sell_grid((n_rows + kSliceRows - 1) / kSliceRows);
+}
+
 Layout layout(const cf_matrix* A, int precision, int precond, void* base, int64_t lsum) {
   const int64_t n = n_dofs(A);
   const bool nb3 = A->format == CF_FMT_NB3;
@@ -2314,9 +2346,7 @@ Layout layout(const cf_matrix* A, int precision, int precond, void* base, int64_
   L.Ap = c.take<double>(n);
   L.dinv = c.take<double>(precond == CF_PC_BLOCK3 ? 3 * n : n);
   const int64_t parts =
-      nb3 ? std::max<int64_t>(pcg_grid(),
-                              kXcd * sell_grid((A->n_rows + kSliceRows - 1) / kSliceRows))
-          : pcg_grid();
+      nb3 ? std::max<int64_t>(pcg_grid(), sell_parts(A->n_rows)) : pcg_grid();
   L.partials = c.take<double>(4 * size_t(parts));
   L.partials_b = c.take<double>(2 * size_t(pcg_vec_grid()));
   L.state = c.take<PcgState>(1);
@@ -2334,29 +2364,7 @@ Layout layout(const cf_matrix* A, int precision, int precond, void* base, int64_
     L.z0 = c.take<double>(n);
     L.z1 = c.take<double>(n);
   }
-  if (nb3) {
-    const int64_t N = A->n_rows;
-    L.n_slices = (N + kSliceRows - 1) / kSliceRows;
-    L.rows = c.take<uint32_t>(N);
-    L.koff = c.take<int32_t>(L.n_slices + 1);
-    L.slice_len = c.take<int32_t>(L.n_slices + 1);
-    L.xsplit = c.take<int32_t>(kXcd + 1);
-    L.scol = c.take<int32_t>(size_t(64) * lsum);
-    L.sval64 = c.take<double>(size_t(9 * 64) * lsum);
-    if (precision == CF_PREC_MIXED) L.sval32 = c.take<float>(size_t(9 * 64) * lsum);
-    if (precision == CF_PREC_MIXED16) L.sval16 = c.take<_Float16>(size_t(9 * 64) * lsum);
-    L.keys_in = c.take<uint64_t>(N);
-    L.keys_out = c.take<uint64_t>(N);
-    L.ids_in = c.take<uint32_t>(N);
-    size_t sort_bytes = 0, scan_bytes = 0;
-    const int end_bit = bits_for((N / kSigma + 1)) + 32;
-    rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint64_t*)nullptr, (uint64_t*)nullptr,
-                              (uint32_t*)nullptr, (uint32_t*)nullptr, size_t(N), 0, end_bit);
-    rocprim::exclusive_scan(nullptr, scan_bytes, (int32_t*)nullptr, (int32_t*)nullptr, 0,
-                            size_t(L.n_slices + 1), rocprim::plus<int32_t>());
-    L.tmp_bytes = std::max(sort_bytes, scan_bytes);
-    L.tmp = c.take<char>(L.tmp_bytes);
-  }
+  if (nb3) sell_layout(A, precision, lsum, c, L);
   L.total = c.used;
   return L;
 }
@@ -2434,6 +2442,50 @@ extern "C" int cf_spmv(const cf_matrix* A, const double* x, double* y, const uin
   hipStream_t s = cf::as_stream(stream);
   return launch_spmv<double, double, double>(A, A->values, x, y, fixed, nullptr, nullptr,
                                              pcg_grid(), s);
+}
+
+namespace {
+
+// y = |A| |x| (fixed rows 0), one thread per DOF row: the magnitude that
+// bounds the FP64 rounding of A x (used once per solve, not on a hot path).
+__global__ void spmv_abs(int32_t format, int64_t n_rows, const int32_t* __restrict__ ptr,
+                         const int32_t* __restrict__ idx, const double* __restrict__ val,
+                         const double* __restrict__ x, const uint8_t* __restrict__ fixed,
+                         double* __restrict__ y) {
+  const int64_t n = format == CF_FMT_NB3 ? 3 * n_rows : n_rows;
+  for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    double acc = 0.0;
+    if (format == CF_FMT_NB3) {
+      const int64_t i = t / 3;
+      const int c = int(t - 3 * i);
+      const int32_t base = ptr[i], len = ptr[i + 1] - base;
+      const double* v = val + int64_t(9) * base + int64_t(c) * 3 * len;
+      for (int32_t k = 0; k < len; ++k) {
+        const int64_t j3 = 3 * int64_t(idx[base + k]);
+        acc += fabs(v[3 * k]) * fabs(x[j3]) + fabs(v[3 * k + 1]) * fabs(x[j3 + 1]) +
+               fabs(v[3 * k + 2]) * fabs(x[j3 + 2]);
+      }
+    } else {
+      for (int32_t k = ptr[t]; k < ptr[t + 1]; ++k) acc += fabs(val[k]) * fabs(x[idx[k]]);
+    }
+    y[t] = (fixed && fixed[t]) ? 0.0 : acc;
+  }
+}
+
+}  // namespace
+
+extern "C" int cf_spmv_abs(const cf_matrix* A, const double* x, double* y, const uint8_t* fixed,
+                           void* stream) {
+  int rc = check_matrix(A);
+  if (rc) return rc;
+  CF_REQUIRE(x && y, "cf_spmv_abs: null vector");
+  hipStream_t s = cf::as_stream(stream);
+  const int64_t n = n_dofs(A);
+  spmv_abs<<<cf::stride_grid(n, kBlock), kBlock, 0, s>>>(A->format, A->n_rows, A->indptr,
+                                                         A->indices, A->values, x, fixed, y);
+  CF_LAUNCHED();
+  return CF_OK;
 }
 
 extern "C" int cf_pcg_workspace_bytes(const cf_matrix* A, int32_t precision, int32_t precond,
@@ -2671,18 +2723,25 @@ constexpr int kPowerSteps = 40;
 // Enqueues the power iteration for the largest eigenvalue of S A S on
 // stream s (power(v, y): y = S A S v on that stream); the square of the
 // estimate ends in state->pad, read with read_state.
-template <class Power>
+// Vectors of the low-precision part (Chebyshev, power iteration) are carved
+// as raw bytes; T is float, or double for the FP64 twin of the solvers.
+template <class T>
+T* lp(float* p) { return reinterpret_cast<T*>(p); }
+
+template <class T = float, class Power>
 int power_launch(Power&& power, int64_t n, const Layout& L, hipStream_t s) {
   const int G = pcg_grid();
   int rc;
-  power_start<<<G, kBlock, 0, s>>>(n, L.s, L.ph);
+  T* ph = lp<T>(L.ph);
+  T* Aph = lp<T>(L.Aph);
+  power_start<T><<<G, kBlock, 0, s>>>(n, L.s, ph);
   CF_LAUNCHED();
   for (int k = 0; k < kPowerSteps; ++k) {
-    if ((rc = power(L.ph, L.Aph))) return rc;
-    norm2_partials<<<G, kBlock, 0, s>>>(n, L.Aph, L.partials);
+    if ((rc = power(ph, Aph))) return rc;
+    norm2_partials<T><<<G, kBlock, 0, s>>>(n, Aph, L.partials);
     CF_LAUNCHED();
     if ((rc = fin(kFinStore, L, s, &L.state->pad))) return rc;
-    power_scale<<<G, kBlock, 0, s>>>(n, L.Aph, L.ph, L.state);
+    power_scale<T><<<G, kBlock, 0, s>>>(n, Aph, ph, L.state);
     CF_LAUNCHED();
   }
   return CF_OK;
@@ -2697,11 +2756,12 @@ int power_launch(Power&& power, int64_t n, const Layout& L, hipStream_t s) {
 // and interval ratio of the polynomial.  SQ: cheb works on S-scaled vectors
 // in three-term form (d = z_k in, dn = z_{k-1} in and z_{k+1} out; see
 // fcg_update and mf_gather_cheb32).
-template <bool SQ, class Outer, class Power, class Cheb, class Pre2L = std::nullptr_t>
+template <bool SQ, class T, class Outer, class Power, class Cheb,
+          class Pre2L = std::nullptr_t>
 int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, double gersh,
                   ChebShape shape, int64_t n, const double* b, double* x, const uint8_t* fixed, double tol,
                   int maxit, const Layout& L, cf_pcg_stats* st, hipStream_t s,
-                  Pre2L&& pre2l = nullptr, const float* zh2l = nullptr, int extra_products = 0,
+                  Pre2L&& pre2l = nullptr, const T* zh2l = nullptr, int extra_products = 0,
                   double power_est = 0.0) {
   constexpr bool kTwoLevel = !std::is_same<std::decay_t<Pre2L>, std::nullptr_t>::value;
   const int G = pcg_grid();
@@ -2710,7 +2770,7 @@ int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, do
   // power_est > 0: the caller ran power_launch already (state->pad = its square)
   double est = power_est;
   if (!(est > 0.0)) {
-    if ((rc = power_launch(power, n, L, s))) return rc;
+    if ((rc = power_launch<T>(power, n, L, s))) return rc;
     if ((rc = read_state(L.state, &h, s))) return rc;
     est = std::sqrt(h.pad);
   }
@@ -2727,16 +2787,18 @@ int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, do
   }
   const double amin = bmax / shape.ratio;
   const double theta = 0.5 * (bmax + amin), delta = 0.5 * (bmax - amin), sigma = theta / delta;
-  float cd[kChebMaxSteps], cw[kChebMaxSteps];
+  T cd[kChebMaxSteps], cw[kChebMaxSteps];
   double rho0 = 1.0 / sigma;
   for (int k = 1; k < shape.steps; ++k) {
     const double rho1 = 1.0 / (2.0 * sigma - rho0);
-    cd[k] = float(rho1 * rho0);
-    cw[k] = float(2.0 * rho1 / delta);
+    cd[k] = T(rho1 * rho0);
+    cw[k] = T(2.0 * rho1 / delta);
     rho0 = rho1;
   }
-  const float inv_theta = float(1.0 / theta);
-  float* D[2] = {L.ph, L.Aph};
+  const T inv_theta = T(1.0 / theta);
+  T* D[2] = {lp<T>(L.ph), lp<T>(L.Aph)};
+  T* rh = lp<T>(L.rh);
+  T* yh = lp<T>(L.yh);
   double* Z[2] = {L.z0, L.z1};
   // One outer iteration is: product (p.Ap partials), fcg_update_fused (x, r
   // and the Chebyshev start), the Chebyshev steps, fcg_zdots_fused (stop test,
@@ -2748,20 +2810,20 @@ int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, do
       // two-level: the hook updates x, r and forms the preconditioned residual in zh2l
       int r2 = pre2l(apply_step, slot, sample, cd, cw, inv_theta, shape.steps);
       if (r2) return r2;
-      fcg_zdots_fused<SQ><<<GV, kBlock, 0, s>>>(n, L.r, L.s, zh2l, zc, zo, L.state, L.partials_b,
-                                                GV, apply_step, L.partials);
+      fcg_zdots_fused<SQ, T><<<GV, kBlock, 0, s>>>(n, L.r, L.s, zh2l, zc, zo, L.state,
+                                                   L.partials_b, GV, apply_step, L.partials);
       CF_LAUNCHED();
       fcg_direction_fused<<<GV, kBlock, 0, s>>>(n, zc, L.p, L.state, L.partials, GV, slot);
       CF_LAUNCHED();
       return CF_OK;
     }
     if (apply_step) {
-      fcg_update_fused<SQ><<<GV, kBlock, 0, s>>>(n, x, L.r, L.p, L.Ap, L.s, L.rh, D[0], L.yh,
-                                             inv_theta, L.state, L.partials, outer_parts,
-                                             L.partials_b, slot);
+      fcg_update_fused<SQ, T><<<GV, kBlock, 0, s>>>(n, x, L.r, L.p, L.Ap, L.s, rh, D[0], yh,
+                                                    inv_theta, L.state, L.partials, outer_parts,
+                                                    L.partials_b, slot);
     } else {
-      fcg_update<SQ><<<G, kBlock, 0, s>>>(n, x, L.r, L.p, L.Ap, L.s, L.rh, D[0], L.yh, inv_theta, 0,
-                                      L.state, L.partials);
+      fcg_update<SQ, T><<<G, kBlock, 0, s>>>(n, x, L.r, L.p, L.Ap, L.s, rh, D[0], yh, inv_theta,
+                                             0, L.state, L.partials);
     }
     CF_LAUNCHED();
     int r2;
@@ -2772,9 +2834,9 @@ int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, do
       if (k == 1) prof_end(tok, s);
     }
     // SQ: the last three-term step left z in D[(steps - 1) & 1]
-    const float* zh = SQ ? D[(shape.steps - 1) & 1] : L.yh;
-    fcg_zdots_fused<SQ><<<GV, kBlock, 0, s>>>(n, L.r, L.s, zh, zc, zo, L.state, L.partials_b, GV,
-                                          apply_step, L.partials);
+    const T* zh = SQ ? D[(shape.steps - 1) & 1] : yh;
+    fcg_zdots_fused<SQ, T><<<GV, kBlock, 0, s>>>(n, L.r, L.s, zh, zc, zo, L.state, L.partials_b,
+                                                 GV, apply_step, L.partials);
     CF_LAUNCHED();
     fcg_direction_fused<<<GV, kBlock, 0, s>>>(n, zc, L.p, L.state, L.partials, GV, slot);
     CF_LAUNCHED();
@@ -2935,8 +2997,9 @@ int pcg_fcg_cheb(const cf_matrix* A, VT* sval, const double* b, double* x, const
     CF_LAUNCHED();
     return CF_OK;
   };
-  return fcg_cheb_core<false>(outer, GS, power, cheb, gersh, sizeof(VT) == 2 ? kChebF16 : kChebF32, n,
-                       b, x, fixed, tol, maxit, L, st, s);
+  return fcg_cheb_core<false, float>(outer, GS, power, cheb, gersh,
+                                     sizeof(VT) == 2 ? kChebF16 : kChebF32, n, b, x, fixed, tol,
+                                     maxit, L, st, s);
 }
 
 // Scalar CSR systems (P1 heat, capacity-dominated theta-method steps): FP32
@@ -3672,37 +3735,44 @@ extern "C" int cf_pcg_solve_mf_t10_plan(int64_t n_nodes, int64_t n_elem, const i
 
 namespace {
 
+// Workspace of the Chebyshev / two-level solvers.  The vectors of the
+// low-precision part (Chebyshev iterates, power iteration, inner geometry,
+// scaling and run partials) hold lpb bytes per value: 4 (FP32) or 8 (the
+// FP64 twin of the same solver).
 struct MfMixed {
   double* fe;      // FP64 run partials of the outer product (component planes)
   double* blocks;  // diagonal 3x3 blocks
-  float* geo32;    // geometry rounded to FP32
-  float* fe32;     // FP32 run partials of the inner products (AoS, 3 per run)
-  float* s32;      // Jacobi scaling in FP32
-  MfRuns R;          // run plan shared by the FP64 and FP32 products
+  void* geoL;      // geometry in the inner precision
+  void* feL;       // inner run partials (AoS, 3 per run)
+  void* sL;        // Jacobi scaling in the inner precision
+  MfRuns R;          // run plan shared by the FP64 and inner products
 };
 
-void mf_mixed_layout_base(int64_t n_nodes, int64_t n_elem, cf::Carve& c, Layout* L, MfMixed* M) {
+void mf_mixed_layout_base(int64_t n_nodes, int64_t n_elem, cf::Carve& c, Layout* L, MfMixed* M,
+                          int lpb, const cf_matrix* A) {
   const int64_t n = 3 * n_nodes;
   L->r = c.take<double>(n);
   L->p = c.take<double>(n);
   L->Ap = c.take<double>(n);
   L->dinv = c.take<double>(n);
-  L->partials = c.take<double>(4 * size_t(std::max<int64_t>(pcg_grid(), mf_gather_grid(n_nodes))));
+  int64_t parts = std::max<int64_t>(pcg_grid(), mf_gather_grid(n_nodes));
+  if (A) parts = std::max<int64_t>(parts, sell_parts(A->n_rows));
+  L->partials = c.take<double>(4 * size_t(parts));
   L->partials_b = c.take<double>(2 * size_t(pcg_vec_grid()));
   L->state = c.take<PcgState>(1);
   L->bad = c.take<int32_t>(1);
   L->s = c.take<double>(n);
-  L->rh = c.take<float>(n);
-  L->ph = c.take<float>(n);
-  L->Aph = c.take<float>(n);
-  L->yh = c.take<float>(n);
+  L->rh = reinterpret_cast<float*>(c.take<char>(n * lpb));
+  L->ph = reinterpret_cast<float*>(c.take<char>(n * lpb));
+  L->Aph = reinterpret_cast<float*>(c.take<char>(n * lpb));
+  L->yh = reinterpret_cast<float*>(c.take<char>(n * lpb));
   L->z0 = c.take<double>(n);
   L->z1 = c.take<double>(n);
   M->fe = c.take<double>(30 * n_elem);
   M->blocks = c.take<double>(9 * n_nodes);
-  M->geo32 = c.take<float>(10 * n_elem);
-  M->fe32 = c.take<float>(30 * n_elem);
-  M->s32 = c.take<float>(n);
+  M->geoL = c.take<char>(10 * n_elem * lpb);
+  M->feL = c.take<char>(30 * n_elem * lpb);
+  M->sL = c.take<char>(n * lpb);
   mf_runs_layout(n_nodes, n_elem, c, &M->R);
 }
 
@@ -4150,13 +4220,13 @@ coarse_gemv(int64_t nc, const float* __restrict__ Ainv, const double* __restrict
   }
 }
 
-// u (FP32) = [ADD ? u : 0] + P zc (zero on fixed DOFs)
-template <bool ADD>
+// u (inner precision T) = [ADD ? u : 0] + P zc (zero on fixed DOFs)
+template <bool ADD, class T>
 __global__ void coarse_prolong(int64_t n_nodes, const int32_t* __restrict__ agg,
                                const double* __restrict__ frame, const double* __restrict__ x,
                                const double* __restrict__ y, const double* __restrict__ z,
                                const uint8_t* __restrict__ fixed, const double* __restrict__ zc,
-                               float* __restrict__ u, const PcgState* state) {
+                               T* __restrict__ u, const PcgState* state) {
   if (state && state->halt) return;
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n_nodes;
        i += int64_t(gridDim.x) * blockDim.x) {
@@ -4177,7 +4247,7 @@ __global__ void coarse_prolong(int64_t n_nodes, const int32_t* __restrict__ agg,
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const int64_t d = 3 * i + k;
-      const float v = (fixed && fixed[d]) ? 0.0f : float(s[k]);
+      const T v = (fixed && fixed[d]) ? T(0) : T(s[k]);
       u[d] = ADD ? u[d] + v : v;
     }
   }
@@ -4447,15 +4517,15 @@ struct Coarse {
   double* rc;
   double* zc;
   double* probe;       // 6 probe vectors, 6 products
-  float* u32;          // preconditioned residual being built
-  float* r32;          // FP32 copy of the outer residual
-  float* r2;           // residual after the smoothing step
+  void* u;             // preconditioned residual being built (inner precision)
+  void* rl;            // copy of the outer residual in the inner precision
+  void* r2;            // residual after the smoothing step (inner precision)
   int32_t* info;
   void* tmp;
   size_t tmp_bytes = 0;
 };
 
-void coarse_layout(int64_t n_nodes, int64_t n_elem, cf::Carve& c, Coarse* C) {
+void coarse_layout(int64_t n_nodes, int64_t n_elem, cf::Carve& c, Coarse* C, int lpb) {
   const int64_t n = 3 * n_nodes, n_tiles = (n_elem + kBlock - 1) / kBlock;
   const int64_t G = (n_tiles + kAggTiles - 1) / kAggTiles, ncmax = 6 * G;
   const int64_t maxp = kPairsPerElem * n_elem;
@@ -4481,9 +4551,9 @@ void coarse_layout(int64_t n_nodes, int64_t n_elem, cf::Carve& c, Coarse* C) {
   C->rc = c.take<double>(ncmax);
   C->zc = c.take<double>(ncmax);
   C->probe = c.take<double>(12 * n);
-  C->u32 = c.take<float>(n);
-  C->r32 = c.take<float>(n);
-  C->r2 = c.take<float>(n);
+  C->u = c.take<char>(n * lpb);
+  C->rl = c.take<char>(n * lpb);
+  C->r2 = c.take<char>(n * lpb);
   C->info = c.take<int32_t>(2);
   size_t b1 = 0, b2 = 0, b3 = 0, b4 = 0;
   rocprim::radix_sort_pairs(nullptr, b1, (uint32_t*)nullptr, (uint32_t*)nullptr,
@@ -4501,12 +4571,15 @@ void coarse_layout(int64_t n_nodes, int64_t n_elem, cf::Carve& c, Coarse* C) {
   C->tmp = c.take<char>(C->tmp_bytes);
 }
 
-// workspace of the mixed matrix-free solver; C (two-level only) may be null
+// workspace of the Chebyshev / two-level solvers; C (two-level only) and A
+// (assembled outer operator: its sliced copy, lsum slice steps) may be null
 size_t mf_mixed_layout(int64_t n_nodes, int64_t n_elem, void* base, Layout* L, MfMixed* M,
-                       Coarse* C) {
+                       Coarse* C, int lpb = 4, const cf_matrix* A = nullptr,
+                       int64_t lsum = 0) {
   cf::Carve c(base);
-  mf_mixed_layout_base(n_nodes, n_elem, c, L, M);
-  if (C) coarse_layout(n_nodes, n_elem, c, C);
+  mf_mixed_layout_base(n_nodes, n_elem, c, L, M, lpb, A);
+  if (C) coarse_layout(n_nodes, n_elem, c, C, lpb);
+  if (A) sell_layout(A, CF_PREC_FP64, lsum, c, *L);
   L->total = c.used;
   return c.used;
 }
@@ -4740,7 +4813,7 @@ int coarse_factor(Coarse* C, int64_t n, hipStream_t s) {
   const int64_t nt = (nc + 31) / 32;
   coarse_fill_upper<<<int(std::min<int64_t>(nt * nt, 65535)), kBlock, 0, s>>>(nc, C->A);
   CF_LAUNCHED();
-  to_f32<<<cf::stride_grid(nc * nc, kBlock), kBlock, 0, s>>>(nc * nc, C->A, C->A32);
+  to_type<float><<<cf::stride_grid(nc * nc, kBlock), kBlock, 0, s>>>(nc * nc, C->A, C->A32);
   CF_LAUNCHED();
   return CF_OK;
 }
@@ -4834,14 +4907,18 @@ class HostTask {
 
 namespace {
 
-// Mixed precision with the matrix-free operator (Chebyshev-only or, with
-// xyz, two-level).
-int mf_mixed_solve(const char* who, int64_t n_nodes, int64_t n_elem, const int32_t* conn,
-                   const double* geo, double lam, double mu, const int32_t* offs,
-                   const uint32_t* perm, const double* xs, const double* ys, const double* zs,
-                   bool two_level, const double* b, double* x, const uint8_t* fixed, double tol,
-                   int32_t maxit, void* ws, size_t ws_bytes, cf_pcg_stats* stats_host,
-                   void* stream) {
+// Flexible CG preconditioned by the matrix-free Chebyshev polynomial in
+// precision T (float: the mixed solvers; double: their FP64 twins), alone or,
+// with xyz, inside the two-level cycle.  The outer FP64 products are
+// matrix-free, or, when A is given, products with the assembled NB3 matrix A
+// (its sliced FP64 copy), so that the system solved is A's own.
+template <class T>
+int mf_mixed_solve(const char* who, const cf_matrix* A, int64_t n_nodes, int64_t n_elem,
+                   const int32_t* conn, const double* geo, double lam, double mu,
+                   const int32_t* offs, const uint32_t* perm, const double* xs, const double* ys,
+                   const double* zs, bool two_level, const double* b, double* x,
+                   const uint8_t* fixed, double tol, int32_t maxit, void* ws, size_t ws_bytes,
+                   cf_pcg_stats* stats_host, void* stream) {
   MfArgs a{n_nodes, n_elem, conn, geo, lam, mu, offs, perm};
   int rc = mf_check(a);
   if (rc) return rc;
@@ -4849,10 +4926,19 @@ int mf_mixed_solve(const char* who, int64_t n_nodes, int64_t n_elem, const int32
   CF_REQUIRE(!two_level || (xs && ys && zs), "%s: node coordinates required", who);
   CF_REQUIRE(10 * n_elem < (int64_t(1) << 31), "%s: too many elements", who);
   CF_REQUIRE(tol > 0.0 && maxit > 0, "%s: tol and maxit must be positive", who);
+  int64_t lsum = 0;
+  if (A) {
+    if ((rc = check_matrix(A))) return rc;
+    CF_REQUIRE(A->format == CF_FMT_NB3 && A->n_rows == n_nodes,
+               "%s: the assembled operator must be the NB3 matrix of the mesh (%d node rows, "
+               "%lld nodes)", who, int(A->n_rows), (long long)n_nodes);
+    if ((rc = sell_bound(A, &lsum))) return rc;
+  }
   Layout L{};
   MfMixed M{};
   Coarse C{};
-  const size_t need = mf_mixed_layout(n_nodes, n_elem, ws, &L, &M, two_level ? &C : nullptr);
+  const size_t need = mf_mixed_layout(n_nodes, n_elem, ws, &L, &M, two_level ? &C : nullptr,
+                                      int(sizeof(T)), A, lsum);
   if (ws_bytes < need) {
     cf::set_error("%s: workspace %zu < %zu bytes", who, ws_bytes, need);
     return CF_ERR_WORKSPACE;
@@ -4860,6 +4946,9 @@ int mf_mixed_solve(const char* who, int64_t n_nodes, int64_t n_elem, const int32
   hipStream_t s = cf::as_stream(stream);
   const int G = pcg_grid();
   const int64_t n = 3 * n_nodes;
+  T* geoL = static_cast<T*>(M.geoL);
+  T* feL = static_cast<T*>(M.feL);
+  T* sL = static_cast<T*>(M.sL);
   *stats_host = cf_pcg_stats{};
   CF_HIP(hipMemsetAsync(L.state, 0, sizeof(PcgState), s));
   CF_HIP(hipMemsetAsync(L.bad, 0, sizeof(int32_t), s));
@@ -4878,43 +4967,56 @@ int mf_mixed_solve(const char* who, int64_t n_nodes, int64_t n_elem, const int32
   }
   make_scale<<<G, kBlock, 0, s>>>(n, L.dinv, L.s);
   CF_LAUNCHED();
-  to_f32<<<G, kBlock, 0, s>>>(n, L.s, M.s32);
+  to_type<T><<<G, kBlock, 0, s>>>(n, L.s, sL);
   CF_LAUNCHED();
-  geo_to_f32<<<cf::stride_grid(10 * n_elem, kBlock), kBlock, 0, s>>>(10 * n_elem, geo, M.geo32);
+  to_type<T><<<cf::stride_grid(10 * n_elem, kBlock), kBlock, 0, s>>>(10 * n_elem, geo, geoL);
   CF_LAUNCHED();
   if ((rc = mf_runs_build(n_nodes, n_elem, offs, perm, &M.R, s))) return rc;
+  int GS = 0;
+  if (A && ((rc = build_sell(A, L, s, &GS)) || (rc = fill_sell<double>(A, L, nullptr, L.sval64, s))))
+    return rc;
   const int ge = cf::stride_grid(n_elem, kBlock);
   const int gg = int(mf_gather_grid(n_nodes));
+  const int32_t* xsplit = L.xsplit;
   auto outer = [&](const double* xin, double* y, PcgState* state) {
+    if (A) {
+      spmv_sell<double, double, double><<<GS, kBlock, 0, s>>>(
+          A->n_rows, L.n_slices, L.rows, L.koff, L.scol, L.sval64, xin, y, fixed, state,
+          L.partials, xsplit);
+      CF_LAUNCHED();
+      return CF_OK;
+    }
     return mf_apply(a, M.R, xin, y, fixed, M.fe, state, state ? L.partials : nullptr, s);
   };
+  const int outer_parts = A ? GS : gg;
+  T* rl = static_cast<T*>(C.rl);
+  T* u = static_cast<T*>(C.u);
+  T* r2v = static_cast<T*>(C.r2);
   // the power iteration multiplies plain vectors (S K S v); the Chebyshev
   // steps run on S-scaled vectors, so their element kernel gathers no S.
   // Step k reads z_k (din) and z_{k-1} (dout, none at k = 1) and overwrites
   // dout with z_{k+1}.  Modes 3 and 4: products of the two-level form.
   // zout: where mode 5 adds its step (else the step overwrites dout); ps:
   // the stream (default s)
-  auto product32 = [&](const float* din, float* dout, int mode, float cdk, float cwk,
-                       const PcgState* state, float* zout = nullptr, hipStream_t ps = nullptr) {
+  auto product = [&](const T* din, T* dout, int mode, T cdk, T cwk, const PcgState* state,
+                     T* zout = nullptr, hipStream_t ps = nullptr) {
     if (!ps) ps = s;
     if (mode == 0)
-      mf_t10_elem32<false><<<ge, kBlock, 0, ps>>>(n_elem, conn, M.geo32, float(lam), float(mu),
-                                                  M.s32, din, M.R.rank, M.R.toff, M.R.tstart,
-                                                  M.fe32, state);
+      mf_t10_elem_lp<T, false><<<ge, kBlock, 0, ps>>>(n_elem, conn, geoL, T(lam), T(mu), sL, din,
+                                                      M.R.rank, M.R.toff, M.R.tstart, feL, state);
     else
-      mf_t10_elem32<true><<<ge, kBlock, 0, ps>>>(n_elem, conn, M.geo32, float(lam), float(mu),
-                                                 M.s32, din, M.R.rank, M.R.toff, M.R.tstart,
-                                                 M.fe32, state);
+      mf_t10_elem_lp<T, true><<<ge, kBlock, 0, ps>>>(n_elem, conn, geoL, T(lam), T(mu), sL, din,
+                                                     M.R.rank, M.R.toff, M.R.tstart, feL, state);
     CF_LAUNCHED();
-    mf_gather_cheb32<<<gg, kBlock, 0, ps>>>(n_nodes, M.R.nro, M.R.nq, M.fe32, M.s32, mode, din,
-                                            dout, zout ? zout : dout, L.rh, cdk, cwk,
-                                            two_level ? C.r32 : nullptr, state);
+    mf_gather_cheb<T><<<gg, kBlock, 0, ps>>>(n_nodes, M.R.nro, M.R.nq, feL, sL, mode, din, dout,
+                                             zout ? zout : dout, lp<T>(L.rh), cdk, cwk,
+                                             two_level ? rl : nullptr, state);
     CF_LAUNCHED();
     return CF_OK;
   };
-  auto power = [&](const float* v, float* y) { return product32(v, y, 0, 0.0f, 0.0f, nullptr); };
-  auto cheb = [&](int k, float cdk, float cwk, const float* din, float* dout) {
-    return product32(din, dout, k == 1 ? 1 : 2, cdk, cwk, L.state);
+  auto power = [&](const T* v, T* y) { return product(v, y, 0, T(0), T(0), nullptr); };
+  auto cheb = [&](int k, T cdk, T cwk, const T* din, T* dout) {
+    return product(din, dout, k == 1 ? 1 : 2, cdk, cwk, L.state);
   };
   // Two-level: the power iteration runs on a side stream while this stream
   // builds, factors and inverts the coarse matrix (whose host colouring and
@@ -4930,12 +5032,12 @@ int mf_mixed_solve(const char* who, int64_t n_nodes, int64_t n_elem, const int32
       join.s = s;
       join.done = side->done;
     }
-    auto power_side = [&](const float* v, float* y) {
-      return product32(v, y, 0, 0.0f, 0.0f, nullptr, nullptr, sp);
+    auto power_side = [&](const T* v, T* y) {
+      return product(v, y, 0, T(0), T(0), nullptr, nullptr, sp);
     };
     // its ~200 launches go out from a second host thread
     HostTask launches([&]() {
-      int r = power_launch(power_side, n, L, sp);
+      int r = power_launch<T>(power_side, n, L, sp);
       if (side && hipEventRecord(side->done, sp) != hipSuccess && r == CF_OK) {
         cf::set_error("two-level: hipEventRecord on the side stream failed");
         r = CF_ERR_HIP;
@@ -4952,8 +5054,8 @@ int mf_mixed_solve(const char* who, int64_t n_nodes, int64_t n_elem, const int32
     if (side) CF_HIP(hipStreamWaitEvent(s, side->done, 0));
   }
   if (!two_level)
-    return fcg_cheb_core<true>(outer, gg, power, cheb, 0.0, kChebF32, n, b, x, fixed, tol, maxit,
-                               L, stats_host, s);
+    return fcg_cheb_core<true, T>(outer, outer_parts, power, cheb, 0.0, kChebF32, n, b, x, fixed,
+                                  tol, maxit, L, stats_host, s);
   // two-level: z1 = C r; z2 = z1 + Cheb(r - K z1); z = z2 + C (r - K z2)
   const int GV = pcg_vec_grid();
   const int ga = int(std::min<int64_t>(C.n_agg, 65535));
@@ -4965,46 +5067,46 @@ int mf_mixed_solve(const char* who, int64_t n_nodes, int64_t n_elem, const int32
     coarse_gemv<<<gc, kBlock, 0, s>>>(C.nc, C.A32, C.rc, C.zc, L.state);
     CF_LAUNCHED();
     if (add)
-      coarse_prolong<true><<<G, kBlock, 0, s>>>(n_nodes, C.agg, C.frame, xs, ys, zs, fixed, C.zc,
-                                                C.u32, L.state);
+      coarse_prolong<true, T><<<G, kBlock, 0, s>>>(n_nodes, C.agg, C.frame, xs, ys, zs, fixed,
+                                                   C.zc, u, L.state);
     else
-      coarse_prolong<false><<<G, kBlock, 0, s>>>(n_nodes, C.agg, C.frame, xs, ys, zs, fixed, C.zc,
-                                                 C.u32, L.state);
+      coarse_prolong<false, T><<<G, kBlock, 0, s>>>(n_nodes, C.agg, C.frame, xs, ys, zs, fixed,
+                                                    C.zc, u, L.state);
     CF_LAUNCHED();
     return CF_OK;
   };
-  float* D[2] = {L.ph, L.Aph};
-  auto pre2l = [&](int apply_step, int slot, int sample, const float* cd, const float* cw,
-                   float inv_theta, int steps) {
+  T* D[2] = {lp<T>(L.ph), lp<T>(L.Aph)};
+  auto pre2l = [&](int apply_step, int slot, int sample, const T* cd, const T* cw, T inv_theta,
+                   int steps) {
     if (apply_step)
-      fcg_update2l<true><<<GV, kBlock, 0, s>>>(n, x, L.r, L.p, L.Ap, C.r32, L.state, L.partials,
-                                               gg, L.partials_b, slot);
+      fcg_update2l<true, T><<<GV, kBlock, 0, s>>>(n, x, L.r, L.p, L.Ap, rl, L.state, L.partials,
+                                                  outer_parts, L.partials_b, slot);
     else
-      fcg_update2l<false><<<GV, kBlock, 0, s>>>(n, x, L.r, L.p, L.Ap, C.r32, L.state, nullptr, 0,
-                                                L.partials_b, slot);
+      fcg_update2l<false, T><<<GV, kBlock, 0, s>>>(n, x, L.r, L.p, L.Ap, rl, L.state, nullptr, 0,
+                                                   L.partials_b, slot);
     CF_LAUNCHED();
     int r2;
     if ((r2 = coarse_apply(L.r, false))) return r2;                       // u = C r
-    if ((r2 = product32(C.u32, D[0], 3, 0.0f, inv_theta, L.state))) return r2;  // Cheb start
+    if ((r2 = product(u, D[0], 3, T(0), inv_theta, L.state))) return r2;  // Cheb start
     for (int k = 1; k < steps; ++k) {
       int tok = -1;
       if (k == 1) prof_begin(1, sample, s, &tok);
       if (k > 1 && k == steps - 1)  // the last step adds its iterate to u = z1 (mode 5)
-        r2 = product32(D[(k - 1) & 1], D[k & 1], 5, cd[k], cw[k], L.state, C.u32);
+        r2 = product(D[(k - 1) & 1], D[k & 1], 5, cd[k], cw[k], L.state, u);
       else
         r2 = cheb(k, cd[k], cw[k], D[(k - 1) & 1], D[k & 1]);
       if (r2) return r2;
       if (k == 1) prof_end(tok, s);
     }
     if (steps == 2) {
-      vec_add32<<<G, kBlock, 0, s>>>(n, C.u32, D[1], L.state);  // u = z1 + Cheb
+      vec_add<T><<<G, kBlock, 0, s>>>(n, u, D[1], L.state);  // u = z1 + Cheb
       CF_LAUNCHED();
     }
-    if ((r2 = product32(C.u32, C.r2, 4, 0.0f, 0.0f, L.state))) return r2;   // r2 = r - K u
-    return coarse_apply(C.r2, true);                                         // u += C r2
+    if ((r2 = product(u, r2v, 4, T(0), T(0), L.state))) return r2;  // r2 = r - K u
+    return coarse_apply(r2v, true);                                 // u += C r2
   };
-  return fcg_cheb_core<true>(outer, gg, power, cheb, 0.0, kCheb2L, n, b, x, fixed, tol, maxit,
-                             L, stats_host, s, pre2l, C.u32, 2, power_est);
+  return fcg_cheb_core<true, T>(outer, outer_parts, power, cheb, 0.0, kCheb2L, n, b, x, fixed,
+                                tol, maxit, L, stats_host, s, pre2l, u, 2, power_est);
 }
 
 }  // namespace
@@ -5023,9 +5125,9 @@ extern "C" int cf_pcg_solve_mf_t10_mixed(int64_t n_nodes, int64_t n_elem, const 
                                          const double* b, double* x, const uint8_t* fixed,
                                          double tol, int32_t maxit, void* ws, size_t ws_bytes,
                                          cf_pcg_stats* stats_host, void* stream) {
-  return mf_mixed_solve("cf_pcg_solve_mf_t10_mixed", n_nodes, n_elem, conn, geo, lam, mu, offs,
-                        perm, nullptr, nullptr, nullptr, false, b, x, fixed, tol, maxit, ws,
-                        ws_bytes, stats_host, stream);
+  return mf_mixed_solve<float>("cf_pcg_solve_mf_t10_mixed", nullptr, n_nodes, n_elem, conn, geo,
+                               lam, mu, offs, perm, nullptr, nullptr, nullptr, false, b, x, fixed,
+                               tol, maxit, ws, ws_bytes, stats_host, stream);
 }
 
 extern "C" int cf_mf_t10_twolevel_workspace_bytes(int64_t n_nodes, int64_t n_elem,
@@ -5047,9 +5149,52 @@ extern "C" int cf_pcg_solve_mf_t10_twolevel(int64_t n_nodes, int64_t n_elem, con
                                             const uint8_t* fixed, double tol, int32_t maxit,
                                             void* ws, size_t ws_bytes, cf_pcg_stats* stats_host,
                                             void* stream) {
-  return mf_mixed_solve("cf_pcg_solve_mf_t10_twolevel", n_nodes, n_elem, conn, geo, lam, mu, offs,
-                        perm, x_nodes, y_nodes, z_nodes, true, b, x, fixed, tol, maxit, ws,
-                        ws_bytes, stats_host, stream);
+  return mf_mixed_solve<float>("cf_pcg_solve_mf_t10_twolevel", nullptr, n_nodes, n_elem, conn,
+                               geo, lam, mu, offs, perm, x_nodes, y_nodes, z_nodes, true, b, x,
+                               fixed, tol, maxit, ws, ws_bytes, stats_host, stream);
+}
+
+extern "C" int cf_t10_twolevel_workspace_bytes(const cf_matrix* A, int64_t n_nodes,
+                                               int64_t n_elem, int32_t precision,
+                                               size_t* bytes) {
+  CF_REQUIRE(n_nodes > 0 && n_elem > 0 && bytes, "cf_t10_twolevel_workspace_bytes: bad arguments");
+  CF_REQUIRE(precision == CF_PREC_FP64 || precision == CF_PREC_MIXED,
+             "cf_t10_twolevel_workspace_bytes: precision must be CF_PREC_FP64 or CF_PREC_MIXED");
+  int64_t lsum = 0;
+  int rc;
+  if (A) {
+    if ((rc = check_matrix(A))) return rc;
+    CF_REQUIRE(A->format == CF_FMT_NB3 && A->n_rows == n_nodes,
+               "cf_t10_twolevel_workspace_bytes: A must be the NB3 matrix of the mesh");
+    if ((rc = sell_bound(A, &lsum))) return rc;
+  }
+  Layout L{};
+  MfMixed M{};
+  Coarse C{};
+  *bytes = mf_mixed_layout(n_nodes, n_elem, nullptr, &L, &M, &C,
+                           precision == CF_PREC_FP64 ? 8 : 4, A, lsum);
+  return CF_OK;
+}
+
+extern "C" int cf_pcg_solve_t10_twolevel(const cf_matrix* A, int64_t n_nodes, int64_t n_elem,
+                                         const int32_t* conn, const double* geo, double lam,
+                                         double mu, const int32_t* offs, const uint32_t* perm,
+                                         const double* x_nodes, const double* y_nodes,
+                                         const double* z_nodes, const double* b, double* x,
+                                         const uint8_t* fixed, int32_t precision, double tol,
+                                         int32_t maxit, void* ws, size_t ws_bytes,
+                                         cf_pcg_stats* stats_host, void* stream) {
+  const char* who = "cf_pcg_solve_t10_twolevel";
+  if (precision == CF_PREC_MIXED)
+    return mf_mixed_solve<float>(who, A, n_nodes, n_elem, conn, geo, lam, mu, offs, perm,
+                                 x_nodes, y_nodes, z_nodes, true, b, x, fixed, tol, maxit, ws,
+                                 ws_bytes, stats_host, stream);
+  if (precision == CF_PREC_FP64)
+    return mf_mixed_solve<double>(who, A, n_nodes, n_elem, conn, geo, lam, mu, offs, perm,
+                                  x_nodes, y_nodes, z_nodes, true, b, x, fixed, tol, maxit, ws,
+                                  ws_bytes, stats_host, stream);
+  cf::set_error("%s: precision must be CF_PREC_FP64 or CF_PREC_MIXED", who);
+  return CF_ERR_ARG;
 }
 
 extern "C" int cf_invert_permutation(int64_t n, const uint32_t* perm, uint32_t* inv,

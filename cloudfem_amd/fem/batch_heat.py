@@ -197,7 +197,8 @@ def transient_heat_batch(meshes, k=1.0, rho_cp=1.0, f=0.0, neumann=None, robin=N
     last_batch_stats.clear()
     last_batch_stats.update({name: getattr(st, name) for name, _ in N.CfBatchStats._fields_})
     last_batch_stats.update({"autotune": tune, "setup_s": t_run - t_start,
-                             "run_s": t_done - t_run})
+                             "run_s": t_done - t_run, "rows": int(dm.n_nodes),
+                             "nnz": int(pat.nnz)})
     N.check(rc, "cf_batch_theta_run")
     times = np.cumsum(np.concatenate([[0.0], np.full(steps, h)]))
     times[-1] = t_end

@@ -86,11 +86,14 @@ def assemble_elasticity_tet10(mesh, E, nu, body=None, traction=None, T=None, alp
 def solve_device(dm, vals, rhs, mesh, bc, lam=None, mu=None, **pcg_kw):
     """Solve on the device; returns (u (3N,) tensor, stats).
 
-    With solver option operator="matfree" the stiffness is applied matrix-free
-    from the geometry record (needs lam, mu; precision "mixed" adds the FP32
-    matrix-free Chebyshev preconditioner, and precond "twolevel" wraps it in
-    the rigid-body coarse correction); otherwise the assembled NB3 matrix
-    `vals` is used.
+    precond "twolevel" selects the two-level solver (rigid-body coarse
+    correction around a matrix-free Chebyshev polynomial, FP32 inside with
+    precision "mixed", FP64 with "fp64"), whose outer products use the
+    assembled NB3 matrix `vals` or, with solver option operator="matfree",
+    the matrix-free operator.  Otherwise operator="matfree" applies the
+    stiffness matrix-free from the geometry record (needs lam, mu; precision
+    "mixed" adds the FP32 matrix-free Chebyshev preconditioner), and the
+    assembled NB3 matrix is used by default.
     """
     n3 = 3 * dm.n_nodes
     if bc:
@@ -101,15 +104,23 @@ def solve_device(dm, vals, rhs, mesh, bc, lam=None, mu=None, **pcg_kw):
     fixed = to_dev(mask.astype(np.uint8), torch.uint8, dm.dev) if mask.any() else None
     o = solver_mod.options()
     prec = pcg_kw.get("precision", o["precision"])
-    if o["operator"] == "matfree" and lam is not None and prec == "fp64":
+    precond = pcg_kw.get("precond", o["precond"])
+    kw = {k: v for k, v in pcg_kw.items() if k in ("tol", "maxit")}
+    matfree = o["operator"] == "matfree" and lam is not None
+    if precond == "twolevel" and lam is not None:
+        A = None
+        if not matfree:
+            if vals is None:
+                vals = dm.matrix_values(N.OP_T10_ELAST, lam=lam, mu=mu)
+            A = DeviceMatrix(N.FMT_NB3, dm.pattern(), vals)
+        stats = solver_mod.pcg_t10_twolevel(dm, lam, mu, rhs, x, fixed, A=A,
+                                            precision="fp64" if prec == "fp64" else "mixed",
+                                            **kw)
+    elif matfree and prec == "fp64":
         kw = {k: v for k, v in pcg_kw.items() if k != "precision"}
         stats = solver_mod.pcg_matfree_t10(dm, lam, mu, rhs, x, fixed, **kw)
-    elif o["operator"] == "matfree" and lam is not None and prec in ("mixed", "mixed16"):
-        kw = {k: v for k, v in pcg_kw.items() if k in ("tol", "maxit")}
-        if pcg_kw.get("precond", o["precond"]) == "twolevel":
-            stats = solver_mod.pcg_matfree_t10_twolevel(dm, lam, mu, rhs, x, fixed, **kw)
-        else:
-            stats = solver_mod.pcg_matfree_t10_mixed(dm, lam, mu, rhs, x, fixed, **kw)
+    elif matfree and prec in ("mixed", "mixed16"):
+        stats = solver_mod.pcg_matfree_t10_mixed(dm, lam, mu, rhs, x, fixed, **kw)
     else:
         if vals is None:
             vals = dm.matrix_values(N.OP_T10_ELAST, lam=lam, mu=mu)

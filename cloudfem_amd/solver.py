@@ -15,10 +15,12 @@ problems:
              test is the FP64 residual in every case, so the error against
              FP64 is bounded by the same tolerance (cf_pcg_solve)
   precond    "jacobi" (default) or "block3" (3x3 node blocks, vector
-             problems, FP64 only); "twolevel" with operator "matfree" and
-             precision "mixed": the FP32 Chebyshev polynomial wrapped in a
-             coarse correction on the rigid-body modes of node aggregates
-             (cf_pcg_solve_mf_t10_twolevel)
+             problems, FP64 only); "twolevel" (Tet10 elasticity): a matrix-free
+             Chebyshev polynomial, FP32 with precision "mixed" or FP64 with
+             "fp64", wrapped in a coarse correction on the rigid-body modes of
+             node aggregates (cf_pcg_solve_t10_twolevel); the outer FP64
+             products use the assembled matrix (operator "assembled") or the
+             matrix-free operator ("matfree")
   tol        relative residual ||b_f - A_ff x_f|| / ||b_f|| (default 1e-12)
   maxit      iteration cap (default 200000)
   operator   "assembled" (default: CSR/NB3 SpMV) or "matfree" (Tet10
@@ -97,6 +99,23 @@ class DeviceMatrix:
         return N.CfMatrix(self.fmt, p.n_rows, p.nnz, p.indptr.data_ptr(),
                           p.indices.data_ptr(), self.values.data_ptr())
 
+    def abs_matvec(self, x, fixed=None):
+        """|A| |x| with fixed rows zeroed (cf_spmv_abs)."""
+        y = torch.empty_like(x)
+        A = self.cstruct()
+        N.check(N.load().cf_spmv_abs(ctypes.byref(A), ptr(x), ptr(y), ptr(fixed),
+                                     stream_handle()), "cf_spmv_abs")
+        return y
+
+    def fp64_floor(self, x, b, fixed=None):
+        """eps || |A_ff| |x_f| || / ||b_f||: the smallest relative residual
+        FP64 resolves for this system at the solution x."""
+        xf = x if fixed is None else torch.where(fixed.bool(), torch.zeros_like(x), x)
+        bf = b if fixed is None else torch.where(fixed.bool(), torch.zeros_like(b), b)
+        y = self.abs_matvec(xf, fixed)
+        return float(torch.finfo(torch.float64).eps * torch.linalg.vector_norm(y)
+                     / torch.linalg.vector_norm(bf))
+
     def matvec(self, x, fixed=None):
         y = torch.empty_like(x)
         A = self.cstruct()
@@ -160,30 +179,48 @@ def pcg_matfree_t10_mixed(dm, lam, mu, b, x, fixed=None, tol=None, maxit=None):
     return stats
 
 
-def pcg_matfree_t10_twolevel(dm, lam, mu, b, x, fixed=None, tol=None, maxit=None):
-    """Two-level mixed precision with the matrix-free Tet10 operator: FP64
-    flexible CG, preconditioned by a coarse correction on the rigid-body modes
-    of node aggregates around the FP32 matrix-free Chebyshev polynomial
-    (cf_pcg_solve_mf_t10_twolevel)."""
+def pcg_t10_twolevel(dm, lam, mu, b, x, fixed=None, A=None, precision="mixed", tol=None,
+                     maxit=None):
+    """Two-level solver for Tet10 elasticity (cf_pcg_solve_t10_twolevel):
+    FP64 flexible CG preconditioned by a coarse correction on the rigid-body
+    modes of node aggregates around the matrix-free Chebyshev polynomial,
+    which runs in FP32 (precision "mixed") or FP64 ("fp64", the same solver
+    with every inner product in FP64).  The outer products use the assembled
+    NB3 matrix A (a DeviceMatrix) when given, else the FP64 matrix-free
+    operator."""
     o = options()
     tol = float(tol if tol is not None else o["tol"])
     maxit = int(maxit if maxit is not None else o["maxit"])
+    prec = {"fp64": N.PREC_FP64, "mixed": N.PREC_MIXED}.get(precision)
+    if prec is None:
+        raise ValueError(f"two-level precision {precision!r} is not available (fp64, mixed)")
     lib = N.load()
     conn, geo, offs, perm = dm.matfree_arrays()
+    As = A.cstruct() if A is not None else None
+    Ap = ctypes.byref(As) if As is not None else None
     nb = ctypes.c_size_t()
-    N.check(lib.cf_mf_t10_twolevel_workspace_bytes(dm.n_nodes, dm.n_elem, ctypes.byref(nb)),
-            "cf_mf_t10_twolevel_workspace_bytes")
+    N.check(lib.cf_t10_twolevel_workspace_bytes(Ap, dm.n_nodes, dm.n_elem, prec,
+                                                ctypes.byref(nb)),
+            "cf_t10_twolevel_workspace_bytes")
     ws = workspace(nb.value, b.device)
     st = N.CfPcgStats()
-    rc = lib.cf_pcg_solve_mf_t10_twolevel(dm.n_nodes, dm.n_elem, ptr(conn), ptr(geo), float(lam),
-                                          float(mu), ptr(offs), ptr(perm), ptr(dm.x), ptr(dm.y),
-                                          ptr(dm.z), ptr(b), ptr(x), ptr(fixed), tol, maxit,
-                                          ptr(ws), nb.value, ctypes.byref(st), stream_handle())
-    stats = {**_base_stats(st), "precision": "mixed",
-             "precond": "two-level", "tol": tol, "operator": "matfree"}
+    rc = lib.cf_pcg_solve_t10_twolevel(Ap, dm.n_nodes, dm.n_elem, ptr(conn), ptr(geo),
+                                       float(lam), float(mu), ptr(offs), ptr(perm), ptr(dm.x),
+                                       ptr(dm.y), ptr(dm.z), ptr(b), ptr(x), ptr(fixed), prec, tol,
+                                       maxit, ptr(ws), nb.value, ctypes.byref(st),
+                                       stream_handle())
+    stats = {**_base_stats(st), "precision": precision, "precond": "two-level", "tol": tol,
+             "operator": "assembled" if A is not None else "matfree"}
     last_stats.value = stats
-    N.check(rc, "cf_pcg_solve_mf_t10_twolevel")
+    N.check(rc, "cf_pcg_solve_t10_twolevel")
     return stats
+
+
+def pcg_matfree_t10_twolevel(dm, lam, mu, b, x, fixed=None, tol=None, maxit=None):
+    """Two-level mixed precision with the matrix-free Tet10 operator
+    outside and FP32 matrix-free products inside (pcg_t10_twolevel)."""
+    return pcg_t10_twolevel(dm, lam, mu, b, x, fixed, A=None, precision="mixed", tol=tol,
+                            maxit=maxit)
 
 
 def pcg_matfree_t10(dm, lam, mu, b, x, fixed=None, precond=None, tol=None, maxit=None,

@@ -275,6 +275,12 @@ int cf_copy_masked(int64_t n, const uint8_t* mask, const double* src,
 int cf_spmv(const cf_matrix* A, const double* x, double* y,
             const uint8_t* fixed, void* stream);
 
+/* ---- y = |A| |x| (fixed rows 0): no reference counterpart; it gives the
+ *      FP64 rounding floor eps ||A_ff| |x_f|| / ||b_f|| below which the
+ *      relative residual of a solve cannot be resolved.                  */
+int cf_spmv_abs(const cf_matrix* A, const double* x, double* y, const uint8_t* fixed,
+                void* stream);
+
 /* ---- preconditioned CG on the free-DOF system (replaces spsolve /
  *      apply_dirichlet_vec3 at elastic3d.py:220-232, tet_poisson.py:121,
  *      tet10.py:177, elastic_tet10.py:225-228).
@@ -418,6 +424,30 @@ int cf_pcg_solve_mf_t10_twolevel(int64_t n_nodes, int64_t n_elem, const int32_t*
                                  const double* z_nodes, const double* b, double* x,
                                  const uint8_t* fixed, double tol, int32_t maxit, void* ws,
                                  size_t ws_bytes, cf_pcg_stats* stats_host, void* stream);
+
+/* ---- two-level solver, general form (no reference counterpart; replaces
+ *      the spsolve of elastic_tet10.py:214-229 like cf_pcg_solve).  The
+ *      preconditioner is the two-level cycle of cf_pcg_solve_mf_t10_twolevel,
+ *      its Chebyshev polynomial and coarse prolongation run matrix-free in
+ *      `precision`: CF_PREC_MIXED = FP32, CF_PREC_FP64 = FP64 (the same
+ *      algorithm in FP64 throughout, for measuring what precision alone
+ *      gains).  The outer FP64 flexible CG multiplies by A when A is given
+ *      (the assembled NB3 stiffness of the same mesh, node rows = n_nodes;
+ *      the solver builds a sliced copy of it in the workspace), so the system
+ *      solved is A's own -- the matrix cf_assemble_matrix(CF_OP_T10_ELAST)
+ *      returns and the reference builds -- and the FP64 matrix-free operator
+ *      otherwise (the same matrix up to rounding).  Same stop rule and
+ *      residual replacement as cf_pcg_solve.                                */
+int cf_t10_twolevel_workspace_bytes(const cf_matrix* A, int64_t n_nodes, int64_t n_elem,
+                                    int32_t precision, size_t* bytes);
+int cf_pcg_solve_t10_twolevel(const cf_matrix* A, int64_t n_nodes, int64_t n_elem,
+                              const int32_t* conn, const double* geo, double lam, double mu,
+                              const int32_t* offs, const uint32_t* perm,
+                              const double* x_nodes, const double* y_nodes,
+                              const double* z_nodes, const double* b, double* x,
+                              const uint8_t* fixed, int32_t precision, double tol,
+                              int32_t maxit, void* ws, size_t ws_bytes,
+                              cf_pcg_stats* stats_host, void* stream);
 
 /* ---- inv[perm[j]] = j for a permutation of 0..n-1.                     */
 int cf_invert_permutation(int64_t n, const uint32_t* perm, uint32_t* inv,

@@ -180,6 +180,40 @@ def test_matfree_twolevel_meets_fp64_residual(gpu, dims):
     assert err < 1e-10
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+@pytest.mark.parametrize("dims", [(8, 2, 2), (40, 5, 5)])
+def test_twolevel_assembled_outer_matches_spsolve(gpu, precision, dims):
+    """Two-level solver whose outer FP64 products use the assembled NB3
+    matrix (the system the reference solves), with the Chebyshev polynomial
+    and coarse correction matrix-free in FP32 ("mixed") or FP64 ("fp64", the
+    precision twin): residual bar and 1e-10 against SuperLU."""
+    from cloudfem_amd import solver
+    from cloudfem_amd.mesh import box_mesh_tet10
+    from cloudfem_amd.physics import elastic_tet10
+    lims = dict(xlim=(0.0, 0.4), ylim=(0.0, 0.05), zlim=(0.0, 0.05))
+    trac = {"xmax": (0.0, 0.0, -1e6)}
+    bc = {"xmin": (0.0, 0.0, 0.0)}
+    m = box_mesh_tet10(*dims, **lims)
+    with solver.solver_options(operator="assembled", precision=precision, precond="twolevel",
+                               tol=1e-12):
+        u = np.stack(elastic_tet10.solve_elasticity_tet10(m, 210e9, 0.3, bc=bc, traction=trac),
+                     axis=1).ravel()
+    st = solver.last_stats.value
+    assert st["operator"] == "assembled" and st["precond"] == "two-level"
+    assert st["precision"] == precision
+    m0 = om.box_mesh_tet10(*dims, **lims)
+    A0, rhs0 = oel.assemble_elasticity_tet10(m0, 210e9, 0.3, traction=trac)
+    u0 = oel.apply_dirichlet_vec3(A0, rhs0, m0, bc)
+    Aff, bf, free = oel.reduced_system_vec3(A0, rhs0, m0, bc)[:3]
+    floor = fp64_floor(Aff, u0[free], bf)
+    err = rel_err(u, u0)
+    print(f"two-level assembled {precision} {dims}: {st['iterations']} iterations, "
+          f"{st['residual_updates']} inner products, {st['restarts']} restarts, rel. residual "
+          f"{st['rel_residual']:.2e} (floor {floor:.2e}), error vs SuperLU {err:.2e}")
+    assert_residual_bar(st, 1e-12, floor)
+    assert err < 1e-10
+
+
 def test_twolevel_jittered_mesh_body_load(gpu):
     """Two-level solver on an irregular Tet10 mesh (interior and boundary
     nodes of a 36x6x6 Kuhn box jittered by up to 0.2 h, 7,776 elements, six
