@@ -125,7 +125,7 @@ _SIGS = {
     "cf_incidence_workspace_bytes": ([_I64, _I64, _SZ], _I32),
     "cf_incidence_build": ([_P, _I64, _I64, _P, ctypes.c_size_t, _P, _P, _P], _I32),
     "cf_tet_geometry": ([_P, _P, _P, _P, _I64, _I32, _P, _P, _P], _I32),
-    "cf_assemble_matrix": ([_I32, _P, _P, _P, _P, _I64, _I64, _I32, _P, _P, _D, _D,
+    "cf_assemble_matrix": ([_I32, _P, _P, _P, _P, _P, _I64, _I64, _I32, _P, _P, _D, _D,
                             _P, _P], _I32),
     "cf_assemble_rhs": ([_I32, _P, _P, _I64, _I32, _P, _P, _D, _D, _P, _P], _I32),
     "cf_face_rhs": ([_I32, _P, _P, _P, _P, _I64, _I32, _P, _P, _I64, _P, _P, _P], _I32),

@@ -100,42 +100,45 @@ __device__ __forceinline__ void t10_grad(const Geo& g, int q, int a, double out[
   for (int i = 0; i < 3; ++i) out[i] = d0 * g.j[0][i] + d1 * g.j[1][i] + d2 * g.j[2][i];
 }
 
-// Row of stored entry s: last r with ptr[r] <= s.
-__device__ __forceinline__ int64_t row_of(const int32_t* __restrict__ ptr,
-                                          int64_t n_rows, int64_t s) {
-  int64_t lo = 0, hi = n_rows;  // invariant: ptr[lo] <= s < ptr[hi]
-  while (hi - lo > 1) {
-    int64_t mid = (lo + hi) >> 1;
-    if (ptr[mid] <= s)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
 // ------------------------------------------------------------- matrices
 
+// One thread per stored entry (node pair) s: its contributors (e, a, b) are
+// contrib[seg[s] .. seg[s+1]) in ascending element order, and the entry is
+// their sum in that order (scipy's coo->csr sums the reference's triplets in
+// element order too).  Vector operators store the 3x3 block at its place in
+// the DOF-level CSR value array; the row node is conn[e][a] of the first
+// contributor, so no search over ptr is needed.  The Tet10 elasticity
+// reference tensor (900 doubles) is staged in LDS, where the per-lane
+// (a, b) reads cost far less than divergent loads from constant memory.
 template <int OP>
-__global__ void assemble_values(const int32_t* __restrict__ ptr,
-                                const int32_t* __restrict__ seg,
-                                const uint32_t* __restrict__ contrib,
-                                int64_t n_rows, int64_t nnz, int k,
-                                const double* __restrict__ geo,
-                                const double* __restrict__ coef, double lam,
-                                double mu, double* __restrict__ out) {
+__global__ void __launch_bounds__(kBlock)
+assemble_values(const int32_t* __restrict__ ptr, const int32_t* __restrict__ seg,
+                const uint32_t* __restrict__ contrib, const int32_t* __restrict__ conn,
+                int64_t nnz, int k, const double* __restrict__ geo,
+                const double* __restrict__ coef, double lam, double mu,
+                double* __restrict__ out) {
+  constexpr bool kRT = (OP == CF_OP_T10_ELAST);
+  __shared__ double Rs[kRT ? 900 : 1];
+  if constexpr (kRT) {
+    const double* R = &kT10.R[0][0][0][0];
+    for (int t = threadIdx.x; t < 900; t += blockDim.x) Rs[t] = R[t];
+    __syncthreads();
+  }
   const int kk = k * k;
   for (int64_t s = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; s < nnz;
        s += int64_t(gridDim.x) * blockDim.x) {
     double acc[3][3] = {};
     const int32_t j0 = seg[s], j1 = seg[s + 1];
+    int64_t row = 0;
     for (int32_t j = j0; j < j1; ++j) {
       const uint32_t pid = contrib[j];
       const int64_t e = pid / uint32_t(kk);
       const int ab = int(pid - uint32_t(e) * uint32_t(kk));
       const int a = ab / k, b = ab - (ab / k) * k;
+      if constexpr (OP == CF_OP_T10_ELAST || OP == CF_OP_P1_ELAST || OP == CF_OP_P1_VMASS)
+        if (j == j0) row = conn[e * k + a];
       if constexpr (OP == CF_OP_P1_MASS || OP == CF_OP_P1_VMASS) {
-        // rho V (1 + d_ab)/20  (tet_heat.py:204-210, modal3d.py:64-66)
+        // rho V (1 + d_ab)/20  (tet_heat.py:72-94, modal3d.py:64-66)
         const double v = geo[e * 10 + 9];
         acc[0][0] += (coef[e] * v) * ((a == b ? 2.0 : 1.0) / 20.0);
       } else if constexpr (OP == CF_OP_T10_MASS) {
@@ -163,13 +166,13 @@ __global__ void assemble_values(const int32_t* __restrict__ ptr,
         double M[3][3];
         if constexpr (OP == CF_OP_T10_ELAST) {
           // M = Jinv^T R_ab Jinv
+          const double* R = Rs + (a * 10 + b) * 9;
           double T[3][3];
 #pragma unroll
           for (int m = 0; m < 3; ++m)
 #pragma unroll
             for (int i = 0; i < 3; ++i)
-              T[m][i] = kT10.R[a][b][m][0] * g.j[0][i] + kT10.R[a][b][m][1] * g.j[1][i] +
-                        kT10.R[a][b][m][2] * g.j[2][i];
+              T[m][i] = R[3 * m] * g.j[0][i] + R[3 * m + 1] * g.j[1][i] + R[3 * m + 2] * g.j[2][i];
 #pragma unroll
           for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -194,8 +197,7 @@ __global__ void assemble_values(const int32_t* __restrict__ ptr,
       }
     }
     if constexpr (OP == CF_OP_T10_ELAST || OP == CF_OP_P1_ELAST) {
-      const int64_t i = row_of(ptr, n_rows, s);
-      const int64_t base = ptr[i], len = ptr[i + 1] - base, kpos = s - base;
+      const int64_t base = ptr[row], len = ptr[row + 1] - base, kpos = s - base;
       double* o = out + 9 * base + 3 * kpos;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -204,8 +206,7 @@ __global__ void assemble_values(const int32_t* __restrict__ ptr,
         o[c * 3 * len + 2] = acc[c][2];
       }
     } else if constexpr (OP == CF_OP_P1_VMASS) {
-      const int64_t i = row_of(ptr, n_rows, s);
-      const int64_t base = ptr[i], len = ptr[i + 1] - base, kpos = s - base;
+      const int64_t base = ptr[row], len = ptr[row + 1] - base, kpos = s - base;
       double* o = out + 3 * base + kpos;
       o[0] = acc[0][0];
       o[len] = acc[0][0];
@@ -546,14 +547,14 @@ extern "C" int cf_tet_geometry(const double* x, const double* y, const double* z
 
 extern "C" int cf_assemble_matrix(int32_t op, const int32_t* indptr,
                                   const int32_t* indices, const int32_t* seg,
-                                  const uint32_t* contrib, int64_t n_nodes,
-                                  int64_t nnz, int32_t k, const double* geo,
+                                  const uint32_t* contrib, const int32_t* conn,
+                                  int64_t n_nodes, int64_t nnz, int32_t k, const double* geo,
                                   const double* coef, double lam, double mu,
                                   double* values, void* stream) {
   (void)indices;
   CF_REQUIRE(indptr && seg && values && n_nodes > 0 && nnz >= 0,
              "cf_assemble_matrix: bad arguments");
-  CF_REQUIRE(nnz == 0 || (contrib && geo), "cf_assemble_matrix: null inputs");
+  CF_REQUIRE(nnz == 0 || (contrib && geo && conn), "cf_assemble_matrix: null inputs");
   const bool p1 = (op == CF_OP_P1_STIFF || op == CF_OP_P1_MASS || op == CF_OP_P1_ELAST ||
                    op == CF_OP_P1_VMASS);
   CF_REQUIRE(k == (p1 ? 4 : 10), "cf_assemble_matrix: op %d needs k=%d, got %d", op,
@@ -565,8 +566,8 @@ extern "C" int cf_assemble_matrix(int32_t op, const int32_t* indptr,
   const int grid = cf::stride_grid(nnz, kBlock);
 #define CF_CASE(OPV)                                                                  \
   case OPV:                                                                           \
-    assemble_values<OPV><<<grid, kBlock, 0, s>>>(indptr, seg, contrib, n_nodes, nnz, k, \
-                                                 geo, coef, lam, mu, values);        \
+    assemble_values<OPV><<<grid, kBlock, 0, s>>>(indptr, seg, contrib, conn, nnz, k, geo, \
+                                                 coef, lam, mu, values);              \
     break;
   switch (op) {
     CF_CASE(CF_OP_P1_STIFF)

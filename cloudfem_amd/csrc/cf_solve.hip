@@ -1522,42 +1522,80 @@ mf_shared_gather(int64_t n_sn, const int32_t* __restrict__ shared_node,
 // update of cheb_step (or, for the power iteration, just stores S K S d).
 
 
+// Per-element inputs of the low-precision operator in tile-packed order:
+// value a (0..9) of element e = t kBlock + l sits at ((t 10 + a) kBlock + l),
+// so each of the 10 loads of a thread is one contiguous 64-lane access.
+__device__ __forceinline__ int64_t tile_packed(int64_t t, int a) {
+  return (t * 10 + a) * kBlock + threadIdx.x;
+}
+
+template <class T>
+__global__ void mf_pack_tiles(int64_t m, const int32_t* __restrict__ conn,
+                              const double* __restrict__ geo, const uint16_t* __restrict__ rank,
+                              int32_t* __restrict__ connT, T* __restrict__ geoT,
+                              uint16_t* __restrict__ rankT) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < 10 * m;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t e = i / 10;
+    const int a = int(i - 10 * e);
+    const int64_t t = e / kBlock;
+    const int64_t o = (t * 10 + a) * kBlock + (e - t * kBlock);
+    connT[o] = conn[i];
+    rankT[o] = rank[i];
+    geoT[o] = T(geo[i]);
+  }
+}
+
+// three consecutive values of a node (one 12- or 24-byte load)
+template <class T>
+struct Vec3 {
+  T v[3];
+};
+
 // PRE: d is already S-scaled (the Chebyshev vectors of the SQ form), so
 // the kernel gathers it as is; otherwise it gathers S d.  Writes the run
-// partials described above.
+// partials described above.  conn, geo and rank are tile-packed
+// (mf_pack_tiles).
 template <class T, bool PRE>
 __global__ void __launch_bounds__(kBlock)
 mf_t10_elem_lp(int64_t m, const int32_t* __restrict__ conn, const T* __restrict__ geo,
                T lam, T mu, const T* __restrict__ sc, const T* __restrict__ d,
                const uint16_t* __restrict__ rank, const int32_t* __restrict__ toff,
                const uint16_t* __restrict__ tstart, T* __restrict__ part,
-              const PcgState* state) {
+               const PcgState* state) {
   if (state && state->halt) return;
   __shared__ T fl[kTileItems * 3];
   const int64_t n_tiles = (m + kBlock - 1) / kBlock;
   for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < n_tiles; t += gridDim.x) {
     const int64_t e = t * kBlock + threadIdx.x;
     if (e < m) {
-      const T* g = geo + e * 10;
       T J[3][3];
 #pragma unroll
       for (int r = 0; r < 3; ++r)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) J[r][c] = g[3 * r + c];
-      const T V = g[9];
+        for (int c = 0; c < 3; ++c) J[r][c] = geo[tile_packed(t, 3 * r + c)];
+      const T V = geo[tile_packed(t, 9)];
+      int32_t nd[10];
+#pragma unroll
+      for (int a = 0; a < 10; ++a) nd[a] = conn[tile_packed(t, a)];
       T xa[10][3];
 #pragma unroll
       for (int a = 0; a < 10; ++a) {
-        const int64_t nd = conn[e * 10 + a];
+        const Vec3<T> x = *reinterpret_cast<const Vec3<T>*>(d + 3 * int64_t(nd[a]));
+        if constexpr (PRE) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-          xa[a][c] = PRE ? d[3 * nd + c] : sc[3 * nd + c] * d[3 * nd + c];
+          for (int c = 0; c < 3; ++c) xa[a][c] = x.v[c];
+        } else {
+          const Vec3<T> sv = *reinterpret_cast<const Vec3<T>*>(sc + 3 * int64_t(nd[a]));
+#pragma unroll
+          for (int c = 0; c < 3; ++c) xa[a][c] = sv.v[c] * x.v[c];
+        }
       }
       T f[10][3];
       t10_forces_4pt<T>(xa, J, V, lam, mu, f);
 #pragma unroll
       for (int a = 0; a < 10; ++a) {
-        const int r = rank[e * 10 + a];
+        const int r = rank[tile_packed(t, a)];
         fl[3 * r] = f[a][0];
         fl[3 * r + 1] = f[a][1];
         fl[3 * r + 2] = f[a][2];
@@ -2553,6 +2591,20 @@ struct TrueResidual {
     return true;
   }
   double segment_stop2() const { return std::max(bar2, 0.01 * last2); }
+  // A restart the caller requires whatever the residual (the outer operator
+  // changed): the segment then reduces the new true residual 10x.
+  bool forced = false;
+  bool force(double rr2) {
+    last2 = rr2;
+    forced = rr2 > 0.0;
+    if (!forced) return false;
+    prev2 = rr2;
+    ++restarts;
+    return true;
+  }
+  double stop2_after(bool was_forced) const {
+    return was_forced ? 0.01 * last2 : segment_stop2();
+  }
 };
 
 template <int PC, class Apply>
@@ -2756,14 +2808,21 @@ int power_launch(Power&& power, int64_t n, const Layout& L, hipStream_t s) {
 // and interval ratio of the polynomial.  SQ: cheb works on S-scaled vectors
 // in three-term form (d = z_k in, dn = z_{k-1} in and z_{k+1} out; see
 // fcg_update and mf_gather_cheb32).
+// Outer iterations enqueued between host checks of the flexible CG: an
+// iteration is ~25 launches, and those enqueued after convergence still cost
+// their launch on the host, so batches stay short (the check itself is one
+// small read-back per batch).
+constexpr int kFcgBatchMax = 16;
+
 template <bool SQ, class T, class Outer, class Power, class Cheb,
-          class Pre2L = std::nullptr_t>
-int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, double gersh,
+          class Pre2L = std::nullptr_t, class SwitchOuter = std::nullptr_t>
+int fcg_cheb_core(Outer&& outer, const int& outer_parts, Power&& power, Cheb&& cheb, double gersh,
                   ChebShape shape, int64_t n, const double* b, double* x, const uint8_t* fixed, double tol,
                   int maxit, const Layout& L, cf_pcg_stats* st, hipStream_t s,
                   Pre2L&& pre2l = nullptr, const T* zh2l = nullptr, int extra_products = 0,
-                  double power_est = 0.0) {
+                  double power_est = 0.0, SwitchOuter&& switch_outer = nullptr) {
   constexpr bool kTwoLevel = !std::is_same<std::decay_t<Pre2L>, std::nullptr_t>::value;
+  constexpr bool kSwitch = !std::is_same<std::decay_t<SwitchOuter>, std::nullptr_t>::value;
   const int G = pcg_grid();
   int rc;
   PcgState h{};
@@ -2904,10 +2963,15 @@ int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, do
       }
       if ((rc = read_state(L.state, &h, s))) return rc;
       prof_collect();
-      batch = next_batch(h, rr_start, prev_rr, prev_it, batch, 4, 128);
+      batch = next_batch(h, rr_start, prev_rr, prev_it, batch, 2, kFcgBatchMax);
       prev_rr = h.rr;
       prev_it = h.iter;
     }
+    // converged with the first outer operator: the caller may switch to the
+    // final one, which then gets at least one segment of its own
+    bool switched = false;
+    if constexpr (kSwitch)
+      if (h.halt == kConverged) switched = switch_outer();
     // true FP64 residual (r = b - A x, its r.r into pad)
     if ((rc = outer(x, L.Ap, nullptr))) return rc;
     residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->pad);
@@ -2916,9 +2980,10 @@ int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, do
     PcgState fin_state{};
     if ((rc = read_state(L.state, &fin_state, s))) return rc;
     tr.last2 = fin_state.pad;
-    if (h.halt != kConverged || !tr.restart(fin_state.pad)) break;
+    if (h.halt != kConverged) break;
+    if (switched ? !tr.force(fin_state.pad) : !tr.restart(fin_state.pad)) break;
     // residual replacement: a fresh flexible CG from the true residual in r
-    h.stop2 = tr.segment_stop2();
+    h.stop2 = tr.stop2_after(switched);
     h.rr = fin_state.pad;
     h.rz = 1.0;
     h.rzs[0] = h.rzs[1] = 1.0;
@@ -2931,7 +2996,7 @@ int fcg_cheb_core(Outer&& outer, int outer_parts, Power&& power, Cheb&& cheb, do
       ++opened;
     }
     enq = 0;
-    batch = 8;
+    batch = 4;  // restart segments are short
     prev_rr = rr_start = h.rr;
     prev_it = h.iter;
   }
@@ -3742,7 +3807,9 @@ namespace {
 struct MfMixed {
   double* fe;      // FP64 run partials of the outer product (component planes)
   double* blocks;  // diagonal 3x3 blocks
-  void* geoL;      // geometry in the inner precision
+  void* geoL;      // geometry in the inner precision, tile-packed
+  int32_t* connT;  // connectivity, tile-packed
+  uint16_t* rankT; // run-plan ranks, tile-packed
   void* feL;       // inner run partials (AoS, 3 per run)
   void* sL;        // Jacobi scaling in the inner precision
   MfRuns R;          // run plan shared by the FP64 and inner products
@@ -3770,7 +3837,10 @@ void mf_mixed_layout_base(int64_t n_nodes, int64_t n_elem, cf::Carve& c, Layout*
   L->z1 = c.take<double>(n);
   M->fe = c.take<double>(30 * n_elem);
   M->blocks = c.take<double>(9 * n_nodes);
-  M->geoL = c.take<char>(10 * n_elem * lpb);
+  const int64_t packed = 10 * ((n_elem + kBlock - 1) / kBlock) * kBlock;
+  M->geoL = c.take<char>(packed * lpb);
+  M->connT = c.take<int32_t>(packed);
+  M->rankT = c.take<uint16_t>(packed);
   M->feL = c.take<char>(30 * n_elem * lpb);
   M->sL = c.take<char>(n * lpb);
   mf_runs_layout(n_nodes, n_elem, c, &M->R);
@@ -4969,17 +5039,24 @@ int mf_mixed_solve(const char* who, const cf_matrix* A, int64_t n_nodes, int64_t
   CF_LAUNCHED();
   to_type<T><<<G, kBlock, 0, s>>>(n, L.s, sL);
   CF_LAUNCHED();
-  to_type<T><<<cf::stride_grid(10 * n_elem, kBlock), kBlock, 0, s>>>(10 * n_elem, geo, geoL);
-  CF_LAUNCHED();
   if ((rc = mf_runs_build(n_nodes, n_elem, offs, perm, &M.R, s))) return rc;
+  mf_pack_tiles<T><<<cf::stride_grid(10 * n_elem, kBlock), kBlock, 0, s>>>(
+      n_elem, conn, geo, M.R.rank, M.connT, geoL, M.rankT);
+  CF_LAUNCHED();
   int GS = 0;
   if (A && ((rc = build_sell(A, L, s, &GS)) || (rc = fill_sell<double>(A, L, nullptr, L.sval64, s))))
     return rc;
   const int ge = cf::stride_grid(n_elem, kBlock);
   const int gg = int(mf_gather_grid(n_nodes));
   const int32_t* xsplit = L.xsplit;
+  // With A the outer products start matrix-free (FP64, cheap) and switch to
+  // A once that converges: the solver then restarts from A's true residual
+  // and ends on A's own system (the matrix-free operator equals A only up
+  // to rounding, which the conditioning of a clamped beam magnifies).
+  bool use_A = false;
+  int outer_parts = gg;
   auto outer = [&](const double* xin, double* y, PcgState* state) {
-    if (A) {
+    if (use_A) {
       spmv_sell<double, double, double><<<GS, kBlock, 0, s>>>(
           A->n_rows, L.n_slices, L.rows, L.koff, L.scol, L.sval64, xin, y, fixed, state,
           L.partials, xsplit);
@@ -4988,7 +5065,12 @@ int mf_mixed_solve(const char* who, const cf_matrix* A, int64_t n_nodes, int64_t
     }
     return mf_apply(a, M.R, xin, y, fixed, M.fe, state, state ? L.partials : nullptr, s);
   };
-  const int outer_parts = A ? GS : gg;
+  auto switch_outer = [&]() {
+    if (!A || use_A) return false;
+    use_A = true;
+    outer_parts = GS;
+    return true;
+  };
   T* rl = static_cast<T*>(C.rl);
   T* u = static_cast<T*>(C.u);
   T* r2v = static_cast<T*>(C.r2);
@@ -5002,11 +5084,13 @@ int mf_mixed_solve(const char* who, const cf_matrix* A, int64_t n_nodes, int64_t
                      T* zout = nullptr, hipStream_t ps = nullptr) {
     if (!ps) ps = s;
     if (mode == 0)
-      mf_t10_elem_lp<T, false><<<ge, kBlock, 0, ps>>>(n_elem, conn, geoL, T(lam), T(mu), sL, din,
-                                                      M.R.rank, M.R.toff, M.R.tstart, feL, state);
+      mf_t10_elem_lp<T, false><<<ge, kBlock, 0, ps>>>(n_elem, M.connT, geoL, T(lam), T(mu), sL,
+                                                      din, M.rankT, M.R.toff, M.R.tstart, feL,
+                                                      state);
     else
-      mf_t10_elem_lp<T, true><<<ge, kBlock, 0, ps>>>(n_elem, conn, geoL, T(lam), T(mu), sL, din,
-                                                     M.R.rank, M.R.toff, M.R.tstart, feL, state);
+      mf_t10_elem_lp<T, true><<<ge, kBlock, 0, ps>>>(n_elem, M.connT, geoL, T(lam), T(mu), sL,
+                                                     din, M.rankT, M.R.toff, M.R.tstart, feL,
+                                                     state);
     CF_LAUNCHED();
     mf_gather_cheb<T><<<gg, kBlock, 0, ps>>>(n_nodes, M.R.nro, M.R.nq, feL, sL, mode, din, dout,
                                              zout ? zout : dout, lp<T>(L.rh), cdk, cwk,
@@ -5055,7 +5139,8 @@ int mf_mixed_solve(const char* who, const cf_matrix* A, int64_t n_nodes, int64_t
   }
   if (!two_level)
     return fcg_cheb_core<true, T>(outer, outer_parts, power, cheb, 0.0, kChebF32, n, b, x, fixed,
-                                  tol, maxit, L, stats_host, s);
+                                  tol, maxit, L, stats_host, s, nullptr, nullptr, 0, 0.0,
+                                  switch_outer);
   // two-level: z1 = C r; z2 = z1 + Cheb(r - K z1); z = z2 + C (r - K z2)
   const int GV = pcg_vec_grid();
   const int ga = int(std::min<int64_t>(C.n_agg, 65535));
@@ -5106,7 +5191,8 @@ int mf_mixed_solve(const char* who, const cf_matrix* A, int64_t n_nodes, int64_t
     return coarse_apply(r2v, true);                                 // u += C r2
   };
   return fcg_cheb_core<true, T>(outer, outer_parts, power, cheb, 0.0, kCheb2L, n, b, x, fixed,
-                                tol, maxit, L, stats_host, s, pre2l, u, 2, power_est);
+                                tol, maxit, L, stats_host, s, pre2l, u, 2, power_est,
+                                switch_outer);
 }
 
 }  // namespace

@@ -211,7 +211,8 @@ class DeviceMesh:
         width = {N.OP_T10_ELAST: 9, N.OP_P1_ELAST: 9, N.OP_P1_VMASS: 3}.get(op, 1)
         vals = torch.empty(max(width * pat.nnz, 1), dtype=torch.float64, device=self.dev)
         N.check(lib.cf_assemble_matrix(op, ptr(pat.indptr), ptr(pat.indices), ptr(pat.seg),
-                                       ptr(pat.contrib), self.n_nodes, pat.nnz, self.k,
+                                       ptr(pat.contrib), ptr(self.conn), self.n_nodes,
+                                       pat.nnz, self.k,
                                        ptr(geo), ptr(coef), float(lam), float(mu), ptr(vals),
                                        stream_handle()), "cf_assemble_matrix")
         return vals[:width * pat.nnz]

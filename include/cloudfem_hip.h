@@ -168,11 +168,13 @@ int cf_tet_geometry(const double* x, const double* y, const double* z,
  *      which take Lame lam, mu.  Output: scalar ops write values[nnz];
  *      *_ELAST write NB3 values[9*nnz]; P1_VMASS writes the
  *      component-diagonal layout values[3*nnz] (DOF row 3i+c holds
- *      block(i,j)[c][c] for the row's columns j).                          */
+ *      block(i,j)[c][c] for the row's columns j).  conn: the connectivity
+ *      the pattern was built from (the row node of an entry is conn[e][a]
+ *      of its first contributor).                                          */
 int cf_assemble_matrix(int32_t op, const int32_t* indptr,
                        const int32_t* indices, const int32_t* seg,
-                       const uint32_t* contrib, int64_t n_nodes, int64_t nnz,
-                       int32_t k, const double* geo, const double* coef,
+                       const uint32_t* contrib, const int32_t* conn, int64_t n_nodes,
+                       int64_t nnz, int32_t k, const double* geo, const double* coef,
                        double lam, double mu, double* values, void* stream);
 
 /* ---- right-hand sides: rhs[i] (or rhs[3i+c]) += sum over the node's
@@ -431,13 +433,15 @@ int cf_pcg_solve_mf_t10_twolevel(int64_t n_nodes, int64_t n_elem, const int32_t*
  *      its Chebyshev polynomial and coarse prolongation run matrix-free in
  *      `precision`: CF_PREC_MIXED = FP32, CF_PREC_FP64 = FP64 (the same
  *      algorithm in FP64 throughout, for measuring what precision alone
- *      gains).  The outer FP64 flexible CG multiplies by A when A is given
- *      (the assembled NB3 stiffness of the same mesh, node rows = n_nodes;
- *      the solver builds a sliced copy of it in the workspace), so the system
- *      solved is A's own -- the matrix cf_assemble_matrix(CF_OP_T10_ELAST)
- *      returns and the reference builds -- and the FP64 matrix-free operator
- *      otherwise (the same matrix up to rounding).  Same stop rule and
- *      residual replacement as cf_pcg_solve.                                */
+ *      gains).  Without A the outer FP64 flexible CG multiplies by the FP64
+ *      matrix-free operator (the same matrix as the assembled one up to
+ *      rounding).  With A (the assembled NB3 stiffness of the same mesh,
+ *      node rows = n_nodes; the solver builds a sliced copy of it in the
+ *      workspace) the outer products start matrix-free and, once that
+ *      converges, switch to A: the solver restarts from A's true residual
+ *      for at least one segment, so the system solved is A's own -- the
+ *      matrix cf_assemble_matrix(CF_OP_T10_ELAST) returns and the reference
+ *      builds.  Same stop rule and residual replacement as cf_pcg_solve.    */
 int cf_t10_twolevel_workspace_bytes(const cf_matrix* A, int64_t n_nodes, int64_t n_elem,
                                     int32_t precision, size_t* bytes);
 int cf_pcg_solve_t10_twolevel(const cf_matrix* A, int64_t n_nodes, int64_t n_elem,
