@@ -272,6 +272,30 @@ def pcg_matfree_t10(dm, lam, mu, b, x, fixed=None, precond=None, tol=None, maxit
     return stats
 
 
+def matfree_t10_lp_apply(dm, lam, mu, x, precision="mixed", scale=None, reps=0):
+    """y = S K S x with the inner-precision operator of the Chebyshev and
+    two-level solvers (cf_mf_t10_lp_apply): x float32 for "mixed", float64
+    for "fp64"; scale (float64, nullable) gives S.  Returns (y, avg_ms of
+    `reps` timed repetitions, or None)."""
+    prec = {"fp64": N.PREC_FP64, "mixed": N.PREC_MIXED}[precision]
+    want = torch.float64 if precision == "fp64" else torch.float32
+    if x.dtype != want:
+        raise TypeError(f"x must be {want} for precision {precision!r}")
+    lib = N.load()
+    conn, geo, offs, perm = dm.matfree_arrays()
+    nb = ctypes.c_size_t()
+    N.check(lib.cf_mf_t10_lp_workspace_bytes(dm.n_nodes, dm.n_elem, prec, ctypes.byref(nb)),
+            "cf_mf_t10_lp_workspace_bytes")
+    ws = workspace(nb.value, x.device)
+    y = torch.empty_like(x)
+    ms = ctypes.c_float(0.0)
+    N.check(lib.cf_mf_t10_lp_apply(dm.n_nodes, dm.n_elem, ptr(conn), ptr(geo), float(lam),
+                                   float(mu), ptr(offs), ptr(perm), prec, ptr(scale), ptr(x),
+                                   ptr(y), int(reps), ctypes.byref(ms), ptr(ws), nb.value,
+                                   stream_handle()), "cf_mf_t10_lp_apply")
+    return y, (ms.value if reps > 0 else None)
+
+
 def matfree_t10_apply(dm, lam, mu, x, fixed=None, chunked=False):
     """y = K x with the matrix-free Tet10 elasticity operator (chunked: the
     Morton-chunked plan, else the run-sum operator)."""

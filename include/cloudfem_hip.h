@@ -453,6 +453,21 @@ int cf_pcg_solve_t10_twolevel(const cf_matrix* A, int64_t n_nodes, int64_t n_ele
                               int32_t maxit, void* ws, size_t ws_bytes,
                               cf_pcg_stats* stats_host, void* stream);
 
+/* ---- the inner product of the Chebyshev / two-level solvers on its own
+ *      (no reference counterpart; for tests and kernel timing):
+ *      y = S K S x with S = diag(scale) (scale NULL: S = I) in the solvers'
+ *      inner precision -- x, y are float vectors for CF_PREC_MIXED, double
+ *      for CF_PREC_FP64 -- by the same run plan and kernels.  reps > 0
+ *      repeats the product reps more times between two HIP events and
+ *      stores the average milliseconds in *avg_ms_host (synchronous then). */
+int cf_mf_t10_lp_workspace_bytes(int64_t n_nodes, int64_t n_elem, int32_t precision,
+                                 size_t* bytes);
+int cf_mf_t10_lp_apply(int64_t n_nodes, int64_t n_elem, const int32_t* conn, const double* geo,
+                       double lam, double mu, const int32_t* offs, const uint32_t* perm,
+                       int32_t precision, const double* scale, const void* x, void* y,
+                       int32_t reps, float* avg_ms_host, void* ws, size_t ws_bytes,
+                       void* stream);
+
 /* ---- inv[perm[j]] = j for a permutation of 0..n-1.                     */
 int cf_invert_permutation(int64_t n, const uint32_t* perm, uint32_t* inv,
                           void* stream);

@@ -436,6 +436,34 @@ def test_matfree_operator_matches_assembled(gpu):
         assert rel_err(u_m.ravel(), u_a.ravel()) < 1e-10
 
 
+def test_inner_precision_operator_matches_assembled(gpu):
+    """The solvers' inner product S K S x (matrix-free, run plan, tile-packed
+    inputs) against the FP64 assembled matrix: the FP64 twin to rounding,
+    the FP32 one to FP32 rounding, on a jittered-free 12x3x2 box with a
+    random S."""
+    import torch
+    from cloudfem_amd._native import FMT_NB3
+    from cloudfem_amd.device import DeviceMesh
+    from cloudfem_amd.mesh import box_mesh_tet10
+    from cloudfem_amd.physics.elastic3d import lame
+    from cloudfem_amd.solver import DeviceMatrix, matfree_t10_lp_apply
+    m = box_mesh_tet10(12, 3, 2, xlim=(0.0, 0.4), ylim=(0.0, 0.05), zlim=(0.0, 0.05))
+    lam, mu = lame(210e9, 0.3)
+    dm = DeviceMesh(m)
+    A = DeviceMatrix(FMT_NB3, dm.pattern(), dm.matrix_values(5, lam=lam, mu=mu))
+    g = torch.Generator(device="cpu").manual_seed(5)
+    x = torch.randn(3 * m.n_nodes, dtype=torch.float64, generator=g).to("cuda")
+    sc = (torch.rand(3 * m.n_nodes, dtype=torch.float64, generator=g) + 0.5).to("cuda") * 1e-5
+    y_ref = sc * A.matvec(sc * x)
+    y64, _ = matfree_t10_lp_apply(dm, lam, mu, x, precision="fp64", scale=sc)
+    y32, _ = matfree_t10_lp_apply(dm, lam, mu, x.float(), precision="mixed", scale=sc)
+    e64 = float((y64 - y_ref).abs().max() / y_ref.abs().max())
+    e32 = float((y32.double() - y_ref).abs().max() / y_ref.abs().max())
+    print(f"inner product vs assembled: FP64 {e64:.2e}, FP32 {e32:.2e}")
+    assert e64 < 1e-13
+    assert e32 < 1e-5
+
+
 @pytest.mark.parametrize("dims", [(24, 3, 3), (5, 5, 5)])
 def test_chunked_matfree_solve_matches_two_pass(gpu, dims):
     """The LDS-chunked operator (Morton chunks, shared-node partials) and the
