@@ -71,7 +71,7 @@ EXPORTED = (
     "cf_mf_t10_mixed_workspace_bytes", "cf_pcg_solve_mf_t10_mixed",
     "cf_mf_t10_twolevel_workspace_bytes", "cf_pcg_solve_mf_t10_twolevel",
     "cf_t10_twolevel_workspace_bytes", "cf_pcg_solve_t10_twolevel", "cf_spmv_abs",
-    "cf_mf_t10_lp_workspace_bytes", "cf_mf_t10_lp_apply",
+    "cf_pcg_solve_noalloc", "cf_mf_t10_lp_workspace_bytes", "cf_mf_t10_lp_apply",
 )
 
 
@@ -170,6 +170,8 @@ _SIGS = {
     "cf_mf_t10_lp_workspace_bytes": ([_I64, _I64, _I32, _SZ], _I32),
     "cf_mf_t10_lp_apply": ([_I64, _I64, _P, _P, _D, _D, _P, _P, _I32, _P, _P, _P, _I32,
                             ctypes.POINTER(ctypes.c_float), _P, ctypes.c_size_t, _P], _I32),
+    "cf_pcg_solve_noalloc": ([ctypes.POINTER(CfMatrix), _P, _P, _P, _D, _I32,
+                              ctypes.POINTER(CfPcgStats), _P], _I32),
     "cf_pcg_solve_t10_twolevel": ([ctypes.POINTER(CfMatrix), _I64, _I64, _P, _P, _D, _D, _P, _P, _P, _P, _P, _P, _P,
                                    _P, _I32, _D, _I32, _P, ctypes.c_size_t,
                                    ctypes.POINTER(CfPcgStats), _P], _I32),

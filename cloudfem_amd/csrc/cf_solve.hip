@@ -5299,6 +5299,92 @@ extern "C" int cf_invert_permutation(int64_t n, const uint32_t* perm, uint32_t* 
   return CF_OK;
 }
 
+// ------------------------------------------------------------- no-alloc solve
+// cf_pcg_solve_noalloc: cf_pcg_solve for scalar CSR systems in FP64 with
+// Jacobi preconditioning, whose workspace (CG vectors, block partials, solver
+// state) lives in a file-scope device array instead of a caller buffer.
+
+namespace {
+
+constexpr int64_t kNoAllocMaxDofs = int64_t(1) << 18;  // 262,144 unknowns
+// four FP64 vectors of kNoAllocMaxDofs, partials for grids of up to 4096
+// blocks, the state and the flag, each carved at 256-byte alignment
+constexpr size_t kNoAllocBytes = 4 * (size_t(kNoAllocMaxDofs) * 8) + 4 * 4096 * 8 +
+                                 2 * 4096 * 8 + 4 * 256;
+
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// per device: the device address of `scratch` (the host-side name is only a
+// stand-in for it)
+void* scratch_address() {
+  static std::mutex mu;
+  static void* addr[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+    addr[dev] = p;
+  }
+  return addr[dev];
+}
+
+__global__ void zero_words(int64_t n, uint32_t* __restrict__ p) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    p[i] = 0u;
+}
+
+}  // namespace
+
+extern "C" int cf_pcg_solve_noalloc(const cf_matrix* A, const double* b, double* x,
+                                    const uint8_t* fixed, double tol, int32_t maxit,
+                                    cf_pcg_stats* stats_host, void* stream) {
+  int rc = check_matrix(A);
+  if (rc) return rc;
+  CF_REQUIRE(b && x && stats_host, "cf_pcg_solve_noalloc: null argument");
+  CF_REQUIRE(A->format == CF_FMT_CSR,
+             "cf_pcg_solve_noalloc: scalar CSR matrices only (use cf_pcg_solve for NB3)");
+  CF_REQUIRE(tol > 0.0 && maxit > 0, "cf_pcg_solve_noalloc: tol and maxit must be positive");
+  if (int64_t(A->n_rows) > kNoAllocMaxDofs) {
+    cf::set_error("cf_pcg_solve_noalloc: %d unknowns exceed the limit of %lld",
+                  int(A->n_rows), (long long)kNoAllocMaxDofs);
+    return CF_ERR_ARG;
+  }
+  void* base = scratch_address();
+  if (!base) {
+    cf::set_error("cf_pcg_solve_noalloc: hipGetSymbolAddress of the scratch array failed");
+    return CF_ERR_HIP;
+  }
+  Layout L = layout(A, CF_PREC_FP64, CF_PC_JACOBI, base, 0);
+  if (L.total > kNoAllocBytes) {
+    cf::set_error("cf_pcg_solve_noalloc: needs %zu bytes of scratch, has %zu (grid of %d blocks)",
+                  L.total, kNoAllocBytes, pcg_grid());
+    return CF_ERR_WORKSPACE;
+  }
+  hipStream_t s = cf::as_stream(stream);
+  const int G = pcg_grid();
+  *stats_host = cf_pcg_stats{};
+  // the array keeps the previous call's contents: clear the state and flag
+  zero_words<<<1, 64, 0, s>>>(int64_t(sizeof(PcgState) / 4), reinterpret_cast<uint32_t*>(L.state));
+  CF_LAUNCHED();
+  zero_words<<<1, 64, 0, s>>>(1, reinterpret_cast<uint32_t*>(L.bad));
+  CF_LAUNCHED();
+  diag_csr<<<G, kBlock, 0, s>>>(A->n_rows, A->indptr, A->indices, A->values, fixed, L.dinv,
+                                L.bad);
+  CF_LAUNCHED();
+  int32_t bad = 0;
+  CF_HIP(hipMemcpyAsync(&bad, L.bad, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  CF_HIP(hipStreamSynchronize(s));
+  if (bad) {
+    cf::set_error("cf_pcg_solve_noalloc: %d free diagonal entries are not positive "
+                  "(matrix not SPD on the free DOFs)", bad);
+    return CF_ERR_BREAKDOWN;
+  }
+  return pcg_fp64<CF_PC_JACOBI>(A, b, x, fixed, tol, maxit, L, stats_host, s);
+}
+
 // ------------------------------------------------------------- inner product alone
 namespace {
 

@@ -155,6 +155,23 @@ def pcg(A, b, x, fixed=None, precision=None, precond=None, tol=None, maxit=None)
     return stats
 
 
+def pcg_noalloc(A, b, x, fixed=None, tol=None, maxit=None):
+    """cf_pcg_solve_noalloc: FP64 Jacobi PCG on a scalar CSR DeviceMatrix with
+    the library's static scratch array as workspace (nothing is allocated;
+    at most 262,144 unknowns; overlapping calls on one GPU race)."""
+    o = options()
+    tol = float(tol if tol is not None else o["tol"])
+    maxit = int(maxit if maxit is not None else o["maxit"])
+    As = A.cstruct()
+    st = N.CfPcgStats()
+    rc = N.load().cf_pcg_solve_noalloc(ctypes.byref(As), ptr(b), ptr(x), ptr(fixed), tol, maxit,
+                                       ctypes.byref(st), stream_handle())
+    stats = {**_base_stats(st), "precision": "fp64", "precond": "jacobi", "tol": tol}
+    last_stats.value = stats
+    N.check(rc, "cf_pcg_solve_noalloc")
+    return stats
+
+
 def pcg_matfree_t10_mixed(dm, lam, mu, b, x, fixed=None, tol=None, maxit=None):
     """Mixed precision with the matrix-free Tet10 operator: FP64 flexible CG,
     FP32 matrix-free Chebyshev preconditioner (cf_pcg_solve_mf_t10_mixed)."""

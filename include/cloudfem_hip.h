@@ -453,6 +453,20 @@ int cf_pcg_solve_t10_twolevel(const cf_matrix* A, int64_t n_nodes, int64_t n_ele
                               int32_t maxit, void* ws, size_t ws_bytes,
                               cf_pcg_stats* stats_host, void* stream);
 
+/* ---- no-allocation PCG (no reference counterpart): for callers that cannot
+ *      allocate device memory or pass a workspace -- code under HIP graph
+ *      capture, tight serving loops, harnesses that forbid hipMalloc.  Same
+ *      solve as cf_pcg_solve(A, ..., CF_PREC_FP64, CF_PC_JACOBI, ...) for a
+ *      scalar CSR matrix (CF_FMT_CSR only), with the workspace in a static
+ *      device array of the library (8.6 MB per GPU, held until the process
+ *      ends): at most 262,144 unknowns, larger systems return CF_ERR_ARG.
+ *      The call allocates nothing.  Every call on a GPU shares that GPU's
+ *      array, so calls that overlap on one GPU (two streams or two host
+ *      threads) race: run them one at a time.                             */
+int cf_pcg_solve_noalloc(const cf_matrix* A, const double* b, double* x,
+                         const uint8_t* fixed, double tol, int32_t maxit,
+                         cf_pcg_stats* stats_host, void* stream);
+
 /* ---- the inner product of the Chebyshev / two-level solvers on its own
  *      (no reference counterpart; for tests and kernel timing):
  *      y = S K S x with S = diag(scale) (scale NULL: S = I) in the solvers'
