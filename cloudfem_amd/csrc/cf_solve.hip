@@ -1131,6 +1131,22 @@ __global__ void mf_run_len(int64_t n_items, const uint32_t* __restrict__ ridx,
     nro[n] = int32_t(ridx[offs[n]]);
 }
 
+// run4[i] = the node's run ids in tile order, -1 padded, when it has at most
+// four; *over counts the nodes with more (the plan then keeps nro / nq only)
+__global__ void mf_run4_fill(int64_t n_nodes, const int32_t* __restrict__ nro,
+                             const uint32_t* __restrict__ nq, int4* __restrict__ run4,
+                             int32_t* __restrict__ over) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n_nodes;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int32_t b = nro[i], n = nro[i + 1] - b;
+    if (n > 4) atomicAdd(over, 1);
+    int q[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) q[u] = u < n ? int32_t(nq[b + u]) : -1;
+    run4[i] = make_int4(q[0], q[1], q[2], q[3]);
+  }
+}
+
 // tile order of the runs: rq[q] = node-order run id of the q-th smallest
 // start rk; nq[r] = q, tstart[q] = start inside the tile
 __global__ void mf_run_tile_order(int64_t n_runs, const uint32_t* __restrict__ rk_sorted,
@@ -1644,13 +1660,17 @@ mf_t10_elem_lp(int64_t m, const int32_t* __restrict__ conn, const T* __restrict_
 // needs 1.03 rounds at 539,601 nodes: 14.8 vs 13.2 us), and it loads the
 // node vectors before the dependent nro -> nq -> partial chain, whose loads
 // it interleaves over the two nodes.
-template <class T>
+// R4 (every node has at most four runs, the usual case): the node's run ids
+// come from one 16-byte load of run4[i] (-1 = none), so the partial loads
+// wait on one load instead of the nro -> nq chain.
+template <class T, bool R4 = false>
 __global__ void __launch_bounds__(kBlock)
 mf_gather_cheb(int64_t n_nodes, const int32_t* __restrict__ nro,
                const uint32_t* __restrict__ nq, const T* __restrict__ part,
                const T* __restrict__ sc, int mode, const T* __restrict__ zc,
                const T* zp, T* zn, T* __restrict__ w, T cd, T cw,
-               const T* __restrict__ rin, const PcgState* state) {
+               const T* __restrict__ rin, const PcgState* state,
+               const int4* __restrict__ run4 = nullptr) {
   if (state && state->halt) return;
   const int64_t i0 = tile_index(blockIdx.x) * (2 * kBlock) + threadIdx.x;
   // per node: S, then zc and w (modes 1, 2, 5) or rin (modes 3, 4), zp
@@ -1658,12 +1678,17 @@ mf_gather_cheb(int64_t n_nodes, const int32_t* __restrict__ nro,
   T acc[2][3] = {}, sv[2][3] = {}, zv[2][3] = {}, wv[2][3] = {}, pv[2][3] = {},
     uv[2][3] = {};
   int32_t jb[2] = {0, 0}, je[2] = {0, 0};
+  int4 r4[2] = {make_int4(-1, -1, -1, -1), make_int4(-1, -1, -1, -1)};
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int64_t i = i0 + h * kBlock;
     if (i >= n_nodes) continue;
-    jb[h] = nro[i];
-    je[h] = nro[i + 1];
+    if constexpr (R4) {
+      r4[h] = run4[i];
+    } else {
+      jb[h] = nro[i];
+      je[h] = nro[i + 1];
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const int64_t k = 3 * i + c;
@@ -1678,7 +1703,20 @@ mf_gather_cheb(int64_t n_nodes, const int32_t* __restrict__ nro,
       if (mode == 5) uv[h][c] = zn[k];
     }
   }
-  for (int32_t j = 0;; ++j) {
+  if constexpr (R4) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int q[4] = {r4[h].x, r4[h].y, r4[h].z, r4[h].w};
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (q[u] >= 0) {
+          const Vec3<T> v = *reinterpret_cast<const Vec3<T>*>(part + 3 * int64_t(q[u]));
+#pragma unroll
+          for (int c = 0; c < 3; ++c) acc[h][c] += v.v[c];
+        }
+    }
+  }
+  for (int32_t j = 0; !R4; ++j) {
     const bool m0 = jb[0] + j < je[0], m1 = jb[1] + j < je[1];
     if (!m0 && !m1) break;
     if (m0) {
@@ -3336,6 +3374,9 @@ struct MfRuns {
   uint16_t* tstart;  // start of each run inside its tile
   uint32_t* nq;      // runs of each node, tile order
   int32_t* nro;      // first entry of each node in nq
+  int4* run4;        // the same, four per node (-1 padded), when use_run4
+  int32_t* over;     // nodes with more than four runs (device)
+  bool use_run4;     // set by mf_runs_build (host)
   // build temporaries
   uint32_t* keys_in;
   uint32_t* keys_out;
@@ -3356,6 +3397,9 @@ void mf_runs_layout(int64_t n_nodes, int64_t n_elem, cf::Carve& c, MfRuns* R) {
   R->tstart = c.take<uint16_t>(items);
   R->nq = c.take<uint32_t>(items);
   R->nro = c.take<int32_t>(n_nodes + 1);
+  R->run4 = c.take<int4>(n_nodes);
+  R->over = c.take<int32_t>(1);
+  R->use_run4 = false;
   R->rk = c.take<uint32_t>(items);
   R->rl = c.take<uint16_t>(items);
   R->keys_in = c.take<uint32_t>(items + 1);
@@ -3418,6 +3462,14 @@ int mf_runs_build(int64_t n_nodes, int64_t n_elem, const int32_t* offs, const ui
   mf_tile_run_off<<<cf::stride_grid(n_tiles + 1, kBlock), kBlock, 0, s>>>(n_tiles, n_runs,
                                                                           M.keys_out, M.toff);
   CF_LAUNCHED();
+  CF_HIP(hipMemsetAsync(M.over, 0, sizeof(int32_t), s));
+  mf_run4_fill<<<cf::stride_grid(n_nodes, kBlock), kBlock, 0, s>>>(n_nodes, M.nro, M.nq, M.run4,
+                                                                   M.over);
+  CF_LAUNCHED();
+  int32_t over = 0;
+  CF_HIP(hipMemcpyAsync(&over, M.over, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  CF_HIP(hipStreamSynchronize(s));
+  R->use_run4 = over == 0;
   return CF_OK;
 }
 
@@ -5098,9 +5150,15 @@ int mf_mixed_solve(const char* who, const cf_matrix* A, int64_t n_nodes, int64_t
                                                      din, M.rankT, M.R.toff, M.R.tstart, feL,
                                                      state);
     CF_LAUNCHED();
-    mf_gather_cheb<T><<<gg, kBlock, 0, ps>>>(n_nodes, M.R.nro, M.R.nq, feL, sL, mode, din, dout,
-                                             zout ? zout : dout, lp<T>(L.rh), cdk, cwk,
-                                             two_level ? rl : nullptr, state);
+    if (M.R.use_run4)
+      mf_gather_cheb<T, true><<<gg, kBlock, 0, ps>>>(n_nodes, M.R.nro, M.R.nq, feL, sL, mode, din,
+                                                     dout, zout ? zout : dout, lp<T>(L.rh), cdk,
+                                                     cwk, two_level ? rl : nullptr, state,
+                                                     M.R.run4);
+    else
+      mf_gather_cheb<T><<<gg, kBlock, 0, ps>>>(n_nodes, M.R.nro, M.R.nq, feL, sL, mode, din,
+                                               dout, zout ? zout : dout, lp<T>(L.rh), cdk, cwk,
+                                               two_level ? rl : nullptr, state);
     CF_LAUNCHED();
     return CF_OK;
   };
@@ -5431,9 +5489,15 @@ int mf_lp_apply(int64_t n_nodes, int64_t n_elem, const int32_t* conn, const doub
                                                     M.rankT, M.R.toff, M.R.tstart,
                                                     static_cast<T*>(M.feL), nullptr);
     CF_LAUNCHED();
-    mf_gather_cheb<T><<<gg, kBlock, 0, s>>>(n_nodes, M.R.nro, M.R.nq, static_cast<T*>(M.feL),
-                                            sL, 0, nullptr, nullptr, y, nullptr, T(0), T(0),
-                                            nullptr, nullptr);
+    if (M.R.use_run4)
+      mf_gather_cheb<T, true><<<gg, kBlock, 0, s>>>(n_nodes, M.R.nro, M.R.nq,
+                                                    static_cast<T*>(M.feL), sL, 0, nullptr,
+                                                    nullptr, y, nullptr, T(0), T(0), nullptr,
+                                                    nullptr, M.R.run4);
+    else
+      mf_gather_cheb<T><<<gg, kBlock, 0, s>>>(n_nodes, M.R.nro, M.R.nq, static_cast<T*>(M.feL),
+                                              sL, 0, nullptr, nullptr, y, nullptr, T(0), T(0),
+                                              nullptr, nullptr);
     CF_LAUNCHED();
     return CF_OK;
   };
