@@ -4222,31 +4222,56 @@ probe_extract(int64_t n_agg, int n_col, int c, const int32_t* __restrict__ nbc,
   }
 }
 
-// Largest diagonal entry of A (one block; deterministic tree).
-__global__ void __launch_bounds__(kBlock)
-coarse_diag_max(int64_t nc, const double* __restrict__ A, double* __restrict__ out) {
-  __shared__ double sh[kBlock];
-  double m = 0.0;
-  for (int64_t j = threadIdx.x; j < nc; j += blockDim.x) m = fmax(m, A[j + j * nc]);
-  sh[threadIdx.x] = m;
-  __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[0] = sh[0];
-}
 
 // A vanishing diagonal (an aggregate without free DOFs, or a degenerate mode)
 // becomes an identity row and column of the lower triangle rocSOLVER reads.
 // The probed blocks A[h, g] and A[g, h] agree to rounding, and potrf uses the
 // lower one, so no symmetrisation is needed.
+// Coarse modes that the aggregate's free nodes do not support independently:
+// one thread per aggregate factors its 6x6 diagonal block (Cholesky in mode
+// order) and flags every mode whose pivot falls to 1e-10 of the block's
+// largest diagonal entry or below -- a vanishing mode (no free node), or a
+// dependent one (two free nodes, say, leave the rotation about the line
+// through them without stiffness although every diagonal entry is positive).
+__global__ void coarse_dependent_modes(int64_t n_agg, int64_t nc, const double* __restrict__ A,
+                                       int32_t* __restrict__ dep) {
+  for (int64_t g = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; g < n_agg;
+       g += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t o = 6 * g;
+    double L[6][6] = {};
+    double mx = 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) mx = fmax(mx, A[(o + i) + (o + i) * nc]);
+    const double tol = 1e-10 * mx;
+    bool ok[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      double d = A[(o + k) + (o + k) * nc];
+      for (int m = 0; m < k; ++m)
+        if (ok[m]) d -= L[k][m] * L[k][m];
+      ok[k] = d > tol;
+      dep[o + k] = ok[k] ? 0 : 1;
+      if (!ok[k]) continue;
+      L[k][k] = sqrt(d);
+      for (int i = k + 1; i < 6; ++i) {
+        double v = A[(o + i) + (o + k) * nc];
+        for (int m = 0; m < k; ++m)
+          if (ok[m]) v -= L[i][m] * L[k][m];
+        L[i][k] = v / L[k][k];
+      }
+    }
+  }
+}
+
+// Flagged modes get identity rows and columns for the factorisation (lower
+// triangle, column-major nc x nc); coarse_drop_modes then clears them in the
+// inverse, so the coarse correction leaves them out.  Two flagged modes j1 < j2
+// both clear entry (j2, j1): the same value, so the overlap is benign.
 __global__ void coarse_regularize(int64_t nc, double* __restrict__ A,
-                                  const double* __restrict__ dmax) {
-  const double tiny = 1e-12 * dmax[0];
+                                  const int32_t* __restrict__ dep) {
   for (int64_t j = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; j < nc;
        j += int64_t(gridDim.x) * blockDim.x) {
-    if (A[j + j * nc] > tiny) continue;
+    if (!dep[j]) continue;
     for (int64_t i = j + 1; i < nc; ++i) A[i + j * nc] = 0.0;  // column j below the diagonal
     for (int64_t k = 0; k < j; ++k) A[j + k * nc] = 0.0;       // row j left of it
     A[j + j * nc] = 1.0;
@@ -4649,6 +4674,7 @@ struct Coarse {
   void* rl;            // copy of the outer residual in the inner precision
   void* r2;            // residual after the smoothing step (inner precision)
   int32_t* info;
+  int32_t* dep;        // coarse modes dropped as dependent (1) or kept (0)
   void* tmp;
   size_t tmp_bytes = 0;
 };
@@ -4683,6 +4709,7 @@ void coarse_layout(int64_t n_nodes, int64_t n_elem, cf::Carve& c, Coarse* C, int
   C->rl = c.take<char>(n * lpb);
   C->r2 = c.take<char>(n * lpb);
   C->info = c.take<int32_t>(2);
+  C->dep = c.take<int32_t>(ncmax);
   size_t b1 = 0, b2 = 0, b3 = 0, b4 = 0;
   rocprim::radix_sort_pairs(nullptr, b1, (uint32_t*)nullptr, (uint32_t*)nullptr,
                             (uint32_t*)nullptr, (uint32_t*)nullptr, size_t(n_nodes), 0, 32);
@@ -4716,8 +4743,10 @@ size_t mf_mixed_layout(int64_t n_nodes, int64_t n_elem, void* base, Layout* L, M
 // per device for the life of the process; not thread-safe).
 rocblas_handle solver_handle() {
   static rocblas_handle handles[64] = {};
+  static std::mutex mu;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  std::lock_guard<std::mutex> lock(mu);
   if (!handles[dev] && rocblas_create_handle(&handles[dev]) != rocblas_status_success)
     handles[dev] = nullptr;
   return handles[dev];
@@ -4891,12 +4920,26 @@ int coarse_setup(const MfArgs& a, const MfRuns& R, const double* xs, const doubl
     CF_LAUNCHED();
   }
   const int gc = cf::stride_grid(nc, 64);
-  coarse_diag_max<<<1, kBlock, 0, s>>>(nc, C->A, C->zc);
+  coarse_dependent_modes<<<cf::stride_grid(n_agg, 64), 64, 0, s>>>(n_agg, nc, C->A, C->dep);
   CF_LAUNCHED();
-  coarse_regularize<<<gc, 64, 0, s>>>(nc, C->A, C->zc);
+  coarse_regularize<<<gc, 64, 0, s>>>(nc, C->A, C->dep);
   CF_LAUNCHED();
   C->bw = 6 * agg_band + 5;  // scalar half-bandwidth
   return CF_OK;
+}
+
+// Rows and columns of the dropped coarse modes of the inverse set to zero
+// (one thread per mode clears both).
+__global__ void coarse_drop_modes(int64_t nc, double* __restrict__ Ainv,
+                                  const int32_t* __restrict__ dep) {
+  for (int64_t j = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; j < nc;
+       j += int64_t(gridDim.x) * blockDim.x) {
+    if (!dep[j]) continue;
+    for (int64_t i = 0; i < nc; ++i) {
+      Ainv[i + j * nc] = 0.0;
+      Ainv[j + i * nc] = 0.0;
+    }
+  }
 }
 
 // Factor and invert the coarse matrix on stream s, with no host
@@ -4941,6 +4984,8 @@ int coarse_factor(Coarse* C, int64_t n, hipStream_t s) {
   const int64_t nt = (nc + 31) / 32;
   coarse_fill_upper<<<int(std::min<int64_t>(nt * nt, 65535)), kBlock, 0, s>>>(nc, C->A);
   CF_LAUNCHED();
+  coarse_drop_modes<<<cf::stride_grid(nc, 64), 64, 0, s>>>(nc, C->A, C->dep);
+  CF_LAUNCHED();
   to_type<float><<<cf::stride_grid(nc * nc, kBlock), kBlock, 0, s>>>(nc * nc, C->A, C->A32);
   CF_LAUNCHED();
   return CF_OK;
@@ -4965,10 +5010,19 @@ struct Side {
   hipEvent_t ready = nullptr, done = nullptr;
 };
 
+// Per device: serialises two-level solves (they share the side stream and
+// its events) and guards the lazy creation below.
+std::mutex& device_mutex(int dev) {
+  static std::mutex mu[64];
+  return mu[dev];
+}
+
 const Side* side_stream() {
   static Side side[64];
+  static std::mutex mu;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  std::lock_guard<std::mutex> lock(mu);
   Side& S = side[dev];
   if (!S.s && hipStreamCreateWithFlags(&S.s, hipStreamNonBlocking) != hipSuccess) S.s = nullptr;
   if (S.s && !S.ready && hipEventCreateWithFlags(&S.ready, hipEventDisableTiming) != hipSuccess)
@@ -5170,6 +5224,15 @@ int mf_mixed_solve(const char* who, const cf_matrix* A, int64_t n_nodes, int64_t
   // builds, factors and inverts the coarse matrix (whose host colouring and
   // one-workgroup factorisation leave most of the GPU idle).
   double power_est = 0.0;
+  // two-level solves on one device run one at a time (shared side stream,
+  // events and rocBLAS handle); the lock is held until the solve returns
+  std::unique_lock<std::mutex> device_lock;
+  if (two_level) {
+    int dev = 0;
+    CF_HIP(hipGetDevice(&dev));
+    CF_REQUIRE(dev >= 0 && dev < 64, "two-level: device %d out of range", dev);
+    device_lock = std::unique_lock<std::mutex>(device_mutex(dev));
+  }
   SideJoin join;
   if (two_level) {
     const Side* side = side_stream();

@@ -214,6 +214,36 @@ def test_twolevel_assembled_outer_matches_spsolve(gpu, precision, dims):
     assert err < 1e-10
 
 
+@pytest.mark.parametrize("n_free", [1, 2])
+def test_twolevel_aggregate_with_few_free_nodes(gpu, n_free):
+    """Every node clamped but one or two: the one aggregate's six rigid
+    modes then span only 3 or 5 independent directions on its free DOFs,
+    although every coarse diagonal entry is positive.  The dependent modes
+    must be dropped before the coarse factorisation, and the solve must
+    match SuperLU."""
+    from cloudfem_amd import solver
+    from cloudfem_amd.mesh import box_mesh_tet10
+    from cloudfem_amd.physics import elastic_tet10
+    m, m0 = box_mesh_tet10(4, 2, 2), om.box_mesh_tet10(4, 2, 2)
+    keep = [m.n_corners + 7, m.n_corners + 40][:n_free]      # interior midpoints
+    fixed_ids = np.setdiff1d(np.arange(m.n_nodes), keep)
+    for mesh in (m, m0):
+        mesh.boundaries = dict(mesh.boundaries, clamp=fixed_ids)
+    bc = {"clamp": (0.0, 0.0, 0.0)}
+    body = (1e3, -2e3, -7850.0 * 9.81)
+    for op in ("matfree", "assembled"):
+        with solver.solver_options(operator=op, precision="mixed", precond="twolevel",
+                                   tol=1e-12):
+            u = np.stack(elastic_tet10.solve_elasticity_tet10(m, 210e9, 0.3, bc=bc, body=body),
+                         axis=1).ravel()
+        st = solver.last_stats.value
+        A0, rhs0 = oel.assemble_elasticity_tet10(m0, 210e9, 0.3, body=body)
+        u0 = oel.apply_dirichlet_vec3(A0, rhs0, m0, bc)
+        err = rel_err(u, u0)
+        print(f"{n_free} free node(s), {op}: {st['iterations']} iterations, error {err:.2e}")
+        assert err < 1e-10
+
+
 def test_twolevel_jittered_mesh_body_load(gpu):
     """Two-level solver on an irregular Tet10 mesh (interior and boundary
     nodes of a 36x6x6 Kuhn box jittered by up to 0.2 h, 7,776 elements, six
