@@ -1,7 +1,7 @@
 // Batched theta-method heat solves: many independent scalar systems (one per
 // mesh) stored as one block-diagonal CSR, stepped together.
 //
-// Replaces, for a batch of meshes, the per-mesh loop of tet_heat.py:356-396
+// Replaces, for a batch of meshes, the per-mesh loop of tet_heat.py:234-274
 // ((C + th h K) T+ = (C - (1-th) h K) T + h F with one LU) by warm-started
 // CG per step with per-system scalars: every launch covers all systems,
 // every CG scalar (alpha, beta, residual, stop test) is per system, and a

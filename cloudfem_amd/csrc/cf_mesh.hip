@@ -4,7 +4,7 @@
 // tet10_mesh.py:77-112 numbers the midpoint of the sorted corner pair (a, b)
 // n_corners + (rank of (a, b) among the unique pairs, lexicographic), the
 // order np.unique(axis=0) produces; here one stable radix sort of the 64-bit
-// keys a * n + b gives the same order.  tet_heat.py:219-242 (and the Tet10
+// keys a * n + b gives the same order.  tet_heat.py:97-120 (and the Tet10
 // and elasticity copies) keep a face when its sorted corner triple occurs in
 // exactly one element and all three corners are in the named set, listed in
 // first-occurrence order (element order, then local face order) with that
@@ -25,7 +25,7 @@ constexpr int kBlock = 256;
 
 // local edges of the midpoint slots 4..9 (tet10_mesh.py:33)
 __constant__ int kEdge[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
-// P1 faces (tet_heat.py:233) and Tet10 faces: corners + midpoints (tet10.py:287-292)
+// P1 faces (tet_heat.py:111) and Tet10 faces: corners + midpoints (tet10.py:287-292)
 __constant__ int kFace4[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
 __constant__ int kFace10[4][6] = {
     {0, 1, 2, 4, 5, 6}, {0, 3, 1, 7, 8, 4}, {0, 2, 3, 6, 9, 7}, {1, 3, 2, 8, 9, 5}};

@@ -1,6 +1,6 @@
 """Theta-method time stepping on the GPU (shared by P1 and Tet10 heat).
 
-Replaces the LU-once loop of tet_heat.py:356-396 / tet10.py:504-548:
+Replaces the LU-once loop of tet_heat.py:234-274 / tet10.py:504-548:
     (C + th h K) T+ = (C - (1-th) h K) T + h F,
 with the Dirichlet elimination done by solving the free-DOF system (the
 same system the reference's apply_dirichlet + per-step rhs - A[:, F] v

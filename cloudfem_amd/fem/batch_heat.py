@@ -1,6 +1,6 @@
 """Batched theta-method transient heat on many P1 meshes at once.
 
-The reference runs `transient_heat_tet` (tet_heat.py:319-396) one mesh at a
+The reference runs `transient_heat_tet` (tet_heat.py:197-274) one mesh at a
 time: assemble K, F, C, factorize C + th h K once, then one triangular solve
 per step.  `transient_heat_batch` runs the same recursion with the same
 per-mesh semantics for a list of meshes: the meshes become one
@@ -13,7 +13,7 @@ Per mesh i, like transient_heat_tet: k, rho_cp, f evaluated at element
 centroids (a float, a callable(x, y, z), or here also an (M_i,) array of
 per-element values, the data the reference feeds through its nearest-centroid
 lookup, induction3d.py:111-123); Neumann q A/3 and Robin h A (1+d)/12,
-h T_inf A/3 on the named boundaries (tet_heat.py:258-294); h = t_end /
+h T_inf A/3 on the named boundaries (tet_heat.py:136-172); h = t_end /
 ceil(t_end / dt); theta from `method`.  Dirichlet conditions are not part of
 the batched path (the batched configuration has none): use
 transient_heat_tet for them.

@@ -22,13 +22,13 @@ def _mass_device(dm, mesh, rho_cp):
 
 def assemble_mass_tet(mesh, rho_cp=1.0):
     """Consistent capacity C = sum_e rho_cp(x_c) V (1 + d_ab)/20
-    (tet_heat.py:194-216)."""
+    (tet_heat.py:72-94)."""
     dm = DeviceMesh(mesh)
     return scalar_csr(dm.pattern(), _mass_device(dm, mesh, rho_cp))
 
 
 def _boundary_faces(mesh, name):
-    """Exterior faces (F, 3) of a named boundary (tet_heat.py:219-242)."""
+    """Exterior faces (F, 3) of a named boundary (tet_heat.py:97-120)."""
     return _boundary_faces_np(mesh, name)
 
 
@@ -40,7 +40,7 @@ def _boundary_device(dm, mesh, neumann, robin):
     """(Robin face pattern + values or None, load vector) on the device.
 
     All Robin faces of all names form one face matrix, as the reference
-    gathers them into one COO (tet_heat.py:278-293).
+    gathers them into one COO (tet_heat.py:156-171).
     """
     F = torch.zeros(dm.n_nodes, dtype=torch.float64, device=dm.dev)
     for name, q in (neumann or {}).items():
@@ -73,21 +73,21 @@ def _face_csr(n, term):
 
 def assemble_boundary_terms(mesh, neumann=None, robin=None):
     """(K_add csr_matrix, F_add (N,)): Neumann q A/3; Robin h A (1+d)/12 and
-    h T_inf A/3 (tet_heat.py:258-294)."""
+    h T_inf A/3 (tet_heat.py:136-172)."""
     dm = DeviceMesh(mesh)
     term, F = _boundary_device(dm, mesh, neumann, robin)
     return _face_csr(dm.n_nodes, term), F.cpu().numpy()
 
 
 def _dirichlet_values(mesh, bc):
-    """{node: value} of a bc dict (tet_heat.py:297-312)."""
+    """{node: value} of a bc dict (tet_heat.py:175-190)."""
     mask, vals = dirichlet_scalar(mesh, bc)
     return {int(i): float(vals[i]) for i in np.nonzero(mask)[0]}
 
 
 def transient_heat_tet(mesh, k=1.0, rho_cp=1.0, f=0.0, bc=None, neumann=None, robin=None,
                        t_end=1.0, dt=0.1, T0=0.0, method="cn"):
-    """Theta-method transient heat on a P1 mesh (tet_heat.py:319-396).
+    """Theta-method transient heat on a P1 mesh (tet_heat.py:197-274).
 
     Returns (times (S+1,), T_history (S+1, N)).
     """

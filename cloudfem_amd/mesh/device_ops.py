@@ -2,7 +2,7 @@
 
 tet4_to_tet10_device mirrors tet4_to_tet10 (tet10_mesh.py:77-112) and
 boundary_faces_device / boundary_faces10_device mirror the exterior-face scans
-of tet_heat.py:219-242 / tet10.py:329-352, with the sorts and scans on the
+of tet_heat.py:97-120 / tet10.py:329-352, with the sorts and scans on the
 device (cf_tet10_edges, cf_edge_midpoints, cf_boundary_faces).  The results
 are returned as host arrays, identical to the host versions.
 """

@@ -1,7 +1,7 @@
 """Exterior boundary faces of a named node set (host-side mesh processing).
 
 Same rule and order as the reference's dictionary scans
-(tet_heat.py:219-242, tet10.py:329-352, elastic_tet10.py:68-91): a face is
+(tet_heat.py:97-120, tet10.py:329-352, elastic_tet10.py:68-91): a face is
 exterior iff its sorted corner triple occurs in exactly one element; it is
 kept iff all three corners are in the set; faces are listed in order of
 first occurrence (element order, then local face order) with the node
@@ -10,7 +10,18 @@ order of that occurrence.  Done with one sort of 64-bit face keys.
 
 import numpy as np
 
-# P1 local faces (a,b,c), (a,b,d), (a,c,d), (b,c,d) (tet_heat.py:233)
+# From this many elements up the scan runs on the GPU when one is visible
+# (cf_boundary_faces: one device sort of the face keys); same faces, same order.
+DEVICE_MIN_ELEMENTS = 50_000
+
+
+def _use_device(mesh):
+    if np.asarray(mesh.elements).shape[0] < DEVICE_MIN_ELEMENTS:
+        return False
+    import torch
+    return torch.cuda.is_available()
+
+# P1 local faces (a,b,c), (a,b,d), (a,c,d), (b,c,d) (tet_heat.py:111)
 P1_FACES = ((0, 1, 2), (0, 1, 3), (0, 2, 3), (1, 2, 3))
 # Tet10 faces as corners + edge midpoints (tet10.py:287-292)
 T10_FACES = (((0, 1, 2), (4, 5, 6)), ((0, 3, 1), (7, 8, 4)),
@@ -47,6 +58,9 @@ def _exterior_faces(mesh, local_faces, width):
 def boundary_faces(mesh, name):
     """(F, 3) P1 faces of boundary `name` (KeyError if unknown)."""
     ids = np.asarray(mesh.boundaries[name], dtype=int)
+    if _use_device(mesh):
+        from .device_ops import boundary_faces_device
+        return boundary_faces_device(mesh, name)
     tris = _exterior_faces(mesh, P1_FACES, 3)
     member = np.zeros(mesh.n_nodes, dtype=bool)
     member[ids] = True
@@ -56,6 +70,9 @@ def boundary_faces(mesh, name):
 def boundary_faces10(mesh, name):
     """(F, 6) Tet10 faces [c0, c1, c2, m01, m12, m20] of boundary `name`."""
     ids = np.asarray(mesh.boundaries[name], dtype=int)
+    if _use_device(mesh):
+        from .device_ops import boundary_faces10_device
+        return boundary_faces10_device(mesh, name)
     full = _exterior_faces(mesh, [c + m for c, m in T10_FACES], 6)
     member = np.zeros(mesh.n_nodes, dtype=bool)
     member[ids] = True

@@ -11,6 +11,19 @@ import numpy as np
 
 from .tet_mesh import TetMesh, box_mesh
 
+# Meshes from this many elements up are enriched on the GPU when one is
+# visible (cf_tet10_edges / cf_edge_midpoints: radix sort of the edge keys on
+# the device); smaller ones on the host, where the transfers would dominate.
+# Both give the same numbering, coordinates and boundary sets.
+DEVICE_MIN_ELEMENTS = 50_000
+
+
+def _use_device(n_elements):
+    if n_elements < DEVICE_MIN_ELEMENTS:
+        return False
+    import torch
+    return torch.cuda.is_available()
+
 # Local edges of midpoint slots 4..9 (tet10_mesh.py:33)
 EDGES = ((0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3))
 
@@ -36,7 +49,7 @@ class Tet10Mesh:
         return self.elements.shape[0]
 
     def element_volumes(self):
-        """Corner-Jacobian volumes (tet10_mesh.py:168-176)."""
+        """Corner-Jacobian volumes abs(det) / 6 (tet10_mesh.py:66-74)."""
         if self._volumes is None:
             p = self.nodes[self.elements[:, :4]]
             jac = np.stack([p[:, 1] - p[:, 0], p[:, 2] - p[:, 0], p[:, 3] - p[:, 0]], axis=2)
@@ -56,6 +69,9 @@ def _unique_edges(els, n0):
 def tet4_to_tet10(mesh):
     """Enrich a P1 mesh with edge-midpoint nodes (tet10_mesh.py:77-112)."""
     els = np.asarray(mesh.elements, dtype=int)
+    if _use_device(els.shape[0]):
+        from .device_ops import tet4_to_tet10_device
+        return tet4_to_tet10_device(mesh)
     n0 = mesh.n_nodes
     uniq, inv = _unique_edges(els, max(n0, 1))
     mids = 0.5 * (mesh.nodes[uniq[:, 0]] + mesh.nodes[uniq[:, 1]])

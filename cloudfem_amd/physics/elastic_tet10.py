@@ -86,7 +86,9 @@ def assemble_elasticity_tet10(mesh, E, nu, body=None, traction=None, T=None, alp
 def solve_device(dm, vals, rhs, mesh, bc, lam=None, mu=None, **pcg_kw):
     """Solve on the device; returns (u (3N,) tensor, stats).
 
-    precond "twolevel" selects the two-level solver (rigid-body coarse
+    The default (precision and precond "auto") is the two-level solver with
+    FP32 inside and the assembled matrix outside.  precond "twolevel"
+    selects the two-level solver (rigid-body coarse
     correction around a matrix-free Chebyshev polynomial, FP32 inside with
     precision "mixed", FP64 with "fp64"), whose outer products use the
     assembled NB3 matrix `vals` or, with solver option operator="matfree",
@@ -105,6 +107,14 @@ def solve_device(dm, vals, rhs, mesh, bc, lam=None, mu=None, **pcg_kw):
     o = solver_mod.options()
     prec = pcg_kw.get("precision", o["precision"])
     precond = pcg_kw.get("precond", o["precond"])
+    if lam is not None:
+        # Tet10 elasticity: the two-level solver by default, FP32 inside (it
+        # solves the assembled matrix to the same FP64 residual bar)
+        prec = "mixed" if prec == "auto" else prec
+        precond = "twolevel" if precond == "auto" else precond
+    prec = solver_mod.resolve("precision", prec)
+    precond = solver_mod.resolve("precond", precond)
+    pcg_kw = dict(pcg_kw, precision=prec, precond=precond)
     kw = {k: v for k, v in pcg_kw.items() if k in ("tol", "maxit")}
     matfree = o["operator"] == "matfree" and lam is not None
     if precond == "twolevel" and lam is not None:

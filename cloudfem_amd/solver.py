@@ -7,14 +7,17 @@ configuration), and the defaults are chosen so that solutions agree with a
 direct solve to well within the 1e-10 relative bar on the validation
 problems:
 
-  precision  "fp64" (default), "mixed" or "mixed16".  For 3x3-block
+  precision  "auto" (default), "fp64", "mixed" or "mixed16".  "auto" is
+             "mixed" for Tet10 elasticity (with the two-level solver) and
+             "fp64" for every other system.  For 3x3-block
              (vector) systems: FP64 flexible CG preconditioned by an FP32
              Chebyshev polynomial of the Jacobi-scaled matrix, whose copy is
              stored in FP32 ("mixed") or FP16 ("mixed16"); for scalar systems
              both mean FP32 inner CG with FP64 residual replacement.  The stop
              test is the FP64 residual in every case, so the error against
              FP64 is bounded by the same tolerance (cf_pcg_solve)
-  precond    "jacobi" (default) or "block3" (3x3 node blocks, vector
+  precond    "auto" (default: "twolevel" for Tet10 elasticity, "jacobi"
+             otherwise), "jacobi" or "block3" (3x3 node blocks, vector
              problems, FP64 only); "twolevel" (Tet10 elasticity): a matrix-free
              Chebyshev polynomial, FP32 with precision "mixed" or FP64 with
              "fp64", wrapped in a coarse correction on the rigid-body modes of
@@ -39,7 +42,7 @@ import torch
 from . import _native as N
 from .device import ptr, stream_handle, workspace
 
-_DEFAULTS = {"precision": "fp64", "precond": "jacobi", "tol": 1e-12, "maxit": 200000,
+_DEFAULTS = {"precision": "auto", "precond": "auto", "tol": 1e-12, "maxit": 200000,
              "operator": "assembled"}
 _local = threading.local()
 
@@ -65,6 +68,15 @@ def solver_options(**kw):
 
 
 last_stats = threading.local()
+
+# What "auto" means for the general solvers; Tet10 elasticity resolves it to
+# the two-level mixed solver instead (physics.elastic_tet10.solve_device).
+_AUTO = {"precision": "fp64", "precond": "jacobi"}
+
+
+def resolve(name, value):
+    """An option value with "auto" replaced by the general default."""
+    return _AUTO[name] if value == "auto" else value
 
 
 def _base_stats(st):
@@ -130,8 +142,8 @@ def pcg(A, b, x, fixed=None, precision=None, precond=None, tol=None, maxit=None)
     Returns a dict of statistics; raises SolverNotConverged / SolverBreakdown.
     """
     o = options()
-    precision = precision or o["precision"]
-    precond = precond or o["precond"]
+    precision = resolve("precision", precision or o["precision"])
+    precond = resolve("precond", precond or o["precond"])
     tol = float(tol if tol is not None else o["tol"])
     maxit = int(maxit if maxit is not None else o["maxit"])
     prec = {"fp64": N.PREC_FP64, "mixed": N.PREC_MIXED, "mixed16": N.PREC_MIXED16}[precision]
@@ -248,7 +260,7 @@ def pcg_matfree_t10(dm, lam, mu, b, x, fixed=None, precond=None, tol=None, maxit
     gathered per node); chunked=True: the Morton-chunked operator
     (cf_pcg_solve_mf_t10_plan)."""
     o = options()
-    precond = precond or o["precond"]
+    precond = resolve("precond", precond or o["precond"])
     tol = float(tol if tol is not None else o["tol"])
     maxit = int(maxit if maxit is not None else o["maxit"])
     pc = _pc_code(precond)

@@ -53,7 +53,7 @@ extern "C" {
 
 /* ---- matrix operators (cf_assemble_matrix) ------------------------------ */
 #define CF_OP_P1_STIFF 1   /* k_e V G G^T              tet_poisson.py:51-66   */
-#define CF_OP_P1_MASS 2    /* rho_e V (1+d_ab)/20      tet_heat.py:194-216    */
+#define CF_OP_P1_MASS 2    /* rho_e V (1+d_ab)/20      tet_heat.py:72-94    */
 #define CF_OP_T10_STIFF 3  /* V sum_q w_q k_q G G^T    tet10.py:138-165       */
 #define CF_OP_T10_MASS 4   /* rho_e V MASS_REF         tet10.py:185-249       */
 #define CF_OP_T10_ELAST 5  /* V sum_q w_q B^T D B      elastic_tet10.py:164-173 */
@@ -69,8 +69,8 @@ extern "C" {
 #define CF_RHS_T10_THERMAL 6  /* V sum_q w_q (3l+2m) s_q g_aq  elastic_tet10.py:181-190 */
 
 /* ---- face operators (cf_face_rhs, cf_face_matrix) ------------------------ */
-#define CF_FACE_P1_FLUX 1       /* q_f A/3                tet_heat.py:271-276    */
-#define CF_FACE_P1_ROBIN_K 2    /* h_f A (1+d)/12         tet_heat.py:278-290    */
+#define CF_FACE_P1_FLUX 1       /* q_f A/3                tet_heat.py:149-154    */
+#define CF_FACE_P1_ROBIN_K 2    /* h_f A (1+d)/12         tet_heat.py:156-168    */
 #define CF_FACE_T10_FLUX 3      /* A sum_q w6 q_q N6      tet10.py:393-401       */
 #define CF_FACE_T10_ROBIN_K 4   /* A sum_q w6 h_q N6 N6^T tet10.py:413-427       */
 #define CF_FACE_P1_TRACTION 5   /* t_f,c A/3              elastic3d.py:171-186   */
@@ -120,7 +120,7 @@ int cf_version(void);
 const char* cf_last_error(void);
 
 /* ---- node-pair pattern (replaces the pattern half of
- *      scipy coo_matrix(...).tocsr() at tet_poisson.py:68, tet_heat.py:216,
+ *      scipy coo_matrix(...).tocsr() at tet_poisson.py:68, tet_heat.py:94,
  *      tet10.py:162-164 / 247-249, elastic_tet10.py:208-210,
  *      elastic3d.py:188, modal3d.py:75) ------------------------------------
  * Two phases around one host read of nnz:
@@ -143,7 +143,7 @@ int cf_pattern_finalize(int64_t n_elem, int32_t k, int64_t n_nodes,
 /* ---- node incidence: for every node the (element, local slot) items that
  *      touch it, ascending; used for deterministic RHS gathers
  *      (replaces rhs[el] += ... at tet_poisson.py:62, tet10.py:161,
- *      elastic_tet10.py:179/190/206, tet_heat.py:276/291) ---------------- */
+ *      elastic_tet10.py:179/190/206, tet_heat.py:154/291) ---------------- */
 int cf_incidence_workspace_bytes(int64_t n_items, int64_t n_nodes,
                                  size_t* bytes);
 int cf_incidence_build(const int32_t* targets, int64_t n_items,
@@ -209,7 +209,7 @@ int cf_face_matrix(int32_t op, const double* x, const double* y,
 /* ---- sparse add of a sub-pattern into a super-pattern (scalar CSR):
  *      for every stored (i, j) of B, A[i, j] += alpha * B[i, j].  Every
  *      stored entry of B must be stored in A (the face pattern is a subset
- *      of the element pattern).  Replaces K + K_add (tet_heat.py:352).      */
+ *      of the element pattern).  Replaces K + K_add (tet_heat.py:230).      */
 int cf_csr_add_subpattern(int64_t n_rows, const int32_t* a_ptr,
                           const int32_t* a_idx, double* a_val,
                           const int32_t* b_ptr, const int32_t* b_idx,
@@ -265,11 +265,11 @@ int cf_dirichlet_fill(int64_t n, const int32_t* ptr, const int32_t* idx,
                       double* val_out, void* stream);
 
 /* ---- y = a x + b y (theta-method operators C + th h K, per-step loads;
- *      replaces the scipy sums at tet_heat.py:361-362, 386).              */
+ *      replaces the scipy sums at tet_heat.py:239-240, 386).              */
 int cf_axpby(int64_t n, double a, const double* x, double b, double* y,
              void* stream);
 /* ---- dst[i] = src[i] where mask[i] (re-imposing Dirichlet values,
- *      tet_heat.py:388-392).                                              */
+ *      tet_heat.py:266-270).                                              */
 int cf_copy_masked(int64_t n, const uint8_t* mask, const double* src,
                    double* dst, void* stream);
 
@@ -369,7 +369,7 @@ int cf_pcg_solve_mf_t10_plan(int64_t n_nodes, int64_t n_elem, const int32_t* con
  *  (device).  cf_edge_midpoints: x/y/z[n_corners + i] = endpoint mean.
  *  cf_boundary_faces: exterior faces (sorted corner triple in exactly one
  *  element) whose corners are all in `member` (nullable: all), in first-
- *  occurrence order with that occurrence's node order (tet_heat.py:219-242,
+ *  occurrence order with that occurrence's node order (tet_heat.py:97-120,
  *  tet10.py:329-352, elastic_tet10.py:68-91); width 3 (P1 corner faces) or 6
  *  (Tet10 faces, corners + midpoints); faces capacity 4 M x width; *count
  *  (device).  n_nodes < 2^21.  Workspace: cf_mesh_workspace_bytes.       */
@@ -495,7 +495,7 @@ int cf_spmv_profile(int32_t enable, double* avg_ms_fp64, int64_t* n_fp64,
                     double* avg_ms_fp32, int64_t* n_fp32);
 
 /* ---- batched theta-method heat (many meshes, one block-diagonal CSR).
- *      Replaces the per-mesh loop of transient_heat_tet (tet_heat.py:319-396:
+ *      Replaces the per-mesh loop of transient_heat_tet (tet_heat.py:197-274:
  *      (C + th h K) T+ = (C - (1-th) h K) T + h F, factorized once) for a
  *      batch of meshes without Dirichlet nodes: every launch covers all
  *      systems, CG scalars and stop tests are per system (tol relative to

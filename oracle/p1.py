@@ -1,7 +1,7 @@
 """Oracle for P1 scalar assembly, Dirichlet, steady and transient solves.
 
 Restates cloudfem/fem/tet_poisson.py:17-122 and
-cloudfem/fem/tet_heat.py:194-396 (CPU, test-only).
+cloudfem/fem/tet_heat.py:72-274 (CPU, test-only).
 """
 
 import numpy as np
@@ -15,7 +15,7 @@ _GREF = np.array([[-1.0, -1.0, -1.0],
 
 
 def as_fn(v):
-    """float-or-callable(x, y, z) -> callable (tet_heat.py:253-255)."""
+    """float-or-callable(x, y, z) -> callable (tet_heat.py:131-133)."""
     return v if callable(v) else (lambda x, y, z, _v=v: _v)
 
 
@@ -53,7 +53,7 @@ def assemble(mesh, k=1.0, f=0.0):
 
 
 def dirichlet_values(mesh, bc):
-    """{node: value} from a bc dict (tet_heat.py:297-312)."""
+    """{node: value} from a bc dict (tet_heat.py:175-190)."""
     fixed = {}
     if not bc:
         return fixed
@@ -106,7 +106,7 @@ def solve_tet_poisson(mesh, k=1.0, f=0.0, bc=None):
 
 
 def assemble_mass(mesh, rho_cp=1.0):
-    """C = sum_e rho_cp(x_c) V (1 + d_ab)/20 (tet_heat.py:194-216)."""
+    """C = sum_e rho_cp(x_c) V (1 + d_ab)/20 (tet_heat.py:72-94)."""
     n = mesh.n_nodes
     rf = as_fn(rho_cp)
     base = (np.ones((4, 4)) + np.eye(4)) / 20.0
@@ -122,7 +122,7 @@ def assemble_mass(mesh, rho_cp=1.0):
 
 
 def boundary_faces(mesh, name):
-    """Exterior faces of a named set, first-seen order (tet_heat.py:219-242).
+    """Exterior faces of a named set, first-seen order (tet_heat.py:97-120).
 
     Faces (a,b,c), (a,b,d), (a,c,d), (b,c,d) of each element; a face is
     exterior iff its sorted triple occurs once; kept iff all three nodes
@@ -149,7 +149,7 @@ def _face_geom(mesh, tri):
 
 
 def assemble_boundary_terms(mesh, neumann=None, robin=None):
-    """Neumann q A/3 and Robin h A (1+d)/12, h T_inf A/3 (tet_heat.py:258-294)."""
+    """Neumann q A/3 and Robin h A (1+d)/12, h T_inf A/3 (tet_heat.py:136-172)."""
     n = mesh.n_nodes
     F = np.zeros(n)
     ri, ci, vals = [], [], []
@@ -178,7 +178,7 @@ THETA = {"be": 1.0, "implicit_euler": 1.0, "cn": 0.5, "crank_nicolson": 0.5}
 
 def transient_heat(mesh, k=1.0, rho_cp=1.0, f=0.0, bc=None, neumann=None,
                    robin=None, t_end=1.0, dt=0.1, T0=0.0, method="cn"):
-    """Theta method with one LU (tet_heat.py:319-396).
+    """Theta method with one LU (tet_heat.py:197-274).
 
     (C + th h K) T+ = (C - (1-th) h K) T + h F, h = t_end/ceil(t_end/dt),
     Dirichlet by the apply_dirichlet elimination of the system matrix and

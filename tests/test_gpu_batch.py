@@ -1,6 +1,6 @@
 """Batched theta-method heat (many jittered P1 box meshes, per-element k,
 Robin on all faces) against the oracle's per-mesh LU recursion
-(tet_heat.py:319-396) and against the single-mesh GPU path."""
+(tet_heat.py:197-274) and against the single-mesh GPU path."""
 
 import numpy as np
 import pytest

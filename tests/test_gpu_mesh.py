@@ -1,7 +1,9 @@
 """Mesh processing on the GPU against the host restatements (which the CPU
-suite pins to brute force and the reference's rules): Tet4 -> Tet10 edge
-numbering, midpoint coordinates, boundary sets and exterior face lists must
-be identical (bit-exact integers, exact midpoints)."""
+suite pins to brute force and the reference's rules) and, on a mesh large
+enough that the drop-ins route it to the GPU, against the oracle directly:
+Tet4 -> Tet10 edge numbering, midpoint coordinates, boundary sets and
+exterior face lists must be identical (bit-exact integers, exact
+midpoints)."""
 
 import numpy as np
 import pytest
@@ -64,3 +66,38 @@ def test_sphere_fixture(gpu):
     assert np.array_equal(d.elements, h.elements) and np.array_equal(d.nodes, h.nodes)
     assert np.array_equal(d.boundaries["boundary"], h.boundaries["boundary"])
     assert np.array_equal(boundary_faces10_device(h, "boundary"), boundary_faces10(h, "boundary"))
+
+
+def test_large_box_drop_ins_run_on_gpu_and_match_oracle(gpu, monkeypatch):
+    """box_mesh_tet10 / tet4_to_tet10 / boundary_faces10 of a 96,000-element
+    box take the device path (tet10_mesh.py:77-135, tet10.py:329-352 and
+    elastic_tet10.py:68-91 restated on the GPU) and equal the oracle."""
+    from cloudfem_amd.mesh import box_mesh_tet10, device_ops
+    from cloudfem_amd.mesh import faces as faces_mod
+    from cloudfem_amd.mesh import tet10_mesh
+    from oracle import meshes as om
+    from oracle import tet10 as ot10
+    calls = {"edges": 0, "faces": 0}
+    real_edges, real_faces = device_ops.tet4_to_tet10_device, device_ops.boundary_faces10_device
+
+    def edges(m):
+        calls["edges"] += 1
+        return real_edges(m)
+
+    def faces(m, name):
+        calls["faces"] += 1
+        return real_faces(m, name)
+    monkeypatch.setattr(device_ops, "tet4_to_tet10_device", edges)
+    monkeypatch.setattr(device_ops, "boundary_faces10_device", faces)
+    lims = dict(xlim=(0.0, 0.4), ylim=(0.0, 0.1), zlim=(0.0, 0.1))
+    m = box_mesh_tet10(40, 20, 20, **lims)
+    assert m.n_elements >= tet10_mesh.DEVICE_MIN_ELEMENTS and calls["edges"] == 1
+    m0 = om.box_mesh_tet10(40, 20, 20, **lims)
+    assert np.array_equal(m.elements, m0.elements)
+    assert np.array_equal(m.nodes, m0.nodes)
+    assert m.n_corners == m0.n_corners
+    for name in FACES:
+        assert np.array_equal(m.boundaries[name], m0.boundaries[name])
+        f = faces_mod.boundary_faces10(m, name)
+        assert np.array_equal(f, ot10.boundary_faces10(m0, name))
+    assert calls["faces"] == len(FACES)

@@ -413,6 +413,25 @@ def test_tet10_constant_strain_patch(gpu):
     assert np.abs(sig - sig_ex).max() / np.abs(sig_ex).max() < 1e-9
 
 
+def test_default_tet10_elasticity_solver_is_two_level_mixed(gpu):
+    """Without solver options, solve_elasticity_tet10 uses the two-level
+    solver with FP32 inner products on the assembled matrix, and matches
+    SuperLU (the drop-in's default path)."""
+    from cloudfem_amd import solver
+    from cloudfem_amd.mesh import box_mesh_tet10
+    from cloudfem_amd.physics import elastic_tet10
+    lims = dict(xlim=(0.0, 0.4), ylim=(0.0, 0.05), zlim=(0.0, 0.05))
+    trac, bc = {"xmax": (0.0, 0.0, -1e6)}, {"xmin": (0.0, 0.0, 0.0)}
+    m = box_mesh_tet10(24, 3, 3, **lims)
+    u = np.stack(elastic_tet10.solve_elasticity_tet10(m, 210e9, 0.3, bc=bc, traction=trac),
+                 axis=1).ravel()
+    st = solver.last_stats.value
+    assert (st["precond"], st["precision"], st["operator"]) == ("two-level", "mixed", "assembled")
+    m0 = om.box_mesh_tet10(24, 3, 3, **lims)
+    A0, rhs0 = oel.assemble_elasticity_tet10(m0, 210e9, 0.3, traction=trac)
+    assert rel_err(u, oel.apply_dirichlet_vec3(A0, rhs0, m0, bc)) < 1e-10
+
+
 def test_solver_failures_raise(gpu):
     from cloudfem_amd import solver_options
     from cloudfem_amd._native import SolverBreakdown, SolverNotConverged
@@ -458,9 +477,10 @@ def test_matfree_operator_matches_assembled(gpu):
     plan, counts = dm.matfree_plan()
     assert counts[0] > 0 and 0 < counts[1] <= counts[0] and 0 < counts[2] <= m.n_nodes
     bc, trac = {"xmin": (0.0, 0.0, 0.0)}, {"xmax": (0.0, 0.0, -1e6)}
-    u_a = np.stack(elastic_tet10.solve_elasticity_tet10(m, 210e9, 0.3, bc=bc, traction=trac))
+    with solver_options(precision="fp64", precond="jacobi"):
+        u_a = np.stack(elastic_tet10.solve_elasticity_tet10(m, 210e9, 0.3, bc=bc, traction=trac))
     for pc in ("jacobi", "block3"):
-        with solver_options(operator="matfree", precond=pc):
+        with solver_options(operator="matfree", precision="fp64", precond=pc):
             u_m = np.stack(elastic_tet10.solve_elasticity_tet10(m, 210e9, 0.3, bc=bc,
                                                                 traction=trac))
         assert rel_err(u_m.ravel(), u_a.ravel()) < 1e-10
