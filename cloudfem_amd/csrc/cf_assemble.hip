@@ -105,11 +105,18 @@ __device__ __forceinline__ void t10_grad(const Geo& g, int q, int a, double out[
 // One thread per stored entry (node pair) s: its contributors (e, a, b) are
 // contrib[seg[s] .. seg[s+1]) in ascending element order, and the entry is
 // their sum in that order (scipy's coo->csr sums the reference's triplets in
-// element order too).  Vector operators store the 3x3 block at its place in
-// the DOF-level CSR value array; the row node is conn[e][a] of the first
-// contributor, so no search over ptr is needed.  The Tet10 elasticity
-// reference tensor (900 doubles) is staged in LDS, where the per-lane
-// (a, b) reads cost far less than divergent loads from constant memory.
+// element order too).  The Tet10 elasticity reference tensor (900 doubles) is
+// staged in LDS, where the per-lane (a, b) reads cost far less than divergent
+// loads from constant memory.
+//
+// Vector operators store a 3x3 block per entry at its place in the DOF-level
+// CSR value array: row node i = conn[e][a] of the first contributor (no
+// search over ptr), DOF row 3i+c holding block(i, j)[c][0..2] at 3 (s - ptr[i]).
+// A block of kBlock entries first stages its blocks in LDS and then writes
+// them row segment by row segment, three doubles per entry per DOF row, with
+// consecutive lanes on consecutive addresses (the direct per-thread stores
+// touched each 128-byte line in three 8-byte pieces, and measured 0.92 ms at
+// 15.0 M entries for 1.97 GB of HBM traffic).
 template <int OP>
 __global__ void __launch_bounds__(kBlock)
 assemble_values(const int32_t* __restrict__ ptr, const int32_t* __restrict__ seg,
@@ -118,24 +125,28 @@ assemble_values(const int32_t* __restrict__ ptr, const int32_t* __restrict__ seg
                 const double* __restrict__ coef, double lam, double mu,
                 double* __restrict__ out) {
   constexpr bool kRT = (OP == CF_OP_T10_ELAST);
+  constexpr bool kVec9 = (OP == CF_OP_T10_ELAST || OP == CF_OP_P1_ELAST);
   __shared__ double Rs[kRT ? 900 : 1];
+  __shared__ double blk[kVec9 ? kBlock * 9 : 1];
+  __shared__ int64_t rbase[kVec9 ? kBlock : 1];
+  __shared__ int32_t rlen[kVec9 ? kBlock : 1];
   if constexpr (kRT) {
     const double* R = &kT10.R[0][0][0][0];
     for (int t = threadIdx.x; t < 900; t += blockDim.x) Rs[t] = R[t];
     __syncthreads();
   }
   const int kk = k * k;
-  for (int64_t s = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; s < nnz;
-       s += int64_t(gridDim.x) * blockDim.x) {
+  for (int64_t s0 = int64_t(blockIdx.x) * kBlock; s0 < nnz; s0 += int64_t(gridDim.x) * kBlock) {
+    const int64_t s = s0 + threadIdx.x;
     double acc[3][3] = {};
-    const int32_t j0 = seg[s], j1 = seg[s + 1];
     int64_t row = 0;
+    const int32_t j0 = s < nnz ? seg[s] : 0, j1 = s < nnz ? seg[s + 1] : 0;
     for (int32_t j = j0; j < j1; ++j) {
       const uint32_t pid = contrib[j];
       const int64_t e = pid / uint32_t(kk);
       const int ab = int(pid - uint32_t(e) * uint32_t(kk));
       const int a = ab / k, b = ab - (ab / k) * k;
-      if constexpr (OP == CF_OP_T10_ELAST || OP == CF_OP_P1_ELAST || OP == CF_OP_P1_VMASS)
+      if constexpr (kVec9 || OP == CF_OP_P1_VMASS)
         if (j == j0) row = conn[e * k + a];
       if constexpr (OP == CF_OP_P1_MASS || OP == CF_OP_P1_VMASS) {
         // rho V (1 + d_ab)/20  (tet_heat.py:72-94, modal3d.py:64-66)
@@ -161,7 +172,7 @@ assemble_values(const int32_t* __restrict__ ptr, const int32_t* __restrict__ seg
           sum += (kT10.w[q] * coef[e * 5 + q]) * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
         }
         acc[0][0] += g.v * sum;
-      } else if constexpr (OP == CF_OP_T10_ELAST || OP == CF_OP_P1_ELAST) {
+      } else if constexpr (kVec9) {
         Geo g = load_geo(geo, e);
         double M[3][3];
         if constexpr (OP == CF_OP_T10_ELAST) {
@@ -196,23 +207,37 @@ assemble_values(const int32_t* __restrict__ ptr, const int32_t* __restrict__ seg
             acc[i][jj] += lv * M[i][jj] + mv * M[jj][i] + (i == jj ? tr : 0.0);
       }
     }
-    if constexpr (OP == CF_OP_T10_ELAST || OP == CF_OP_P1_ELAST) {
-      const int64_t base = ptr[row], len = ptr[row + 1] - base, kpos = s - base;
-      double* o = out + 9 * base + 3 * kpos;
+    if constexpr (kVec9) {
+      // stage the block, then write the chunk row segment by row segment
+      if (s < nnz) {
+        const int32_t base = ptr[row];
+        rbase[threadIdx.x] = base;
+        rlen[threadIdx.x] = ptr[row + 1] - base;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        o[c * 3 * len + 0] = acc[c][0];
-        o[c * 3 * len + 1] = acc[c][1];
-        o[c * 3 * len + 2] = acc[c][2];
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) blk[threadIdx.x * 9 + c * 3 + d] = acc[c][d];
       }
+      __syncthreads();
+      const int64_t m = nnz - s0 < kBlock ? nnz - s0 : kBlock;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        for (int t = threadIdx.x; t < 3 * m; t += kBlock) {
+          const int q = t / 3, d = t - 3 * (t / 3);
+          const int64_t base = rbase[q];
+          out[9 * base + int64_t(c) * 3 * rlen[q] + 3 * (s0 + q - base) + d] = blk[q * 9 + c * 3 + d];
+        }
+      __syncthreads();
     } else if constexpr (OP == CF_OP_P1_VMASS) {
-      const int64_t base = ptr[row], len = ptr[row + 1] - base, kpos = s - base;
-      double* o = out + 3 * base + kpos;
-      o[0] = acc[0][0];
-      o[len] = acc[0][0];
-      o[2 * len] = acc[0][0];
+      if (s < nnz) {
+        const int64_t base = ptr[row], len = ptr[row + 1] - base, kpos = s - base;
+        double* o = out + 3 * base + kpos;
+        o[0] = acc[0][0];
+        o[len] = acc[0][0];
+        o[2 * len] = acc[0][0];
+      }
     } else {
-      out[s] = acc[0][0];
+      if (s < nnz) out[s] = acc[0][0];
     }
   }
 }
