@@ -2371,17 +2371,52 @@ int bits_for(int64_t n) {
 // Upper bound on the number of steps of the sliced copy: the slice steps of
 // the unsorted rows (sorting inside windows only lowers each slice maximum's
 // rank, so the sorted total never exceeds this).
-int sell_bound(const cf_matrix* A, int64_t* lsum) {
-  std::vector<int32_t> ptr(size_t(A->n_rows) + 1);
-  CF_HIP(hipMemcpy(ptr.data(), A->indptr, ptr.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
-  int64_t total = 0;
-  for (int64_t s0 = 0; s0 < A->n_rows; s0 += kSliceRows) {
+// Steps of the sliced layout before the window sort: per slice of
+// kSliceRows rows, ceil(longest row / kPhases), summed over the slices with
+// integer atomics (exact in any order).  One wave-reduced add per wave.
+__device__ unsigned long long g_sell_steps;
+
+__global__ void __launch_bounds__(kBlock)
+sell_steps_count(int64_t n_rows, const int32_t* __restrict__ ptr,
+                 unsigned long long* __restrict__ out) {
+  const int64_t n_slices = (n_rows + kSliceRows - 1) / kSliceRows;
+  unsigned long long sum = 0;
+  for (int64_t sl = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; sl < n_slices;
+       sl += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t r1 = (sl + 1) * kSliceRows < n_rows ? (sl + 1) * kSliceRows : n_rows;
     int32_t mx = 0;
-    for (int64_t r = s0; r < s0 + kSliceRows && r < A->n_rows; ++r)
-      mx = std::max(mx, ptr[r + 1] - ptr[r]);
-    total += (mx + kPhases - 1) / kPhases;
+    for (int64_t r = sl * kSliceRows; r < r1; ++r) {
+      const int32_t len = ptr[r + 1] - ptr[r];
+      mx = len > mx ? len : mx;
+    }
+    sum += uint64_t((mx + kPhases - 1) / kPhases);
   }
-  *lsum = total;
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) sum += __shfl_xor(sum, m, 64);
+  if ((threadIdx.x & 63) == 0 && sum) atomicAdd(out, sum);
+}
+
+__global__ void zero_u64(unsigned long long* p) { *p = 0; }
+
+// Workspace bound of the sliced copy (total steps of the unsorted slices;
+// the window sort never makes them longer), counted on the device so that
+// only 8 bytes cross to the host.  Calls serialise on a process-wide lock
+// around the per-device counter.  s: the caller's stream (0: null stream).
+int sell_bound(const cf_matrix* A, int64_t* lsum, hipStream_t s = nullptr) {
+  static std::mutex lock;
+  std::lock_guard<std::mutex> guard(lock);
+  unsigned long long* d = nullptr;
+  CF_HIP(hipGetSymbolAddress(reinterpret_cast<void**>(&d), HIP_SYMBOL(g_sell_steps)));
+  zero_u64<<<1, 1, 0, s>>>(d);
+  CF_LAUNCHED();
+  const int64_t n_slices = (int64_t(A->n_rows) + kSliceRows - 1) / kSliceRows;
+  sell_steps_count<<<cf::stride_grid(n_slices, kBlock), kBlock, 0, s>>>(
+      A->n_rows, static_cast<const int32_t*>(A->indptr), d);
+  CF_LAUNCHED();
+  unsigned long long total = 0;
+  CF_HIP(hipMemcpyAsync(&total, d, sizeof(total), hipMemcpyDeviceToHost, s));
+  CF_HIP(hipStreamSynchronize(s));
+  *lsum = int64_t(total);
   return CF_OK;
 }
 
@@ -2452,6 +2487,19 @@ Layout layout(const cf_matrix* A, int precision, int precond, void* base, int64_
 
 // Slice structure of the sliced copy: window sort, slice steps, offsets, and
 // the per-die slice ranges; *product_grid = blocks of the sliced products.
+// Grid of the sliced product: kXcd x the blocks of the longest per-die slice
+// range (reads the die split back, so it synchronises with stream s).
+int sell_product_grid(const Layout& L, hipStream_t s, int* product_grid) {
+  int32_t xs[kXcd + 1];
+  CF_HIP(hipMemcpyAsync(xs, L.xsplit, sizeof(xs), hipMemcpyDeviceToHost, s));
+  CF_HIP(hipStreamSynchronize(s));
+  int64_t most = 0;
+  for (int d = 0; d < kXcd; ++d) most = std::max<int64_t>(most, xs[d + 1] - xs[d]);
+  *product_grid = int(kXcd * sell_grid(most));
+  return CF_OK;
+}
+
+// product_grid null: the caller reads the grid later (sell_product_grid).
 int build_sell(const cf_matrix* A, const Layout& L, hipStream_t s, int* product_grid) {
   const int64_t N = A->n_rows;
   const int grid = cf::stride_grid(N, kBlock);
@@ -2470,13 +2518,7 @@ int build_sell(const cf_matrix* A, const Layout& L, hipStream_t s, int* product_
                                  rocprim::plus<int32_t>(), s));
   sell_xcd_split<<<1, 64, 0, s>>>(L.n_slices, L.koff, L.xsplit);
   CF_LAUNCHED();
-  int32_t xs[kXcd + 1];
-  CF_HIP(hipMemcpyAsync(xs, L.xsplit, sizeof(xs), hipMemcpyDeviceToHost, s));
-  CF_HIP(hipStreamSynchronize(s));
-  int64_t most = 0;
-  for (int d = 0; d < kXcd; ++d) most = std::max<int64_t>(most, xs[d + 1] - xs[d]);
-  *product_grid = int(kXcd * sell_grid(most));
-  return CF_OK;
+  return product_grid ? sell_product_grid(L, s, product_grid) : CF_OK;
 }
 
 // Values of the sliced copy: A (scale null) or S A S, stored as OT.
@@ -3295,14 +3337,14 @@ extern "C" int cf_pcg_solve(const cf_matrix* A, const double* b, double* x,
   CF_REQUIRE(!(precond == CF_PC_BLOCK3 && precision != CF_PREC_FP64),
              "cf_pcg_solve: mixed precision uses Jacobi scaling");
   CF_REQUIRE(tol > 0.0 && maxit > 0, "cf_pcg_solve: tol and maxit must be positive");
+  hipStream_t s = cf::as_stream(stream);
   int64_t lsum = 0;
-  if (A->format == CF_FMT_NB3 && (rc = sell_bound(A, &lsum))) return rc;
+  if (A->format == CF_FMT_NB3 && (rc = sell_bound(A, &lsum, s))) return rc;
   Layout L = layout(A, precision, precond, ws, lsum);
   if (ws_bytes < L.total) {
     cf::set_error("cf_pcg_solve: workspace %zu < %zu bytes", ws_bytes, L.total);
     return CF_ERR_WORKSPACE;
   }
-  hipStream_t s = cf::as_stream(stream);
   const int G = pcg_grid();
   *stats_host = cf_pcg_stats{};
   CF_HIP(hipMemsetAsync(L.state, 0, sizeof(PcgState), s));
@@ -5118,7 +5160,7 @@ int mf_mixed_solve(const char* who, const cf_matrix* A, int64_t n_nodes, int64_t
     CF_REQUIRE(A->format == CF_FMT_NB3 && A->n_rows == n_nodes,
                "%s: the assembled operator must be the NB3 matrix of the mesh (%d node rows, "
                "%lld nodes)", who, int(A->n_rows), (long long)n_nodes);
-    if ((rc = sell_bound(A, &lsum))) return rc;
+    if ((rc = sell_bound(A, &lsum, cf::as_stream(stream)))) return rc;
   }
   Layout L{};
   MfMixed M{};
@@ -5159,8 +5201,11 @@ int mf_mixed_solve(const char* who, const cf_matrix* A, int64_t n_nodes, int64_t
   mf_pack_tiles<T><<<cf::stride_grid(10 * n_elem, kBlock), kBlock, 0, s>>>(
       n_elem, conn, geo, M.R.rank, M.connT, geoL, M.rankT);
   CF_LAUNCHED();
+  // the sliced product's grid is read back when the outer products switch to
+  // A (a synchronisation point anyway), not here
   int GS = 0;
-  if (A && ((rc = build_sell(A, L, s, &GS)) || (rc = fill_sell<double>(A, L, nullptr, L.sval64, s))))
+  if (A && ((rc = build_sell(A, L, s, nullptr)) ||
+            (rc = fill_sell<double>(A, L, nullptr, L.sval64, s))))
     return rc;
   const int ge = cf::stride_grid(n_elem, kBlock);
   const int gg = int(mf_gather_grid(n_nodes));
@@ -5181,8 +5226,10 @@ int mf_mixed_solve(const char* who, const cf_matrix* A, int64_t n_nodes, int64_t
     }
     return mf_apply(a, M.R, xin, y, fixed, M.fe, state, state ? L.partials : nullptr, s);
   };
+  int switch_rc = CF_OK;
   auto switch_outer = [&]() {
     if (!A || use_A) return false;
+    if ((switch_rc = sell_product_grid(L, s, &GS))) return false;
     use_A = true;
     outer_parts = GS;
     return true;
@@ -5268,10 +5315,12 @@ int mf_mixed_solve(const char* who, const cf_matrix* A, int64_t n_nodes, int64_t
     power_est = std::sqrt(h.pad);
     if (side) CF_HIP(hipStreamWaitEvent(s, side->done, 0));
   }
-  if (!two_level)
-    return fcg_cheb_core<true, T>(outer, outer_parts, power, cheb, 0.0, kChebF32, n, b, x, fixed,
-                                  tol, maxit, L, stats_host, s, nullptr, nullptr, 0, 0.0,
-                                  switch_outer);
+  if (!two_level) {
+    rc = fcg_cheb_core<true, T>(outer, outer_parts, power, cheb, 0.0, kChebF32, n, b, x, fixed,
+                                tol, maxit, L, stats_host, s, nullptr, nullptr, 0, 0.0,
+                                switch_outer);
+    return rc ? rc : switch_rc;
+  }
   // two-level: z1 = C r; z2 = z1 + Cheb(r - K z1); z = z2 + C (r - K z2)
   const int GV = pcg_vec_grid();
   const int ga = int(std::min<int64_t>(C.n_agg, 65535));
@@ -5321,9 +5370,10 @@ int mf_mixed_solve(const char* who, const cf_matrix* A, int64_t n_nodes, int64_t
     if ((r2 = product(u, r2v, 4, T(0), T(0), L.state))) return r2;  // r2 = r - K u
     return coarse_apply(r2v, true);                                 // u += C r2
   };
-  return fcg_cheb_core<true, T>(outer, outer_parts, power, cheb, 0.0, kCheb2L, n, b, x, fixed,
-                                tol, maxit, L, stats_host, s, pre2l, u, 2, power_est,
-                                switch_outer);
+  rc = fcg_cheb_core<true, T>(outer, outer_parts, power, cheb, 0.0, kCheb2L, n, b, x, fixed,
+                              tol, maxit, L, stats_host, s, pre2l, u, 2, power_est,
+                              switch_outer);
+  return rc ? rc : switch_rc;
 }
 
 }  // namespace

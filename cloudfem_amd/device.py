@@ -168,7 +168,8 @@ class DeviceMesh:
             _, order = build_incidence(self.conn[:, 0].contiguous(), self.n_nodes, self.dev)
             order = order[:self.n_elem].long()
             # the sort returns element ids; check before they index anything
-            lo, hi = int(order.min()), int(order.max())
+            # (one read-back for both bounds)
+            lo, hi = (torch.stack(torch.aminmax(order)).tolist() if order.numel() else (0, -1))
             if order.numel() != self.n_elem or lo < 0 or hi >= self.n_elem:
                 raise N.CFError(f"element order out of range [{lo}, {hi}] for {self.n_elem}")
             conn = self.conn.index_select(0, order).contiguous()
