@@ -4456,11 +4456,12 @@ __global__ void coarse_prolong(int64_t n_nodes, const int32_t* __restrict__ agg,
 // Aggregates are numbered in the solver's element order, so two aggregates
 // that share an element lie a few ids apart and the coarse matrix is block
 // banded (half-bandwidth 6 * 2 + 5 = 17 at 250 aggregates for a 160x20x20
-// beam).  band_cholesky factors it in one workgroup, band_inverse forms the
-// dense inverse column by column; together they replace rocSOLVER's dense
-// potrf + potri (6.4 ms at 1,500 unknowns, mostly panel kernels and launch
-// gaps) whenever the half-bandwidth is at most kBandW - 2.
-constexpr int kBandW = 64;         // LDS window of the factorisation, ring of the inverse
+// beam).  band_cholesky factors it in one workgroup, band_forward forms
+// L^-1 column by column and one rocBLAS SYRK the dense inverse L^-T L^-1;
+// together they replace rocSOLVER's dense potrf + potri (6.4 ms at 1,500
+// unknowns, mostly panel kernels and launch gaps) whenever the
+// half-bandwidth is at most kBandW - 2.
+constexpr int kBandW = 64;         // LDS window of the factorisation, ring of the substitution
 constexpr int kBandChunk = 32;     // band rows fetched per chunk of factorisation steps
 constexpr int kBandThreads = 256;  // threads of band_cholesky
 constexpr int kBandNV = kBandChunk * kBandW / kBandThreads;
@@ -4562,18 +4563,16 @@ band_cholesky(int nc, int bw, const double* __restrict__ A, double* __restrict__
   if (th == 0) *info = bad;
 }
 
-// Dense inverse from the band factor, one column per lane: lane j solves
-// L y = e_j forward over rows j0 .. nc - 1 (j0 = first column of its wave,
-// y = 0 above row j) and L^T x = y backward over rows nc - 1 .. 0, and
-// stores Ainv[i nc + j] = x_i (y_i first, read back by the backward pass
-// kBandPf rows ahead).  The lane's last kBandW values stay in an LDS ring
-// (lane-major, no bank conflicts); the factor entries are wave-uniform
+// Inverse factor Y = L^-1 from the band factor, one column per lane: lane j
+// solves L y = e_j forward over rows j0 .. nc - 1 (j0 = first column of its
+// wave; y = 0 above row j) and stores Y(i, j) at Y[i nc + j] (rows above j0
+// are left to the caller's zero fill).  The inverse of A = L L^T is then
+// Y^T Y, one rocBLAS SYRK.  The lane's last kBandW values stay in an LDS
+// ring (lane-major, no bank conflicts); the factor entries are wave-uniform
 // loads.  Fixed summation order: bitwise reproducible.
-constexpr int kBandPf = 4;
-
 __global__ void __launch_bounds__(64)
-band_inverse(int nc, int bw, const double* __restrict__ Lc, const double* __restrict__ Lr,
-             const double* __restrict__ dinv, double* __restrict__ Ainv) {
+band_forward(int nc, int bw, const double* __restrict__ Lr, const double* __restrict__ dinv,
+             double* __restrict__ Y) {
   constexpr int W = kBandW;
   __shared__ double ring[W][64];
   const int lane = threadIdx.x;
@@ -4591,101 +4590,84 @@ band_inverse(int nc, int bw, const double* __restrict__ Lc, const double* __rest
     if (t <= tmax) s0 -= lr[t] * ring[(i - t) % W][lane];
     const double y = (s0 + s1) * dinv[i];
     ring[i % W][lane] = y;
-    if (live) Ainv[int64_t(i) * nc + j] = y;
-  }
-  // y_i for the next kBandPf rows of the backward pass (0 above row j0)
-  double pf[kBandPf];
-#pragma unroll
-  for (int u = 0; u < kBandPf; ++u) {
-    const int i = nc - 1 - u;
-    pf[u] = (live && i >= j0) ? Ainv[int64_t(i) * nc + j] : 0.0;
-  }
-  for (int ib = nc - 1; ib >= 0; ib -= kBandPf) {
-#pragma unroll
-    for (int u = 0; u < kBandPf; ++u) {
-      const int i = ib - u;
-      if (i < 0) break;
-      double s0 = pf[u], s1 = 0.0;
-      const int ip = i - kBandPf;
-      pf[u] = (live && ip >= j0) ? Ainv[int64_t(ip) * nc + j] : 0.0;
-      const double* lc = Lc + int64_t(i) * W;
-      const int tmax = bw < nc - 1 - i ? bw : nc - 1 - i;
-      int t = 1;
-      for (; t + 1 <= tmax; t += 2) {
-        s0 -= lc[t] * ring[(i + t) % W][lane];
-        s1 -= lc[t + 1] * ring[(i + t + 1) % W][lane];
-      }
-      if (t <= tmax) s0 -= lc[t] * ring[(i + t) % W][lane];
-      const double x = (s0 + s1) * dinv[i];
-      ring[i % W][lane] = x;
-      if (live) Ainv[int64_t(i) * nc + j] = x;
-    }
+    if (live) Y[int64_t(i) * nc + j] = y;
   }
 }
 
-// band_inverse for half-bandwidths below BWM, with the lane's last BWM - 1
-// values in registers instead of LDS: row r sits in slot (r - j0) mod BWM
-// during the forward pass and (nc - 1 - r) mod BWM during the backward pass,
-// and the step loop is unrolled BWM times, so every slot index is a
-// constant.  The factor arrays are cleared before the factorisation, so the
-// entries beyond the band (and past the last row) are zero and every step
-// runs the same BWM - 1 multiply-adds with no bounds.
+// band_forward for half-bandwidths below BWM, with the lane's last BWM - 1
+// values in registers: row r sits in slot (r - j0) mod BWM and the step loop
+// is unrolled BWM times, so every slot index is a constant.  The factor rows
+// (BWM entries each, dinv beside them) are staged in LDS a chunk of 2 BWM
+// rows at a time, the next chunk's loads in flight while the wave works
+// through the current one (wave-uniform global loads per step left each
+// step waiting on the L2).  The factor arrays are cleared before the
+// factorisation, so the entries beyond the band (and past the last row) are
+// zero and every step runs the same BWM - 1 multiply-adds with no bounds.
 template <int BWM>
 __global__ void __launch_bounds__(64)
-band_inverse_reg(int nc, const double* __restrict__ Lc, const double* __restrict__ Lr,
-                 const double* __restrict__ dinv, double* __restrict__ Ainv) {
-  static_assert(BWM % kBandPf == 0 && BWM <= kBandW, "slot rotation");
+band_forward_reg(int nc, const double* __restrict__ Lr, const double* __restrict__ dinv,
+                 double* __restrict__ Y) {
+  constexpr int CH = 2 * BWM;           // rows per staged chunk
+  constexpr int PER = CH * BWM / 64;    // factor entries each lane fetches per chunk
+  static_assert((CH * BWM) % 64 == 0 && CH <= 64 && BWM <= kBandW, "chunk shape");
+  __shared__ double st[2][CH * BWM];
+  __shared__ double sd[2][CH];
   const int lane = threadIdx.x;
   const int j0 = blockIdx.x * 64, j = j0 + lane;
   const bool live = j < nc;
+  double v[PER], dv = 0.0;
+  auto fetch = [&](int r0) {
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+      const int q = lane + 64 * m;
+      const int r = r0 + q / BWM;
+      v[m] = r < nc ? Lr[int64_t(r) * kBandW + q % BWM] : 0.0;
+    }
+    dv = (lane < CH && r0 + lane < nc) ? dinv[r0 + lane] : 0.0;
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int m = 0; m < PER; ++m) st[buf][lane + 64 * m] = v[m];
+    if (lane < CH) sd[buf][lane] = dv;
+  };
+  fetch(j0);
+  stage(0);
+  __syncthreads();
+  if (j0 + CH < nc) fetch(j0 + CH);
   double h[BWM];
 #pragma unroll
   for (int t = 0; t < BWM; ++t) h[t] = 0.0;
-  for (int ib = j0; ib < nc; ib += BWM) {
+  int buf = 0;
+  for (int ib = j0; ib < nc; ib += CH) {
 #pragma unroll
-    for (int u = 0; u < BWM; ++u) {
-      const int i = ib + u;
+    for (int u2 = 0; u2 < CH; ++u2) {
+      const int u = u2 % BWM;
+      const int i = ib + u2;
       if (i >= nc) break;
-      const double* lr = Lr + int64_t(i) * kBandW;
-      double s0 = i == j ? 1.0 : 0.0, s1 = 0.0;
+      const double* lr = st[buf] + u2 * BWM;
+      // the older terms first, the previous row's last: from one row's
+      // value to the next the chain is one multiply-add and one multiply
+      double s0 = 0.0, s1 = 0.0;
 #pragma unroll
-      for (int t = 1; t < BWM; ++t) {
-        const double v = lr[t] * h[(u - t + BWM) % BWM];
-        if (t & 1) s0 -= v; else s1 -= v;
+      for (int t = BWM - 1; t >= 2; --t) {
+        const double x = lr[t] * h[(u - t + BWM) % BWM];
+        if (t & 1) s0 += x; else s1 += x;
       }
-      const double y = (s0 + s1) * dinv[i];
+      const double rhs = (i == j ? 1.0 : 0.0) - (s0 + s1);
+      const double y = (rhs - lr[1] * h[(u - 1 + BWM) % BWM]) * sd[buf][u2];
       h[u] = y;
-      if (live) Ainv[int64_t(i) * nc + j] = y;
+      if (live) Y[int64_t(i) * nc + j] = y;
     }
-  }
-  double pf[kBandPf];
-#pragma unroll
-  for (int u = 0; u < kBandPf; ++u) {
-    const int i = nc - 1 - u;
-    pf[u] = (live && i >= j0) ? Ainv[int64_t(i) * nc + j] : 0.0;
-  }
-  for (int ib = 0; ib < nc; ib += BWM) {
-#pragma unroll
-    for (int u = 0; u < BWM; ++u) {
-      const int i = nc - 1 - (ib + u);
-      if (i < 0) break;
-      const int q = u % kBandPf;
-      double s0 = pf[q], s1 = 0.0;
-      const int ip = i - kBandPf;
-      pf[q] = (live && ip >= j0) ? Ainv[int64_t(ip) * nc + j] : 0.0;
-      const double* lc = Lc + int64_t(i) * kBandW;
-#pragma unroll
-      for (int t = 1; t < BWM; ++t) {
-        const double v = lc[t] * h[(u - t + BWM) % BWM];
-        if (t & 1) s0 -= v; else s1 -= v;
-      }
-      const double x = (s0 + s1) * dinv[i];
-      h[u] = x;
-      if (live) Ainv[int64_t(i) * nc + j] = x;
-    }
+    // the next chunk (in flight since the last stage) to the other buffer,
+    // then the one after it in flight
+    __syncthreads();
+    stage(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+    if (ib + 2 * CH < nc) fetch(ib + 2 * CH);
   }
 }
-constexpr int kBandReg = 24;  // band_inverse_reg<kBandReg> serves half-bandwidths below it
+constexpr int kBandReg = 24;  // band_forward_reg<kBandReg> serves half-bandwidths below it
 
 }  // namespace
 
@@ -5000,20 +4982,36 @@ int coarse_factor(Coarse* C, int64_t n, hipStream_t s) {
   const bool pin_dense = pin && std::strcmp(pin, "dense") == 0;
   const bool pin_ring = pin && std::strcmp(pin, "ring") == 0;
   CF_HIP(hipMemsetAsync(C->info, 0, 2 * sizeof(int32_t), s));
-  if (!pin_dense && bw <= kBandW - 2 && 2 * nc * kBandW + nc <= 12 * n) {
-    // banded factor and inverse; the probe vectors' space holds the factor
+  if (!pin_dense && bw <= kBandW - 2 && 2 * nc * kBandW + nc + nc * nc <= 12 * n) {
+    // banded factor, Y = L^-1 by forward substitution, A^-1 = Y^T Y; the
+    // probe vectors' space holds the factor and Y
     double* Lc = C->probe;
     double* Lr = Lc + nc * kBandW;
     double* dinv = Lr + nc * kBandW;
+    double* Y = dinv + nc;
     CF_HIP(hipMemsetAsync(Lc, 0, sizeof(double) * 2 * nc * kBandW, s));
+    CF_HIP(hipMemsetAsync(Y, 0, sizeof(double) * nc * nc, s));
     band_cholesky<<<1, kBandThreads, 0, s>>>(int(nc), int(bw), C->A, Lc, Lr, dinv, C->info);
     CF_LAUNCHED();
     const int gi = int((nc + 63) / 64);
     if (bw < kBandReg && !pin_ring)
-      band_inverse_reg<kBandReg><<<gi, 64, 0, s>>>(int(nc), Lc, Lr, dinv, C->A);
+      band_forward_reg<kBandReg><<<gi, 64, 0, s>>>(int(nc), Lr, dinv, Y);
     else
-      band_inverse<<<gi, 64, 0, s>>>(int(nc), int(bw), Lc, Lr, dinv, C->A);
+      band_forward<<<gi, 64, 0, s>>>(int(nc), int(bw), Lr, dinv, Y);
     CF_LAUNCHED();
+    // Y[i nc + j] = Y(i, j) is Y^T column-major, so (Y^T)(Y^T)^T = Y^T Y is
+    // A^-1: one DGEMM (rocBLAS's SYRK splits into several GEMM launches and
+    // took 0.75 ms here against one GEMM's fraction of that)
+    rocblas_handle hnd = solver_handle();
+    CF_REQUIRE(hnd != nullptr, "two-level: no rocBLAS handle");
+    CF_REQUIRE(rocblas_set_stream(hnd, s) == rocblas_status_success,
+               "two-level: rocblas_set_stream");
+    const double one = 1.0, zero = 0.0;
+    CF_REQUIRE(rocblas_dgemm(hnd, rocblas_operation_none, rocblas_operation_transpose,
+                             rocblas_int(nc), rocblas_int(nc), rocblas_int(nc), &one, Y,
+                             rocblas_int(nc), Y, rocblas_int(nc), &zero, C->A,
+                             rocblas_int(nc)) == rocblas_status_success,
+               "two-level: rocblas_dgemm failed");
   } else {
     rocblas_handle hnd = solver_handle();
     CF_REQUIRE(hnd != nullptr, "two-level: no rocBLAS handle");
