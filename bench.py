@@ -21,6 +21,8 @@ each timed after the same full CSR assembly:
   mixed_2l   the same with the inner Chebyshev products in FP32
   mixed_mf2l as mixed_2l with matrix-free FP64 outer products (solves the
              matrix-free operator, equal to the assembled one up to rounding)
+  mixed_2l_mfma  mixed_2l with the FP32 element forces on the matrix cores
+             (mf_t10_elem_mfma, v_mfma_f32_16x16x4f32) instead of VALU
   (also selectable: mixed_mf, mixed, mixed16)
 `value` is the fastest path that solves the assembled matrix and meets the
 bar max(1e-10, 3 x FP64 floor); `config.solver` names it.  DOF counts all
@@ -60,8 +62,8 @@ E_MOD, NU, TRACTION, TOL = 210e9, 0.3, -1e6, 1e-10
 HBM_PEAK_GBS = 8000.0
 CPU_SAMPLE = (56, 6, 6)   # ~9 s of single-core CPU work per run on the GPU host
 CPU_DNF_CAP_S = 30.0
-DEFAULT_PATHS = "fp64,matfree,fp64_2l,mixed_mf2l,mixed_2l"
-ASSEMBLED = {"fp64", "fp64_2l", "mixed_2l", "mixed", "mixed16"}   # solve A itself
+DEFAULT_PATHS = "fp64,matfree,fp64_2l,mixed_mf2l,mixed_2l,mixed_2l_mfma"
+ASSEMBLED = {"fp64", "fp64_2l", "mixed_2l", "mixed_2l_mfma", "mixed", "mixed16"}   # solve A itself
 
 
 def parse():
@@ -244,6 +246,7 @@ OP_NAMES = {
     "mixed_mf": "mf_t10_elem_lp<float> + mf_gather_cheb<float>",
     "mixed_mf2l": "mf_t10_elem_lp<float> + mf_gather_cheb<float>",
     "mixed_2l": "mf_t10_elem_lp<float> + mf_gather_cheb<float>",
+    "mixed_2l_mfma": "mf_t10_elem_mfma + mf_gather_cheb<float>",
     "fp64_2l": "mf_t10_elem_lp<double> + mf_gather_cheb<double>",
     "fp64_mf2l": "mf_t10_elem_lp<double> + mf_gather_cheb<double>",
 }
@@ -303,6 +306,13 @@ def run_cantilever(workload, path_list, steps, warmup, precond64="jacobi", keep_
         elif path in ("mixed_2l", "fp64_2l"):
             stats = pcg_t10_twolevel(dm, lam, mu, rhs, x, fixed, A=A,
                                      precision="mixed" if path == "mixed_2l" else "fp64", **kw)
+        elif path == "mixed_2l_mfma":
+            os.environ["CF_INNER_KERNEL"] = "mfma"
+            try:
+                stats = pcg_t10_twolevel(dm, lam, mu, rhs, x, fixed, A=A, precision="mixed",
+                                         **kw)
+            finally:
+                os.environ.pop("CF_INNER_KERNEL", None)
         elif path == "fp64":
             stats = pcg(A, rhs, x, fixed, precision="fp64", precond=precond64, **kw)
         else:
@@ -537,6 +547,15 @@ def main():
                           "speedup": a["ms_per_step"] / b["ms_per_step"],
                           "inner_product_ms": {"fp64": a["op_avg_ms"], "fp32": b["op_avg_ms"]},
                           "iterations": {"fp64": a["iterations"], "mixed": b["iterations"]}}
+    mfma_vs_valu = None
+    if "ms_per_step" in results.get("mixed_2l", {}) and \
+            "ms_per_step" in results.get("mixed_2l_mfma", {}):
+        a, b = results["mixed_2l_mfma"], results["mixed_2l"]
+        mfma_vs_valu = {"mfma_ms": a["ms_per_step"], "valu_ms": b["ms_per_step"],
+                        "inner_product_ms": {"mfma": a["op_avg_ms"], "valu": b["op_avg_ms"]},
+                        "inner_products": {"mfma": a.get("inner_products"),
+                                           "valu": b.get("inner_products")},
+                        "solution_rel_diff": a.get("rel_diff_vs_fp64")}
     line = {
         "metric": "Tet10 elasticity DOF/s (assemble+PCG to 1e-10), mixed vs FP64 on 1 GPU",
         "value": n_dof / (rb["ms_per_step"] / 1e3),
@@ -561,6 +580,7 @@ def main():
         "paths": results,
         "speedup_vs_fp64": speed,
         "speedup_precision_only": precision_only,
+        "element_forces_mfma_vs_valu": mfma_vs_valu,
         "tip_uz": ctx["tip"],
         "roofline": {"bound": "hbm", "achieved": rb["op_gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s",

@@ -1640,6 +1640,221 @@ mf_t10_elem_lp(int64_t m, const int32_t* __restrict__ conn, const T* __restrict_
   }
 }
 
+// ---- FP32 element forces on the matrix cores.
+// The two contractions of the affine Tet10 operator with the reference
+// gradient table d[q][a][k] = dN_a/dxi_k at the 4 points of the exact rule
+// (point q: L_q = kT4a, the other three kT4b) are batched GEMMs:
+//   G_c[q][k] = sum_a d[q][a][k] u_c[a]          (gradients of u_c)
+//   f_c[a]    = sum_{q,k} d[q][a][k] P_c[q][k]    (forces)
+// with P = (V/4) sigma(U Jinv) Jinv^T per point, as in t10_forces_4pt.  A
+// wave takes 16 elements at a time, four lanes per element: lane l serves
+// element e = l % 16 and point / node group j = l / 16.  With
+// v_mfma_f32_16x16x4f32 (A 16x4: lane l holds A[l%16][l/16]; B 4x16: lane l
+// holds B[l/16][l%16]; D 16x16: lane l, register r holds D[4(l/16)+r][l%16])
+//   stage 1: D1[4q+k][e] = sum_s A1_s[4q+k][i] B1_s[i][e], B1_s[i][e] =
+//            u_c[4s+i] of element e (nodes 10, 11 and k = 3 padded with
+//            zeros), three k-steps: lane (e, j) ends with G_c[j][0..2];
+//   stage 2: D2[a][e] = sum_s A2_s[a][q] B2_s[q][e], B2_s[q][e] = P_c[q][s]
+//            (lane (e, j) holds point j), three k-steps: lane (e, j) ends
+//            with f_c[4j + r], r = 0..3 (a < 10).
+// 18 MFMAs and about 60 VALU multiply-adds per lane per 16 elements, against
+// about 580 multiply-adds per element on one lane in t10_forces_4pt; every
+// lane gathers three nodes instead of ten.  Same run plan, LDS staging and
+// run partials as mf_t10_elem_lp, so the node kernel is shared.
+typedef float cf_float4 __attribute__((ext_vector_type(4)));
+
+struct T10Grad4 {
+  float d[4][10][3];
+};
+
+constexpr T10Grad4 build_t10_grad4() {
+  T10Grad4 t{};
+  for (int q = 0; q < 4; ++q) {
+    double L[4] = {kT4b, kT4b, kT4b, kT4b};
+    L[q] = kT4a;
+    double dL[10][4] = {};
+    for (int i = 0; i < 4; ++i) dL[i][i] = 4.0 * L[i] - 1.0;
+    for (int s = 0; s < 6; ++s) {
+      const int i = kTE[s][0], j = kTE[s][1];
+      dL[4 + s][i] = 4.0 * L[j];
+      dL[4 + s][j] = 4.0 * L[i];
+    }
+    for (int a = 0; a < 10; ++a)
+      for (int k = 0; k < 3; ++k) t.d[q][a][k] = float(dL[a][k + 1] - dL[a][0]);
+  }
+  return t;
+}
+
+__constant__ T10Grad4 kGrad4 = build_t10_grad4();
+
+__device__ __forceinline__ cf_float4 mfma16x16x4(float a, float b, cf_float4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <bool PRE>
+__global__ void __launch_bounds__(kBlock)
+mf_t10_elem_mfma(int64_t m, const int32_t* __restrict__ conn, const float* __restrict__ geo,
+                 float lam, float mu, const float* __restrict__ sc, const float* __restrict__ d,
+                 const uint16_t* __restrict__ rank, const int32_t* __restrict__ toff,
+                 const uint16_t* __restrict__ tstart, float* __restrict__ part,
+                 const PcgState* state) {
+  if (state && state->halt) return;
+  __shared__ float fl[kTileItems * 3];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int el0 = lane & 15, j = lane >> 4;
+  // constant operands: A1_s (row 4q+k, column node 4s+j) and A2_s (row node
+  // el0, column point j, direction s)
+  float a1[3], a2[3];
+  {
+    const int q = el0 >> 2, k = el0 & 3;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+      const int a = 4 * s + j;
+      a1[s] = (k < 3 && a < 10) ? kGrad4.d[q][a][k] : 0.0f;
+      a2[s] = el0 < 10 ? kGrad4.d[j][el0][s] : 0.0f;
+    }
+  }
+  const float lq = 0.25f * lam, mq = 0.25f * mu;
+  const int64_t n_tiles = (m + kBlock - 1) / kBlock;
+  for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < n_tiles; t += gridDim.x) {
+    // All loads of the four groups first, with no branches: an element past
+    // m reads element m - 1 (the padding of the last tile is not written)
+    // and gets V = 0, node slots 10 and 11 read node 9 (their A1 entries are
+    // zero), so every wait is on loads that are already in flight.
+    const int64_t last = m - 1 - t * kBlock;  // last valid element of the tile
+    int64_t base[4];
+    bool ok[4];
+    int32_t nd[4][3];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int el = wave * 64 + g * 16 + el0;  // element within the tile
+      ok[g] = el <= last;
+      base[g] = t * 10 * kBlock + (ok[g] ? el : last);
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        const int a = 4 * s + j < 10 ? 4 * s + j : 9;
+        nd[g][s] = conn[base[g] + int64_t(a) * kBlock];
+      }
+    }
+    float xb[4][3][3];
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        const Vec3<float> x = *reinterpret_cast<const Vec3<float>*>(d + 3 * int64_t(nd[g][s]));
+        if constexpr (PRE) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) xb[g][s][c] = x.v[c];
+        } else {
+          const Vec3<float> sv =
+              *reinterpret_cast<const Vec3<float>*>(sc + 3 * int64_t(nd[g][s]));
+#pragma unroll
+          for (int c = 0; c < 3; ++c) xb[g][s][c] = sv.v[c] * x.v[c];
+        }
+      }
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      float J[3][3];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) J[r][c] = geo[base[g] + int64_t(3 * r + c) * kBlock];
+      const float V = ok[g] ? geo[base[g] + 9 * int64_t(kBlock)] : 0.0f;
+      // stage 1: reference gradients at point j
+      cf_float4 G[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        G[c] = cf_float4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int s = 0; s < 3; ++s) G[c] = mfma16x16x4(a1[s], xb[g][s][c], G[c]);
+      }
+      // point algebra: H = U Jinv, (V/4) sigma, P = (V/4) sigma Jinv^T
+      float H[3][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) H[c][i] = G[c][0] * J[0][i] + G[c][1] * J[1][i] + G[c][2] * J[2][i];
+      const float lv = lq * V, mv = mq * V;
+      const float ltr = lv * (H[0][0] + H[1][1] + H[2][2]);
+      float sg[3][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        sg[c][c] = 2.0f * mv * H[c][c] + ltr;
+#pragma unroll
+        for (int i = c + 1; i < 3; ++i) sg[c][i] = sg[i][c] = mv * (H[c][i] + H[i][c]);
+      }
+      // stage 2: forces of nodes 4j .. 4j + 3
+      cf_float4 F[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        F[c] = cf_float4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+          const float p = sg[c][0] * J[s][0] + sg[c][1] * J[s][1] + sg[c][2] * J[s][2];
+          F[c] = mfma16x16x4(a2[s], p, F[c]);
+        }
+      }
+      if (ok[g]) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int a = 4 * j + r;
+          if (a < 10) {
+            const int rk = rank[base[g] + int64_t(a) * kBlock];
+            fl[3 * rk] = F[0][r];
+            fl[3 * rk + 1] = F[1][r];
+            fl[3 * rk + 2] = F[2][r];
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // one partial per run of the tile (runs cover the tile's items)
+    const int64_t k0 = t * kTileItems;
+    const int nt = int((k0 + kTileItems < 10 * m ? k0 + kTileItems : 10 * m) - k0);
+    const int32_t q0 = toff[t], q1 = toff[t + 1];
+    for (int32_t q = q0 + threadIdx.x; q < q1; q += kBlock) {
+      const int kb = tstart[q], ke = q + 1 < q1 ? int(tstart[q + 1]) : nt;
+      float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+      for (int k = kb; k < ke; ++k) {
+        s0 += fl[3 * k];
+        s1 += fl[3 * k + 1];
+        s2 += fl[3 * k + 2];
+      }
+      float* o = part + 3 * int64_t(q);
+      o[0] = s0;
+      o[1] = s1;
+      o[2] = s2;
+    }
+    __syncthreads();
+  }
+}
+
+// Which FP32 element kernel the inner products use: CF_INNER_KERNEL=mfma
+// (mf_t10_elem_mfma) or =valu (mf_t10_elem_lp<float>); read at each solve.
+bool inner_mfma() {
+  const char* v = std::getenv("CF_INNER_KERNEL");
+  if (v && std::strcmp(v, "mfma") == 0) return true;
+  if (v && std::strcmp(v, "valu") == 0) return false;
+  return false;
+}
+
+// FP32 element forces of the Chebyshev / power products (PRE as in
+// mf_t10_elem_lp) with the kernel chosen by inner_mfma().
+template <class T, bool PRE>
+void launch_elem_lp(bool mfma, int grid, hipStream_t s, int64_t m, const int32_t* conn,
+                    const T* geo, T lam, T mu, const T* sc, const T* d, const uint16_t* rank,
+                    const int32_t* toff, const uint16_t* tstart, T* part, const PcgState* state) {
+  if constexpr (std::is_same_v<T, float>) {
+    if (mfma) {
+      mf_t10_elem_mfma<PRE><<<grid, kBlock, 0, s>>>(m, conn, geo, lam, mu, sc, d, rank, toff,
+                                                    tstart, part, state);
+      return;
+    }
+  }
+  mf_t10_elem_lp<T, PRE><<<grid, kBlock, 0, s>>>(m, conn, geo, lam, mu, sc, d, rank, toff,
+                                                 tstart, part, state);
+}
+
 // node sums of the FP32 forces (the node's run partials in tile order).
 // mode 0 (power iteration): zn = S sum.  Modes 3 and 4 serve the two-level
 // preconditioner (product K zc of its coarse iterate): mode 3 starts the
@@ -5241,17 +5456,16 @@ int mf_mixed_solve(const char* who, const cf_matrix* A, int64_t n_nodes, int64_t
   // dout with z_{k+1}.  Modes 3 and 4: products of the two-level form.
   // zout: where mode 5 adds its step (else the step overwrites dout); ps:
   // the stream (default s)
+  const bool use_mfma = inner_mfma();
   auto product = [&](const T* din, T* dout, int mode, T cdk, T cwk, const PcgState* state,
                      T* zout = nullptr, hipStream_t ps = nullptr) {
     if (!ps) ps = s;
     if (mode == 0)
-      mf_t10_elem_lp<T, false><<<ge, kBlock, 0, ps>>>(n_elem, M.connT, geoL, T(lam), T(mu), sL,
-                                                      din, M.rankT, M.R.toff, M.R.tstart, feL,
-                                                      state);
+      launch_elem_lp<T, false>(use_mfma, ge, ps, n_elem, M.connT, geoL, T(lam), T(mu), sL, din,
+                               M.rankT, M.R.toff, M.R.tstart, feL, state);
     else
-      mf_t10_elem_lp<T, true><<<ge, kBlock, 0, ps>>>(n_elem, M.connT, geoL, T(lam), T(mu), sL,
-                                                     din, M.rankT, M.R.toff, M.R.tstart, feL,
-                                                     state);
+      launch_elem_lp<T, true>(use_mfma, ge, ps, n_elem, M.connT, geoL, T(lam), T(mu), sL, din,
+                              M.rankT, M.R.toff, M.R.tstart, feL, state);
     CF_LAUNCHED();
     if (M.R.use_run4)
       mf_gather_cheb<T, true><<<gg, kBlock, 0, ps>>>(n_nodes, M.R.nro, M.R.nq, feL, sL, mode, din,
@@ -5599,10 +5813,10 @@ int mf_lp_apply(int64_t n_nodes, int64_t n_elem, const int32_t* conn, const doub
   CF_LAUNCHED();
   const int ge = cf::stride_grid(n_elem, kBlock);
   const int gg = int(mf_gather_grid(n_nodes));
+  const bool use_mfma = inner_mfma();
   auto product = [&]() {
-    mf_t10_elem_lp<T, false><<<ge, kBlock, 0, s>>>(n_elem, M.connT, geoL, T(lam), T(mu), sL, x,
-                                                    M.rankT, M.R.toff, M.R.tstart,
-                                                    static_cast<T*>(M.feL), nullptr);
+    launch_elem_lp<T, false>(use_mfma, ge, s, n_elem, M.connT, geoL, T(lam), T(mu), sL, x,
+                             M.rankT, M.R.toff, M.R.tstart, static_cast<T*>(M.feL), nullptr);
     CF_LAUNCHED();
     if (M.R.use_run4)
       mf_gather_cheb<T, true><<<gg, kBlock, 0, s>>>(n_nodes, M.R.nro, M.R.nq,

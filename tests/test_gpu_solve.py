@@ -514,6 +514,76 @@ def test_inner_precision_operator_matches_assembled(gpu):
     assert e32 < 1e-5
 
 
+@pytest.mark.parametrize("dims", [(12, 3, 2), (5, 5, 5), (24, 5, 5)])
+def test_inner_product_mfma_matches_valu(gpu, monkeypatch, dims):
+    """The matrix-core element kernel (mf_t10_elem_mfma, CF_INNER_KERNEL=mfma)
+    against the VALU kernel and the FP64 assembled matrix: S K S x with a
+    random S (the power iteration's form, S gathered in the kernel) on boxes
+    whose last 256-element tile is partial (432, 750 and 3,600 elements)."""
+    import torch
+    from cloudfem_amd._native import FMT_NB3
+    from cloudfem_amd.device import DeviceMesh
+    from cloudfem_amd.mesh import box_mesh_tet10
+    from cloudfem_amd.physics.elastic3d import lame
+    from cloudfem_amd.solver import DeviceMatrix, matfree_t10_lp_apply
+    m = box_mesh_tet10(*dims, xlim=(0.0, 0.4), ylim=(0.0, 0.05), zlim=(0.0, 0.05))
+    lam, mu = lame(210e9, 0.3)
+    dm = DeviceMesh(m)
+    A = DeviceMatrix(FMT_NB3, dm.pattern(), dm.matrix_values(5, lam=lam, mu=mu))
+    g = torch.Generator(device="cpu").manual_seed(sum(dims))
+    x = torch.randn(3 * m.n_nodes, dtype=torch.float64, generator=g).to("cuda")
+    sc = (torch.rand(3 * m.n_nodes, dtype=torch.float64, generator=g) + 0.5).to("cuda") * 1e-5
+    y_ref = sc * A.matvec(sc * x)
+    ys = {}
+    for kern in ("valu", "mfma"):
+        monkeypatch.setenv("CF_INNER_KERNEL", kern)
+        ys[kern], _ = matfree_t10_lp_apply(dm, lam, mu, x.float(), precision="mixed", scale=sc)
+    scale = float(y_ref.abs().max())
+    e_ref = float((ys["mfma"].double() - y_ref).abs().max()) / scale
+    e_valu = float((ys["mfma"] - ys["valu"]).abs().max()) / scale
+    print(f"{dims} MFMA inner product: vs assembled FP64 {e_ref:.2e}, vs VALU {e_valu:.2e}")
+    # FP32 rounding of x, S, geometry and the sums (the VALU kernel: ~1.6e-7)
+    assert e_ref < 1e-6
+    assert e_valu < 1e-6
+
+
+@pytest.mark.parametrize("dims", [(8, 2, 2), (40, 5, 5)])
+def test_twolevel_mfma_inner_matches_spsolve(gpu, monkeypatch, dims):
+    """The two-level mixed solver with its FP32 Chebyshev products on the
+    matrix cores: residual bar and 1e-10 against SuperLU, and within a few
+    outer iterations of the VALU kernel's solve."""
+    from cloudfem_amd import solver
+    from cloudfem_amd.mesh import box_mesh_tet10
+    from cloudfem_amd.physics import elastic_tet10
+    lims = dict(xlim=(0.0, 0.4), ylim=(0.0, 0.05), zlim=(0.0, 0.05))
+    trac = {"xmax": (0.0, 0.0, -1e6)}
+    bc = {"xmin": (0.0, 0.0, 0.0)}
+    m = box_mesh_tet10(*dims, **lims)
+    m0 = om.box_mesh_tet10(*dims, **lims)
+    A0, rhs0 = oel.assemble_elasticity_tet10(m0, 210e9, 0.3, traction=trac)
+    u0 = oel.apply_dirichlet_vec3(A0, rhs0, m0, bc)
+    Aff, bf, free = oel.reduced_system_vec3(A0, rhs0, m0, bc)[:3]
+    floor = fp64_floor(Aff, u0[free], bf)
+    its = {}
+    for kern in ("valu", "mfma"):
+        monkeypatch.setenv("CF_INNER_KERNEL", kern)
+        with solver.solver_options(operator="assembled", precision="mixed", precond="twolevel",
+                                   tol=1e-12):
+            u = np.stack(elastic_tet10.solve_elasticity_tet10(m, 210e9, 0.3, bc=bc,
+                                                              traction=trac),
+                         axis=1).ravel()
+        st = solver.last_stats.value
+        its[kern] = st["iterations"]
+        err = rel_err(u, u0)
+        print(f"{dims} inner kernel {kern}: {st['iterations']} iterations, rel. residual "
+              f"{st['rel_residual']:.2e} (floor {floor:.2e}), error vs SuperLU {err:.2e}")
+        assert_residual_bar(st, 1e-12, floor)
+        assert err < 1e-10
+    # FP32 rounding differs between the kernels, so the iteration counts may
+    # differ a little (measured: 189 / 193 at 8x2x2, 178 / 159 at 40x5x5)
+    assert its["mfma"] <= 1.25 * its["valu"] + 5
+
+
 @pytest.mark.parametrize("dims", [(24, 3, 3), (5, 5, 5)])
 def test_chunked_matfree_solve_matches_two_pass(gpu, dims):
     """The LDS-chunked operator (Morton chunks, shared-node partials) and the
