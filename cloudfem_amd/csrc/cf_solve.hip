@@ -25,6 +25,7 @@
 // 1517), so the stopping test is applied to the true FP64 residual.
 #include <cmath>
 #include <cstdlib>
+#include <limits>
 #include <cstring>
 #include <string>
 #include <mutex>
@@ -63,6 +64,7 @@ struct PcgState {
   int32_t halt;   // Status
   uint32_t ticket;
   double rzs[2];  // r.z by iteration parity (the fused CG kernels read one, write the other)
+  double floor2;  // || |A_ff| |x_f| ||^2 at the last floor check
 };
 
 // ------------------------------------------------------------- reductions
@@ -2203,7 +2205,7 @@ __global__ void sell_fill(int64_t n, int64_t n_slices, const int32_t* __restrict
 // iteration with the next steps' column indices loaded one iteration ahead,
 // so the x gathers of a step never wait on its own index load.  Returns the
 // row (or -1) whose 3 sums `o` this lane holds after the phase reduction.
-template <class VT, class XT, class AT>
+template <class VT, class XT, class AT, bool ABS = false>
 __device__ __forceinline__ int64_t sell_rows(int64_t n, int64_t sl,
                                              const uint32_t* __restrict__ rows,
                                              const int32_t* __restrict__ koff,
@@ -2228,6 +2230,10 @@ __device__ __forceinline__ int64_t sell_rows(int64_t n, int64_t sl,
       xv[u][0] = ok ? AT(x[c3]) : AT(0);
       xv[u][1] = ok ? AT(x[c3 + 1]) : AT(0);
       xv[u][2] = ok ? AT(x[c3 + 2]) : AT(0);
+      if constexpr (ABS) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) xv[u][q] = fabs(xv[u][q]);
+      }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -2235,6 +2241,10 @@ __device__ __forceinline__ int64_t sell_rows(int64_t n, int64_t sl,
       const bool ok = g + u < g1 && (!kMask || c[u] >= 0);
 #pragma unroll
       for (int q = 0; q < 9; ++q) v[u][q] = ok ? AT(ld(p + q * 64)) : AT(0);
+      if constexpr (ABS) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) v[u][q] = fabs(v[u][q]);
+      }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)
@@ -2287,6 +2297,50 @@ spmv_sell(int64_t n, int64_t n_slices, const uint32_t* __restrict__ rows,
   }
   if (!state) return;
   double v1[1] = {dot};
+  block_partials<1>(v1, partials);
+}
+
+// || |A_ff| |x_f| ||^2 partials on the sliced copy (xf: x with its fixed
+// entries zeroed; fixed rows are dropped): the rounding floor of the FP64
+// residual b_f - A_ff x_f is about eps || |A_ff| |x_f| || (Higham, Accuracy
+// and Stability of Numerical Algorithms, 2002, sec. 3.5).  Grid: as spmv_sell.
+__global__ void __launch_bounds__(kBlock)
+sell_abs_norm(int64_t n, int64_t n_slices, const uint32_t* __restrict__ rows,
+              const int32_t* __restrict__ koff, const int32_t* __restrict__ scol,
+              const double* __restrict__ sval, const double* __restrict__ xf,
+              const uint8_t* __restrict__ fixed, double* __restrict__ partials,
+              const int32_t* __restrict__ xsplit) {
+  int64_t end;
+  const int64_t sl0 = sell_first_slice(xsplit, &end);
+  double sq = 0.0;
+  for (int64_t sl = sl0; sl < sl0 + kSlicesPerWave && sl < end; ++sl) {
+    double o[3];
+    const int64_t r = sell_rows<double, double, double, true>(n, sl, rows, koff, scol, sval, xf,
+                                                              o);
+    if (r >= 0) {
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc)
+        if (!(fixed && fixed[3 * r + cc])) sq += o[cc] * o[cc];
+    }
+  }
+  double v1[1] = {sq};
+  block_partials<1>(v1, partials);
+}
+
+// The same for a scalar CSR matrix (grid-stride rows).
+__global__ void __launch_bounds__(kBlock)
+csr_abs_norm(int64_t n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ idx,
+             const double* __restrict__ val, const double* __restrict__ xf,
+             const uint8_t* __restrict__ fixed, double* __restrict__ partials) {
+  double sq = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    if (fixed && fixed[i]) continue;
+    double acc = 0.0;
+    for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) acc += fabs(val[k]) * fabs(xf[idx[k]]);
+    sq += acc * acc;
+  }
+  double v1[1] = {sq};
   block_partials<1>(v1, partials);
 }
 
@@ -2864,24 +2918,38 @@ int next_batch(const PcgState& h, double rr_start, double prev_rr, int prev_it, 
 
 // Residual replacement at the end of a CG solve.  CG stops on its
 // recursive residual, which drifts from the true one b - A x by rounding.
-// When the true residual misses the bar, CG restarts from it (the true
-// residual replaces the recursive one) for a segment that ends when the
-// recursive residual has fallen 10x below the restart value (or to the bar).
-// If a restart does not halve the true residual, the bar lies below what
-// FP64 can resolve for this system (about eps ||A_ff| |x_f|| / ||b_f||):
-// the solve stops with floor_limited set and the true residual reported.
+// When the true residual misses the bar, the solver first measures the FP64
+// floor eps || |A_ff| |x_f| || (where the operator has stored values): a
+// true residual within kFloorFactor of it is what FP64 can resolve for this
+// system, so the solve stops there with floor_limited set.  Otherwise CG
+// restarts from the true residual (it replaces the recursive one) for a
+// segment that ends when the recursive residual has fallen 10x below the
+// restart value (or to the bar); a restart that does not halve the true
+// residual also ends the solve with floor_limited set.
 struct TrueResidual {
   static constexpr int kMaxRestarts = 8;
+  // a true residual within this factor of the FP64 floor is accepted
+  static constexpr double kFloorFactor = 2.0;
   double bar2;            // (tol ||b_f||)^2
   double last2 = 0.0;     // last true r.r
   double prev2 = HUGE_VAL;// true r.r at the previous restart
+  double floor_rr = 0.0;  // (kFloorFactor eps || |A_ff| |x_f| ||)^2, 0 = unknown
   int restarts = 0;
   int floor_limited = 0;
   explicit TrueResidual(double bar) : bar2(bar) {}
+  // || |A_ff| |x_f| ||^2 measured at the current iterate
+  void set_floor(double absnorm2) {
+    const double e = kFloorFactor * std::numeric_limits<double>::epsilon();
+    floor_rr = absnorm2 > 0.0 && std::isfinite(absnorm2) ? e * e * absnorm2 : 0.0;
+  }
   // true r.r of the current iterate -> whether to restart from it
   bool restart(double rr2) {
     last2 = rr2;
     if (!(rr2 > bar2)) return false;  // met (or NaN: reported, no restart)
+    if (rr2 <= floor_rr) {            // at the FP64 floor: nothing left to gain
+      floor_limited = 1;
+      return false;
+    }
     if (restarts >= kMaxRestarts || rr2 > 0.25 * prev2) {
       floor_limited = 1;
       return false;
@@ -2912,10 +2980,33 @@ struct TrueResidual {
   }
 };
 
-template <int PC, class Apply>
+// FP64 floor of a converged iterate whose true residual misses the bar:
+// abs_norm(xf) enqueues || |A_ff| |xf| ||^2 into state->floor2 (returns
+// CF_OK, < 0 when the current operator has no stored values, or an error).
+// L.Ap is free at this point and holds x_f.
+template <class AbsNorm>
+int measure_floor(AbsNorm& abs_norm, int64_t n, const double* x, const uint8_t* fixed,
+                  const Layout& L, TrueResidual& tr, hipStream_t s) {
+  if constexpr (std::is_same_v<std::decay_t<AbsNorm>, std::nullptr_t>) {
+    return CF_OK;
+  } else {
+    split_vector<<<pcg_grid(), kBlock, 0, s>>>(n, x, fixed, 0, L.Ap);
+    CF_LAUNCHED();
+    const int fr = abs_norm(static_cast<const double*>(L.Ap));
+    if (fr < 0) return CF_OK;
+    if (fr) return fr;
+    PcgState t{};
+    const int rc = read_state(L.state, &t, s);
+    if (rc) return rc;
+    tr.set_floor(t.floor2);
+    return CF_OK;
+  }
+}
+
+template <int PC, class Apply, class AbsNorm = std::nullptr_t>
 int pcg_fp64_core(Apply&& apply, int apply_parts, int64_t n, int64_t units, const double* b,
                   double* x, const uint8_t* fixed, double tol, int maxit, const Layout& L,
-                  cf_pcg_stats* st, hipStream_t s) {
+                  cf_pcg_stats* st, hipStream_t s, AbsNorm&& abs_norm = nullptr) {
   const int G = pcg_grid();
   int rc;
   PcgState h{};
@@ -2998,6 +3089,9 @@ int pcg_fp64_core(Apply&& apply, int apply_parts, int64_t n, int64_t units, cons
     PcgState ts{};
     if ((rc = read_state(L.state, &ts, s))) return rc;
     tr.last2 = ts.pad;
+    if (h.halt == kConverged && ts.pad > tr.bar2 && tr.floor_rr == 0.0 &&
+        (rc = measure_floor(abs_norm, n, x, fixed, L, tr, s)))
+      return rc;
     if (h.halt != kConverged || !tr.restart(ts.pad)) break;
     // residual replacement: restart CG from the true residual now in r
     cg_start<PC><<<G, kBlock, 0, s>>>(units, L.r, L.p, L.dinv, L.state, L.partials);
@@ -3051,7 +3145,19 @@ int pcg_fp64(const cf_matrix* A, const double* b, double* x, const uint8_t* fixe
     return launch_spmv<double, double, double>(A, A->values, xin, y, fixed, state, partials, G,
                                                s);
   };
-  return pcg_fp64_core<PC>(apply, GS, n, units, b, x, fixed, tol, maxit, L, st, s);
+  auto abs_norm = [&](const double* xf) -> int {
+    if (A->format == CF_FMT_NB3) {
+      sell_abs_norm<<<GS, kBlock, 0, s>>>(A->n_rows, L.n_slices, L.rows, L.koff, L.scol,
+                                          L.sval64, xf, fixed, L.partials, xsplit);
+      CF_LAUNCHED();
+      return fin(kFinStore, L, s, &L.state->floor2, 0.0, GS);
+    }
+    csr_abs_norm<<<G, kBlock, 0, s>>>(A->n_rows, A->indptr, A->indices, A->values, xf, fixed,
+                                      L.partials);
+    CF_LAUNCHED();
+    return fin(kFinStore, L, s, &L.state->floor2, 0.0, G);
+  };
+  return pcg_fp64_core<PC>(apply, GS, n, units, b, x, fixed, tol, maxit, L, st, s, abs_norm);
 }
 
 // NB3 systems: FP64 flexible CG, preconditioned by S p(S A S) S with p the
@@ -3118,14 +3224,23 @@ int power_launch(Power&& power, int64_t n, const Layout& L, hipStream_t s) {
 // their launch on the host, so batches stay short (the check itself is one
 // small read-back per batch).
 constexpr int kFcgBatchMax = 16;
+// Relative recursive residual at which a two-operator solve switches to its
+// final outer operator (the assembled matrix).  Measured at 384,000 Tet10
+// elements: switching at 1e-6 or 1e-7 gives 56 outer iterations, switching
+// after convergence (with a restart from the final operator's residual) 60,
+// switching at 1e-8 71 (the operators' rounding difference is no longer
+// small against r).
+constexpr double kSwitchRel = 1e-6;
 
 template <bool SQ, class T, class Outer, class Power, class Cheb,
-          class Pre2L = std::nullptr_t, class SwitchOuter = std::nullptr_t>
+          class Pre2L = std::nullptr_t, class SwitchOuter = std::nullptr_t,
+          class AbsNorm = std::nullptr_t>
 int fcg_cheb_core(Outer&& outer, const int& outer_parts, Power&& power, Cheb&& cheb, double gersh,
                   ChebShape shape, int64_t n, const double* b, double* x, const uint8_t* fixed, double tol,
                   int maxit, const Layout& L, cf_pcg_stats* st, hipStream_t s,
                   Pre2L&& pre2l = nullptr, const T* zh2l = nullptr, int extra_products = 0,
-                  double power_est = 0.0, SwitchOuter&& switch_outer = nullptr) {
+                  double power_est = 0.0, SwitchOuter&& switch_outer = nullptr,
+                  AbsNorm&& abs_norm = nullptr) {
   constexpr bool kTwoLevel = !std::is_same<std::decay_t<Pre2L>, std::nullptr_t>::value;
   constexpr bool kSwitch = !std::is_same<std::decay_t<SwitchOuter>, std::nullptr_t>::value;
   const int G = pcg_grid();
@@ -3256,6 +3371,9 @@ int fcg_cheb_core(Outer&& outer, const int& outer_parts, Power&& power, Cheb&& c
   double rr_start = h.rr;
   int prev_it = 0;
   TrueResidual tr(tol * tol * bb);
+  bool switched_mid = false;
+  double switch2 = 0.0;  // switch the outer operator when r.r <= switch2
+  if constexpr (kSwitch) switch2 = kSwitchRel * kSwitchRel * bb;
   for (;;) {
     while (h.halt == kRunning) {
       for (int t = 0; t < batch; ++t, ++enq) {
@@ -3268,6 +3386,20 @@ int fcg_cheb_core(Outer&& outer, const int& outer_parts, Power&& power, Cheb&& c
       }
       if ((rc = read_state(L.state, &h, s))) return rc;
       prof_collect();
+      if constexpr (kSwitch) {
+        // Switch the outer operator while the recursive residual is still far
+        // above the rounding-level difference of the two operators: the true
+        // residual of the final operator replaces r in place, p and the CG
+        // scalars are kept (residual replacement, van der Vorst and Ye, SIAM
+        // J. Sci. Comput. 22 (2000) 1635), so no Krylov information is lost.
+        if (!switched_mid && h.halt == kRunning && h.rr <= switch2 && switch_outer()) {
+          switched_mid = true;
+          if ((rc = outer(x, L.Ap, nullptr))) return rc;
+          residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials,
+                                        &L.state->pad);
+          CF_LAUNCHED();
+        }
+      }
       batch = next_batch(h, rr_start, prev_rr, prev_it, batch, 2, kFcgBatchMax);
       prev_rr = h.rr;
       prev_it = h.iter;
@@ -3276,7 +3408,7 @@ int fcg_cheb_core(Outer&& outer, const int& outer_parts, Power&& power, Cheb&& c
     // final one, which then gets at least one segment of its own
     bool switched = false;
     if constexpr (kSwitch)
-      if (h.halt == kConverged) switched = switch_outer();
+      if (h.halt == kConverged && !switched_mid) switched = switch_outer();
     // true FP64 residual (r = b - A x, its r.r into pad)
     if ((rc = outer(x, L.Ap, nullptr))) return rc;
     residual<<<G, kBlock, 0, s>>>(n, b, L.Ap, fixed, L.r, L.state, L.partials, &L.state->pad);
@@ -3286,6 +3418,9 @@ int fcg_cheb_core(Outer&& outer, const int& outer_parts, Power&& power, Cheb&& c
     if ((rc = read_state(L.state, &fin_state, s))) return rc;
     tr.last2 = fin_state.pad;
     if (h.halt != kConverged) break;
+    if (!switched && fin_state.pad > tr.bar2 && tr.floor_rr == 0.0 &&
+        (rc = measure_floor(abs_norm, n, x, fixed, L, tr, s)))
+      return rc;
     if (switched ? !tr.force(fin_state.pad) : !tr.restart(fin_state.pad)) break;
     // residual replacement: a fresh flexible CG from the true residual in r
     h.stop2 = tr.stop2_after(switched);
@@ -3367,9 +3502,16 @@ int pcg_fcg_cheb(const cf_matrix* A, VT* sval, const double* b, double* x, const
     CF_LAUNCHED();
     return CF_OK;
   };
+  auto abs_norm = [&](const double* xf) -> int {
+    sell_abs_norm<<<GS, kBlock, 0, s>>>(A->n_rows, L.n_slices, L.rows, L.koff, L.scol, L.sval64,
+                                        xf, fixed, L.partials, xsplit);
+    CF_LAUNCHED();
+    return fin(kFinStore, L, s, &L.state->floor2, 0.0, GS);
+  };
   return fcg_cheb_core<false, float>(outer, GS, power, cheb, gersh,
                                      sizeof(VT) == 2 ? kChebF16 : kChebF32, n, b, x, fixed, tol,
-                                     maxit, L, st, s);
+                                     maxit, L, st, s, nullptr, nullptr, 0, 0.0, nullptr,
+                                     abs_norm);
 }
 
 // Scalar CSR systems (P1 heat, capacity-dominated theta-method steps): FP32
@@ -5527,10 +5669,18 @@ int mf_mixed_solve(const char* who, const cf_matrix* A, int64_t n_nodes, int64_t
     power_est = std::sqrt(h.pad);
     if (side) CF_HIP(hipStreamWaitEvent(s, side->done, 0));
   }
+  // FP64 floor of the final operator when it is A (matrix-free: none stored)
+  auto abs_norm = [&](const double* xf) -> int {
+    if (!use_A) return -1;
+    sell_abs_norm<<<GS, kBlock, 0, s>>>(A->n_rows, L.n_slices, L.rows, L.koff, L.scol, L.sval64,
+                                        xf, fixed, L.partials, xsplit);
+    CF_LAUNCHED();
+    return fin(kFinStore, L, s, &L.state->floor2, 0.0, GS);
+  };
   if (!two_level) {
     rc = fcg_cheb_core<true, T>(outer, outer_parts, power, cheb, 0.0, kChebF32, n, b, x, fixed,
                                 tol, maxit, L, stats_host, s, nullptr, nullptr, 0, 0.0,
-                                switch_outer);
+                                switch_outer, abs_norm);
     return rc ? rc : switch_rc;
   }
   // two-level: z1 = C r; z2 = z1 + Cheb(r - K z1); z = z2 + C (r - K z2)
@@ -5584,7 +5734,7 @@ int mf_mixed_solve(const char* who, const cf_matrix* A, int64_t n_nodes, int64_t
   };
   rc = fcg_cheb_core<true, T>(outer, outer_parts, power, cheb, 0.0, kCheb2L, n, b, x, fixed,
                               tol, maxit, L, stats_host, s, pre2l, u, 2, power_est,
-                              switch_outer);
+                              switch_outer, abs_norm);
   return rc ? rc : switch_rc;
 }
 

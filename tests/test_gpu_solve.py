@@ -274,6 +274,44 @@ def test_twolevel_jittered_mesh_body_load(gpu):
     assert err < 1e-10
 
 
+@pytest.mark.parametrize("path", ["fp64", "mixed_2l"])
+def test_tol_below_fp64_floor_stops_at_floor(gpu, path):
+    """tol = 1e-15 lies far below the FP64 floor eps || |A_ff| |x_f| || /
+    ||b_f|| of a 40x5x5 cantilever: the solver measures the floor when its
+    true residual misses tol and stops once it is within 2x that floor
+    (floor_limited; no restart is spent below it), and the solution still
+    meets 1e-10 against SuperLU."""
+    from cloudfem_amd import solver
+    from cloudfem_amd.mesh import box_mesh_tet10
+    from cloudfem_amd.physics import elastic_tet10
+    lims = dict(xlim=(0.0, 0.4), ylim=(0.0, 0.05), zlim=(0.0, 0.05))
+    trac = {"xmax": (0.0, 0.0, -1e6)}
+    bc = {"xmin": (0.0, 0.0, 0.0)}
+    m = box_mesh_tet10(40, 5, 5, **lims)
+    opts = (dict(operator="assembled", precision="fp64", precond="jacobi") if path == "fp64"
+            else dict(operator="assembled", precision="mixed", precond="twolevel"))
+    with solver.solver_options(tol=1e-15, **opts):
+        u = np.stack(elastic_tet10.solve_elasticity_tet10(m, 210e9, 0.3, bc=bc, traction=trac),
+                     axis=1).ravel()
+    st = solver.last_stats.value
+    m0 = om.box_mesh_tet10(40, 5, 5, **lims)
+    A0, rhs0 = oel.assemble_elasticity_tet10(m0, 210e9, 0.3, traction=trac)
+    u0 = oel.apply_dirichlet_vec3(A0, rhs0, m0, bc)
+    Aff, bf, free = oel.reduced_system_vec3(A0, rhs0, m0, bc)[:3]
+    floor = fp64_floor(Aff, u0[free], bf)
+    err = rel_err(u, u0)
+    print(f"{path} tol 1e-15: {st['iterations']} iterations, {st['restarts']} restarts, "
+          f"rel. residual {st['rel_residual']:.2e} (floor {floor:.2e}), floor_limited "
+          f"{st['floor_limited']}, error vs SuperLU {err:.2e}")
+    assert st["floor_limited"] == 1
+    assert st["rel_residual"] <= 2.0 * floor * 1.05
+    # a restart is taken only while the true residual is above 2x the floor
+    # (FP64 Jacobi: its recursion drifts over 2,300 iterations, one restart
+    # brings it to 8.8e-11); none is spent below it
+    assert st["restarts"] <= 1
+    assert err < 1e-10
+
+
 @pytest.mark.parametrize("dims", [(40, 5, 5), (80, 6, 6)])
 def test_twolevel_coarse_factor_paths(gpu, monkeypatch, dims):
     """The three factorisations of the coarse matrix (banded Cholesky with
