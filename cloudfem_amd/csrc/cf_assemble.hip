@@ -4,9 +4,10 @@
 // scipy merge the triplets (e.g. elastic_tet10.py:164-173, 208-210).  Here
 // nothing per element is materialised except the 10-double geometry record
 // (inverse Jacobian + volume).  Each stored node pair of the CSR pattern is
-// owned by one thread, which walks its contributor list (the element node
-// pairs that map to it, ascending element order) and recomputes each local
-// entry from the geometry record and constant reference tables.  For affine
+// owned by one thread, which adds its contributors (the element node pairs
+// that map to it, ascending element order); the contributors themselves are
+// computed one per thread, in rounds, from the geometry record and constant
+// reference tables.  For affine
 // elements every local stiffness is a contraction of the element's inverse
 // Jacobian with a reference tensor R_ab = sum_q w_q dN_a/dxi (dN_b/dxi)^T:
 //   Tet10 elasticity   K_ab = V (lam M + mu M^T + mu tr(M) I),  M = Jinv^T R_ab Jinv
@@ -102,21 +103,106 @@ __device__ __forceinline__ void t10_grad(const Geo& g, int q, int a, double out[
 
 // ------------------------------------------------------------- matrices
 
-// One thread per stored entry (node pair) s: its contributors (e, a, b) are
-// contrib[seg[s] .. seg[s+1]) in ascending element order, and the entry is
-// their sum in that order (scipy's coo->csr sums the reference's triplets in
-// element order too).  The Tet10 elasticity reference tensor (900 doubles) is
-// staged in LDS, where the per-lane (a, b) reads cost far less than divergent
-// loads from constant memory.
+// The part of element e's geometry record operator OP reads (mass
+// operators: the volume only).
+template <int OP>
+__device__ __forceinline__ Geo load_geo_for(const double* __restrict__ geo, int64_t e) {
+  if constexpr (OP == CF_OP_P1_MASS || OP == CF_OP_P1_VMASS || OP == CF_OP_T10_MASS) {
+    Geo g{};
+    g.v = geo[e * 10 + 9];
+    return g;
+  } else {
+    return load_geo(geo, e);
+  }
+}
+
+// Value of one contributor pid = e k^2 + a k + b (element e, local pair
+// (a, b)) to its stored entry: a 3x3 block for the vector operators, v[0]
+// for the scalar ones.  g: the geometry record of e, already loaded.
+template <int OP>
+__device__ __forceinline__ void contributor_value(uint32_t pid, int k, const Geo& g,
+                                                  const double* __restrict__ geo,
+                                                  const double* __restrict__ coef, double lam,
+                                                  double mu, const double* __restrict__ Rs,
+                                                  double (&v)[9]) {
+  constexpr bool kVec9 = (OP == CF_OP_T10_ELAST || OP == CF_OP_P1_ELAST);
+  const uint32_t kk = uint32_t(k * k);
+  const int64_t e = pid / kk;
+  const int ab = int(pid - uint32_t(e) * kk);
+  const int a = ab / k, b = ab - (ab / k) * k;
+  (void)geo;
+  if constexpr (OP == CF_OP_P1_MASS || OP == CF_OP_P1_VMASS) {
+    // rho V (1 + d_ab)/20  (tet_heat.py:72-94, modal3d.py:64-66)
+    v[0] = (coef[e] * g.v) * ((a == b ? 2.0 : 1.0) / 20.0);
+  } else if constexpr (OP == CF_OP_T10_MASS) {
+    v[0] = (coef[e] * g.v) * kT10.M[a][b];
+  } else if constexpr (OP == CF_OP_P1_STIFF) {
+    double ga[3], gb[3];
+    p1_grad(g, a, ga);
+    p1_grad(g, b, gb);
+    v[0] = (coef[e] * g.v) * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
+  } else if constexpr (OP == CF_OP_T10_STIFF) {
+    double sum = 0.0;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      double ga[3], gb[3];
+      t10_grad(g, q, a, ga);
+      t10_grad(g, q, b, gb);
+      sum += (kT10.w[q] * coef[e * 5 + q]) * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
+    }
+    v[0] = g.v * sum;
+  } else if constexpr (kVec9) {
+    double M[3][3];
+    if constexpr (OP == CF_OP_T10_ELAST) {
+      // M = Jinv^T R_ab Jinv
+      const double* R = Rs + (a * 10 + b) * 9;
+      double T[3][3];
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+          T[m][i] = R[3 * m] * g.j[0][i] + R[3 * m + 1] * g.j[1][i] + R[3 * m + 2] * g.j[2][i];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj)
+          M[i][jj] = g.j[0][i] * T[0][jj] + g.j[1][i] * T[1][jj] + g.j[2][i] * T[2][jj];
+    } else {
+      double ga[3], gb[3];
+      p1_grad(g, a, ga);
+      p1_grad(g, b, gb);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj) M[i][jj] = ga[i] * gb[jj];
+    }
+    const double lv = lam * g.v, mv = mu * g.v;
+    const double tr = mv * (M[0][0] + M[1][1] + M[2][2]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj)
+        v[3 * i + jj] = lv * M[i][jj] + mv * M[jj][i] + (i == jj ? tr : 0.0);
+  }
+}
+
+// A block owns kBlock consecutive stored entries (node pairs) s0 .. s0 + 255
+// and their contributors contrib[seg[s0] .. seg[s0 + 256]) (element node
+// pairs mapping to them, ascending element order within an entry).  The
+// contributors are computed in rounds of kBlock, one per thread, so every
+// lane works whatever the entries' contributor counts (1 to ~24 for Tet10):
+// a round stages its values in LDS, then each entry thread adds the staged
+// contributors of its own entry, in order.  The sum of an entry is therefore
+// its contributors' sum in ascending element order (scipy's coo->csr sums
+// the reference's triplets in that order too), bitwise reproducible, with no
+// atomics.
 //
 // Vector operators store a 3x3 block per entry at its place in the DOF-level
 // CSR value array: row node i = conn[e][a] of the first contributor (no
 // search over ptr), DOF row 3i+c holding block(i, j)[c][0..2] at 3 (s - ptr[i]).
-// A block of kBlock entries first stages its blocks in LDS and then writes
-// them row segment by row segment, three doubles per entry per DOF row, with
-// consecutive lanes on consecutive addresses (the direct per-thread stores
-// touched each 128-byte line in three 8-byte pieces, and measured 0.92 ms at
-// 15.0 M entries for 1.97 GB of HBM traffic).
+// The block's 3x3 results are staged in LDS and written row segment by row
+// segment, consecutive lanes on consecutive addresses (direct per-thread
+// stores touch each 128-byte line in three 8-byte pieces).
 template <int OP>
 __global__ void __launch_bounds__(kBlock)
 assemble_values(const int32_t* __restrict__ ptr, const int32_t* __restrict__ seg,
@@ -126,118 +212,84 @@ assemble_values(const int32_t* __restrict__ ptr, const int32_t* __restrict__ seg
                 double* __restrict__ out) {
   constexpr bool kRT = (OP == CF_OP_T10_ELAST);
   constexpr bool kVec9 = (OP == CF_OP_T10_ELAST || OP == CF_OP_P1_ELAST);
+  constexpr int NV = kVec9 ? 9 : 1;
   __shared__ double Rs[kRT ? 900 : 1];
-  __shared__ double blk[kVec9 ? kBlock * 9 : 1];
+  __shared__ double stage[kBlock * NV];
+  __shared__ int32_t segl[kBlock + 1];
   __shared__ int64_t rbase[kVec9 ? kBlock : 1];
   __shared__ int32_t rlen[kVec9 ? kBlock : 1];
   if constexpr (kRT) {
     const double* R = &kT10.R[0][0][0][0];
     for (int t = threadIdx.x; t < 900; t += blockDim.x) Rs[t] = R[t];
-    __syncthreads();
   }
-  const int kk = k * k;
   for (int64_t s0 = int64_t(blockIdx.x) * kBlock; s0 < nnz; s0 += int64_t(gridDim.x) * kBlock) {
+    const int64_t m = nnz - s0 < kBlock ? nnz - s0 : kBlock;  // entries of this chunk
     const int64_t s = s0 + threadIdx.x;
-    double acc[3][3] = {};
+    const bool own = threadIdx.x < m;
+    segl[threadIdx.x] = seg[own ? s : s0 + m];
+    if (threadIdx.x == 0) segl[kBlock] = seg[s0 + m];
+    __syncthreads();
+    const int32_t c0 = segl[0], c1 = segl[kBlock];
+    const int32_t jb = segl[threadIdx.x], je = own ? segl[threadIdx.x + 1] : jb;
+    // row of the entry from its first contributor
     int64_t row = 0;
-    const int32_t j0 = s < nnz ? seg[s] : 0, j1 = s < nnz ? seg[s + 1] : 0;
-    for (int32_t j = j0; j < j1; ++j) {
-      const uint32_t pid = contrib[j];
-      const int64_t e = pid / uint32_t(kk);
-      const int ab = int(pid - uint32_t(e) * uint32_t(kk));
-      const int a = ab / k, b = ab - (ab / k) * k;
-      if constexpr (kVec9 || OP == CF_OP_P1_VMASS)
-        if (j == j0) row = conn[e * k + a];
-      if constexpr (OP == CF_OP_P1_MASS || OP == CF_OP_P1_VMASS) {
-        // rho V (1 + d_ab)/20  (tet_heat.py:72-94, modal3d.py:64-66)
-        const double v = geo[e * 10 + 9];
-        acc[0][0] += (coef[e] * v) * ((a == b ? 2.0 : 1.0) / 20.0);
-      } else if constexpr (OP == CF_OP_T10_MASS) {
-        const double v = geo[e * 10 + 9];
-        acc[0][0] += (coef[e] * v) * kT10.M[a][b];
-      } else if constexpr (OP == CF_OP_P1_STIFF) {
-        Geo g = load_geo(geo, e);
-        double ga[3], gb[3];
-        p1_grad(g, a, ga);
-        p1_grad(g, b, gb);
-        acc[0][0] += (coef[e] * g.v) * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
-      } else if constexpr (OP == CF_OP_T10_STIFF) {
-        Geo g = load_geo(geo, e);
-        double sum = 0.0;
-#pragma unroll
-        for (int q = 0; q < 5; ++q) {
-          double ga[3], gb[3];
-          t10_grad(g, q, a, ga);
-          t10_grad(g, q, b, gb);
-          sum += (kT10.w[q] * coef[e * 5 + q]) * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
-        }
-        acc[0][0] += g.v * sum;
-      } else if constexpr (kVec9) {
-        Geo g = load_geo(geo, e);
-        double M[3][3];
-        if constexpr (OP == CF_OP_T10_ELAST) {
-          // M = Jinv^T R_ab Jinv
-          const double* R = Rs + (a * 10 + b) * 9;
-          double T[3][3];
-#pragma unroll
-          for (int m = 0; m < 3; ++m)
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-              T[m][i] = R[3 * m] * g.j[0][i] + R[3 * m + 1] * g.j[1][i] + R[3 * m + 2] * g.j[2][i];
-#pragma unroll
-          for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int jj = 0; jj < 3; ++jj)
-              M[i][jj] = g.j[0][i] * T[0][jj] + g.j[1][i] * T[1][jj] + g.j[2][i] * T[2][jj];
-        } else {
-          double ga[3], gb[3];
-          p1_grad(g, a, ga);
-          p1_grad(g, b, gb);
-#pragma unroll
-          for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int jj = 0; jj < 3; ++jj) M[i][jj] = ga[i] * gb[jj];
-        }
-        const double lv = lam * g.v, mv = mu * g.v;
-        const double tr = mv * (M[0][0] + M[1][1] + M[2][2]);
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-          for (int jj = 0; jj < 3; ++jj)
-            acc[i][jj] += lv * M[i][jj] + mv * M[jj][i] + (i == jj ? tr : 0.0);
+    if constexpr (kVec9 || OP == CF_OP_P1_VMASS) {
+      if (own) {
+        const uint32_t pid = contrib[jb];
+        const uint32_t e = pid / uint32_t(k * k);
+        const int a = int(pid - e * uint32_t(k * k)) / k;
+        row = conn[int64_t(e) * k + a];
       }
     }
-    if constexpr (kVec9) {
-      // stage the block, then write the chunk row segment by row segment
-      if (s < nnz) {
-        const int32_t base = ptr[row];
-        rbase[threadIdx.x] = base;
-        rlen[threadIdx.x] = ptr[row + 1] - base;
+    double acc[NV];
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
+    for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+    for (int32_t base = c0; base < c1; base += kBlock) {
+      const int32_t j = base + int32_t(threadIdx.x);
+      if (j < c1) {
+        const uint32_t pid = contrib[j];
+        const Geo g = load_geo_for<OP>(geo, int64_t(pid / uint32_t(k * k)));
+        double v[9];
+        contributor_value<OP>(pid, k, g, geo, coef, lam, mu, Rs, v);
 #pragma unroll
-          for (int d = 0; d < 3; ++d) blk[threadIdx.x * 9 + c * 3 + d] = acc[c][d];
+        for (int q = 0; q < NV; ++q) stage[q * kBlock + threadIdx.x] = v[q];
       }
       __syncthreads();
-      const int64_t m = nnz - s0 < kBlock ? nnz - s0 : kBlock;
+      const int32_t lo = jb > base ? jb : base;
+      const int32_t hi = je < base + kBlock ? je : base + kBlock;
+      for (int32_t jj = lo; jj < hi; ++jj)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) acc[q] += stage[q * kBlock + (jj - base)];
+      __syncthreads();
+    }
+    if constexpr (kVec9) {
+      // stage the blocks, then write the chunk row segment by row segment
+      if (own) {
+        const int32_t b0 = ptr[row];
+        rbase[threadIdx.x] = b0;
+        rlen[threadIdx.x] = ptr[row + 1] - b0;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) stage[threadIdx.x * 9 + q] = acc[q];
+      }
+      __syncthreads();
 #pragma unroll
       for (int c = 0; c < 3; ++c)
         for (int t = threadIdx.x; t < 3 * m; t += kBlock) {
           const int q = t / 3, d = t - 3 * (t / 3);
-          const int64_t base = rbase[q];
-          out[9 * base + int64_t(c) * 3 * rlen[q] + 3 * (s0 + q - base) + d] = blk[q * 9 + c * 3 + d];
+          const int64_t b0 = rbase[q];
+          out[9 * b0 + int64_t(c) * 3 * rlen[q] + 3 * (s0 + q - b0) + d] = stage[q * 9 + c * 3 + d];
         }
       __syncthreads();
     } else if constexpr (OP == CF_OP_P1_VMASS) {
-      if (s < nnz) {
-        const int64_t base = ptr[row], len = ptr[row + 1] - base, kpos = s - base;
-        double* o = out + 3 * base + kpos;
-        o[0] = acc[0][0];
-        o[len] = acc[0][0];
-        o[2 * len] = acc[0][0];
+      if (own) {
+        const int64_t b0 = ptr[row], len = ptr[row + 1] - b0, kpos = s - b0;
+        double* o = out + 3 * b0 + kpos;
+        o[0] = acc[0];
+        o[len] = acc[0];
+        o[2 * len] = acc[0];
       }
     } else {
-      if (s < nnz) out[s] = acc[0][0];
+      if (own) out[s] = acc[0];
     }
   }
 }
