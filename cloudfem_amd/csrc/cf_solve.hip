@@ -67,6 +67,12 @@ struct PcgState {
   double floor2;  // || |A_ff| |x_f| ||^2 at the last floor check
 };
 
+// three consecutive values of a node (one 12- or 24-byte load)
+template <class T>
+struct Vec3 {
+  T v[3];
+};
+
 // ------------------------------------------------------------- reductions
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -1271,12 +1277,15 @@ mf_gather_runs(int64_t n_nodes, const int32_t* __restrict__ nro, const uint32_t*
   for (int h = 0; h < 2; ++h) {
     const int64_t i = i0 + h * kBlock;
     if (i >= n_nodes) continue;
+    // one 24-byte store per node (three 8-byte stores wrote lines in pieces)
+    Vec3<double> o;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const int64_t d = 3 * i + c;
-      const double o = fixed && fixed[d] ? 0.0 : acc[h][c];
-      y[d] = o;
-      if (state) dot += x[d] * o;
+    for (int c = 0; c < 3; ++c) o.v[c] = fixed && fixed[3 * i + c] ? 0.0 : acc[h][c];
+    *reinterpret_cast<Vec3<double>*>(y + 3 * i) = o;
+    if (state) {
+      const Vec3<double> xv = *reinterpret_cast<const Vec3<double>*>(x + 3 * i);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) dot += xv.v[c] * o.v[c];
     }
   }
   if (!state) return;
@@ -1565,11 +1574,6 @@ __global__ void mf_pack_tiles(int64_t m, const int32_t* __restrict__ conn,
   }
 }
 
-// three consecutive values of a node (one 12- or 24-byte load)
-template <class T>
-struct Vec3 {
-  T v[3];
-};
 
 // PRE: d is already S-scaled (the Chebyshev vectors of the SQ form), so
 // the kernel gathers it as is; otherwise it gathers S d.  Writes the run
@@ -1906,19 +1910,23 @@ mf_gather_cheb(int64_t n_nodes, const int32_t* __restrict__ nro,
       jb[h] = nro[i];
       je[h] = nro[i + 1];
     }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const int64_t k = 3 * i + c;
-      if (mode != 4) sv[h][c] = sc[k];
-      if (mode == 3 || mode == 4) {
-        wv[h][c] = rin[k];
-      } else if (mode != 0) {
-        zv[h][c] = zc[k];
-        wv[h][c] = w[k];
-      }
-      if (mode == 2 || mode == 5) pv[h][c] = zp[k];
-      if (mode == 5) uv[h][c] = zn[k];
+    // one 12- or 24-byte access per node vector (three scalar accesses per
+    // node wrote each line in pieces: twice the bytes of zn reached HBM)
+    auto ld3 = [&](const T* v, T (&o)[3]) {
+      const Vec3<T> t = *reinterpret_cast<const Vec3<T>*>(v + 3 * i);
+      o[0] = t.v[0];
+      o[1] = t.v[1];
+      o[2] = t.v[2];
+    };
+    if (mode != 4) ld3(sc, sv[h]);
+    if (mode == 3 || mode == 4) {
+      ld3(rin, wv[h]);
+    } else if (mode != 0) {
+      ld3(zc, zv[h]);
+      ld3(w, wv[h]);
     }
+    if (mode == 2 || mode == 5) ld3(zp, pv[h]);
+    if (mode == 5) ld3(zn, uv[h]);
   }
   if constexpr (R4) {
 #pragma unroll
@@ -1951,25 +1959,27 @@ mf_gather_cheb(int64_t n_nodes, const int32_t* __restrict__ nro,
   for (int h = 0; h < 2; ++h) {
     const int64_t i = i0 + h * kBlock;
     if (i >= n_nodes) continue;
+    Vec3<T> out, wout;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const int64_t k = 3 * i + c;
       const T s = sv[h][c], o = acc[h][c];
       if (mode == 0) {
-        zn[k] = s * o;
+        out.v[c] = s * o;
       } else if (mode == 3) {
         const T wk = s * s * (wv[h][c] - o);
-        w[k] = wk;
-        zn[k] = cw * wk;
+        wout.v[c] = wk;
+        out.v[c] = cw * wk;
       } else if (mode == 4) {
-        zn[k] = wv[h][c] - o;
+        out.v[c] = wv[h][c] - o;
       } else {
         const T zk = zv[h][c];
         const T dz = mode == 1 ? zk : zk - pv[h][c];
         const T znew = zk + cd * dz + cw * (wv[h][c] - s * s * o);
-        zn[k] = mode == 5 ? uv[h][c] + znew : znew;
+        out.v[c] = mode == 5 ? uv[h][c] + znew : znew;
       }
     }
+    *reinterpret_cast<Vec3<T>*>(zn + 3 * i) = out;
+    if (mode == 3) *reinterpret_cast<Vec3<T>*>(w + 3 * i) = wout;
   }
 }
 
@@ -2230,10 +2240,6 @@ __device__ __forceinline__ int64_t sell_rows(int64_t n, int64_t sl,
       xv[u][0] = ok ? AT(x[c3]) : AT(0);
       xv[u][1] = ok ? AT(x[c3 + 1]) : AT(0);
       xv[u][2] = ok ? AT(x[c3 + 2]) : AT(0);
-      if constexpr (ABS) {
-#pragma unroll
-        for (int q = 0; q < 3; ++q) xv[u][q] = fabs(xv[u][q]);
-      }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -2241,19 +2247,29 @@ __device__ __forceinline__ int64_t sell_rows(int64_t n, int64_t sl,
       const bool ok = g + u < g1 && (!kMask || c[u] >= 0);
 #pragma unroll
       for (int q = 0; q < 9; ++q) v[u][q] = ok ? AT(ld(p + q * 64)) : AT(0);
-      if constexpr (ABS) {
-#pragma unroll
-        for (int q = 0; q < 9; ++q) v[u][q] = fabs(v[u][q]);
-      }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)
       c[u] = g + U + u < g1 ? scol[int64_t(g + U + u) * 64 + lane] : -1;
+    if constexpr (ABS) {
+      // |A| |x|: absolute values at the use (FMA source modifiers), so the
+      // loads keep their batched waits
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      a0 += v[u][0] * xv[u][0] + v[u][1] * xv[u][1] + v[u][2] * xv[u][2];
-      a1 += v[u][3] * xv[u][0] + v[u][4] * xv[u][1] + v[u][5] * xv[u][2];
-      a2 += v[u][6] * xv[u][0] + v[u][7] * xv[u][1] + v[u][8] * xv[u][2];
+      for (int u = 0; u < U; ++u) {
+        a0 += fabs(v[u][0]) * fabs(xv[u][0]) + fabs(v[u][1]) * fabs(xv[u][1]) +
+              fabs(v[u][2]) * fabs(xv[u][2]);
+        a1 += fabs(v[u][3]) * fabs(xv[u][0]) + fabs(v[u][4]) * fabs(xv[u][1]) +
+              fabs(v[u][5]) * fabs(xv[u][2]);
+        a2 += fabs(v[u][6]) * fabs(xv[u][0]) + fabs(v[u][7]) * fabs(xv[u][1]) +
+              fabs(v[u][8]) * fabs(xv[u][2]);
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        a0 += v[u][0] * xv[u][0] + v[u][1] * xv[u][1] + v[u][2] * xv[u][2];
+        a1 += v[u][3] * xv[u][0] + v[u][4] * xv[u][1] + v[u][5] * xv[u][2];
+        a2 += v[u][6] * xv[u][0] + v[u][7] * xv[u][1] + v[u][8] * xv[u][2];
+      }
     }
   }
 #pragma unroll
