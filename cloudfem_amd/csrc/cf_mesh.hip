@@ -19,6 +19,12 @@
 
 #include "cf_common.hpp"
 
+// Device radix sorts run rocPRIM's onesweep algorithm at every size above a
+// single block: its default falls back to a merge sort up to 1 M items (about
+// 20 short launches per sort, against one per 8-bit digit here).
+using SortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                              rocprim::default_config, 0>;
+
 namespace {
 
 constexpr int kBlock = 256;
@@ -171,7 +177,7 @@ MeshWs mesh_layout(int64_t items, void* base) {
   W.flags = c.take<int32_t>(items);
   W.scan = c.take<int32_t>(items);
   size_t a = 0, b = 0;
-  rocprim::radix_sort_pairs(nullptr, a, (uint64_t*)nullptr, (uint64_t*)nullptr,
+  rocprim::radix_sort_pairs<SortConfig>(nullptr, a, (uint64_t*)nullptr, (uint64_t*)nullptr,
                             (uint32_t*)nullptr, (uint32_t*)nullptr, size_t(items), 0, 64);
   rocprim::inclusive_scan(nullptr, b, (int32_t*)nullptr, (int32_t*)nullptr, size_t(items),
                           rocprim::plus<int32_t>());
@@ -210,7 +216,7 @@ extern "C" int cf_tet10_edges(int64_t n_elem, const int32_t* conn4, int32_t stri
   CF_LAUNCHED();
   const uint64_t max_key = uint64_t(n_corners) * uint64_t(n_corners);
   size_t tb = W.tmp_bytes;
-  CF_HIP(rocprim::radix_sort_pairs(W.tmp, tb, W.keys_a, W.keys_b, W.ids_a, W.ids_b, size_t(m6), 0,
+  CF_HIP(rocprim::radix_sort_pairs<SortConfig>(W.tmp, tb, W.keys_a, W.keys_b, W.ids_a, W.ids_b, size_t(m6), 0,
                                    bits_for(max_key), s));
   run_heads<<<g, kBlock, 0, s>>>(m6, W.keys_b, W.flags);
   CF_LAUNCHED();
@@ -257,7 +263,7 @@ extern "C" int cf_boundary_faces(int64_t n_elem, const int32_t* conn, int32_t k,
   face_keys<<<g, kBlock, 0, s>>>(n_elem, conn, k, n_nodes, W.keys_a, W.ids_a);
   CF_LAUNCHED();
   size_t tb = W.tmp_bytes;
-  CF_HIP(rocprim::radix_sort_pairs(W.tmp, tb, W.keys_a, W.keys_b, W.ids_a, W.ids_b, size_t(nf), 0,
+  CF_HIP(rocprim::radix_sort_pairs<SortConfig>(W.tmp, tb, W.keys_a, W.keys_b, W.ids_a, W.ids_b, size_t(nf), 0,
                                    bits_for(uint64_t(n_nodes) * uint64_t(n_nodes) *
                                             uint64_t(n_nodes)),
                                    s));

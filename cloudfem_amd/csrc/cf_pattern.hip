@@ -19,6 +19,12 @@
 
 #include "cf_common.hpp"
 
+// Device radix sorts run rocPRIM's onesweep algorithm at every size above a
+// single block: its default falls back to a merge sort up to 1 M items (about
+// 20 short launches per sort, against one per 8-bit digit here).
+using SortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                              rocprim::default_config, 0>;
+
 namespace {
 
 constexpr int kBlock = 256;
@@ -137,7 +143,7 @@ struct PatternWs {
 
 int pattern_layout(void* base, int64_t n, int end_bit, PatternWs* w) {
   size_t sort_bytes = 0, scan_bytes = 0;
-  CF_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint64_t*)nullptr,
+  CF_HIP(rocprim::radix_sort_pairs<SortConfig>(nullptr, sort_bytes, (uint64_t*)nullptr,
                                    (uint64_t*)nullptr, (uint32_t*)nullptr,
                                    (uint32_t*)nullptr, size_t(n), 0, end_bit));
   CF_HIP(rocprim::inclusive_scan(nullptr, scan_bytes, (int32_t*)nullptr,
@@ -342,7 +348,7 @@ struct RowWs {
 
 int row_layout(cf::Carve& c, int64_t n_items, int64_t n_nodes, RowWs* w) {
   size_t sort_bytes = 0, scan_bytes = 0;
-  CF_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint32_t*)nullptr,
+  CF_HIP(rocprim::radix_sort_pairs<SortConfig>(nullptr, sort_bytes, (uint32_t*)nullptr,
                                    (uint32_t*)nullptr, (uint32_t*)nullptr,
                                    (uint32_t*)nullptr, size_t(n_items), 0,
                                    bits_for(n_nodes + 1)));
@@ -398,7 +404,7 @@ int global_sort(const int32_t* conn, int64_t n, int k, int sh, const PatternWs& 
   pair_keys<<<grid, kBlock, 0, s>>>(conn, n, k, sh, w.keys_in, w.ids_in);
   CF_LAUNCHED();
   size_t tb = w.tmp_bytes;
-  CF_HIP(rocprim::radix_sort_pairs(w.tmp, tb, w.keys_in, w.keys_out, w.ids_in,
+  CF_HIP(rocprim::radix_sort_pairs<SortConfig>(w.tmp, tb, w.keys_in, w.keys_out, w.ids_in,
                                    contrib, size_t(n), 0, 2 * sh, s));
   run_heads<<<grid, kBlock, 0, s>>>(w.keys_out, n, w.head);
   CF_LAUNCHED();
@@ -443,7 +449,7 @@ extern "C" int cf_pattern_sort(const int32_t* conn, int64_t n_elem, int32_t k,
   item_keys<<<igrid, kBlock, 0, s>>>(conn, items, r.keys_in, r.ids_in);
   CF_LAUNCHED();
   size_t tb = r.tmp_bytes;
-  CF_HIP(rocprim::radix_sort_pairs(r.tmp, tb, r.keys_in, r.keys_out, r.ids_in, r.perm,
+  CF_HIP(rocprim::radix_sort_pairs<SortConfig>(r.tmp, tb, r.keys_in, r.keys_out, r.ids_in, r.perm,
                                    size_t(items), 0, bits_for(n_nodes + 1), s));
   incidence_offsets<<<cf::stride_grid(n_nodes + 1, kBlock), kBlock, 0, s>>>(r.keys_out, items,
                                                                            n_nodes, r.offs);
@@ -527,7 +533,7 @@ struct IncidenceWs {
 
 int incidence_layout(void* base, int64_t n, int end_bit, IncidenceWs* w) {
   size_t sort_bytes = 0;
-  CF_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint32_t*)nullptr,
+  CF_HIP(rocprim::radix_sort_pairs<SortConfig>(nullptr, sort_bytes, (uint32_t*)nullptr,
                                    (uint32_t*)nullptr, (uint32_t*)nullptr,
                                    (uint32_t*)nullptr, size_t(n), 0, end_bit));
   cf::Carve c(base);
@@ -571,7 +577,7 @@ extern "C" int cf_incidence_build(const int32_t* targets, int64_t n_items,
     item_keys<<<grid, kBlock, 0, s>>>(targets, n_items, w.keys_in, w.ids_in);
     CF_LAUNCHED();
     size_t tb = w.tmp_bytes;
-    CF_HIP(rocprim::radix_sort_pairs(w.tmp, tb, w.keys_in, w.keys_out, w.ids_in,
+    CF_HIP(rocprim::radix_sort_pairs<SortConfig>(w.tmp, tb, w.keys_in, w.keys_out, w.ids_in,
                                      perm, size_t(n_items), 0, end_bit, s));
   }
   incidence_offsets<<<cf::stride_grid(n_nodes + 1, kBlock), kBlock, 0, s>>>(w.keys_out, n_items,

@@ -42,6 +42,12 @@
 #include "cf_common.hpp"
 #include "cf_tables.hpp"
 
+// Device radix sorts run rocPRIM's onesweep algorithm at every size above a
+// single block: its default falls back to a merge sort up to 1 M items (about
+// 20 short launches per sort, against one per 8-bit digit here).
+using SortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                              rocprim::default_config, 0>;
+
 namespace {
 
 constexpr int kBlock = 256;
@@ -2723,7 +2729,7 @@ void sell_layout(const cf_matrix* A, int precision, int64_t lsum, cf::Carve& c, 
   L.ids_in = c.take<uint32_t>(N);
   size_t sort_bytes = 0, scan_bytes = 0;
   const int end_bit = bits_for((N / kSigma + 1)) + 32;
-  rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint64_t*)nullptr, (uint64_t*)nullptr,
+  rocprim::radix_sort_pairs<SortConfig>(nullptr, sort_bytes, (uint64_t*)nullptr, (uint64_t*)nullptr,
                             (uint32_t*)nullptr, (uint32_t*)nullptr, size_t(N), 0, end_bit);
   rocprim::exclusive_scan(nullptr, scan_bytes, (int32_t*)nullptr, (int32_t*)nullptr, 0,
                           size_t(L.n_slices + 1), rocprim::plus<int32_t>());
@@ -2792,7 +2798,7 @@ int build_sell(const cf_matrix* A, const Layout& L, hipStream_t s, int* product_
   CF_LAUNCHED();
   size_t tb = L.tmp_bytes;
   const int end_bit = bits_for((N / kSigma + 1)) + 32;
-  CF_HIP(rocprim::radix_sort_pairs(L.tmp, tb, L.keys_in, L.keys_out, L.ids_in, L.rows, size_t(N),
+  CF_HIP(rocprim::radix_sort_pairs<SortConfig>(L.tmp, tb, L.keys_in, L.keys_out, L.ids_in, L.rows, size_t(N),
                                    0, end_bit, s));
   CF_HIP(hipMemsetAsync(L.slice_len + L.n_slices, 0, sizeof(int32_t), s));
   sell_slice_len<<<cf::stride_grid(L.n_slices, kBlock), kBlock, 0, s>>>(N, L.n_slices, A->indptr,
@@ -3826,7 +3832,7 @@ void mf_runs_layout(int64_t n_nodes, int64_t n_elem, cf::Carve& c, MfRuns* R) {
   R->ids_in = c.take<uint32_t>(items);
   R->tslot = c.take<uint32_t>(items);
   size_t sort_bytes = 0, scan_bytes = 0;
-  rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
+  rocprim::radix_sort_pairs<SortConfig>(nullptr, sort_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
                             (uint32_t*)nullptr, (uint32_t*)nullptr, size_t(items), 0, 32);
   rocprim::exclusive_scan(nullptr, scan_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, 0u,
                           size_t(items + 1), rocprim::plus<uint32_t>());
@@ -3844,7 +3850,7 @@ int mf_runs_build(int64_t n_nodes, int64_t n_elem, const int32_t* offs, const ui
   mf_store_keys<<<grid, kBlock, 0, s>>>(items, perm, M.keys_in, M.ids_in);
   CF_LAUNCHED();
   size_t tb = M.tmp_bytes;
-  CF_HIP(rocprim::radix_sort_pairs(M.tmp, tb, M.keys_in, M.keys_out, M.ids_in, M.tslot,
+  CF_HIP(rocprim::radix_sort_pairs<SortConfig>(M.tmp, tb, M.keys_in, M.keys_out, M.ids_in, M.tslot,
                                    size_t(items), 0,
                                    bits_for((n_elem + kBlock - 1) / kBlock + 1), s));
   mf_store_rank<<<grid, kBlock, 0, s>>>(items, perm, M.tslot, M.rank);
@@ -3873,8 +3879,8 @@ int mf_runs_build(int64_t n_nodes, int64_t n_elem, const int32_t* offs, const ui
   iota_u32<<<grid, kBlock, 0, s>>>(n_runs, M.ids_in);
   CF_LAUNCHED();
   tb = M.tmp_bytes;
-  CF_HIP(rocprim::radix_sort_pairs(M.tmp, tb, M.rk, M.keys_out, M.ids_in, M.tslot,
-                                   size_t(n_runs), 0, 32, s));
+  CF_HIP(rocprim::radix_sort_pairs<SortConfig>(M.tmp, tb, M.rk, M.keys_out, M.ids_in, M.tslot,
+                                   size_t(n_runs), 0, bits_for(items + 1), s));
   mf_run_tile_order<<<grid, kBlock, 0, s>>>(n_runs, M.keys_out, M.tslot, M.nq, M.tstart);
   CF_LAUNCHED();
   const int64_t n_tiles = (n_elem + kBlock - 1) / kBlock;
@@ -3987,9 +3993,9 @@ MfPlan mf_plan_layout(int64_t n_elem, void* base) {
   P.flags = c.take<int32_t>(P.m10);
   P.scan = c.take<int32_t>(P.m10);
   size_t a = 0, b = 0, d = 0;
-  rocprim::radix_sort_pairs(nullptr, a, (uint64_t*)nullptr, (uint64_t*)nullptr,
+  rocprim::radix_sort_pairs<SortConfig>(nullptr, a, (uint64_t*)nullptr, (uint64_t*)nullptr,
                             (uint32_t*)nullptr, (uint32_t*)nullptr, size_t(P.m10), 0, 64);
-  rocprim::radix_sort_pairs(nullptr, b, (uint32_t*)nullptr, (uint32_t*)nullptr,
+  rocprim::radix_sort_pairs<SortConfig>(nullptr, b, (uint32_t*)nullptr, (uint32_t*)nullptr,
                             (uint32_t*)nullptr, (uint32_t*)nullptr, size_t(n_elem), 0, 30);
   rocprim::inclusive_scan(nullptr, d, (int32_t*)nullptr, (int32_t*)nullptr, size_t(P.m10),
                           rocprim::plus<int32_t>());
@@ -4017,7 +4023,7 @@ int mf_plan_build(const MfArgs& A, const double* x, const double* y, const doubl
   uint32_t* mkeys = reinterpret_cast<uint32_t*>(P.keys_a);
   uint32_t* morder = reinterpret_cast<uint32_t*>(P.keys_b);
   size_t tb = P.tmp_bytes;
-  CF_HIP(rocprim::radix_sort_pairs(P.tmp, tb, P.vals_a, mkeys, P.vals_b, morder, size_t(m), 0, 30,
+  CF_HIP(rocprim::radix_sort_pairs<SortConfig>(P.tmp, tb, P.vals_a, mkeys, P.vals_b, morder, size_t(m), 0, 30,
                                    s));
   mfp_permute<<<g10, kBlock, 0, s>>>(m, morder, A.conn, A.geo, P.conn_c, P.geo_c);
   CF_LAUNCHED();
@@ -4027,7 +4033,7 @@ int mf_plan_build(const MfArgs& A, const double* x, const double* y, const doubl
   int chunk_bits = 1;
   while ((int64_t(1) << chunk_bits) < P.n_chunks + 1) ++chunk_bits;
   tb = P.tmp_bytes;
-  CF_HIP(rocprim::radix_sort_pairs(P.tmp, tb, P.keys_a, P.keys_b, P.vals_a, P.vals_b,
+  CF_HIP(rocprim::radix_sort_pairs<SortConfig>(P.tmp, tb, P.keys_a, P.keys_b, P.vals_a, P.vals_b,
                                    size_t(m10), 0, 32 + chunk_bits, s));
   mfp_heads<<<g10, kBlock, 0, s>>>(m10, P.keys_b, nullptr, P.flags);
   CF_LAUNCHED();
@@ -4042,7 +4048,7 @@ int mf_plan_build(const MfArgs& A, const double* x, const double* y, const doubl
                                          A.offs, P.keys_a, P.vals_a, P.pair_out);
   CF_LAUNCHED();
   tb = P.tmp_bytes;
-  CF_HIP(rocprim::radix_sort_pairs(P.tmp, tb, P.keys_a, P.keys_b, P.vals_a, P.vals_b,
+  CF_HIP(rocprim::radix_sort_pairs<SortConfig>(P.tmp, tb, P.keys_a, P.keys_b, P.vals_a, P.vals_b,
                                    size_t(m10), 0, 64, s));
   mfp_count_shared<<<g10, kBlock, 0, s>>>(m10, P.keys_b, P.counts);
   CF_LAUNCHED();
@@ -5112,13 +5118,13 @@ void coarse_layout(int64_t n_nodes, int64_t n_elem, cf::Carve& c, Coarse* C, int
   C->info = c.take<int32_t>(2);
   C->dep = c.take<int32_t>(ncmax);
   size_t b1 = 0, b2 = 0, b3 = 0, b4 = 0;
-  rocprim::radix_sort_pairs(nullptr, b1, (uint32_t*)nullptr, (uint32_t*)nullptr,
+  rocprim::radix_sort_pairs<SortConfig>(nullptr, b1, (uint32_t*)nullptr, (uint32_t*)nullptr,
                             (uint32_t*)nullptr, (uint32_t*)nullptr, size_t(n_nodes), 0, 32);
   rocprim::exclusive_scan(nullptr, b2, (uint32_t*)nullptr, (uint32_t*)nullptr, 0u,
                           size_t(G + 1), rocprim::plus<uint32_t>());
   rocprim::exclusive_scan(nullptr, b3, (uint32_t*)nullptr, (uint32_t*)nullptr, 0u,
                           size_t(n_elem + 1), rocprim::plus<uint32_t>());
-  rocprim::radix_sort_keys(nullptr, b4, (uint64_t*)nullptr, (uint64_t*)nullptr, size_t(maxp), 0,
+  rocprim::radix_sort_keys<SortConfig>(nullptr, b4, (uint64_t*)nullptr, (uint64_t*)nullptr, size_t(maxp), 0,
                            64);
   size_t b5 = 0;
   rocprim::unique(nullptr, b5, (uint64_t*)nullptr, (uint64_t*)nullptr, (uint32_t*)nullptr,
@@ -5184,7 +5190,7 @@ int coarse_setup(const MfArgs& a, const MfRuns& R, const double* xs, const doubl
       CF_HIP(hipStreamSynchronize(s));
       CF_REQUIRE(n_agg > 0 && int64_t(n_agg) <= G, "two-level: bad aggregate count %u", n_agg);
       tb = C->tmp_bytes;
-      CF_HIP(rocprim::radix_sort_pairs(C->tmp, tb, C->keys_in, C->keys_out, C->ids_in,
+      CF_HIP(rocprim::radix_sort_pairs<SortConfig>(C->tmp, tb, C->keys_in, C->keys_out, C->ids_in,
                                        C->agg_nodes, size_t(n_nodes), 0,
                                        bits_for(int64_t(n_agg) + 1), s));
       agg_offsets<<<cf::stride_grid(n_agg + 1, kBlock), kBlock, 0, s>>>(n_nodes, n_agg,
@@ -5212,7 +5218,7 @@ int coarse_setup(const MfArgs& a, const MfRuns& R, const double* xs, const doubl
                                       C->keys_in, C->ids_in);
       CF_LAUNCHED();
       tb = C->tmp_bytes;
-      CF_HIP(rocprim::radix_sort_pairs(C->tmp, tb, C->keys_in, C->keys_out, C->ids_in,
+      CF_HIP(rocprim::radix_sort_pairs<SortConfig>(C->tmp, tb, C->keys_in, C->keys_out, C->ids_in,
                                        C->agg_nodes, size_t(n_nodes), 0, 32, s));
       slab_count<<<1, 64, 0, s>>>(n_nodes, C->keys_out, C->used);
       CF_LAUNCHED();
@@ -5249,7 +5255,7 @@ int coarse_setup(const MfArgs& a, const MfRuns& R, const double* xs, const doubl
       agg_pair_fill<<<ge, kBlock, 0, s>>>(n_elem, a.conn, C->agg, C->poff, n_agg, C->pkeys_in);
       CF_LAUNCHED();
       tb = C->tmp_bytes;
-      CF_HIP(rocprim::radix_sort_keys(C->tmp, tb, C->pkeys_in, C->pkeys_out, size_t(n_pairs), 0,
+      CF_HIP(rocprim::radix_sort_keys<SortConfig>(C->tmp, tb, C->pkeys_in, C->pkeys_out, size_t(n_pairs), 0,
                                       bits_for(int64_t(n_agg) * n_agg + 1), s));
       // distinct pairs only (about 3 per aggregate instead of one per element
       // on a slab boundary) cross to the host
@@ -5573,10 +5579,15 @@ int mf_mixed_solve(const char* who, const cf_matrix* A, int64_t n_nodes, int64_t
       n_elem, conn, geo, M.R.rank, M.connT, geoL, M.rankT);
   CF_LAUNCHED();
   // the sliced product's grid is read back when the outer products switch to
-  // A (a synchronisation point anyway), not here
+  // A (a synchronisation point anyway), not here.  Two-level solves build
+  // the sliced copy on the side stream, after the power iteration and beside
+  // the coarse set-up: the outer products switch to A only many iterations
+  // later, and the coarse set-up leaves most of the GPU idle.
   int GS = 0;
-  if (A && ((rc = build_sell(A, L, s, nullptr)) ||
-            (rc = fill_sell<double>(A, L, nullptr, L.sval64, s))))
+  const bool sell_on_side = two_level && A;
+  if (A && !sell_on_side &&
+      ((rc = build_sell(A, L, s, nullptr)) ||
+       (rc = fill_sell<double>(A, L, nullptr, L.sval64, s))))
     return rc;
   const int ge = cf::stride_grid(n_elem, kBlock);
   const int gg = int(mf_gather_grid(n_nodes));
@@ -5641,9 +5652,10 @@ int mf_mixed_solve(const char* who, const cf_matrix* A, int64_t n_nodes, int64_t
   auto cheb = [&](int k, T cdk, T cwk, const T* din, T* dout) {
     return product(din, dout, k == 1 ? 1 : 2, cdk, cwk, L.state);
   };
-  // Two-level: the power iteration runs on a side stream while this stream
-  // builds, factors and inverts the coarse matrix (whose host colouring and
-  // one-workgroup factorisation leave most of the GPU idle).
+  // Two-level: the power iteration (and, with A, the sliced copy of A) runs
+  // on a side stream while this stream builds, factors and inverts the
+  // coarse matrix (whose host colouring and one-workgroup factorisation leave
+  // most of the GPU idle).
   double power_est = 0.0;
   // two-level solves on one device run one at a time (shared side stream,
   // events and rocBLAS handle); the lock is held until the solve returns
@@ -5670,6 +5682,8 @@ int mf_mixed_solve(const char* who, const cf_matrix* A, int64_t n_nodes, int64_t
     // its ~200 launches go out from a second host thread
     HostTask launches([&]() {
       int r = power_launch<T>(power_side, n, L, sp);
+      if (r == CF_OK && sell_on_side) r = build_sell(A, L, sp, nullptr);
+      if (r == CF_OK && sell_on_side) r = fill_sell<double>(A, L, nullptr, L.sval64, sp);
       if (side && hipEventRecord(side->done, sp) != hipSuccess && r == CF_OK) {
         cf::set_error("two-level: hipEventRecord on the side stream failed");
         r = CF_ERR_HIP;
