@@ -103,6 +103,14 @@ __device__ __forceinline__ void t10_grad(const Geo& g, int q, int a, double out[
 
 // ------------------------------------------------------------- matrices
 
+// Nodes per element of operator OP (a compile-time divisor for the
+// contributor decode).
+template <int OP>
+constexpr int op_nodes() {
+  return (OP == CF_OP_P1_STIFF || OP == CF_OP_P1_MASS || OP == CF_OP_P1_ELAST ||
+          OP == CF_OP_P1_VMASS) ? 4 : 10;
+}
+
 // The part of element e's geometry record operator OP reads (mass
 // operators: the volume only).
 template <int OP>
@@ -126,10 +134,12 @@ __device__ __forceinline__ void contributor_value(uint32_t pid, int k, const Geo
                                                   double mu, const double* __restrict__ Rs,
                                                   double (&v)[9]) {
   constexpr bool kVec9 = (OP == CF_OP_T10_ELAST || OP == CF_OP_P1_ELAST);
-  const uint32_t kk = uint32_t(k * k);
+  constexpr int K = op_nodes<OP>();
+  constexpr uint32_t kk = uint32_t(K * K);
+  (void)k;
   const int64_t e = pid / kk;
   const int ab = int(pid - uint32_t(e) * kk);
-  const int a = ab / k, b = ab - (ab / k) * k;
+  const int a = ab / K, b = ab - (ab / K) * K;
   (void)geo;
   if constexpr (OP == CF_OP_P1_MASS || OP == CF_OP_P1_VMASS) {
     // rho V (1 + d_ab)/20  (tet_heat.py:72-94, modal3d.py:64-66)
@@ -213,8 +223,11 @@ assemble_values(const int32_t* __restrict__ ptr, const int32_t* __restrict__ seg
   constexpr bool kRT = (OP == CF_OP_T10_ELAST);
   constexpr bool kVec9 = (OP == CF_OP_T10_ELAST || OP == CF_OP_P1_ELAST);
   constexpr int NV = kVec9 ? 9 : 1;
+  // staged contributor values, contributor-major, padded to 10 doubles (80 B)
+  // so that one contributor is five aligned 16-byte LDS reads
+  constexpr int NS = kVec9 ? 10 : 1;
   __shared__ double Rs[kRT ? 900 : 1];
-  __shared__ double stage[kBlock * NV];
+  __shared__ __align__(16) double stage[kBlock * NS];
   __shared__ int32_t segl[kBlock + 1];
   __shared__ int64_t rbase[kVec9 ? kBlock : 1];
   __shared__ int32_t rlen[kVec9 ? kBlock : 1];
@@ -236,9 +249,10 @@ assemble_values(const int32_t* __restrict__ ptr, const int32_t* __restrict__ seg
     if constexpr (kVec9 || OP == CF_OP_P1_VMASS) {
       if (own) {
         const uint32_t pid = contrib[jb];
-        const uint32_t e = pid / uint32_t(k * k);
-        const int a = int(pid - e * uint32_t(k * k)) / k;
-        row = conn[int64_t(e) * k + a];
+        constexpr int K = op_nodes<OP>();
+        const uint32_t e = pid / uint32_t(K * K);
+        const int a = int(pid - e * uint32_t(K * K)) / K;
+        row = conn[int64_t(e) * K + a];
       }
     }
     double acc[NV];
@@ -248,18 +262,35 @@ assemble_values(const int32_t* __restrict__ ptr, const int32_t* __restrict__ seg
       const int32_t j = base + int32_t(threadIdx.x);
       if (j < c1) {
         const uint32_t pid = contrib[j];
-        const Geo g = load_geo_for<OP>(geo, int64_t(pid / uint32_t(k * k)));
+        const Geo g = load_geo_for<OP>(geo, int64_t(pid / uint32_t(op_nodes<OP>() * op_nodes<OP>())));
         double v[9];
         contributor_value<OP>(pid, k, g, geo, coef, lam, mu, Rs, v);
+        if constexpr (kVec9) {
+          double2* o = reinterpret_cast<double2*>(stage + threadIdx.x * NS);
 #pragma unroll
-        for (int q = 0; q < NV; ++q) stage[q * kBlock + threadIdx.x] = v[q];
+          for (int q = 0; q < 4; ++q) o[q] = make_double2(v[2 * q], v[2 * q + 1]);
+          o[4] = make_double2(v[8], 0.0);
+        } else {
+          stage[threadIdx.x] = v[0];
+        }
       }
       __syncthreads();
       const int32_t lo = jb > base ? jb : base;
       const int32_t hi = je < base + kBlock ? je : base + kBlock;
-      for (int32_t jj = lo; jj < hi; ++jj)
+      for (int32_t jj = lo; jj < hi; ++jj) {
+        if constexpr (kVec9) {
+          const double2* o = reinterpret_cast<const double2*>(stage + (jj - base) * NS);
 #pragma unroll
-        for (int q = 0; q < NV; ++q) acc[q] += stage[q * kBlock + (jj - base)];
+          for (int q = 0; q < 4; ++q) {
+            const double2 t = o[q];
+            acc[2 * q] += t.x;
+            acc[2 * q + 1] += t.y;
+          }
+          acc[8] += o[4].x;
+        } else {
+          acc[0] += stage[jj - base];
+        }
+      }
       __syncthreads();
     }
     if constexpr (kVec9) {
