@@ -100,11 +100,18 @@ class DeviceMesh:
         self.k = int(els.shape[1])
         if self.k not in (4, 10):
             raise ValueError(f"elements must have 4 or 10 nodes, got {self.k}")
-        if self.n_elem and (els.min() < 0 or els.max() >= self.n_nodes):
-            raise IndexError("element node index out of range")
+        if self.n_nodes >= 2 ** 31 or (els.dtype.kind != "i" and els.dtype.kind != "u"):
+            raise ValueError("element node ids must be integers below 2**31")
         xyz = to_dev(nodes.T.copy(), torch.float64, self.dev)
         self.x, self.y, self.z = xyz[0], xyz[1], xyz[2]
-        self.conn = to_dev(els.astype(np.int32), torch.int32, self.dev)
+        if self.n_elem and els.dtype.itemsize > 4 and (els.min() < 0 or
+                                                       els.max() >= self.n_nodes):
+            raise IndexError("element node index out of range")
+        self.conn = to_dev(np.ascontiguousarray(els, dtype=np.int32), torch.int32, self.dev)
+        # int32 ids (no host pass above): checked on the device before any kernel uses them
+        if self.n_elem and els.dtype.itemsize <= 4 and bool(
+                ((self.conn < 0) | (self.conn >= self.n_nodes)).any()):
+            raise IndexError("element node index out of range")
         self._geo = None
         self._pattern = None
         self._incidence = None

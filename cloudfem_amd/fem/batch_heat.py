@@ -61,11 +61,27 @@ def _element_values(v, mesh):
 
 
 def _union(meshes):
+    """One mesh of all meshes (node ids offset per mesh), int32 connectivity
+    written in place (the device layer's type)."""
     nodes = np.concatenate([np.asarray(m.nodes, dtype=float) for m in meshes])
     offs = np.concatenate([[0], np.cumsum([m.n_nodes for m in meshes])]).astype(np.int64)
-    elems = np.concatenate([np.asarray(m.elements)[:, :4] + offs[i]
-                            for i, m in enumerate(meshes)])
-    return TetMesh(nodes, elems), offs
+    if offs[-1] >= 2 ** 31:
+        raise ValueError(f"transient_heat_batch: {offs[-1]} nodes exceed int32 node ids")
+    elems = np.empty((sum(m.n_elements for m in meshes), 4), dtype=np.int32)
+    e0 = 0
+    for i, m in enumerate(meshes):
+        e = np.asarray(m.elements)[:, :4]
+        np.add(e, offs[i], out=elems[e0:e0 + e.shape[0]], casting="unsafe")
+        e0 += e.shape[0]
+    return TetMesh.wrap(nodes, elems), offs
+
+
+def _needs_points(v):
+    """Whether a coefficient spec (float, callable, or a tuple of them) is
+    evaluated at points; constants only need the face count."""
+    if isinstance(v, (list, tuple)):
+        return any(callable(x) for x in v)
+    return callable(v)
 
 
 def _union_faces(um, dm, meshes, offs, names):
@@ -88,7 +104,13 @@ def _faces_and_coef(meshes, offs, specs, pick, union=None):
         for name, v in (specs[0] or {}).items():
             fc = union[name]
             if fc.shape[0]:
-                cen = union["_nodes"][fc].mean(axis=1)
+                if _needs_points(v):
+                    key = ("_centroids", name)
+                    if key not in union:   # once per face set, shared by the callers
+                        union[key] = union["_nodes"][fc].mean(axis=1)
+                    cen = union[key]
+                else:
+                    cen = np.broadcast_to(0.0, (fc.shape[0], 3))
                 faces.append(fc)
                 vals.append(pick(v, cen))
         if not faces:

@@ -26,6 +26,17 @@ class TetMesh:
         self.boundaries = dict(boundaries) if boundaries else {}
         self._volumes = None
 
+    @classmethod
+    def wrap(cls, nodes, elements, boundaries=None):
+        """A mesh over existing arrays, without dtype conversion or copies
+        (e.g. the int32 connectivity of a union of meshes)."""
+        self = cls.__new__(cls)
+        self.nodes = nodes
+        self.elements = elements
+        self.boundaries = dict(boundaries) if boundaries else {}
+        self._volumes = None
+        return self
+
     @property
     def n_nodes(self):
         return self.nodes.shape[0]
