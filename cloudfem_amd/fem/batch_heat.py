@@ -29,7 +29,7 @@ from .. import _native as N
 from ..autotune import batch_spmv_config
 from ..coef import centroids, eval_scalar
 from ..device import DeviceMesh, ptr, stream_handle, to_dev, workspace
-from ..mesh.device_ops import boundary_faces_device
+from ..mesh.device_ops import boundary_faces_multi_device
 from ..mesh.faces import boundary_faces
 from ..mesh.tet_mesh import TetMesh, box_mesh
 from . import _transient
@@ -85,16 +85,14 @@ def _needs_points(v):
 
 
 def _union_faces(um, dm, meshes, offs, names):
-    """{name: (F, 3) union faces} by one device scan per name over the union
-    connectivity (disjoint meshes: the union's exterior faces are the meshes'
-    exterior faces, in mesh order then occurrence order)."""
-    out = {}
+    """{name: (F, 3) union faces} from one device scan of the union's
+    exterior faces (disjoint meshes: the union's exterior faces are the
+    meshes' exterior faces, in mesh order then occurrence order)."""
     for name in names:
         um.boundaries[name] = np.concatenate(
             [np.asarray(m.boundaries[name], dtype=np.int64) + offs[i]
              for i, m in enumerate(meshes)])
-        out[name] = boundary_faces_device(um, name, conn=dm.conn)
-    return out
+    return boundary_faces_multi_device(um, names, conn=dm.conn)
 
 
 def _faces_and_coef(meshes, offs, specs, pick, union=None):

@@ -73,6 +73,31 @@ def _faces(mesh, name, width, conn=None):
     return faces[:int(cnt.item())].cpu().numpy().astype(int)
 
 
+def boundary_faces_multi_device(mesh, names, conn=None):
+    """{name: (F, 3) exterior P1 faces of boundary `name`} for several names
+    from one device scan: a name's faces are the exterior faces whose three
+    nodes all lie on it, in the scan's order (the subsequence a scan with
+    that name's node set returns)."""
+    dev = require_gpu()
+    if conn is None:
+        conn = to_dev(np.asarray(mesh.elements).astype(np.int32), torch.int32, dev)
+    m, k = conn.shape
+    faces = torch.empty((4 * m, 3), dtype=torch.int32, device=dev)
+    cnt = torch.zeros(1, dtype=torch.int64, device=dev)
+    ws, nb = _ws(m, dev)
+    N.check(N.load().cf_boundary_faces(m, ptr(conn), k, mesh.n_nodes, None, 3, ptr(faces),
+                                       ptr(cnt), ptr(ws), nb, stream_handle()),
+            "cf_boundary_faces")
+    ext = faces[:int(cnt.item())].long()
+    out = {}
+    for name in names:
+        member = torch.zeros(mesh.n_nodes, dtype=torch.bool, device=dev)
+        member[to_dev(np.asarray(mesh.boundaries[name], dtype=np.int64), torch.int64, dev)] = True
+        keep = member[ext[:, 0]] & member[ext[:, 1]] & member[ext[:, 2]]
+        out[name] = ext[keep].cpu().numpy()
+    return out
+
+
 def boundary_faces_device(mesh, name, conn=None):
     """(F, 3) exterior P1 faces of boundary `name` (KeyError if unknown);
     conn: the connectivity already on the device (int32), if at hand."""

@@ -50,6 +50,27 @@ def test_boundary_faces_match_host(gpu):
         assert np.array_equal(boundary_faces10_device(m10, name), boundary_faces10(m10, name))
 
 
+def test_multi_name_face_scan_matches_host(gpu):
+    """One device scan for several boundary names (the batched solver's
+    union mesh) gives each name's host face list, in order."""
+    from cloudfem_amd.fem.batch_heat import _union
+    from cloudfem_amd.mesh.device_ops import boundary_faces_multi_device
+    from cloudfem_amd.mesh.faces import boundary_faces
+    for m in _meshes():
+        got = boundary_faces_multi_device(m, FACES)
+        for name in FACES:
+            assert np.array_equal(got[name], boundary_faces(m, name)), name
+    ms = _meshes()
+    um, offs = _union(ms)
+    for name in FACES:
+        um.boundaries[name] = np.concatenate(
+            [np.asarray(m.boundaries[name]) + offs[i] for i, m in enumerate(ms)])
+    got = boundary_faces_multi_device(um, FACES)
+    for name in FACES:
+        want = np.concatenate([boundary_faces(m, name) + offs[i] for i, m in enumerate(ms)])
+        assert np.array_equal(got[name], want), name
+
+
 def test_sphere_fixture(gpu):
     """The unstructured sphere fixture (Delaunay + smoothing, irregular
     connectivity): Tet10 numbering and exterior faces as on the host."""
