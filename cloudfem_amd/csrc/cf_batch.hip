@@ -10,10 +10,13 @@
 // Work is cut into tiles of consecutive rows that never straddle two
 // systems; one workgroup owns one tile and writes one partial per dot
 // product, and a fold kernel (one workgroup per system) sums the tiles of
-// its system in a fixed order, so results are deterministic.  A whole time
-// step (right-hand side + K CG iterations) is captured once as a hipGraph and
-// replayed per step; the host reads the per-system flags after each replay
-// and replays a shorter iteration graph for the rare system that needs more.
+// its system in a fixed order, so results are deterministic.  Each step's
+// solve starts from the linear extrapolation 2 T_n - T_{n-1} of the last two
+// states.  A whole time step (right-hand side + K CG iterations, K from the
+// second step, the first that starts from an extrapolated state) is captured
+// once as a hipGraph and replayed per step; the host reads the per-system
+// flags after each replay and replays a shorter iteration graph for the rare
+// system that needs more.
 //
 // Precision: FP64 CG, or FP32 CG on the Jacobi-scaled matrix S A S inside
 // FP64 iterative refinement (residual r = b - A x and the solution in FP64;
@@ -335,6 +338,19 @@ __global__ void bt_copy(int64_t n, const double* __restrict__ x, double* __restr
     y[i] = x[i];
 }
 
+// Start of a step's solve: the linear extrapolation 2 T_n - T_{n-1} of the
+// last two states (T holds T_n, Tprev T_{n-1}; Tprev = T_0 before the first
+// step, which then starts from T_0); Tprev becomes T_n.  The step's
+// right-hand side R T_n + h F is formed before this.
+__global__ void bt_extrapolate(int64_t n, double* __restrict__ T, double* __restrict__ Tprev) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const double t = T[i];
+    T[i] = 2.0 * t - Tprev[i];
+    Tprev[i] = t;
+  }
+}
+
 // dinv = 1 / A_ii (Jacobi), sc = 1 / sqrt(A_ii); bad counts non-positive diagonals
 __global__ void bt_diag(int64_t n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ idx,
                         const double* __restrict__ val, double* __restrict__ dinv,
@@ -375,7 +391,7 @@ constexpr int kLaneChoices[3] = {4, 8, 16};
 constexpr int kTileMult[3] = {1, 2, 4};  // rows per tile = kBlock / L * mult
 
 struct BatchLayout {
-  double *b, *r, *p, *Ap, *dinv, *sc;
+  double *b, *r, *p, *Ap, *dinv, *sc, *Tprev;
   float *rh, *ph, *Aph, *dh, *vals32;
   double* partials;
   SysState* st;
@@ -405,6 +421,7 @@ BatchLayout batch_layout(int32_t n_sys, const int32_t* rows_host, int64_t nnz, i
   L.Ap = c.take<double>(n);
   L.dinv = c.take<double>(n);
   L.sc = c.take<double>(n);
+  L.Tprev = c.take<double>(n);
   if (precision != CF_PREC_FP64) {
     L.rh = c.take<float>(n);
     L.ph = c.take<float>(n);
@@ -510,10 +527,20 @@ int enqueue_cg_iterations(int k, int lanes, const Tiles& t, const Problem& P,
   return CF_OK;
 }
 
-int enqueue_fp64_step(int k, int lanes, const Tiles& t, const Problem& P, const BatchLayout& L,
-                      hipStream_t s) {
+// b = R T_n + h F, then the extrapolated start (bt_extrapolate)
+int enqueue_rhs_and_start(int lanes, const Tiles& t, const Problem& P, const BatchLayout& L,
+                          int64_t n, hipStream_t s) {
   int rc;
-  if ((rc = launch_spmv<double, double, kRhs, 0>(lanes, t, P, P.R, P.T, L.b, L, s)) ||
+  if ((rc = launch_spmv<double, double, kRhs, 0>(lanes, t, P, P.R, P.T, L.b, L, s))) return rc;
+  bt_extrapolate<<<cf::stride_grid(n, kBlock), kBlock, 0, s>>>(n, P.T, L.Tprev);
+  CF_LAUNCHED();
+  return CF_OK;
+}
+
+int enqueue_fp64_step(int k, int lanes, const Tiles& t, const Problem& P, const BatchLayout& L,
+                      int64_t n, hipStream_t s) {
+  int rc;
+  if ((rc = enqueue_rhs_and_start(lanes, t, P, L, n, s)) ||
       (rc = launch_spmv<double, double, kPlain, 0>(lanes, t, P, P.A, P.T, L.Ap, L, s)))
     return rc;
   bt_start<<<t.n, kBlock, 0, s>>>(t, L.b, L.Ap, L.dinv, L.r, L.p, L.partials);
@@ -550,9 +577,9 @@ int enqueue_sweep(int k_in, int lanes, const Tiles& t, const Problem& P, const B
 }
 
 int enqueue_mixed_step(int sweeps, int k_in, int lanes, const Tiles& t, const Problem& P,
-                       const BatchLayout& L, hipStream_t s) {
+                       const BatchLayout& L, int64_t n, hipStream_t s) {
   int rc;
-  if ((rc = launch_spmv<double, double, kRhs, 0>(lanes, t, P, P.R, P.T, L.b, L, s)) ||
+  if ((rc = enqueue_rhs_and_start(lanes, t, P, L, n, s)) ||
       (rc = launch_spmv<double, double, kPlain, 0>(lanes, t, P, P.A, P.T, L.Ap, L, s)))
     return rc;
   bt_residual<<<t.n, kBlock, 0, s>>>(t, L.b, L.Ap, L.r, L.st, 0, L.partials);
@@ -764,8 +791,8 @@ extern "C" int cf_batch_theta_run(int32_t n_sys, const int32_t* sys_rows, const 
 
   std::vector<SysState> hs;
   const bool mixed = precision == CF_PREC_MIXED;
-  // Size the step graph from the first step: run it eagerly with small
-  // iteration graphs until every system has converged.
+  // iteration graphs for the eager steps and for systems the step graph
+  // leaves unconverged
   const int kProbe = mixed ? 4 : 8;
   const int kInner = 24;   // FP32 iterations per sweep (inner stop at 1e-5 usually first)
   Graph probe;
@@ -775,41 +802,53 @@ extern "C" int cf_batch_theta_run(int32_t n_sys, const int32_t* sys_rows, const 
     rc = capture(cs, &probe,
                  [&](hipStream_t q) { return enqueue_cg_iterations(kProbe, lanes, t, P, L, q); });
   if (rc) return rc;
-  // first step, eagerly
-  if (mixed)
-    rc = enqueue_mixed_step(0, kInner, lanes, t, P, L, cs);
-  else
-    rc = enqueue_fp64_step(0, lanes, t, P, L, cs);
-  if (rc) return rc;
-  int first_rounds = 0;
-  for (;;) {
-    if ((rc = read_flags(L, n_sys, &hs, cs))) return rc;
-    bool running = false;
-    for (const auto& x : hs) running |= (x.halt[0] == kRun);
-    if (!running) break;
-    CF_HIP(hipGraphLaunch(probe.e, cs));
-    ++first_rounds;
-    ++stats->graph_replays;
-  }
+  // T_{-1} = T_0: the first step starts from T_0
+  bt_copy<<<cf::stride_grid(n, kBlock), kBlock, 0, cs>>>(n, T, L.Tprev);
+  CF_LAUNCHED();
+  // The first two steps run eagerly, with small iteration graphs until every
+  // system has converged; the step graph is sized from the second, the first
+  // that starts from an extrapolated state (the first step starts from T_0
+  // and usually needs more iterations than the later ones).
+  const int n_eager = n_steps < 2 ? n_steps : 2;
   int worst = 0, worst_inner = 0;
-  for (const auto& x : hs) {
-    worst = std::max(worst, x.iters[0]);
-    worst_inner = std::max(worst_inner, x.worst_inner);
+  for (int32_t k = 0; k < n_eager; ++k) {
+    if (mixed)
+      rc = enqueue_mixed_step(0, kInner, lanes, t, P, L, n, cs);
+    else
+      rc = enqueue_fp64_step(0, lanes, t, P, L, n, cs);
+    if (rc) return rc;
+    for (;;) {
+      if ((rc = read_flags(L, n_sys, &hs, cs))) return rc;
+      bool running = false;
+      for (const auto& x : hs) running |= (x.halt[0] == kRun);
+      if (!running) break;
+      CF_HIP(hipGraphLaunch(probe.e, cs));
+      ++stats->graph_replays;
+    }
+    worst = worst_inner = 0;
+    for (const auto& x : hs) {
+      worst = std::max(worst, x.iters[0]);
+      worst_inner = std::max(worst_inner, x.worst_inner);
+    }
+    if ((rc = post_step(hist ? hist + (k + 1) * n : nullptr, n, t, P, L, cs))) return rc;
   }
-  // FP32 iterations per sweep in the step graph: the first step's worst + 2
+  // FP32 iterations per sweep in the step graph: the worst so far + 2
   const int k_in = std::min(kInner, worst_inner + 2);
-  if ((rc = post_step(hist ? hist + n : nullptr, n, t, P, L, cs))) return rc;
-  // step graph: rhs + enough iterations for the first step's worst system
+  // step graph: rhs + enough iterations for the second step's worst system
   const int k_step = mixed ? std::max(1, worst) : worst + 2;
   Graph step;
-  if (mixed)
-    rc = capture(cs, &step,
-                 [&](hipStream_t q) { return enqueue_mixed_step(k_step, k_in, lanes, t, P, L, q); });
-  else
-    rc = capture(cs, &step, [&](hipStream_t q) { return enqueue_fp64_step(k_step, lanes, t, P, L, q); });
-  if (rc) return rc;
+  if (n_steps > n_eager) {
+    if (mixed)
+      rc = capture(cs, &step, [&](hipStream_t q) {
+        return enqueue_mixed_step(k_step, k_in, lanes, t, P, L, n, q);
+      });
+    else
+      rc = capture(cs, &step,
+                   [&](hipStream_t q) { return enqueue_fp64_step(k_step, lanes, t, P, L, n, q); });
+    if (rc) return rc;
+  }
   stats->step_iterations = k_step;
-  for (int32_t k = 1; k < n_steps; ++k) {
+  for (int32_t k = n_eager; k < n_steps; ++k) {
     CF_HIP(hipGraphLaunch(step.e, cs));
     ++stats->graph_replays;
     for (;;) {
@@ -834,7 +873,6 @@ extern "C" int cf_batch_theta_run(int32_t n_sys, const int32_t* sys_rows, const 
     const double rel = x.bb > 0.0 ? std::sqrt(x.rr[0] / x.bb) : 0.0;
     stats->max_rel_residual = std::max(stats->max_rel_residual, rel);
   }
-  (void)first_rounds;
   if (stats->failed) {
     cf::set_error("cf_batch_theta_run: %d system-steps did not converge (tol %.1e, maxit %d)",
                   stats->failed, tol, maxit);
