@@ -4,8 +4,9 @@ Replaces the LU-once loop of tet_heat.py:234-274 / tet10.py:504-548:
     (C + th h K) T+ = (C - (1-th) h K) T + h F,
 with the Dirichlet elimination done by solving the free-DOF system (the
 same system the reference's apply_dirichlet + per-step rhs - A[:, F] v
-produces).  Each step is one SpMV for the right-hand side and one
-warm-started PCG solve; the history stays in HBM until the end.
+produces).  Each step is one SpMV for the right-hand side and one PCG
+solve that starts from the extrapolation 2 T_n - T_{n-1} of the last two
+states; the history stays in HBM until the end.
 """
 
 import numpy as np
@@ -71,7 +72,11 @@ def theta_run(dm, K_vals, C_vals, F, theta, t_end, steps, h, T0, mask, fvals,
         extra = step_load(s) if step_load is not None else None
         if extra is not None:
             axpby(h, extra, 1.0, b)
-        pcg(A, b, T, fixed)   # warm start from the previous step
+        if s > 0:
+            # start from the extrapolation 2 T_n - T_{n-1} (prescribed
+            # values are kept: they are the same in both states)
+            axpby(-1.0, hist[s - 1], 2.0, T)
+        pcg(A, b, T, fixed)
         hist[s + 1].copy_(T)
     times = np.cumsum(np.concatenate([[0.0], np.full(steps, h)]))
     times[-1] = t_end
