@@ -4335,9 +4335,9 @@ namespace {
 
 // ------------------------------------------------------------- two-level preconditioner
 // Coarse space of the mixed matrix-free solver: the six rigid-body modes of
-// node aggregates.  Aggregate of a node = the group of kAggTiles element
-// tiles (kAggTiles * kBlock elements in the caller's element order) that
-// holds the first element touching it.  Basis rows of node i in aggregate g:
+// node aggregates.  Aggregate of a node = the group of agg_tiles_for(tiles)
+// consecutive element tiles (in the caller's element order) that holds the
+// first element touching it.  Basis rows of node i in aggregate g:
 // translations e_k and rotations (e_k x (x_i - c_g)) / L_g (c_g centroid, L_g
 // RMS node distance from it), zero on fixed DOFs.  Coarse matrix
 // A_c = P^T K_ff P by probing: aggregates get a distance-2 colouring, the
@@ -4347,7 +4347,20 @@ namespace {
 // factored and inverted with rocSOLVER; the preconditioner is
 //   z1 = C r,  z2 = z1 + Cheb(r - K z1),  z = z2 + C (r - K z2),
 // C = P A_c^-1 P^T, Cheb the FP32 Chebyshev polynomial of S K S.
-constexpr int kAggTiles = 6;  // 1536 elements: 2, 3, 4, 6, 8, 12, 16, 24 tiles measured
+constexpr int kAggTiles = 6;  // the most tiles per aggregate (1536 elements)
+constexpr int kAggTarget = 250;  // aggregates aimed at (a 1,500-unknown coarse problem)
+
+// Tiles per aggregate for a mesh of n_tiles element tiles: about kAggTarget
+// aggregates, 2 to kAggTiles tiles each.  Measured in full Tet10 cantilever
+// solves: at 384,000 elements (1,500 tiles) 6 tiles per aggregate are best
+// (57.9 ms; 4: 60.1, 3: 62.6, 2: 78.9 ms, the coarse factorisation growing
+// with the coarse size; 8 to 24 tiles were slower too), at 48,000 elements
+// (188 tiles) 1 to 3 are (22.3 ms, 45 to 50 outer iterations; 6: 30.2 ms,
+// 71: too coarse a coarse space).
+int agg_tiles_for(int64_t n_tiles) {
+  const int64_t at = (n_tiles + kAggTarget - 1) / kAggTarget;
+  return int(std::min<int64_t>(kAggTiles, std::max<int64_t>(2, at)));
+}
 constexpr int kMaxColors = 64;
 constexpr int kPairsPerElem = 45;  // distinct aggregate pairs of 10 nodes
 
@@ -4370,13 +4383,13 @@ __device__ __forceinline__ void rigid_row(int m, double dx, double dy, double dz
 
 // group of the first element touching node i (-1: none); used[g] = 1
 __global__ void agg_first(int64_t n_nodes, const int32_t* __restrict__ offs,
-                          const uint32_t* __restrict__ perm, int32_t* __restrict__ graw,
-                          uint32_t* __restrict__ used) {
+                          const uint32_t* __restrict__ perm, int agg_tiles,
+                          int32_t* __restrict__ graw, uint32_t* __restrict__ used) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n_nodes;
        i += int64_t(gridDim.x) * blockDim.x) {
     int32_t g = -1;
     if (offs[i] < offs[i + 1]) {
-      g = int32_t(perm[offs[i]] / uint32_t(kTileItems * kAggTiles));
+      g = int32_t(perm[offs[i]] / uint32_t(kTileItems * agg_tiles));
       used[g] = 1u;
     }
     graw[i] = g;
@@ -5054,6 +5067,7 @@ namespace {
 
 struct Coarse {
   int64_t n_groups = 0, n_agg = 0, nc = 0;  // host
+  int agg_tiles = kAggTiles;                 // element tiles per aggregate
   int64_t bw = 0;                            // half-bandwidth of the coarse matrix
   int n_col = 0;
   int32_t* graw;
@@ -5088,7 +5102,9 @@ struct Coarse {
 
 void coarse_layout(int64_t n_nodes, int64_t n_elem, cf::Carve& c, Coarse* C, int lpb) {
   const int64_t n = 3 * n_nodes, n_tiles = (n_elem + kBlock - 1) / kBlock;
-  const int64_t G = (n_tiles + kAggTiles - 1) / kAggTiles, ncmax = 6 * G;
+  const int at = agg_tiles_for(n_tiles);
+  const int64_t G = (n_tiles + at - 1) / at, ncmax = 6 * G;
+  C->agg_tiles = at;
   const int64_t maxp = kPairsPerElem * n_elem;
   C->n_groups = G;
   C->graw = c.take<int32_t>(n_nodes);
@@ -5178,7 +5194,7 @@ int coarse_setup(const MfArgs& a, const MfRuns& R, const double* xs, const doubl
     if (attempt == 0) {
       // first-touching element group, compacted ids
       CF_HIP(hipMemsetAsync(C->used, 0, sizeof(uint32_t) * (G + 1), s));
-      agg_first<<<gn, kBlock, 0, s>>>(n_nodes, a.offs, a.perm, C->graw, C->used);
+      agg_first<<<gn, kBlock, 0, s>>>(n_nodes, a.offs, a.perm, C->agg_tiles, C->graw, C->used);
       CF_LAUNCHED();
       tb = C->tmp_bytes;
       CF_HIP(rocprim::exclusive_scan(C->tmp, tb, C->used, C->gid, 0u, size_t(G + 1),

@@ -405,8 +405,9 @@ int cf_pcg_solve_mf_t10_mixed(int64_t n_nodes, int64_t n_elem, const int32_t* co
  *      product, preconditioned by z1 = C r, z2 = z1 + Cheb(r - K z1),
  *      z = z2 + C (r - K z2): Cheb the FP32 matrix-free Chebyshev polynomial
  *      of cf_pcg_solve_mf_t10_mixed, C = P A_c^-1 P^T a coarse correction on
- *      the six rigid-body modes of node aggregates (the group of 1536
- *      consecutive elements that first touches a node), A_c = P^T K_ff P
+ *      the six rigid-body modes of node aggregates (the group of 512 to
+ *      1536 consecutive elements, about 250 groups per mesh, that first
+ *      touches a node), A_c = P^T K_ff P
  *      assembled by probing with distance-2 coloured aggregates (no atomics)
  *      and inverted by a banded Cholesky factorisation and a column-parallel
  *      inverse when its half-bandwidth is at most 62 (the aggregates follow
