@@ -1068,7 +1068,21 @@ __device__ __forceinline__ void t10_forces(const int32_t* __restrict__ cn,
 //   toff[t]           first run of tile t (runs numbered tile by tile, by
 //                     node inside a tile); tstart[q] = first k of run q
 //   nq[j]             runs of node n at j in [nro[n], nro[n+1]), tile order
-constexpr int kTileItems = kBlock * 10;
+// Elements per tile (threads per element block).  Smaller tiles need fewer
+// waves per barrier but give more runs: FP32 product at 384,000 elements
+// 25.3 us with 256, 27.0 with 128, 29.4 with 64.
+constexpr int kTile = 256;
+constexpr int kTileItems = kTile * 10;
+// the two-level aggregates count elements in units of 256, whatever the tile
+constexpr int kAggUnitItems = 2560;
+
+// Element-kernel grid: one block per tile, at most 8 blocks of 256 threads
+// per CU (more tiles: blocks take several, xcd_tile order).
+int elem_grid(int64_t m) {
+  const int64_t n_tiles = (m + kTile - 1) / kTile;
+  const int64_t cap = int64_t(cf::cu_count()) * 8 * (kBlock / kTile);
+  return int(n_tiles < 1 ? 1 : (n_tiles < cap ? n_tiles : cap));
+}
 
 __global__ void iota_u32(int64_t n, uint32_t* __restrict__ v) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
@@ -1191,7 +1205,7 @@ __global__ void mf_tile_run_off(int64_t n_tiles, int64_t n_runs,
 // FP64 element forces -> run partials part[c * n_runs + q] (component
 // planes), the tile's forces staged in LDS one component at a time (20 KB per
 // block, so the 3 waves per SIMD its registers allow stay resident).
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kTile)
 mf_t10_elem(int64_t m, const int32_t* __restrict__ conn, const double* __restrict__ geo,
             double lam, double mu, const double* __restrict__ x,
             const uint16_t* __restrict__ rank, const int32_t* __restrict__ toff,
@@ -1199,9 +1213,9 @@ mf_t10_elem(int64_t m, const int32_t* __restrict__ conn, const double* __restric
             const PcgState* state) {
   if (state && state->halt) return;
   __shared__ double fl[kTileItems];
-  const int64_t n_tiles = (m + kBlock - 1) / kBlock;
+  const int64_t n_tiles = (m + kTile - 1) / kTile;
   for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < n_tiles; t += gridDim.x) {
-    const int64_t e = t * kBlock + threadIdx.x;
+    const int64_t e = t * kTile + threadIdx.x;
     double f[10][3];
     int rk[10];
     if (e < m) {
@@ -1233,7 +1247,7 @@ mf_t10_elem(int64_t m, const int32_t* __restrict__ conn, const double* __restric
         for (int a = 0; a < 10; ++a) fl[rk[a]] = f[a][c];
       }
       __syncthreads();
-      for (int32_t q = q0 + threadIdx.x; q < q1; q += kBlock) {
+      for (int32_t q = q0 + threadIdx.x; q < q1; q += kTile) {
         const int kb = tstart[q], ke = q + 1 < q1 ? int(tstart[q + 1]) : nt;
         double sum = 0.0;
         for (int k = kb; k < ke; ++k) sum += fl[k];
@@ -1560,7 +1574,7 @@ mf_shared_gather(int64_t n_sn, const int32_t* __restrict__ shared_node,
 // value a (0..9) of element e = t kBlock + l sits at ((t 10 + a) kBlock + l),
 // so each of the 10 loads of a thread is one contiguous 64-lane access.
 __device__ __forceinline__ int64_t tile_packed(int64_t t, int a) {
-  return (t * 10 + a) * kBlock + threadIdx.x;
+  return (t * 10 + a) * kTile + threadIdx.x;
 }
 
 template <class T>
@@ -1572,8 +1586,8 @@ __global__ void mf_pack_tiles(int64_t m, const int32_t* __restrict__ conn,
        i += int64_t(gridDim.x) * blockDim.x) {
     const int64_t e = i / 10;
     const int a = int(i - 10 * e);
-    const int64_t t = e / kBlock;
-    const int64_t o = (t * 10 + a) * kBlock + (e - t * kBlock);
+    const int64_t t = e / kTile;
+    const int64_t o = (t * 10 + a) * kTile + (e - t * kTile);
     connT[o] = conn[i];
     rankT[o] = rank[i];
     geoT[o] = T(geo[i]);
@@ -1586,7 +1600,7 @@ __global__ void mf_pack_tiles(int64_t m, const int32_t* __restrict__ conn,
 // partials described above.  conn, geo and rank are tile-packed
 // (mf_pack_tiles).
 template <class T, bool PRE>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kTile)
 mf_t10_elem_lp(int64_t m, const int32_t* __restrict__ conn, const T* __restrict__ geo,
                T lam, T mu, const T* __restrict__ sc, const T* __restrict__ d,
                const uint16_t* __restrict__ rank, const int32_t* __restrict__ toff,
@@ -1594,9 +1608,9 @@ mf_t10_elem_lp(int64_t m, const int32_t* __restrict__ conn, const T* __restrict_
                const PcgState* state) {
   if (state && state->halt) return;
   __shared__ T fl[kTileItems * 3];
-  const int64_t n_tiles = (m + kBlock - 1) / kBlock;
+  const int64_t n_tiles = (m + kTile - 1) / kTile;
   for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < n_tiles; t += gridDim.x) {
-    const int64_t e = t * kBlock + threadIdx.x;
+    const int64_t e = t * kTile + threadIdx.x;
     if (e < m) {
       T J[3][3];
 #pragma unroll
@@ -1635,7 +1649,7 @@ mf_t10_elem_lp(int64_t m, const int32_t* __restrict__ conn, const T* __restrict_
     const int64_t k0 = t * kTileItems;
     const int nt = int((k0 + kTileItems < 10 * m ? k0 + kTileItems : 10 * m) - k0);
     const int32_t q0 = toff[t], q1 = toff[t + 1];
-    for (int32_t q = q0 + threadIdx.x; q < q1; q += kBlock) {
+    for (int32_t q = q0 + threadIdx.x; q < q1; q += kTile) {
       const int kb = tstart[q], ke = q + 1 < q1 ? int(tstart[q + 1]) : nt;
       T s0 = T(0), s1 = T(0), s2 = T(0);
       for (int k = kb; k < ke; ++k) {
@@ -1704,7 +1718,7 @@ __device__ __forceinline__ cf_float4 mfma16x16x4(float a, float b, cf_float4 c) 
 }
 
 template <bool PRE>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kTile)
 mf_t10_elem_mfma(int64_t m, const int32_t* __restrict__ conn, const float* __restrict__ geo,
                  float lam, float mu, const float* __restrict__ sc, const float* __restrict__ d,
                  const uint16_t* __restrict__ rank, const int32_t* __restrict__ toff,
@@ -1727,13 +1741,13 @@ mf_t10_elem_mfma(int64_t m, const int32_t* __restrict__ conn, const float* __res
     }
   }
   const float lq = 0.25f * lam, mq = 0.25f * mu;
-  const int64_t n_tiles = (m + kBlock - 1) / kBlock;
+  const int64_t n_tiles = (m + kTile - 1) / kTile;
   for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < n_tiles; t += gridDim.x) {
     // All loads of the four groups first, with no branches: an element past
     // m reads element m - 1 (the padding of the last tile is not written)
     // and gets V = 0, node slots 10 and 11 read node 9 (their A1 entries are
     // zero), so every wait is on loads that are already in flight.
-    const int64_t last = m - 1 - t * kBlock;  // last valid element of the tile
+    const int64_t last = m - 1 - t * kTile;  // last valid element of the tile
     int64_t base[4];
     bool ok[4];
     int32_t nd[4][3];
@@ -1741,11 +1755,11 @@ mf_t10_elem_mfma(int64_t m, const int32_t* __restrict__ conn, const float* __res
     for (int g = 0; g < 4; ++g) {
       const int el = wave * 64 + g * 16 + el0;  // element within the tile
       ok[g] = el <= last;
-      base[g] = t * 10 * kBlock + (ok[g] ? el : last);
+      base[g] = t * 10 * kTile + (ok[g] ? el : last);
 #pragma unroll
       for (int s = 0; s < 3; ++s) {
         const int a = 4 * s + j < 10 ? 4 * s + j : 9;
-        nd[g][s] = conn[base[g] + int64_t(a) * kBlock];
+        nd[g][s] = conn[base[g] + int64_t(a) * kTile];
       }
     }
     float xb[4][3][3];
@@ -1770,8 +1784,8 @@ mf_t10_elem_mfma(int64_t m, const int32_t* __restrict__ conn, const float* __res
 #pragma unroll
       for (int r = 0; r < 3; ++r)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) J[r][c] = geo[base[g] + int64_t(3 * r + c) * kBlock];
-      const float V = ok[g] ? geo[base[g] + 9 * int64_t(kBlock)] : 0.0f;
+        for (int c = 0; c < 3; ++c) J[r][c] = geo[base[g] + int64_t(3 * r + c) * kTile];
+      const float V = ok[g] ? geo[base[g] + 9 * int64_t(kTile)] : 0.0f;
       // stage 1: reference gradients at point j
       cf_float4 G[3];
 #pragma unroll
@@ -1811,7 +1825,7 @@ mf_t10_elem_mfma(int64_t m, const int32_t* __restrict__ conn, const float* __res
         for (int r = 0; r < 4; ++r) {
           const int a = 4 * j + r;
           if (a < 10) {
-            const int rk = rank[base[g] + int64_t(a) * kBlock];
+            const int rk = rank[base[g] + int64_t(a) * kTile];
             fl[3 * rk] = F[0][r];
             fl[3 * rk + 1] = F[1][r];
             fl[3 * rk + 2] = F[2][r];
@@ -1824,7 +1838,7 @@ mf_t10_elem_mfma(int64_t m, const int32_t* __restrict__ conn, const float* __res
     const int64_t k0 = t * kTileItems;
     const int nt = int((k0 + kTileItems < 10 * m ? k0 + kTileItems : 10 * m) - k0);
     const int32_t q0 = toff[t], q1 = toff[t + 1];
-    for (int32_t q = q0 + threadIdx.x; q < q1; q += kBlock) {
+    for (int32_t q = q0 + threadIdx.x; q < q1; q += kTile) {
       const int kb = tstart[q], ke = q + 1 < q1 ? int(tstart[q + 1]) : nt;
       float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
       for (int k = kb; k < ke; ++k) {
@@ -1858,12 +1872,12 @@ void launch_elem_lp(bool mfma, int grid, hipStream_t s, int64_t m, const int32_t
                     const int32_t* toff, const uint16_t* tstart, T* part, const PcgState* state) {
   if constexpr (std::is_same_v<T, float>) {
     if (mfma) {
-      mf_t10_elem_mfma<PRE><<<grid, kBlock, 0, s>>>(m, conn, geo, lam, mu, sc, d, rank, toff,
+      mf_t10_elem_mfma<PRE><<<grid, kTile, 0, s>>>(m, conn, geo, lam, mu, sc, d, rank, toff,
                                                     tstart, part, state);
       return;
     }
   }
-  mf_t10_elem_lp<T, PRE><<<grid, kBlock, 0, s>>>(m, conn, geo, lam, mu, sc, d, rank, toff,
+  mf_t10_elem_lp<T, PRE><<<grid, kTile, 0, s>>>(m, conn, geo, lam, mu, sc, d, rank, toff,
                                                  tstart, part, state);
 }
 
@@ -3834,7 +3848,7 @@ struct MfRuns {
 
 void mf_runs_layout(int64_t n_nodes, int64_t n_elem, cf::Carve& c, MfRuns* R) {
   const int64_t items = 10 * n_elem;
-  const int64_t n_tiles = (n_elem + kBlock - 1) / kBlock;
+  const int64_t n_tiles = (n_elem + kTile - 1) / kTile;
   R->n_runs = 0;
   R->rank = c.take<uint16_t>(items);
   R->toff = c.take<int32_t>(n_tiles + 1);
@@ -3871,7 +3885,7 @@ int mf_runs_build(int64_t n_nodes, int64_t n_elem, const int32_t* offs, const ui
   size_t tb = M.tmp_bytes;
   CF_HIP(rocprim::radix_sort_pairs<SortConfig>(M.tmp, tb, M.keys_in, M.keys_out, M.ids_in, M.tslot,
                                    size_t(items), 0,
-                                   bits_for((n_elem + kBlock - 1) / kBlock + 1), s));
+                                   bits_for((n_elem + kTile - 1) / kTile + 1), s));
   mf_store_rank<<<grid, kBlock, 0, s>>>(items, perm, M.tslot, M.rank);
   CF_LAUNCHED();
   uint32_t* head = M.keys_in;
@@ -3902,7 +3916,7 @@ int mf_runs_build(int64_t n_nodes, int64_t n_elem, const int32_t* offs, const ui
                                    size_t(n_runs), 0, bits_for(items + 1), s));
   mf_run_tile_order<<<grid, kBlock, 0, s>>>(n_runs, M.keys_out, M.tslot, M.nq, M.tstart);
   CF_LAUNCHED();
-  const int64_t n_tiles = (n_elem + kBlock - 1) / kBlock;
+  const int64_t n_tiles = (n_elem + kTile - 1) / kTile;
   mf_tile_run_off<<<cf::stride_grid(n_tiles + 1, kBlock), kBlock, 0, s>>>(n_tiles, n_runs,
                                                                           M.keys_out, M.toff);
   CF_LAUNCHED();
@@ -3956,7 +3970,7 @@ size_t mf_layout(int64_t n_nodes, int64_t n_elem, int precond, void* base, Layou
 int mf_apply(const MfArgs& a, const MfRuns& R, const double* xin, double* y,
              const uint8_t* fixed, double* part, PcgState* state, double* partials,
              hipStream_t s) {
-  mf_t10_elem<<<cf::stride_grid(a.n_elem, kBlock), kBlock, 0, s>>>(
+  mf_t10_elem<<<elem_grid(a.n_elem), kTile, 0, s>>>(
       a.n_elem, a.conn, a.geo, a.lam, a.mu, xin, R.rank, R.toff, R.tstart, R.n_runs, part, state);
   CF_LAUNCHED();
   mf_gather_runs<<<int(mf_gather_grid(a.n_nodes)), kBlock, 0, s>>>(
@@ -4339,7 +4353,7 @@ void mf_mixed_layout_base(int64_t n_nodes, int64_t n_elem, cf::Carve& c, Layout*
   L->z1 = c.take<double>(n);
   M->fe = c.take<double>(30 * n_elem);
   M->blocks = c.take<double>(9 * n_nodes);
-  const int64_t packed = 10 * ((n_elem + kBlock - 1) / kBlock) * kBlock;
+  const int64_t packed = 10 * ((n_elem + kTile - 1) / kTile) * kTile;
   M->geoL = c.take<char>(packed * lpb);
   M->connT = c.take<int32_t>(packed);
   M->rankT = c.take<uint16_t>(packed);
@@ -4408,7 +4422,7 @@ __global__ void agg_first(int64_t n_nodes, const int32_t* __restrict__ offs,
        i += int64_t(gridDim.x) * blockDim.x) {
     int32_t g = -1;
     if (offs[i] < offs[i + 1]) {
-      g = int32_t(perm[offs[i]] / uint32_t(kTileItems * agg_tiles));
+      g = int32_t(perm[offs[i]] / uint32_t(kAggUnitItems * agg_tiles));
       used[g] = 1u;
     }
     graw[i] = g;
@@ -5120,7 +5134,7 @@ struct Coarse {
 };
 
 void coarse_layout(int64_t n_nodes, int64_t n_elem, cf::Carve& c, Coarse* C, int lpb) {
-  const int64_t n = 3 * n_nodes, n_tiles = (n_elem + kBlock - 1) / kBlock;
+  const int64_t n = 3 * n_nodes, n_tiles = (10 * n_elem + kAggUnitItems - 1) / kAggUnitItems;
   const int at = agg_tiles_for(n_tiles);
   const int64_t G = (n_tiles + at - 1) / at, ncmax = 6 * G;
   C->agg_tiles = at;
@@ -5624,7 +5638,7 @@ int mf_mixed_solve(const char* who, const cf_matrix* A, int64_t n_nodes, int64_t
       ((rc = build_sell(A, L, s, nullptr)) ||
        (rc = fill_sell<double>(A, L, nullptr, L.sval64, s))))
     return rc;
-  const int ge = cf::stride_grid(n_elem, kBlock);
+  const int ge = elem_grid(n_elem);
   const int gg = int(mf_gather_grid(n_nodes));
   const int32_t* xsplit = L.xsplit;
   // With A the outer products start matrix-free (FP64, cheap) and switch to
@@ -6026,7 +6040,7 @@ int mf_lp_apply(int64_t n_nodes, int64_t n_elem, const int32_t* conn, const doub
   mf_pack_tiles<T><<<cf::stride_grid(10 * n_elem, kBlock), kBlock, 0, s>>>(
       n_elem, conn, geo, M.R.rank, M.connT, geoL, M.rankT);
   CF_LAUNCHED();
-  const int ge = cf::stride_grid(n_elem, kBlock);
+  const int ge = elem_grid(n_elem);
   const int gg = int(mf_gather_grid(n_nodes));
   const bool use_mfma = inner_mfma();
   auto product = [&]() {
