@@ -85,3 +85,17 @@ def test_batch_64_jittered_meshes(gpu):
     for i in order:
         _, h0 = _oracle(meshes[i], ks[i], 5.0, 0.1)
         assert _rel(out64[i][1], h0) < 1e-9
+
+
+def test_batch_per_mesh_constants(gpu):
+    """One float per mesh for k and rho_cp (expanded per element on the
+    device) and a nonzero constant source, against the oracle."""
+    from cloudfem_amd.fem.batch_heat import jittered_box_batch, transient_heat_batch
+    meshes, _ = jittered_box_batch(3, lo=2, hi=4, seed=7)
+    kv, rcv = [1.0, 2.5, 0.7], [RHO_CP, 0.5 * RHO_CP, 2.0 * RHO_CP]
+    out = transient_heat_batch(meshes, k=kv, rho_cp=rcv, f=5e3, robin=ROBIN, t_end=0.5,
+                               dt=0.1, T0=100.0, method="cn")
+    for m, kk, rc, (_, hist) in zip(meshes, kv, rcv, out):
+        _, h0 = op1.transient_heat(m, k=kk, rho_cp=rc, f=5e3, robin=ROBIN, t_end=0.5, dt=0.1,
+                                   T0=100.0, method="cn")
+        assert _rel(hist, h0) < 1e-9
