@@ -274,6 +274,37 @@ def test_twolevel_jittered_mesh_body_load(gpu):
     assert err < 1e-10
 
 
+def test_twolevel_unstructured_sphere(gpu):
+    """Default Tet10 solver (two-level mixed, assembled outside) on the
+    unstructured sphere fixture (Delaunay + smoothing, irregular
+    connectivity), clamped on the surface nodes below z = -0.25, under
+    gravity, against SuperLU."""
+    import os
+    from cloudfem_amd import solver
+    from cloudfem_amd.physics import elastic_tet10
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "sphere_mesh_0p5_0p22.npz"))
+    bnd = z["boundary"]
+    cap = bnd[z["nodes"][bnd, 2] < -0.25]
+    m = om.tet4_to_tet10(om.OracleTetMesh(z["nodes"], z["elements"],
+                                          {"boundary": bnd, "cap": cap}))
+    body = (0.0, 0.0, -7850.0 * 9.81)
+    bc = {"cap": (0.0, 0.0, 0.0)}
+    with solver.solver_options(tol=1e-12):
+        u = np.stack(elastic_tet10.solve_elasticity_tet10(m, 210e9, 0.3, bc=bc, body=body),
+                     axis=1).ravel()
+    st = solver.last_stats.value
+    assert st["precond"] == "two-level" and st["precision"] == "mixed"
+    A0, rhs0 = oel.assemble_elasticity_tet10(m, 210e9, 0.3, body=body)
+    u0 = oel.apply_dirichlet_vec3(A0, rhs0, m, bc)
+    err = rel_err(u, u0)
+    Aff, bf, free = oel.reduced_system_vec3(A0, rhs0, m, bc)[:3]
+    floor = fp64_floor(Aff, u0[free], bf)
+    print(f"sphere: {len(cap)} clamped corner nodes, {st['iterations']} outer iterations, "
+          f"rel. residual {st['rel_residual']:.2e} (floor {floor:.2e}), error {err:.2e}")
+    assert_residual_bar(st, 1e-12, floor)
+    assert err < 1e-10
+
+
 @pytest.mark.parametrize("path", ["fp64", "mixed_2l"])
 def test_tol_below_fp64_floor_stops_at_floor(gpu, path):
     """tol = 1e-15 lies far below the FP64 floor eps || |A_ff| |x_f| || /
