@@ -30,7 +30,7 @@ import torch
 from .. import _native as N
 from ..autotune import batch_spmv_config
 from ..coef import centroids, eval_scalar
-from ..device import DeviceMesh, ptr, stream_handle, to_dev, workspace
+from ..device import DeviceMesh, ptr, require_gpu, stream_handle, to_dev, workspace
 from ..mesh.device_ops import boundary_faces_multi_device
 from ..mesh.faces import boundary_faces
 from ..mesh.tet_mesh import TetMesh, box_mesh
@@ -62,16 +62,26 @@ def _element_values(v, mesh):
     return eval_scalar(v, centroids(mesh))
 
 
-def _union(meshes):
+def _union(meshes, pinned=False):
     """One mesh of all meshes (node ids offset per mesh), int32 connectivity
     written in place (the device layer's type), the meshes' slices filled by
-    a few host threads (NumPy releases the GIL in these copies)."""
+    a few host threads (NumPy releases the GIL in these copies).  pinned:
+    the coordinates (as x, y, z rows) and the connectivity live in
+    page-locked host tensors, returned as well, so that they upload by DMA
+    without a staging copy."""
     offs = np.concatenate([[0], np.cumsum([m.n_nodes for m in meshes])]).astype(np.int64)
     if offs[-1] >= 2 ** 31:
         raise ValueError(f"transient_heat_batch: {offs[-1]} nodes exceed int32 node ids")
     eoffs = np.concatenate([[0], np.cumsum([m.n_elements for m in meshes])]).astype(np.int64)
-    nodes = np.empty((int(offs[-1]), 3), dtype=float)
-    elems = np.empty((int(eoffs[-1]), 4), dtype=np.int32)
+    n, ne = int(offs[-1]), int(eoffs[-1])
+    if pinned:
+        xyz_t = torch.empty((3, n), dtype=torch.float64, pin_memory=True)
+        conn_t = torch.empty((ne, 4), dtype=torch.int32, pin_memory=True)
+        nodes, elems = xyz_t.numpy().T, conn_t.numpy()
+    else:
+        xyz_t = conn_t = None
+        nodes = np.empty((n, 3), dtype=float)
+        elems = np.empty((ne, 4), dtype=np.int32)
 
     def fill(i):
         m = meshes[i]
@@ -86,7 +96,20 @@ def _union(meshes):
     else:
         for i in range(len(meshes)):
             fill(i)
-    return TetMesh.wrap(nodes, elems), offs
+    um = TetMesh.wrap(nodes, elems)
+    return (um, offs, xyz_t, conn_t) if pinned else (um, offs)
+
+
+def _device_union(meshes, dev):
+    """(union TetMesh, node offsets, DeviceMesh) with the union's arrays
+    uploaded from page-locked memory (the meshes' ids were range-checked
+    when the meshes were built; the union's are checked on the device)."""
+    um, offs, xyz_t, conn_t = _union(meshes, pinned=True)
+    xyz = xyz_t.to(dev, non_blocking=True)
+    conn = conn_t.to(dev, non_blocking=True)
+    if conn.numel() and bool(((conn < 0) | (conn >= um.n_nodes)).any()):
+        raise IndexError("element node index out of range")
+    return um, offs, DeviceMesh.from_device(xyz[0], xyz[1], xyz[2], conn)
 
 
 def _needs_points(v):
@@ -181,10 +204,9 @@ def transient_heat_batch(meshes, k=1.0, rho_cp=1.0, f=0.0, neumann=None, robin=N
     theta = _transient.check_method(method)
     steps, h = _transient.step_count(t_end, dt)
     prec = {"fp64": N.PREC_FP64, "mixed": N.PREC_MIXED}[precision]
-    um, offs = _union(meshes)
     ks, rcs, fs = (_per_mesh(v, nm, w) for v, w in ((k, "k"), (rho_cp, "rho_cp"), (f, "f")))
-    dm = DeviceMesh(um)
-    dev = dm.dev
+    dev = require_gpu()
+    um, offs, dm = _device_union(meshes, dev)
     K = dm.matrix_values(N.OP_P1_STIFF, coef=_element_coef(ks, meshes, dev))
     C = dm.matrix_values(N.OP_P1_MASS, coef=_element_coef(rcs, meshes, dev))
     F = torch.zeros(dm.n_nodes, dtype=torch.float64, device=dev)
