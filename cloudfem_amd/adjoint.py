@@ -14,7 +14,7 @@ import numpy as np
 import torch
 
 from . import _native as N
-from .coef import centroids, dirichlet_scalar, eval_scalar
+from .coef import LazyPoints, centroid_points, dirichlet_scalar, eval_scalar
 from .device import DeviceMesh, ptr, stream_handle, to_dev
 from .solver import DeviceMatrix, pcg
 
@@ -61,11 +61,11 @@ def steady_diffusion(mesh, kappa, f=0.0, bc=None, tol=1e-12, dm=None):
         raise ValueError("steady_diffusion needs Dirichlet data (K is singular otherwise)")
     rhs = torch.zeros(dm.n_nodes, dtype=torch.float64, device=dm.dev)
     if dm.k == 4:
-        fv = eval_scalar(f, centroids(mesh))
+        fv = eval_scalar(f, centroid_points(mesh))
         dm.add_rhs(N.RHS_P1_SOURCE, to_dev(fv, torch.float64, dm.dev), rhs)
     else:
         from .fem.tet10 import _qpoints
-        fv = eval_scalar(f, _qpoints(mesh).reshape(-1, 3))
+        fv = eval_scalar(f, LazyPoints(5 * dm.n_elem, lambda: _qpoints(mesh).reshape(-1, 3)))
         dm.add_rhs(N.RHS_T10_SOURCE, to_dev(fv, torch.float64, dm.dev), rhs)
     mask, vals = dirichlet_scalar(mesh, bc)
     fixed = to_dev(mask.astype(np.uint8), torch.uint8, dm.dev)

@@ -14,7 +14,7 @@ import scipy.sparse as sp
 import torch
 
 from .. import _native as N
-from ..coef import centroids, dirichlet_scalar, eval_scalar
+from ..coef import LazyPoints, centroid_points, dirichlet_scalar, eval_scalar
 from ..device import DeviceMesh, scalar_csr, to_dev
 from ..mesh.faces import boundary_faces10 as _faces10
 from ..mesh.tet10_mesh import EDGES
@@ -87,7 +87,7 @@ def _assemble_device(mesh, k=1.0, f=0.0, dm=None):
     dm = dm or DeviceMesh(mesh)
     if dm.k != 10:
         raise ValueError("Tet10 assembly needs a 10-node mesh")
-    xq = _qpoints(mesh).reshape(-1, 3)
+    xq = LazyPoints(5 * dm.n_elem, lambda: _qpoints(mesh).reshape(-1, 3))
     kv = to_dev(eval_scalar(k, xq), torch.float64, dm.dev)
     fv = to_dev(eval_scalar(f, xq), torch.float64, dm.dev)
     vals = dm.matrix_values(N.OP_T10_STIFF, coef=kv)
@@ -140,7 +140,7 @@ _MASS_REF = _mass_reference()
 
 
 def _mass_device(dm, mesh, rho_cp):
-    rc = to_dev(eval_scalar(rho_cp, centroids(mesh)), torch.float64, dm.dev)
+    rc = to_dev(eval_scalar(rho_cp, centroid_points(mesh)), torch.float64, dm.dev)
     return dm.matrix_values(N.OP_T10_MASS, coef=rc)
 
 

@@ -6,7 +6,7 @@ import scipy.sparse as sp
 import torch
 
 from .. import _native as N
-from ..coef import centroids, dirichlet_scalar, eval_scalar
+from ..coef import centroid_points, dirichlet_scalar, eval_scalar
 from ..device import DeviceMesh, scalar_csr, to_dev
 from ..mesh.faces import boundary_faces as _boundary_faces_np
 from . import _transient
@@ -16,7 +16,7 @@ _THETA = _transient.THETA
 
 
 def _mass_device(dm, mesh, rho_cp):
-    rc = to_dev(eval_scalar(rho_cp, centroids(mesh)), torch.float64, dm.dev)
+    rc = to_dev(eval_scalar(rho_cp, centroid_points(mesh)), torch.float64, dm.dev)
     return dm.matrix_values(N.OP_P1_MASS, coef=rc)
 
 

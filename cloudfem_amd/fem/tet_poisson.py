@@ -12,7 +12,7 @@ import scipy.sparse as sp
 import torch
 
 from .. import _native as N
-from ..coef import centroids, dirichlet_scalar, eval_scalar
+from ..coef import centroid_points, dirichlet_scalar, eval_scalar
 from ..device import DeviceMesh, scalar_csr, to_dev
 from ..solver import DeviceMatrix, pcg
 from . import _dirichlet
@@ -35,7 +35,7 @@ def _tet_gradients(p):
 def _assemble_device(mesh, k=1.0, f=0.0, dm=None):
     """(DeviceMesh, stiffness values on the node pattern, rhs) on the GPU."""
     dm = dm or DeviceMesh(mesh)
-    cen = centroids(mesh)
+    cen = centroid_points(mesh)
     kv = to_dev(eval_scalar(k, cen), torch.float64, dm.dev)
     fv = to_dev(eval_scalar(f, cen), torch.float64, dm.dev)
     vals = dm.matrix_values(N.OP_P1_STIFF, coef=kv)

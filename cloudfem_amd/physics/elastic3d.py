@@ -4,7 +4,7 @@ import numpy as np
 import torch
 
 from .. import _native as N
-from ..coef import centroids, dirichlet_vec3, eval_components
+from ..coef import centroid_points, centroids, dirichlet_vec3, eval_components
 from ..device import DeviceMesh, to_dev, vector_csr
 from ..mesh.faces import boundary_faces as _faces_p1
 from ..solver import DeviceMatrix, pcg
@@ -41,7 +41,7 @@ def _assemble_device(mesh, E, nu, body=None, traction=None, T=None, alpha=0.0, T
     vals = dm.matrix_values(N.OP_P1_ELAST, lam=lam, mu=mu)
     rhs = torch.zeros(3 * dm.n_nodes, dtype=torch.float64, device=dm.dev)
     if body is not None:
-        bv = eval_components(body, centroids(mesh))
+        bv = eval_components(body, centroid_points(mesh))
         if np.any(bv):
             dm.add_rhs(N.RHS_P1_BODY, to_dev(bv.ravel(), torch.float64, dm.dev), rhs)
     if T is not None and alpha != 0.0:

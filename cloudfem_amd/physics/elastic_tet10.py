@@ -14,7 +14,7 @@ import numpy as np
 import torch
 
 from .. import _native as N
-from ..coef import centroids, dirichlet_vec3, eval_components, eval_scalar
+from ..coef import LazyPoints, dirichlet_vec3, eval_components, eval_scalar
 from ..device import DeviceMesh, to_dev, vector_csr
 from ..fem.tet10 import quadrature, shape_functions
 from ..mesh.faces import T10_FACES as FACES10
@@ -50,7 +50,7 @@ def _assemble_device(mesh, E, nu, body=None, traction=None, T=None, alpha=0.0, T
     vals = dm.matrix_values(N.OP_T10_ELAST, lam=lam, mu=mu) if values else None
     rhs = torch.zeros(3 * dm.n_nodes, dtype=torch.float64, device=dm.dev)
     if body is not None:
-        xq = _qpoints(mesh).reshape(-1, 3)
+        xq = LazyPoints(5 * dm.n_elem, lambda: _qpoints(mesh).reshape(-1, 3))
         bq = eval_components(body, xq)
         if np.any(bq):
             dm.add_rhs(N.RHS_T10_BODY, to_dev(bq.ravel(), torch.float64, dm.dev), rhs)

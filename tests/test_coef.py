@@ -81,3 +81,24 @@ def test_dirichlet_vec3_reducing_bc():
     np.testing.assert_array_equal(vals[3 * ids + 1], _per_point(NORM, m.nodes[ids]))
     assert not mask[3 * ids].any() and mask[3 * ids + 2].all()
     assert len(set(vals[3 * ids + 1])) > 1
+
+
+def test_lazy_points_built_only_for_callables():
+    """A constant needs only the point count: LazyPoints builds its points
+    for callables only, and the values equal those at the built points."""
+    import numpy as np
+    from cloudfem_amd.coef import LazyPoints, eval_components, eval_scalar
+    built = []
+    pts = np.array([[0.0, 0.0, 0.0], [1.0, 2.0, 3.0], [0.5, 0.5, 0.5]])
+
+    def build():
+        built.append(1)
+        return pts
+    lp = LazyPoints(3, build)
+    assert np.array_equal(eval_scalar(2.5, lp), np.full(3, 2.5)) and not built
+    assert np.array_equal(eval_components((1.0, None, 2.0), lp), eval_components((1.0, None, 2.0), pts))
+    assert not built
+    f = lambda x, y, z: x + 2 * y - z  # noqa: E731
+    assert np.array_equal(eval_scalar(f, lp), eval_scalar(f, pts)) and len(built) == 1
+    assert np.array_equal(eval_components((f, 0.0, f), lp), eval_components((f, 0.0, f), pts))
+    assert len(built) == 1
