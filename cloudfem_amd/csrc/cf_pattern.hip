@@ -275,13 +275,19 @@ __device__ int row_sort(int c, int k, int64_t item0, const int32_t* __restrict__
 }
 
 // One wave per row (4 rows per block): contributor lists and row lengths
-// (rowlen[n_nodes] = 0 for the scan); rows beyond kRowCap pairs set *over.
+// (rowlen[n_nodes] = 0 for the scan); rows beyond kRowCap pairs set
+// flags[0].  LARGE = false sorts the rows of at most 256 pairs (4 per lane:
+// every row of a Tet10 box mesh) and sets flags[2] for longer ones, which
+// the LARGE = true launch then sorts (8 or 16 per lane); kept apart, the
+// common kernel needs 35 instead of 78 VGPRs (Tet10 160x20x20: 446 against
+// 481 us).
+template <bool LARGE>
 __global__ void __launch_bounds__(kBlock)
 row_pattern(int64_t n_nodes, int k, const int32_t* __restrict__ conn,
             const int32_t* __restrict__ offs, const uint32_t* __restrict__ perm,
             uint32_t* __restrict__ contrib, int32_t* __restrict__ hcol,
             int32_t* __restrict__ hpos, int32_t* __restrict__ rowlen,
-            int32_t* __restrict__ over) {
+            int32_t* __restrict__ flags) {
   const int lane = threadIdx.x & 63;
   const int64_t i = blockIdx.x * int64_t(kBlock / 64) + threadIdx.x / 64;
   if (i > n_nodes) return;
@@ -290,19 +296,29 @@ row_pattern(int64_t n_nodes, int k, const int32_t* __restrict__ conn,
     const int64_t o0 = offs[i];
     const int64_t c = (offs[i + 1] - o0) * k;
     const int64_t cbase = o0 * k;
-    if (c > kRowCap) {
-      if (lane == 0) *over = 1;
-    } else if (c > 512) {
-      cnt = row_sort<16>(int(c), k, o0, conn, perm, contrib, hcol, hpos, cbase);
-    } else if (c > 256) {
-      cnt = row_sort<8>(int(c), k, o0, conn, perm, contrib, hcol, hpos, cbase);
-    } else if (c > 128) {
-      cnt = row_sort<4>(int(c), k, o0, conn, perm, contrib, hcol, hpos, cbase);
-    } else if (c > 64) {
-      cnt = row_sort<2>(int(c), k, o0, conn, perm, contrib, hcol, hpos, cbase);
-    } else if (c > 0) {
-      cnt = row_sort<1>(int(c), k, o0, conn, perm, contrib, hcol, hpos, cbase);
+    if constexpr (LARGE) {
+      if (c <= 256 || c > kRowCap) return;
+      if (c > 512) {
+        cnt = row_sort<16>(int(c), k, o0, conn, perm, contrib, hcol, hpos, cbase);
+      } else {
+        cnt = row_sort<8>(int(c), k, o0, conn, perm, contrib, hcol, hpos, cbase);
+      }
+    } else {
+      if (c > kRowCap) {
+        if (lane == 0) flags[0] = 1;
+      } else if (c > 256) {
+        if (lane == 0) flags[2] = 1;
+        return;   // the LARGE launch writes this row
+      } else if (c > 128) {
+        cnt = row_sort<4>(int(c), k, o0, conn, perm, contrib, hcol, hpos, cbase);
+      } else if (c > 64) {
+        cnt = row_sort<2>(int(c), k, o0, conn, perm, contrib, hcol, hpos, cbase);
+      } else if (c > 0) {
+        cnt = row_sort<1>(int(c), k, o0, conn, perm, contrib, hcol, hpos, cbase);
+      }
     }
+  } else if (LARGE) {
+    return;
   }
   if (lane == 0) rowlen[i] = cnt;
 }
@@ -341,7 +357,8 @@ struct RowWs {
   int32_t* offs;
   int32_t* rowlen;
   int32_t* rowptr;
-  int32_t* flags;  // [0] a row beyond kRowCap, [1] 1: the global sort ran
+  int32_t* flags;  // [0] a row beyond kRowCap, [1] 1: the global sort ran,
+                   // [2] a row of 257 .. kRowCap pairs
   void* tmp;
   size_t tmp_bytes;
 };
@@ -361,7 +378,7 @@ int row_layout(cf::Carve& c, int64_t n_items, int64_t n_nodes, RowWs* w) {
   w->offs = c.take<int32_t>(n_nodes + 1);
   w->rowlen = c.take<int32_t>(n_nodes + 1);
   w->rowptr = c.take<int32_t>(n_nodes + 1);
-  w->flags = c.take<int32_t>(2);
+  w->flags = c.take<int32_t>(3);
   w->tmp_bytes = sort_bytes > scan_bytes ? sort_bytes : scan_bytes;
   w->tmp = c.take<char>(w->tmp_bytes);
   return CF_OK;
@@ -437,7 +454,7 @@ extern "C" int cf_pattern_sort(const int32_t* conn, int64_t n_elem, int32_t k,
     return CF_ERR_WORKSPACE;
   }
   hipStream_t s = cf::as_stream(stream);
-  CF_HIP(hipMemsetAsync(r.flags, 0, 2 * sizeof(int32_t), s));
+  CF_HIP(hipMemsetAsync(r.flags, 0, 3 * sizeof(int32_t), s));
   if (n == 0) {
     store_count<<<1, 1, 0, s>>>(w.incl, 0, nnz_dev);
     CF_LAUNCHED();
@@ -459,12 +476,18 @@ extern "C" int cf_pattern_sort(const int32_t* conn, int64_t n_elem, int32_t k,
   // distinct columns and positions of each row
   int32_t* hcol = reinterpret_cast<int32_t*>(w.keys_in);
   int32_t* hpos = reinterpret_cast<int32_t*>(w.keys_out);
-  row_pattern<<<int(rblocks), kBlock, 0, s>>>(n_nodes, k, conn, r.offs, r.perm, contrib, hcol,
-                                                hpos, r.rowlen, r.flags);
+  row_pattern<false><<<int(rblocks), kBlock, 0, s>>>(n_nodes, k, conn, r.offs, r.perm, contrib,
+                                                       hcol, hpos, r.rowlen, r.flags);
   CF_LAUNCHED();
-  int32_t over = 0;
-  CF_HIP(hipMemcpyAsync(&over, r.flags, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  int32_t fl[3] = {0, 0, 0};
+  CF_HIP(hipMemcpyAsync(fl, r.flags, sizeof(fl), hipMemcpyDeviceToHost, s));
   CF_HIP(hipStreamSynchronize(s));
+  const int32_t over = fl[0];
+  if (!over && fl[2]) {
+    row_pattern<true><<<int(rblocks), kBlock, 0, s>>>(n_nodes, k, conn, r.offs, r.perm, contrib,
+                                                        hcol, hpos, r.rowlen, r.flags);
+    CF_LAUNCHED();
+  }
   if (over) {
     set_last<<<1, 1, 0, s>>>(r.flags + 1, 1);
     CF_LAUNCHED();
