@@ -413,7 +413,7 @@ int cf_pcg_solve_mf_t10_mixed(int64_t n_nodes, int64_t n_elem, const int32_t* co
  *      inverse when its half-bandwidth is at most 62 (the aggregates follow
  *      the element order), else by rocSOLVER (dpotrf + dpotri; rocBLAS
  *      allocates its own small workspace and handle, once per device).  The
- *      power iteration for the Chebyshev bound runs on a side stream (created
+ *      Lanczos estimate for the Chebyshev bound runs on a side stream (created
  *      once per device) beside the coarse set-up, launched from a second host
  *      thread; on return the caller's stream is ordered after both.  Solves
  *      that overlap on one device share that stream: run them one at a time.
