@@ -5072,9 +5072,25 @@ band_cholesky(int nc, int bw, const double* __restrict__ A, double* __restrict__
   __syncthreads();
   band_fetch(nc, bw, A, 1 + bw + kBandChunk, v);
   int32_t bad = 0;
+  // this thread's first trailing entry (a0, b0), kept in registers
+  const int a0 = th < np ? (pab[th] & 255) : 0, b0 = th < np ? (pab[th] >> 8) : 0;
   for (int k0 = 0; k0 < nc; k0 += kBandChunk) {
     for (int s = 0; s < kBandChunk && k0 + s < nc; ++s) {
       const int k = k0 + s, ks = k % W;
+      // One phase per column: the column's entries are read unscaled and
+      // scaled by 1 / l where they are used (the same products as storing
+      // the scaled column first), so the update needs no barrier of its own
+      // and its reads go out beside the pivot's (1,500 unknowns, half-
+      // bandwidth 17: 853 against 936 us).
+      const double wout = (th > 0 && th <= bw && k + th < nc) ? win[(k + th) % W][ks] : 0.0;
+      double wa = 0.0, wb = 0.0, wab = 0.0;
+      const bool upd = th < np && k + 1 + a0 < nc;
+      const int ia = (k + 1 + a0) % W, jb = (k + 1 + b0) % W;
+      if (upd) {
+        wa = win[ia][ks];
+        wb = win[jb][ks];
+        wab = win[ia][jb];
+      }
       double d = win[ks][ks];
       if (!(d > 0.0)) {
         if (bad == 0) bad = k + 1;
@@ -5084,8 +5100,7 @@ band_cholesky(int nc, int bw, const double* __restrict__ A, double* __restrict__
       if (th <= bw) {
         const int i = k + th;
         if (i < nc) {
-          const double lik = th == 0 ? l : win[i % W][ks] * rl;
-          if (th > 0) win[i % W][ks] = lik;
+          const double lik = th == 0 ? l : wout * rl;
           Lc[int64_t(k) * W + th] = lik;
           Lr[int64_t(i) * W + th] = lik;
         } else {
@@ -5093,12 +5108,12 @@ band_cholesky(int nc, int bw, const double* __restrict__ A, double* __restrict__
         }
         if (th == 0) dinv[k] = rl;
       }
-      __syncthreads();
-      for (int p = th; p < np; p += kBandThreads) {
+      if (upd) win[ia][jb] = wab - (wa * rl) * (wb * rl);
+      for (int p = th + kBandThreads; p < np; p += kBandThreads) {
         const int a = pab[p] & 255, b = pab[p] >> 8;  // entry (k + 1 + a, k + 1 + b)
         if (k + 1 + a < nc) {
           const int i = (k + 1 + a) % W, j = (k + 1 + b) % W;
-          win[i][j] -= win[i][ks] * win[j][ks];
+          win[i][j] -= (win[i][ks] * rl) * (win[j][ks] * rl);
         }
       }
       const int r = k + 1 + bw;
