@@ -79,6 +79,12 @@ struct Vec3 {
   T v[3];
 };
 
+// two values staged as one aligned 8- or 16-byte LDS access
+template <class T>
+struct alignas(2 * sizeof(T)) Vec2 {
+  T v[2];
+};
+
 // ------------------------------------------------------------- reductions
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -1607,10 +1613,18 @@ mf_t10_elem_lp(int64_t m, const int32_t* __restrict__ conn, const T* __restrict_
                const uint16_t* __restrict__ tstart, T* __restrict__ part,
                const PcgState* state) {
   if (state && state->halt) return;
-  __shared__ T fl[kTileItems * 3];
+  // Forces are staged in two passes: (x, y) as one 8- or 16-byte pair per
+  // item, then z in the same buffer.  Two components per item instead of
+  // three keep the block at 20 KB of LDS (FP32), so six blocks fit a CU
+  // and a 384,000-element mesh (1,500 tiles) runs in one round of
+  // resident blocks instead of 1.17.
+  __shared__ Vec2<T> fl[kTileItems];
+  T* fz = reinterpret_cast<T*>(fl);
   const int64_t n_tiles = (m + kTile - 1) / kTile;
   for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < n_tiles; t += gridDim.x) {
     const int64_t e = t * kTile + threadIdx.x;
+    T fzl[10];
+    uint16_t rk[10];
     if (e < m) {
       T J[3][3];
 #pragma unroll
@@ -1638,10 +1652,12 @@ mf_t10_elem_lp(int64_t m, const int32_t* __restrict__ conn, const T* __restrict_
       t10_forces_4pt<T>(xa, J, V, lam, mu, f);
 #pragma unroll
       for (int a = 0; a < 10; ++a) {
-        const int r = rank[tile_packed(t, a)];
-        fl[3 * r] = f[a][0];
-        fl[3 * r + 1] = f[a][1];
-        fl[3 * r + 2] = f[a][2];
+        rk[a] = rank[tile_packed(t, a)];
+        Vec2<T> v;
+        v.v[0] = f[a][0];
+        v.v[1] = f[a][1];
+        fl[rk[a]] = v;
+        fzl[a] = f[a][2];
       }
     }
     __syncthreads();
@@ -1651,16 +1667,27 @@ mf_t10_elem_lp(int64_t m, const int32_t* __restrict__ conn, const T* __restrict_
     const int32_t q0 = toff[t], q1 = toff[t + 1];
     for (int32_t q = q0 + threadIdx.x; q < q1; q += kTile) {
       const int kb = tstart[q], ke = q + 1 < q1 ? int(tstart[q + 1]) : nt;
-      T s0 = T(0), s1 = T(0), s2 = T(0);
+      T s0 = T(0), s1 = T(0);
       for (int k = kb; k < ke; ++k) {
-        s0 += fl[3 * k];
-        s1 += fl[3 * k + 1];
-        s2 += fl[3 * k + 2];
+        const Vec2<T> v = fl[k];
+        s0 += v.v[0];
+        s1 += v.v[1];
       }
       T* o = part + 3 * int64_t(q);
       o[0] = s0;
       o[1] = s1;
-      o[2] = s2;
+    }
+    __syncthreads();
+    if (e < m) {
+#pragma unroll
+      for (int a = 0; a < 10; ++a) fz[rk[a]] = fzl[a];
+    }
+    __syncthreads();
+    for (int32_t q = q0 + threadIdx.x; q < q1; q += kTile) {
+      const int kb = tstart[q], ke = q + 1 < q1 ? int(tstart[q + 1]) : nt;
+      T s2 = T(0);
+      for (int k = kb; k < ke; ++k) s2 += fz[k];
+      part[3 * int64_t(q) + 2] = s2;
     }
     __syncthreads();
   }
